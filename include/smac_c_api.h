@@ -1,0 +1,209 @@
+/*
+ * smac_c_api.h -- C-ABI of the MI355X (gfx950) SMAC projection step.
+ *
+ * One context owns every device array of one staggered (MAC) grid plus one HIP
+ * stream. All entry points are plain C: opaque handle, plain pointers and sizes,
+ * int status codes (0 = ok, negative = SMAC_ERR_*). Non-convergence of the
+ * pressure solve is not an error; it is reported in smac_solve_stats.
+ *
+ * Each entry point below names the reference interface it replaces
+ * (bi-rugaoishii/myDEMCFD, paths relative to the reference root).
+ *
+ * Layout of host buffers (smac_upload / smac_download) is the reference's
+ * padded x-fastest layout, MyArray<T,3>::operator() (CFD/MyArray.h:35-57):
+ *   cells   (Nx+2)(Ny+2)(Nz+2)            index (k*(Ny+2)+j)*(Nx+2)+i
+ *   x-faces (Nx+3)(Ny+2)(Nz+2)            index (k*(Ny+2)+j)*(Nx+3)+i
+ *   y-faces (Nx+2)(Ny+3)(Nz+2)            index (k*(Ny+3)+j)*(Nx+2)+i
+ *   z-faces (Nx+2)(Ny+2)(Nz+3)            index (k*(Ny+2)+j)*(Nx+2)+i
+ * (CFD/memberList/gridMembers.def:14-128). Cell i in 1..Nx is interior; x-face
+ * i lies between cells i-1 and i; x-faces 1 and Nx+1 are the walls.
+ */
+#ifndef SMAC_C_API_H
+#define SMAC_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMAC_API_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    SMAC_OK = 0,
+    SMAC_ERR_INVALID_ARG = -1,   /* null pointer, bad enum, bad count        */
+    SMAC_ERR_INVALID_GRID = -2,  /* sizes the multigrid hierarchy cannot use  */
+    SMAC_ERR_HIP = -3,           /* a HIP runtime call failed                 */
+    SMAC_ERR_NO_DEVICE = -4,     /* no gfx950 device visible                  */
+    SMAC_ERR_OOM = -5,           /* device allocation failed                  */
+    SMAC_ERR_NAN = -6,           /* NaN/Inf met in the solve                  */
+    SMAC_ERR_UNSUPPORTED = -7    /* option combination not built              */
+};
+
+/* ---- enums mirrored from CFD/Enums.h:3-36 -------------------------------- */
+enum smac_bc_type {              /* BoundaryType (CFD/Enums.h:10-16)          */
+    SMAC_BC_NOSLIP = 0,
+    SMAC_BC_SLIP = 1,
+    SMAC_BC_INFLOW = 2,
+    SMAC_BC_OUTFLOW = 3,
+    SMAC_BC_OUTLET = 4
+};
+
+enum smac_side { SMAC_XMIN = 0, SMAC_XMAX, SMAC_YMIN, SMAC_YMAX, SMAC_ZMIN, SMAC_ZMAX };
+
+/* Fields addressable through smac_upload/smac_download. Names follow the
+ * reference members of CFD/memberList/gridMembers.def. */
+enum smac_field {
+    SMAC_FIELD_U = 0,        /* f_vx_       x-faces */
+    SMAC_FIELD_V = 1,        /* f_vy_       y-faces */
+    SMAC_FIELD_W = 2,        /* f_vz_       z-faces */
+    SMAC_FIELD_USTAR = 3,    /* f_vx_star_  x-faces */
+    SMAC_FIELD_VSTAR = 4,    /* f_vy_star_  y-faces */
+    SMAC_FIELD_WSTAR = 5,    /* f_vz_star_  z-faces */
+    SMAC_FIELD_P = 6,        /* p_          cells   */
+    SMAC_FIELD_DP = 7,       /* p_delta_    cells   */
+    SMAC_FIELD_RHS = 8,      /* rhs_        cells   */
+    SMAC_FIELD_ALPHA = 9,    /* alpha_      cells   (VoF fraction, drives rho/mu) */
+    SMAC_FIELD_COUNT = 10
+};
+
+/* PCG_TYPES (CFD/Enums.h:28-33) plus the north-star multigrid. */
+enum smac_precond {
+    SMAC_PRECOND_JACOBI = 0,     /* STANDARD_PCG / PURENEUMANN_STANDARD_PCG  */
+    SMAC_PRECOND_MG = 1          /* RBGS V-cycle geometric multigrid          */
+};
+
+enum smac_precision {
+    SMAC_PREC_FP64 = 0,          /* everything in FP64                        */
+    SMAC_PREC_MIXED_FP32 = 1,    /* V-cycle storage+arithmetic FP32           */
+    SMAC_PREC_MIXED_BF16 = 2     /* V-cycle storage BF16, FP32 arithmetic     */
+};
+
+enum smac_stop_rule {
+    SMAC_STOP_REF_LINF_DT = 0,   /* max|r|*dt < tol  (G_PCGSolver.cu:884-903) */
+    SMAC_STOP_REL_L2 = 1         /* ||r||_2 < tol*||b||_2                     */
+};
+
+/* ---- descriptors --------------------------------------------------------- */
+typedef struct smac_bc_entry {   /* one row of the bc table,
+                                    boundaryConditionMembers.def:1-4          */
+    int32_t type;                /* enum smac_bc_type */
+    double vx, vy, vz;           /* wall / inflow velocity */
+} smac_bc_entry;
+
+typedef struct smac_grid_desc {
+    int32_t nx, ny, nz;          /* interior cells per direction              */
+    double sizex, sizey, sizez;  /* domain extents (SMACSolver::set_calc_properties) */
+    double originx, originy, originz;
+    int32_t periodic[3];         /* 1 = periodic in that direction (extension:
+                                    the reference has no periodic BC)         */
+    uint8_t side_bcid[6];        /* bc id of each wall side, indexed by smac_side
+                                    (StaggeredGrid::set_boundary_id,
+                                    StaggeredGrid.h:75-115); unused if periodic */
+    int32_t num_bc;              /* rows of bc_table (<= 256)                 */
+    const smac_bc_entry* bc_table;
+} smac_grid_desc;
+
+typedef struct smac_fluid_desc { /* G_SMACSolver::rho0_.. gx_ (G_SMACSolver.h:11-16) */
+    double rho0, rho1;           /* phase densities                           */
+    double mu0, mu1;             /* phase viscosities                         */
+    double gx, gy, gz;           /* gravity                                   */
+} smac_fluid_desc;
+
+typedef struct smac_solver_opts {
+    int32_t precond;             /* enum smac_precond                         */
+    int32_t precision;           /* enum smac_precision                       */
+    int32_t stop_rule;           /* enum smac_stop_rule                       */
+    double tol;                  /* G_PressureSolverBase::tol_                */
+    int32_t check_every;         /* convergence check cadence (ref: 10 Jacobi, 5 GMG) */
+    int32_t max_iter;            /* ref: 2000                                 */
+    int32_t mg_levels;           /* 0 = automatic                             */
+    int32_t nu1, nu2;            /* pre/post RBGS sweeps per level            */
+    int32_t flexible;            /* 1 = Polak-Ribiere beta (auto-on for mixed) */
+} smac_solver_opts;
+
+typedef struct smac_solve_stats {
+    int32_t iters;               /* PCG iterations (G_PCGSolver.cu:921)       */
+    int32_t converged;
+    int32_t pure_neumann;
+    int32_t mg_levels;
+    double rel_res;              /* ||r||_2/||b||_2 at exit                   */
+    double linf_res_dt;          /* max|r|*dt at exit                         */
+    double bnorm;                /* ||b - mean b||_2                          */
+    double ms_predictor, ms_solve, ms_correct; /* device time, when profiling */
+} smac_solve_stats;
+
+typedef struct smac_ctx smac_ctx;
+
+/* ---- lifecycle ------------------------------------------------------------
+ * Replaces G_SMACSolver::set_calc_properties + solver_malloc + set_block_grid
+ * (CFD/G_SMACSolver.h:25,84,90-100; CFD/G_SMACSolver.cu:1451-1496) and the host
+ * setup SMACSolver::set_face_type / set_cell_type / set_face_internal_direction
+ * (CFD/SMACSolver.cpp:37-248). All fields start at zero (the reference
+ * calloc's its host arrays, CFD/SMACSolver.cpp:423-440). */
+int smac_create(const smac_grid_desc* grid, const smac_fluid_desc* fluid, smac_ctx** out);
+/* Replaces G_SMACSolver::solver_free (CFD/G_SMACSolver.cu:1433-1449). */
+void smac_destroy(smac_ctx* ctx);
+/* Replaces the rho/mu/g members written by the driver (CFD/main.cu:130-133,203-206). */
+int smac_set_fluid(smac_ctx* ctx, const smac_fluid_desc* fluid);
+
+/* ---- data movement ----------------------------------------------------------
+ * Replaces G_SMACSolver::cpuTogpu / gpuTocpu (CFD/G_SMACSolver.h:82-83,
+ * CFD/G_SMACSolver.cu:1507-1535) for one field. count = element count of the
+ * field in the reference layout (above). Synchronous. */
+int smac_upload(smac_ctx* ctx, int32_t field, const double* host, size_t count);
+int smac_download(smac_ctx* ctx, int32_t field, double* host, size_t count);
+/* Same, from/to a device buffer (e.g. a torch tensor) in the reference layout. */
+int smac_upload_device(smac_ctx* ctx, int32_t field, const double* dev, size_t count);
+int smac_download_device(smac_ctx* ctx, int32_t field, double* dev, size_t count);
+/* Element count of a field in the reference layout. */
+size_t smac_field_count(const smac_ctx* ctx, int32_t field);
+
+/* ---- the step ---------------------------------------------------------------
+ * Property assembly from alpha: G_SMACSolver::update_properties_by_alpha +
+ * update_boundary_faces (CFD/G_SMACSolver_VoF.cu:1544-1566,1944-1950), the t=0
+ * host form SMACSolver::update_properties_by_alpha_initial
+ * (CFD/SMACSolver.cpp:283-421) when initial != 0 (rho_old = rho). */
+int smac_update_properties(smac_ctx* ctx, int32_t initial);
+/* Mass flux mf = rho0*u_f (alpha flux = 0):
+ * G_SMACSolver::compute_mass_flux_from_alpha_flux (CFD/G_SMACSolver_VoF.cu:1708-1795). */
+int smac_compute_mass_flux(smac_ctx* ctx, double dt);
+/* G_SMACSolver::get_vof_vstar_rhouu_consistent + update_vstar_boundary
+ * (CFD/G_SMACSolver.h:71,74; CFD/G_SMACSolver.cu:1044-1069). */
+int smac_predictor(smac_ctx* ctx, double dt);
+/* G_SMACSolver::solve_poisson -> G_PCGSolver::solve (CFD/G_SMACSolver.cu:1499-1501,
+ * CFD/pressure_solver/G_PCGSolver.cu:592-617): RHS assembly
+ * (G_PressureSolverBase.cu:49-118), operator build (G_PCGSolver.cu:126-212) and the
+ * preconditioned CG on A dp = b, dp starting from 0. */
+int smac_pressure_solve(smac_ctx* ctx, double dt, const smac_solver_opts* opts,
+                        smac_solve_stats* stats);
+/* G_SMACSolver::correct_vof_velocity (CFD/G_SMACSolver.cu:1072-1142,1269-1275):
+ * u = u* - beta*dt*grad(dp) on interior faces, p += dp, dp = 0. */
+int smac_correct(smac_ctx* ctx, double dt);
+/* One whole step: properties (if variable), mass flux, predictor, solve, correct,
+ * in the order of the driver loop CFD/main.cu:298-317. */
+int smac_step(smac_ctx* ctx, double dt, const smac_solver_opts* opts, smac_solve_stats* stats);
+/* G_SMACSolver::calc_cfl (CFD/G_SMACSolver.cu:1365-1430): convective and viscous
+ * CFL numbers for time step dt. */
+int smac_calc_cfl(smac_ctx* ctx, double dt, double* cfl_conv, double* cfl_visc);
+
+/* Fill opts with the defaults of the north-star solver (MG, FP64, REL_L2, 1e-8). */
+void smac_default_opts(smac_solver_opts* opts);
+/* Device-time profiling of the step stages and of named kernels (hipEvents). */
+int smac_set_profiling(smac_ctx* ctx, int32_t on);
+/* Accumulated count and device milliseconds of kernel `name` since profiling was
+ * switched on. Returns SMAC_ERR_INVALID_ARG for an unknown name. */
+int smac_kernel_time(smac_ctx* ctx, const char* name, int64_t* launches, double* ms);
+/* Synchronize the context stream. */
+int smac_sync(smac_ctx* ctx);
+/* HIP stream of the context (hipStream_t), for callers that order work with it. */
+void* smac_stream(smac_ctx* ctx);
+const char* smac_strerror(int status);
+const char* smac_backend_name(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMAC_C_API_H */

@@ -1,0 +1,35 @@
+"""MI355X (gfx950) SMAC projection step.
+
+`load()` returns the HIP C-ABI library (mydemcfd_amd/lib/libsmac_hip.so) and raises
+if it is missing or no device is present; there is no CPU fallback.
+`SMACContext` is a context on that library.
+"""
+from __future__ import annotations
+
+import ctypes as _C
+import os as _os
+
+from . import _abi
+from ._abi import Context as _Context, SmacError  # noqa: F401
+
+HERE = _os.path.dirname(_os.path.abspath(__file__))
+LIB_PATH = _os.path.join(HERE, "lib", "libsmac_hip.so")
+
+_lib = None
+
+
+def load():
+    """Load the HIP library. Raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not _os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        _lib = _abi.bind(_C.CDLL(LIB_PATH, mode=_C.RTLD_LOCAL))
+    return _lib
+
+
+class SMACContext(_Context):
+    """A smac_ctx on the HIP library (device arrays live on the current GPU)."""
+
+    def __init__(self, *a, **kw):
+        super().__init__(load(), *a, **kw)

@@ -1,0 +1,135 @@
+"""Workload definitions and seeded initial conditions.
+
+Every array is float64 in the reference padded layout (numpy shape (z, y, x),
+x fastest; see _abi.field_shape). Face index f along its own axis sits at
+coordinate origin + (f-1)*h; cell index c at origin + (c-1/2)*h, as in
+StaggeredGrid::get_cell_coord (CFD/StaggeredGrid.h:121-142).
+
+Workloads:
+  cavity   lid-driven cavity, unit cube, all walls no-slip, lid u=1 on y-max
+           (the reference's gravity convention is -y, CFD/config.json), Re=UL/nu.
+  tg       triply periodic [0,2pi)^3 Taylor-Green, u=sin x cos y cos z,
+           v=-cos x sin y cos z, w=0, plus a seeded solenoidal perturbation.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _abi
+
+
+def cavity_desc(n=64, re=100.0, lid=1.0):
+    """Context kwargs for the lid-driven cavity: bc id 1 = no-slip at rest on
+    five sides, bc id 2 = no-slip moving with u=lid on the y-max side."""
+    nu = lid * 1.0 / re
+    return dict(n=(n, n, n), size=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), periodic=(0, 0, 0),
+                side_bcid=(1, 1, 1, 2, 1, 1),
+                bc_table=((_abi.BC_NOSLIP, 0.0, 0.0, 0.0), (_abi.BC_NOSLIP, 0.0, 0.0, 0.0),
+                          (_abi.BC_NOSLIP, lid, 0.0, 0.0)),
+                rho=(1.0, 1.0), mu=(nu, nu), g=(0.0, 0.0, 0.0))
+
+
+def periodic_desc(n=256, re=1600.0, length=2.0 * math.pi):
+    nu = 1.0 / re
+    nn = n if isinstance(n, (tuple, list)) else (n, n, n)
+    return dict(n=tuple(nn), size=(length, length, length), origin=(0.0, 0.0, 0.0), periodic=(1, 1, 1),
+                side_bcid=(0,) * 6, bc_table=((_abi.BC_NOSLIP, 0.0, 0.0, 0.0),),
+                rho=(1.0, 1.0), mu=(nu, nu), g=(0.0, 0.0, 0.0))
+
+
+def _embed_periodic(a0, field):
+    """Place a periodic (z, y, x) array of interior values (face/cell index 1..N)
+    into the reference padded layout, filling ghosts and duplicate faces."""
+    nz, ny, nx = a0.shape
+    shape = _abi.field_shape((nx, ny, nz), field)
+    out = np.empty(shape, dtype=np.float64)
+    iz = (np.arange(shape[0]) - 1) % nz
+    iy = (np.arange(shape[1]) - 1) % ny
+    ix = (np.arange(shape[2]) - 1) % nx
+    out[...] = a0[np.ix_(iz, iy, ix)]
+    return out
+
+
+def taylor_green(n, length=2.0 * math.pi):
+    """(u, v, w) interior arrays (z, y, x), N^3 each, of the Taylor-Green field."""
+    h = length / n
+    xf = np.arange(n) * h            # face coordinate of face index 1..N
+    xc = (np.arange(n) + 0.5) * h    # cell centre
+    u = np.sin(xf)[None, None, :] * np.cos(xc)[None, :, None] * np.cos(xc)[:, None, None]
+    v = -np.cos(xc)[None, None, :] * np.sin(xf)[None, :, None] * np.cos(xc)[:, None, None]
+    w = np.zeros((n, n, n))
+    return u, v, w
+
+
+def discrete_curl(Ax, Ay, Az, h):
+    """MAC-grid curl of an edge-centred vector potential on a periodic box:
+    exactly (to rounding) divergence free under the staggered divergence."""
+    r = np.roll
+    u = (r(Az, -1, 1) - Az) / h - (r(Ay, -1, 0) - Ay) / h
+    v = (r(Ax, -1, 0) - Ax) / h - (r(Az, -1, 2) - Az) / h
+    w = (r(Ay, -1, 2) - Ay) / h - (r(Ax, -1, 1) - Ax) / h
+    return u, v, w
+
+
+def white_solenoidal(n, seed, rms, length=2.0 * math.pi):
+    """Curl of an i.i.d. N(0,1) edge potential, scaled to the given velocity RMS."""
+    rng = np.random.default_rng(seed)
+    A = [rng.standard_normal((n, n, n)) for _ in range(3)]
+    u, v, w = discrete_curl(*A, length / n)
+    s = math.sqrt((np.mean(u * u) + np.mean(v * v) + np.mean(w * w)) / 3.0)
+    return u * (rms / s), v * (rms / s), w * (rms / s)
+
+
+def spectral_solenoidal(n, seed, kmax=32, slope=-5.0 / 3.0, length=2.0 * math.pi):
+    """Random-phase solenoidal field with shell energy E(k) ~ k^slope for
+    1 <= |k| <= kmax: potential amplitude |A(k)| ~ k^((slope-4)/2), curled
+    discretely so the field is divergence free on the MAC grid. Unit RMS."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)
+    kz = k1[:, None, None]
+    ky = k1[None, :, None]
+    kx = np.fft.rfftfreq(n, d=1.0 / n)[None, None, :]
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    amp = np.where((kk >= 1.0) & (kk <= kmax), np.power(np.maximum(kk, 1.0), (slope - 4.0) / 2.0), 0.0)
+    A = []
+    for _ in range(3):
+        ph = rng.uniform(0.0, 2.0 * math.pi, size=amp.shape)
+        mag = rng.standard_normal(amp.shape)
+        spec = amp * mag * np.exp(1j * ph)
+        A.append(np.fft.irfftn(spec, s=(n, n, n)).real)
+        del spec, ph, mag
+    u, v, w = discrete_curl(*A, length / n)
+    s = math.sqrt((np.mean(u * u) + np.mean(v * v) + np.mean(w * w)) / 3.0)
+    return u / s, v / s, w / s
+
+
+def tg_fields(n, seed=0x5EED0002, pert="white", pert_rms=1e-3, energy_frac=0.05):
+    """Taylor-Green plus perturbation, as reference-layout (u, v, w).
+    pert="white": curl of white noise scaled to RMS pert_rms (config 2);
+    pert="k53": k^-5/3 spectrum, k<=32, scaled to energy_frac of the total
+    kinetic energy (config 3)."""
+    u, v, w = taylor_green(n)
+    if pert == "white":
+        pu, pv, pw = white_solenoidal(n, seed, pert_rms)
+    elif pert == "k53":
+        pu, pv, pw = spectral_solenoidal(n, seed)
+        e_tg = 0.5 * (np.mean(u * u) + np.mean(v * v) + np.mean(w * w))
+        e_p = 0.5 * (np.mean(pu * pu) + np.mean(pv * pv) + np.mean(pw * pw))
+        scale = math.sqrt(energy_frac / (1.0 - energy_frac) * e_tg / e_p)
+        pu, pv, pw = pu * scale, pv * scale, pw * scale
+    elif pert is None:
+        pu = pv = pw = 0.0
+    else:
+        raise ValueError(pert)
+    return (_embed_periodic(u + pu, _abi.FIELD_U), _embed_periodic(v + pv, _abi.FIELD_V),
+            _embed_periodic(w + pw, _abi.FIELD_W))
+
+
+def cfl_dt(ctx, cfl_target, dt_probe=1.0, eps=1e-12):
+    """dt with max(CFL_conv, CFL_visc) = cfl_target, from the CFL of a probe dt
+    (both CFL numbers are linear in dt); the rule of CFDTime::updateTime
+    (CFD/CFDTime.cpp:22-48) with the clamps removed."""
+    cc, cv = ctx.calc_cfl(dt_probe)
+    return dt_probe * cfl_target / (max(cc, cv) + eps)

@@ -1,0 +1,188 @@
+// smac_dev.hpp -- device-side geometry, indexing and boundary getters of the
+// MI355X SMAC step. Every array (cells and the three face families) uses one
+// padded box (nx+3)(ny+3)(nz+3), x fastest; reference index (i,j,k) keeps its
+// meaning (cell i in 1..nx interior, x-face i between cells i-1 and i).
+//
+// Arithmetic contract: the library is compiled with -ffp-contract=off and every
+// expression below is evaluated in the order written, so FP64 results are
+// bit-identical to the CPU restatement of the reference kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace smac {
+
+enum { F_INTERIOR = 0, F_BOUNDARY = 1, F_GHOST = 2 };  // CFD/Enums.h:3-8
+enum { BC_NOSLIP = 0, BC_SLIP = 1, BC_INFLOW = 2, BC_OUTFLOW = 3, BC_OUTLET = 4 };  // CFD/Enums.h:10-16
+
+// Geometry + boundary table + uniform fluid properties, passed by value.
+struct Geo {
+  int n[3];        // interior cells
+  int px, py;      // pitches (nx+3, ny+3)
+  int per[3];      // periodic flags
+  double inv_d[3], inv_d2[3];
+  // bc slots: 0..5 = the six sides (xmin,xmax,ymin,ymax,zmin,zmax), 6 = bc id 0
+  int btype[7];
+  double bv[7][3];
+  // uniform properties (single phase): rho0, mu0, beta0 = 1/(0.5*(rho0+rho0))
+  double rho0, mu0, beta0, g[3];
+};
+
+struct I3 {
+  int v[3];
+};
+
+__host__ __device__ __forceinline__ I3 mk(int i, int j, int k) {
+  I3 r;
+  r.v[0] = i;
+  r.v[1] = j;
+  r.v[2] = k;
+  return r;
+}
+__host__ __device__ __forceinline__ I3 sh(I3 a, int d, int s) {
+  a.v[d] += s;
+  return a;
+}
+__host__ __device__ __forceinline__ int wrapi(int i, int n) { return i < 1 ? i + n : (i > n ? i - n : i); }
+__host__ __device__ __forceinline__ int W(const Geo& g, int d, int i) { return g.per[d] ? wrapi(i, g.n[d]) : i; }
+__host__ __device__ __forceinline__ I3 wr(const Geo& g, I3 a) {
+  a.v[0] = W(g, 0, a.v[0]);
+  a.v[1] = W(g, 1, a.v[1]);
+  a.v[2] = W(g, 2, a.v[2]);
+  return a;
+}
+__host__ __device__ __forceinline__ size_t idx(const Geo& g, int i, int j, int k) {
+  return ((size_t)k * (size_t)g.py + (size_t)j) * (size_t)g.px + (size_t)i;
+}
+__host__ __device__ __forceinline__ size_t IX(const Geo& g, I3 a) { return idx(g, a.v[0], a.v[1], a.v[2]); }
+
+// Analytic face type of a box without internal walls
+// (SMACSolver::set_face_type, CFD/SMACSolver.cpp:104-202); q already wrapped.
+__device__ __forceinline__ int face_type(const Geo& g, int a, I3 q) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (d == a) continue;
+    if (q.v[d] < 1 || q.v[d] > g.n[d]) return F_GHOST;
+  }
+  int i = q.v[a], n = g.n[a];
+  if (g.per[a]) return (i >= 1 && i <= n) ? F_INTERIOR : F_GHOST;
+  if (i == 1 || i == n + 1) return F_BOUNDARY;
+  if (i >= 2 && i <= n) return F_INTERIOR;
+  return F_GHOST;
+}
+// bc slot of an a-face (6 = bc id 0 off the walls)
+__device__ __forceinline__ int face_slot(const Geo& g, int a, I3 q) {
+  if (g.per[a]) return 6;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (d == a) continue;
+    if (q.v[d] < 1 || q.v[d] > g.n[d]) return 6;
+  }
+  if (q.v[a] == 1) return 2 * a;
+  if (q.v[a] == g.n[a] + 1) return 2 * a + 1;
+  return 6;
+}
+
+// Velocity arrays of the three components.
+struct Vel {
+  const double* c[3];
+};
+
+// d_get_v{x,y,z}_{x,y,z}face (CFD/G_BoundaryFunctions.h:9-97)
+__device__ __forceinline__ double get_face(const Geo& g, const Vel& V, int a, I3 q) {
+  q = wr(g, q);
+  int t = face_type(g, a, q);
+  if (t == F_INTERIOR) return V.c[a][IX(g, q)];
+  if (t == F_BOUNDARY) {
+    int s = face_slot(g, a, q);
+    int bt = g.btype[s];
+    if (bt == BC_INFLOW) return (1. - 0.0) * g.bv[s][a];
+    if (bt == BC_NOSLIP) return 0.;
+    if (bt == BC_OUTLET) return V.c[a][IX(g, q)];
+    if (bt == BC_SLIP && a != 0) return V.c[a][IX(g, q)];
+  }
+  return 0.;
+}
+
+// d_get_v{a}_{b}dir (CFD/G_BoundaryFunctions.h:100-367)
+__device__ __forceinline__ double get_dir(const Geo& g, const Vel& V, int a, int b, I3 q, int s) {
+  q = wr(g, q);
+  double vin = V.c[a][IX(g, q)];
+  I3 q2 = wr(g, sh(q, b, s));
+  if (face_type(g, a, q2) == F_INTERIOR) return V.c[a][IX(g, q2)];
+  I3 qf = q;
+  qf.v[b] = s > 0 ? q.v[b] + 1 : q.v[b];
+  double vw = 0.0;
+  int count = 0;
+  if (face_type(g, b, qf) == F_BOUNDARY) {
+    vw += g.bv[face_slot(g, b, qf)][a];
+    count++;
+  }
+  I3 qf2 = wr(g, sh(qf, a, -1));
+  if (face_type(g, b, qf2) == F_BOUNDARY) {
+    vw += g.bv[face_slot(g, b, qf2)][a];
+    count++;
+  }
+  if (count > 0) vw /= (double)count;
+  int bt = g.btype[face_slot(g, b, qf)];
+  if (bt == BC_SLIP || bt == BC_OUTLET) return vin;
+  return 2.0 * vw - vin;
+}
+
+// d_minmod (CFD/G_SMACSolver.cu:67-88)
+__device__ __forceinline__ double minmod(double dp, double dm) {
+  double prod = dp * dm;
+  double s = 0.;
+  if (prod > 0.) s = fabs(dp) < fabs(dm) ? dp : dm;
+  return s;
+}
+// sgn (CFD/G_SMACSolver.h:130-139)
+__device__ __forceinline__ int sgn(double a) { return a < 0 ? -1 : 1; }
+
+// Uniform-property Poisson coefficients: a = beta0/h^2 on interior faces, the
+// diagonal sums all six (wall faces carry beta = 0), off-diagonals vanish at
+// walls (k_build_vof_poisson_A, CFD/pressure_solver/G_PCGSolver.cu:126-212).
+struct Coef {
+  double a[6];  // xp, xm, yp, ym, zp, zm
+  double diag;
+};
+__device__ __forceinline__ bool wall_face(const int per, int i, int n) { return !per && (i == 1 || i == n + 1); }
+
+__device__ __forceinline__ Coef coef_uniform(const int n[3], const int per[3], const double inv_d2[3],
+                                             double beta0, int i, int j, int k) {
+  Coef c;
+  int q[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double bp = wall_face(per[d], q[d] + 1, n[d]) ? 0.0 : beta0;
+    double bm = wall_face(per[d], q[d], n[d]) ? 0.0 : beta0;
+    c.a[2 * d] = bp * inv_d2[d];
+    c.a[2 * d + 1] = bm * inv_d2[d];
+  }
+  c.diag = c.a[0] + c.a[1] + c.a[2] + c.a[3] + c.a[4] + c.a[5];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (wall_face(per[d], q[d] + 1, n[d])) c.a[2 * d] = 0.;
+    if (wall_face(per[d], q[d], n[d])) c.a[2 * d + 1] = 0.;
+  }
+  return c;
+}
+
+// Canonical wave64 halving tree: lane l += lane l+s, s = 32..1; lane 0 holds
+// the result. Identical association to the CPU oracle's tree().
+__device__ __forceinline__ double wave_tree_sum(double v) {
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) v = v + __shfl_down(v, s, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) {
+    double o = __shfl_down(v, s, 64);
+    if (o > v || o != o) v = o;
+  }
+  return v;
+}
+
+}  // namespace smac

@@ -1,0 +1,77 @@
+"""Loader for the CPU FP64 oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+The oracle library exports the same C-ABI as the HIP product, plus a few
+smac_oracle_* hooks for operator property tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from mydemcfd_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "libsmac_oracle.so")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        l = C.CDLL(LIB, mode=C.RTLD_LOCAL)
+        _abi.bind(l)
+        l.smac_oracle_apply_A.restype = C.c_int
+        l.smac_oracle_apply_A.argtypes = [C.c_void_p, _abi._D, _abi._D]
+        l.smac_oracle_apply_mg.restype = C.c_int
+        l.smac_oracle_apply_mg.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), _abi._D, _abi._D]
+        l.smac_oracle_csum.restype = C.c_double
+        l.smac_oracle_csum.argtypes = [C.c_void_p, _abi._D]
+        l.smac_oracle_threads.restype = C.c_int
+        l.smac_oracle_threads.argtypes = []
+        _lib = l
+    return _lib
+
+
+class OracleContext(_abi.Context):
+    """A smac_ctx of the CPU oracle."""
+
+    def __init__(self, *a, **kw):
+        super().__init__(lib(), *a, **kw)
+
+    def apply_A(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._check(self.lib.smac_oracle_apply_A(self.h, x.ctypes.data_as(_abi._D), y.ctypes.data_as(_abi._D)),
+                    "apply_A")
+        return y
+
+    def apply_mg(self, r, **kw):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        o = self.opts(**kw)
+        self._check(self.lib.smac_oracle_apply_mg(self.h, C.byref(o), r.ctypes.data_as(_abi._D),
+                                                  z.ctypes.data_as(_abi._D)), "apply_mg")
+        return z
+
+    def csum(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        return self.lib.smac_oracle_csum(self.h, x.ctypes.data_as(_abi._D))
+
+
+def threads() -> int:
+    return lib().smac_oracle_threads()

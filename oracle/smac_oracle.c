@@ -1,0 +1,1313 @@
+/*
+ * smac_oracle.c -- CPU FP64 restatement of the reference SMAC projection step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP product
+ * (mydemcfd_amd/csrc). Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it. It exports the same C-ABI as the product
+ * (include/smac_c_api.h) so both can be driven by identical test code.
+ *
+ * Reference: bi-rugaoishii/myDEMCFD. The reference CPU SMACSolver has no step
+ * (CFD/SMACSolver.cpp holds setup only; CFD/main.cu:60-69 refuses CPU runs), so
+ * this file restates the reference's GPU kernels, with eps (void fraction) = 1,
+ * surface tension = 0, no IBM solid and no CFD-DEM coupling impulse:
+ *   predictor          CFD/G_SMACSolver.cu:67-88 (minmod), 90-395 (x), 397-717 (y), 721-1012 (z)
+ *   boundary getters   CFD/G_BoundaryFunctions.h:9-486, star BC G_BoundaryFunctions.cu:3-71
+ *   properties         CFD/SMACSolver.cpp:283-421, CFD/G_SMACSolver_VoF.cu:1374-1566,1798-1950
+ *   mass flux          CFD/G_SMACSolver_VoF.cu:1708-1790
+ *   Poisson RHS        CFD/pressure_solver/G_PressureSolverBase.cu:49-118
+ *   operator, SpMV     CFD/pressure_solver/G_PCGSolver.cu:87-245
+ *   Jacobi PCG         CFD/pressure_solver/G_PCGSolver.cu:307-586
+ *   mean removal       CFD/pressure_solver/G_PressureSolverBase.cu:142-155
+ *   correction         CFD/G_SMACSolver.cu:1072-1142,1269-1275
+ *   CFL                CFD/G_SMACSolver.cu:1365-1430
+ * Extensions the reference lacks (periodic directions, the RBGS multigrid
+ * V-cycle preconditioner, the relative-L2 stopping rule) follow the build's own
+ * specification; the GPU product implements the same arithmetic.
+ *
+ * Determinism contract shared with the GPU product: no FMA contraction
+ * (-ffp-contract=off), every expression evaluated in the order written here,
+ * and every sum over interior cells taken in the canonical order of csum()
+ * (row partials over 64 lanes, halving tree, then a 1024-wide final pass).
+ */
+#include "../include/smac_c_api.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { F_INTERIOR = 0, F_BOUNDARY = 1, F_GHOST = 2 };   /* CFD/Enums.h:3-8 */
+
+#define MAXL 12
+
+typedef struct level {
+    int n[3];            /* interior cells */
+    int p[3];            /* pitches n+3 */
+    size_t tot;
+    double inv_d2[3];
+    double *e, *f, *res; /* correction, rhs, residual (cells) */
+    double *b[3];        /* face coefficients beta */
+} level;
+
+struct smac_ctx {
+    int n[3], p[3];
+    size_t tot;
+    double d[3], inv_d[3], inv_d2[3];
+    int per[3];
+    uint8_t side_bcid[6];
+    int nbc;
+    int bctype[256];
+    double bcv[256][3];
+    smac_fluid_desc fl;
+    /* fields, all in the common (n+3)^3 layout */
+    double *vel[3], *vst[3], *p_, *dp, *rhs, *alpha;
+    double *rho, *rho_old, *inv_rho, *mu;
+    double *beta[3], *inv_rhof[3], *muf[3], *mf[3], *rhof[3];
+    /* PCG */
+    double *r, *z, *dvec, *Ad;
+    double *Adiag, *Aoff[6];
+    /* multigrid */
+    int nlev;
+    level lev[MAXL];
+    double* cinv;        /* dense pseudo-inverse on the coarsest level */
+    int cinv_n;
+    int mg_ready;
+    int props_ready;
+};
+
+/* ---------------------------------------------------------------- indexing */
+static inline size_t IDX(const struct smac_ctx* c, int i, int j, int k) {
+    return ((size_t)k * (size_t)c->p[1] + (size_t)j) * (size_t)c->p[0] + (size_t)i;
+}
+static inline int wrap(int i, int n) { return i < 1 ? i + n : (i > n ? i - n : i); }
+/* resolve an index along dimension d: periodic directions wrap into 1..n */
+static inline int W(const struct smac_ctx* c, int d, int i) { return c->per[d] ? wrap(i, c->n[d]) : i; }
+
+typedef struct { int v[3]; } ivec;
+static inline ivec iv(int i, int j, int k) { ivec r = {{i, j, k}}; return r; }
+static inline ivec add(ivec a, int d, int s) { a.v[d] += s; return a; }
+static inline ivec wr(const struct smac_ctx* c, ivec a) {
+    for (int d = 0; d < 3; d++) a.v[d] = W(c, d, a.v[d]);
+    return a;
+}
+static inline size_t IX(const struct smac_ctx* c, ivec a) { return IDX(c, a.v[0], a.v[1], a.v[2]); }
+
+/* Face type of the a-face at (already wrapped) index q: analytic form of
+ * SMACSolver::set_face_type (CFD/SMACSolver.cpp:104-202) for a box without
+ * internal walls; periodic directions have no walls. */
+static int face_type(const struct smac_ctx* c, int a, ivec q) {
+    for (int d = 0; d < 3; d++) {
+        if (d == a) continue;
+        if (q.v[d] < 1 || q.v[d] > c->n[d]) return F_GHOST;
+    }
+    int i = q.v[a], n = c->n[a];
+    if (c->per[a]) return (i >= 1 && i <= n) ? F_INTERIOR : F_GHOST;
+    if (i == 1 || i == n + 1) return F_BOUNDARY;
+    if (i >= 2 && i <= n) return F_INTERIOR;
+    return F_GHOST;
+}
+/* bc id of an a-face; 0 off the walls as in the calloc'd reference array
+ * (StaggeredGrid::set_boundary_id, CFD/StaggeredGrid.h:75-115). */
+static int face_bcid(const struct smac_ctx* c, int a, ivec q) {
+    int i = q.v[a];
+    if (c->per[a]) return 0;
+    for (int d = 0; d < 3; d++) {
+        if (d == a) continue;
+        if (q.v[d] < 1 || q.v[d] > c->n[d]) return 0;
+    }
+    if (i == 1) return c->side_bcid[2 * a];
+    if (i == c->n[a] + 1) return c->side_bcid[2 * a + 1];
+    return 0;
+}
+/* cell type: interior iff inside (SMACSolver::set_cell_type, CFD/SMACSolver.cpp:206-248) */
+static inline int cell_interior(const struct smac_ctx* c, ivec q) {
+    for (int d = 0; d < 3; d++)
+        if (q.v[d] < 1 || q.v[d] > c->n[d]) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------ boundary getters */
+/* d_get_v{x,y,z}_{x,y,z}face (CFD/G_BoundaryFunctions.h:9-97) */
+static double get_face(const struct smac_ctx* c, int a, ivec q) {
+    q = wr(c, q);
+    int t = face_type(c, a, q);
+    if (t == F_INTERIOR) return c->vel[a][IX(c, q)];
+    if (t == F_BOUNDARY) {
+        int bid = face_bcid(c, a, q);
+        int bt = c->bctype[bid];
+        if (bt == SMAC_BC_INFLOW) return (1. - 0.0) * c->bcv[bid][a];
+        if (bt == SMAC_BC_NOSLIP) return 0.;
+        if (bt == SMAC_BC_OUTLET) return c->vel[a][IX(c, q)];
+        if (bt == SMAC_BC_SLIP && a != 0) return c->vel[a][IX(c, q)];
+    }
+    return 0.;
+}
+/* d_get_v{a}_{b}dir (CFD/G_BoundaryFunctions.h:100-367): the a-velocity one
+ * step along b, mirrored across a wall. q is an interior a-face. */
+static double get_dir(const struct smac_ctx* c, int a, int b, ivec q, int s) {
+    q = wr(c, q);
+    double vin = c->vel[a][IX(c, q)];
+    ivec q2 = wr(c, add(q, b, s));
+    if (face_type(c, a, q2) == F_INTERIOR) return c->vel[a][IX(c, q2)];
+    ivec qf = q;
+    qf.v[b] = s > 0 ? q.v[b] + 1 : q.v[b];
+    double vw = 0.0;
+    int count = 0;
+    if (face_type(c, b, qf) == F_BOUNDARY) {
+        vw += c->bcv[face_bcid(c, b, qf)][a];
+        count++;
+    }
+    ivec qf2 = wr(c, add(qf, a, -1));
+    if (face_type(c, b, qf2) == F_BOUNDARY) {
+        vw += c->bcv[face_bcid(c, b, qf2)][a];
+        count++;
+    }
+    if (count > 0) vw /= (double)count;
+    int bt = c->bctype[face_bcid(c, b, qf)];
+    if (bt == SMAC_BC_SLIP || bt == SMAC_BC_OUTLET) return vin;
+    return 2.0 * vw - vin;
+}
+
+/* d_minmod (CFD/G_SMACSolver.cu:67-88) */
+static inline double minmod(double dp, double dm) {
+    double prod = dp * dm;
+    double s = 0.;
+    if (prod > 0.) s = fabs(dp) < fabs(dm) ? dp : dm;
+    return s;
+}
+/* sgn (CFD/G_SMACSolver.h:130-139) */
+static inline int sgn(double a) { return a < 0 ? -1 : 1; }
+
+/* ------------------------------------------------------------- predictor */
+/* One component of k_get_vof_vstar_rhouu_consistent_{x,y,z}
+ * (CFD/G_SMACSolver.cu:139-392 / 441-662 / 763-1003) at interior a-face q. */
+static double predict_face(const struct smac_ctx* c, int a, ivec q, double dt) {
+    const double* inv_d = c->inv_d;
+    double v0 = c->vel[a][IX(c, q)];
+    double vp1 = get_face(c, a, add(q, a, +1));
+    double vm1 = get_face(c, a, add(q, a, -1));
+    ivec cm = wr(c, add(q, a, -1)); /* cell on the minus side */
+
+    /* viscous, own direction */
+    double tau_p = c->mu[IX(c, q)] * (vp1 - v0);
+    double tau_m = c->mu[IX(c, cm)] * (v0 - vm1);
+    double tmp = 2. * (tau_p - tau_m) * c->inv_d2[a];
+
+    /* viscous, cross directions in increasing axis order */
+    for (int b = 0; b < 3; b++) {
+        if (b == a) continue;
+        double vbm = get_dir(c, a, b, q, -1);
+        double vbp = get_dir(c, a, b, q, +1);
+        double wpp = get_face(c, b, add(q, b, +1));
+        double wmp = get_face(c, b, add(add(q, b, +1), a, -1));
+        double wp0 = get_face(c, b, q);
+        double wm0 = get_face(c, b, add(q, a, -1));
+        ivec f1 = wr(c, add(q, b, +1)), f2 = wr(c, add(add(q, b, +1), a, -1));
+        ivec f3 = wr(c, q), f4 = wr(c, add(q, a, -1));
+        double mu_p = 0.5 * (c->muf[b][IX(c, f1)] + c->muf[b][IX(c, f2)]);
+        double t_p = mu_p * ((wpp - wmp) * inv_d[a] + (vbp - v0) * inv_d[b]);
+        double mu_m = 0.5 * (c->muf[b][IX(c, f3)] + c->muf[b][IX(c, f4)]);
+        double t_m = mu_m * ((wp0 - wm0) * inv_d[a] + (v0 - vbm) * inv_d[b]);
+        tmp += (t_p - t_m) * inv_d[b];
+    }
+
+    /* convection, MUSCL-minmod, directions x, y, z */
+    for (int e = 0; e < 3; e++) {
+        double mp, mm, up_p, up_m, Mp, Mm;
+        if (e == a) {
+            mp = 0.5 * (c->mf[a][IX(c, wr(c, q))] + c->mf[a][IX(c, wr(c, add(q, a, +1)))]);
+            mm = 0.5 * (c->mf[a][IX(c, wr(c, add(q, a, -1)))] + c->mf[a][IX(c, wr(c, q))]);
+            /* + face */
+            ivec up = mp > 0. ? q : add(q, a, +1);
+            if (face_type(c, a, wr(c, up)) != F_INTERIOR) {
+                up_p = mp > 0. ? v0 : vp1;
+            } else {
+                double dpv = get_face(c, a, add(up, a, +1)) - get_face(c, a, up);
+                double dmv = get_face(c, a, up) - get_face(c, a, add(up, a, -1));
+                double s = minmod(dpv, dmv);
+                int dir = sgn(mp);
+                double uw = mp > 0. ? v0 : vp1;
+                up_p = uw + dir * 0.5 * s;
+            }
+            Mp = mp * up_p;
+            /* - face */
+            up = mm > 0. ? add(q, a, -1) : q;
+            if (face_type(c, a, wr(c, up)) != F_INTERIOR) {
+                up_m = mm > 0. ? vm1 : v0;
+            } else {
+                double dpv = get_face(c, a, add(up, a, +1)) - get_face(c, a, up);
+                double dmv = get_face(c, a, up) - get_face(c, a, add(up, a, -1));
+                double s = minmod(dpv, dmv);
+                int dir = sgn(mm);
+                double uw = mm > 0. ? vm1 : v0;
+                up_m = uw + dir * 0.5 * s;
+            }
+            Mm = mm * up_m;
+        } else {
+            double vcm = get_dir(c, a, e, q, -1);
+            double vcp = get_dir(c, a, e, q, +1);
+            mp = 0.5 * (c->mf[e][IX(c, wr(c, add(add(q, e, +1), a, -1)))] +
+                        c->mf[e][IX(c, wr(c, add(q, e, +1)))]);
+            mm = 0.5 * (c->mf[e][IX(c, wr(c, q))] + c->mf[e][IX(c, wr(c, add(q, a, -1)))]);
+            ivec up = mp > 0. ? q : add(q, e, +1);
+            ivec upw = wr(c, up);
+            if (face_type(c, a, upw) != F_INTERIOR) {
+                up_p = mp > 0. ? v0 : vcp;
+            } else {
+                double vu = c->vel[a][IX(c, upw)];
+                double dpv = get_dir(c, a, e, upw, +1) - vu;
+                double dmv = vu - get_dir(c, a, e, upw, -1);
+                double s = minmod(dpv, dmv);
+                int dir = sgn(mp);
+                double uw = mp > 0. ? v0 : vcp;
+                up_p = uw + dir * 0.5 * s;
+            }
+            Mp = mp * up_p;
+            up = mm > 0. ? add(q, e, -1) : q;
+            upw = wr(c, up);
+            if (face_type(c, a, upw) != F_INTERIOR) {
+                up_m = mm > 0. ? vcm : v0;
+            } else {
+                double vu = c->vel[a][IX(c, upw)];
+                double dpv = get_dir(c, a, e, upw, +1) - vu;
+                double dmv = vu - get_dir(c, a, e, upw, -1);
+                double s = minmod(dpv, dmv);
+                int dir = sgn(mm);
+                double uw = mm > 0. ? vcm : v0;
+                up_m = uw + dir * 0.5 * s;
+            }
+            Mm = mm * up_m;
+        }
+        tmp -= (Mp - Mm) * inv_d[e];
+    }
+
+    double beta = c->beta[a][IX(c, q)];
+    double rho_old_f = 0.5 * (c->rho_old[IX(c, q)] + c->rho_old[IX(c, cm)]);
+    tmp -= (c->p_[IX(c, q)] - c->p_[IX(c, cm)]) * inv_d[a];
+    double g = a == 0 ? c->fl.gx : (a == 1 ? c->fl.gy : c->fl.gz);
+    return beta * (v0 * rho_old_f) + dt * (beta * tmp + g);
+}
+
+/* range of face indices along a that are F_INTERIOR */
+static inline int face_lo(const struct smac_ctx* c, int a) { return c->per[a] ? 1 : 2; }
+static inline int face_hi(const struct smac_ctx* c, int a) { return c->n[a]; }
+
+/* d_get_v{x,y,z}star_{x,y,z}face (CFD/G_BoundaryFunctions.h:369-486) */
+static double get_star_boundary(const struct smac_ctx* c, int a, ivec q) {
+    int bid = face_bcid(c, a, q);
+    int bt = c->bctype[bid];
+    if (bt == SMAC_BC_INFLOW) return (1. - 0.0) * c->bcv[bid][a];
+    if (bt == SMAC_BC_NOSLIP) return 0.;
+    if (bt == SMAC_BC_OUTLET) {
+        int internal = (q.v[a] == 1) ? 0 : -1; /* set_face_internal_direction */
+        return internal < 0 ? c->vst[a][IX(c, add(q, a, -1))] : c->vst[a][IX(c, add(q, a, +1))];
+    }
+    return 0.;
+}
+
+int smac_predictor(smac_ctx* c, double dt) {
+    if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    smac_compute_mass_flux(c, dt);
+    for (int a = 0; a < 3; a++) {
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; d++) { lo[d] = 1; hi[d] = c->n[d]; }
+        lo[a] = face_lo(c, a);
+        hi[a] = face_hi(c, a);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    ivec q = iv(i, j, k);
+                    c->vst[a][IX(c, q)] = predict_face(c, a, q, dt);
+                }
+    }
+    /* update_vstar_boundary: wall faces of each component */
+    for (int a = 0; a < 3; a++) {
+        if (c->per[a]) continue;
+        int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+        for (int side = 0; side < 2; side++) {
+            int fi = side == 0 ? 1 : c->n[a] + 1;
+            for (int t2 = 1; t2 <= c->n[b2]; t2++)
+                for (int t1 = 1; t1 <= c->n[b1]; t1++) {
+                    ivec q;
+                    q.v[a] = fi; q.v[b1] = t1; q.v[b2] = t2;
+                    c->vst[a][IX(c, q)] = get_star_boundary(c, a, q);
+                }
+        }
+    }
+    return SMAC_OK;
+}
+
+/* -------------------------------------------------- canonical reductions */
+/* Sum of f(i,j,k) over interior cells. Row (j,k): lane l accumulates
+ * i = 1+l, 1+l+64, ... in that order; the 64 lanes are combined by a halving
+ * tree (lane l += lane l+s, s = 32..1). Row partials, in row order
+ * (k-1)*Ny+(j-1), are combined by 1024 lanes the same way. The GPU kernels
+ * (wave64 shuffles) produce bit-identical results. */
+typedef double (*cellfn)(const struct smac_ctx*, size_t, const void*);
+
+static double tree(double* a, int n) {
+    for (int s = n / 2; s >= 1; s /= 2)
+        for (int l = 0; l < s; l++) a[l] = a[l] + a[l + s];
+    return a[0];
+}
+static double final_reduce(const double* part, size_t nrows) {
+    double lanes[1024];
+    for (int t = 0; t < 1024; t++) {
+        double acc = 0.0;
+        for (size_t r = (size_t)t; r < nrows; r += 1024) acc += part[r];
+        lanes[t] = acc;
+    }
+    return tree(lanes, 1024);
+}
+
+/* generic dot-like sum: sum_c x[c]*y[c] (y may be NULL: sum x; sq: x*x) */
+enum { RED_SUM = 0, RED_DOT = 1, RED_SQ = 2 };
+static double csum(const struct smac_ctx* c, int mode, const double* x, const double* y) {
+    int nx = c->n[0], ny = c->n[1], nz = c->n[2];
+    size_t nrows = (size_t)ny * (size_t)nz;
+    double* part = (double*)malloc(nrows * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < (long)nrows; r++) {
+        int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+        double lanes[64];
+        for (int l = 0; l < 64; l++) {
+            double acc = 0.0;
+            for (int i = 1 + l; i <= nx; i += 64) {
+                size_t id = IDX(c, i, j, k);
+                double v = mode == RED_SUM ? x[id] : (mode == RED_DOT ? x[id] * y[id] : x[id] * x[id]);
+                acc += v;
+            }
+            lanes[l] = acc;
+        }
+        part[r] = tree(lanes, 64);
+    }
+    double s = final_reduce(part, nrows);
+    free(part);
+    return s;
+}
+static double cmaxabs(const struct smac_ctx* c, const double* x) {
+    double m = 0.0;
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                double v = fabs(x[IDX(c, i, j, k)]);
+                if (v > m || v != v) m = v;
+            }
+    return m;
+}
+
+/* ----------------------------------------------------------- properties */
+int smac_update_properties(smac_ctx* c, int32_t initial) {
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    const smac_fluid_desc* f = &c->fl;
+    size_t tot = c->tot;
+    if (!initial) { /* rho_old <- rho (the swap of update_properties_by_alpha) */
+        double* t = c->rho_old; c->rho_old = c->rho; c->rho = t;
+    }
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                size_t id = IDX(c, i, j, k);
+                double al = c->alpha[id];
+                c->rho[id] = (1. - al) * f->rho0 + al * f->rho1;
+                c->inv_rho[id] = 1. / c->rho[id];
+                c->mu[id] = (1. - al) * f->mu0 + al * f->mu1;
+                if (initial) c->rho_old[id] = c->rho[id];
+            }
+    (void)tot;
+    /* ghost cells: periodic images, or copies of the inner cell
+     * (k_update_cell_boundary_properties_by_alpha, G_SMACSolver_VoF.cu:1244-1307) */
+    for (int k = 0; k <= c->n[2] + 1; k++)
+        for (int j = 0; j <= c->n[1] + 1; j++)
+            for (int i = 0; i <= c->n[0] + 1; i++) {
+                ivec q = iv(i, j, k);
+                if (cell_interior(c, q)) continue;
+                ivec s = q;
+                for (int d = 0; d < 3; d++) {
+                    if (c->per[d]) s.v[d] = wrap(s.v[d], c->n[d]);
+                    else if (s.v[d] < 1) s.v[d] = 1;
+                    else if (s.v[d] > c->n[d]) s.v[d] = c->n[d];
+                }
+                size_t id = IX(c, q), is = IX(c, s);
+                c->rho[id] = c->rho[is];
+                c->mu[id] = c->mu[is];
+                c->inv_rho[id] = c->inv_rho[is];
+                if (initial) c->rho_old[id] = c->rho_old[is];
+            }
+    /* faces (k_update_{x,y,z}_face_properties_by_alpha, G_SMACSolver_VoF.cu:1409-1535;
+     * boundary faces: update_boundary_faces, 1798-1941) */
+    for (int a = 0; a < 3; a++) {
+        for (int k = 0; k < c->p[2]; k++)
+            for (int j = 0; j < c->p[1]; j++)
+                for (int i = 0; i < c->p[0]; i++) {
+                    ivec q = iv(i, j, k);
+                    size_t id = IX(c, q);
+                    int t = face_type(c, a, q);
+                    if (t == F_INTERIOR) {
+                        ivec cm = wr(c, add(q, a, -1));
+                        ivec cp = wr(c, q);
+                        c->rhof[a][id] = 0.5 * (c->rho[IX(c, cp)] + c->rho[IX(c, cm)]);
+                        c->beta[a][id] = 1. / c->rhof[a][id];
+                        c->inv_rhof[a][id] = c->beta[a][id];
+                        c->muf[a][id] = 0.5 * (c->mu[IX(c, cp)] + c->mu[IX(c, cm)]);
+                    } else if (t == F_BOUNDARY) {
+                        int inner = q.v[a] == 1 ? 0 : -1;
+                        ivec ci = add(q, a, inner);
+                        c->rhof[a][id] = c->rho[IX(c, ci)];
+                        c->muf[a][id] = c->mu[IX(c, ci)];
+                        c->beta[a][id] = 0.;
+                        c->inv_rhof[a][id] = 0.;
+                    } else {
+                        c->beta[a][id] = 0.;
+                        c->inv_rhof[a][id] = 0.;
+                    }
+                }
+    }
+    c->props_ready = 1;
+    c->mg_ready = 0;
+    return SMAC_OK;
+}
+
+/* k_compute_mass_flux_from_alpha_flux with zero alpha flux:
+ * mf = rho0*u_f + drho*0 (CFD/G_SMACSolver_VoF.cu:1708-1790) */
+int smac_compute_mass_flux(smac_ctx* c, double dt) {
+    (void)dt;
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    for (int a = 0; a < 3; a++)
+        for (int k = 0; k < c->p[2]; k++)
+            for (int j = 0; j < c->p[1]; j++)
+                for (int i = 0; i < c->p[0]; i++) {
+                    ivec q = iv(i, j, k);
+                    int ok = 1;
+                    for (int d = 0; d < 3; d++)
+                        if (q.v[d] > c->n[d] + (d == a ? 2 : 1)) ok = 0;
+                    if (!ok) continue;
+                    c->mf[a][IX(c, q)] = c->fl.rho0 * get_face(c, a, q);
+                }
+    return SMAC_OK;
+}
+
+/* ---------------------------------------------------------------- Poisson */
+/* base_k_make_poisson_rhs with eps = 1 and no surface tension. */
+static void make_rhs(struct smac_ctx* c, double inv_dt) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                ivec q = iv(i, j, k);
+                double div = (c->vst[0][IX(c, wr(c, add(q, 0, 1)))] - c->vst[0][IX(c, q)]) * c->inv_d[0] +
+                             (c->vst[1][IX(c, wr(c, add(q, 1, 1)))] - c->vst[1][IX(c, q)]) * c->inv_d[1] +
+                             (c->vst[2][IX(c, wr(c, add(q, 2, 1)))] - c->vst[2][IX(c, q)]) * c->inv_d[2];
+                c->rhs[IX(c, q)] = -inv_dt * div;
+            }
+}
+
+/* k_build_vof_poisson_A (CFD/pressure_solver/G_PCGSolver.cu:126-212).
+ * Off-diagonal order: xp, xm, yp, ym, zp, zm. */
+static void build_A(struct smac_ctx* c) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                ivec q = iv(i, j, k);
+                size_t id = IX(c, q);
+                double a6[6];
+                for (int d = 0; d < 3; d++) {
+                    ivec fp = wr(c, add(q, d, 1));
+                    a6[2 * d] = c->beta[d][IX(c, fp)] * c->inv_d2[d];
+                    a6[2 * d + 1] = c->beta[d][id] * c->inv_d2[d];
+                }
+                c->Adiag[id] = a6[0] + a6[1] + a6[2] + a6[3] + a6[4] + a6[5];
+                for (int d = 0; d < 3; d++) {
+                    if (face_type(c, d, wr(c, add(q, d, 1))) == F_BOUNDARY) a6[2 * d] = 0.;
+                    if (face_type(c, d, q) == F_BOUNDARY) a6[2 * d + 1] = 0.;
+                }
+                for (int m = 0; m < 6; m++) c->Aoff[m][id] = a6[m];
+            }
+}
+
+/* (A x)_c = Adiag*x_c - (Axp*x_xp + Axm*x_xm + Ayp*x_yp + Aym*x_ym + Azp*x_zp + Azm*x_zm)
+ * (k_make_pAp, G_PCGSolver.cu:87-124) */
+static inline double apply_A(const struct smac_ctx* c, const double* x, int i, int j, int k) {
+    size_t id = IDX(c, i, j, k);
+    double xp = x[IDX(c, W(c, 0, i + 1), j, k)], xm = x[IDX(c, W(c, 0, i - 1), j, k)];
+    double yp = x[IDX(c, i, W(c, 1, j + 1), k)], ym = x[IDX(c, i, W(c, 1, j - 1), k)];
+    double zp = x[IDX(c, i, j, W(c, 2, k + 1))], zm = x[IDX(c, i, j, W(c, 2, k - 1))];
+    double s = c->Aoff[0][id] * xp + c->Aoff[1][id] * xm + c->Aoff[2][id] * yp +
+               c->Aoff[3][id] * ym + c->Aoff[4][id] * zp + c->Aoff[5][id] * zm;
+    return c->Adiag[id] * x[id] - s;
+}
+
+/* --------------------------------------------------------------- multigrid */
+static int mg_auto_levels(const struct smac_ctx* c) {
+    int n[3] = {c->n[0], c->n[1], c->n[2]};
+    int L = 1;
+    while (L < MAXL) {
+        long cells = (long)n[0] * n[1] * n[2];
+        if (cells <= 512) break;
+        if ((n[0] & 1) || (n[1] & 1) || (n[2] & 1)) break;
+        if (n[0] < 4 || n[1] < 4 || n[2] < 4) break;
+        n[0] /= 2; n[1] /= 2; n[2] /= 2;
+        L++;
+    }
+    return L;
+}
+static int mg_check_levels(const struct smac_ctx* c, int L) {
+    if (L < 1 || L > MAXL) return 0;
+    int n[3] = {c->n[0], c->n[1], c->n[2]};
+    for (int l = 1; l < L; l++) {
+        for (int d = 0; d < 3; d++) {
+            if (n[d] & 1) return 0;
+            n[d] /= 2;
+            if (n[d] < 2) return 0;
+        }
+    }
+    if ((long)n[0] * n[1] * n[2] > 1024) return 0;
+    return 1;
+}
+
+static inline size_t LIDX(const level* L, int i, int j, int k) {
+    return ((size_t)k * (size_t)L->p[1] + (size_t)j) * (size_t)L->p[0] + (size_t)i;
+}
+static inline int LW(const struct smac_ctx* c, const level* L, int d, int i) {
+    return c->per[d] ? wrap(i, L->n[d]) : i;
+}
+static int lface_boundary(const struct smac_ctx* c, const level* L, int d, int i) {
+    return !c->per[d] && (i == 1 || i == L->n[d] + 1);
+}
+
+static void mg_free(struct smac_ctx* c) {
+    for (int l = 0; l < MAXL; l++) {
+        level* L = &c->lev[l];
+        free(L->e); free(L->f); free(L->res);
+        if (l > 0) { free(L->b[0]); free(L->b[1]); free(L->b[2]); }
+        memset(L, 0, sizeof(*L));
+    }
+    free(c->cinv);
+    c->cinv = NULL;
+    c->nlev = 0;
+    c->mg_ready = 0;
+}
+
+/* coefficient of face (I,J,K) of direction d on level l+1: mean of the four
+ * fine faces covering it, ((b00 + b10) + (b01 + b11)) * 0.25 */
+static void mg_coarsen_beta(const struct smac_ctx* c, const level* F, level* C) {
+    for (int d = 0; d < 3; d++) {
+        int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+        for (int K = 0; K < C->p[2]; K++)
+            for (int J = 0; J < C->p[1]; J++)
+                for (int I = 0; I < C->p[0]; I++) {
+                    int q[3] = {I, J, K};
+                    int ok = 1;
+                    for (int e = 0; e < 3; e++) {
+                        int hi = C->n[e] + (e == d ? 1 : 0);
+                        if (q[e] < 1 || q[e] > hi) ok = 0;
+                    }
+                    if (!ok) { C->b[d][LIDX(C, I, J, K)] = 0.; continue; }
+                    int fq[3];
+                    fq[d] = 2 * q[d] - 1;
+                    double v[2][2];
+                    for (int s2 = 0; s2 < 2; s2++)
+                        for (int s1 = 0; s1 < 2; s1++) {
+                            fq[t1] = 2 * q[t1] - 1 + s1;
+                            fq[t2] = 2 * q[t2] - 1 + s2;
+                            v[s2][s1] = F->b[d][LIDX(F, fq[0], fq[1], fq[2])];
+                        }
+                    C->b[d][LIDX(C, I, J, K)] = ((v[0][0] + v[0][1]) + (v[1][0] + v[1][1])) * 0.25;
+                }
+    }
+}
+
+/* level operator: a6 = beta/h^2 per face (xp,xm,yp,ym,zp,zm), diag = sum of all six
+ * (before zeroing walls), off-diagonals zero across walls */
+static inline void lcoef(const struct smac_ctx* c, const level* L, int i, int j, int k, double a6[6], double* diag) {
+    int q[3] = {i, j, k};
+    for (int d = 0; d < 3; d++) {
+        int qp[3] = {i, j, k};
+        qp[d] = q[d] + 1;
+        if (c->per[d]) qp[d] = wrap(qp[d], L->n[d]);
+        a6[2 * d] = L->b[d][LIDX(L, qp[0], qp[1], qp[2])] * L->inv_d2[d];
+        a6[2 * d + 1] = L->b[d][LIDX(L, i, j, k)] * L->inv_d2[d];
+    }
+    *diag = a6[0] + a6[1] + a6[2] + a6[3] + a6[4] + a6[5];
+    for (int d = 0; d < 3; d++) {
+        if (lface_boundary(c, L, d, q[d] + 1)) a6[2 * d] = 0.;
+        if (lface_boundary(c, L, d, q[d])) a6[2 * d + 1] = 0.;
+    }
+}
+static inline double lnbsum(const struct smac_ctx* c, const level* L, const double* x, int i, int j, int k,
+                            const double a6[6]) {
+    double xp = x[LIDX(L, LW(c, L, 0, i + 1), j, k)], xm = x[LIDX(L, LW(c, L, 0, i - 1), j, k)];
+    double yp = x[LIDX(L, i, LW(c, L, 1, j + 1), k)], ym = x[LIDX(L, i, LW(c, L, 1, j - 1), k)];
+    double zp = x[LIDX(L, i, j, LW(c, L, 2, k + 1))], zm = x[LIDX(L, i, j, LW(c, L, 2, k - 1))];
+    return a6[0] * xp + a6[1] * xm + a6[2] * yp + a6[3] * ym + a6[4] * zp + a6[5] * zm;
+}
+
+static void dense_pinv(const struct smac_ctx* c, const level* L, double* out, int pure) {
+    int n = L->n[0] * L->n[1] * L->n[2];
+    double* A = (double*)calloc((size_t)n * n, sizeof(double));
+    for (int k = 1; k <= L->n[2]; k++)
+        for (int j = 1; j <= L->n[1]; j++)
+            for (int i = 1; i <= L->n[0]; i++) {
+                int row = ((k - 1) * L->n[1] + (j - 1)) * L->n[0] + (i - 1);
+                double a6[6], diag;
+                lcoef(c, L, i, j, k, a6, &diag);
+                A[(size_t)row * n + row] += diag;
+                int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+                for (int m = 0; m < 6; m++) {
+                    if (a6[m] == 0.) continue;
+                    int q[3] = {nb[m][0], nb[m][1], nb[m][2]};
+                    for (int d = 0; d < 3; d++)
+                        if (c->per[d]) q[d] = wrap(q[d], L->n[d]);
+                    int col = ((q[2] - 1) * L->n[1] + (q[1] - 1)) * L->n[0] + (q[0] - 1);
+                    A[(size_t)row * n + col] -= a6[m];
+                }
+            }
+    double cshift = pure ? A[0] / (double)n : 0.0;
+    if (pure)
+        for (size_t t = 0; t < (size_t)n * n; t++) A[t] += cshift;
+    /* Cholesky A = L L^T (lower, in place) */
+    for (int j = 0; j < n; j++) {
+        double s = A[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) s -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+        double djj = sqrt(s);
+        A[(size_t)j * n + j] = djj;
+        for (int i = j + 1; i < n; i++) {
+            double t = A[(size_t)i * n + j];
+            for (int k = 0; k < j; k++) t -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            A[(size_t)i * n + j] = t / djj;
+        }
+    }
+    double* y = (double*)malloc((size_t)n * sizeof(double));
+    for (int col = 0; col < n; col++) {
+        for (int i = 0; i < n; i++) {
+            double t = (i == col) ? 1.0 : 0.0;
+            for (int k = 0; k < i; k++) t -= A[(size_t)i * n + k] * y[k];
+            y[i] = t / A[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double t = y[i];
+            for (int k = i + 1; k < n; k++) t -= A[(size_t)k * n + i] * y[k];
+            y[i] = t / A[(size_t)i * n + i];
+        }
+        for (int i = 0; i < n; i++) out[(size_t)i * n + col] = y[i];
+    }
+    if (pure) {
+        double corr = 1.0 / (cshift * (double)n * (double)n);
+        for (size_t t = 0; t < (size_t)n * n; t++) out[t] -= corr;
+    }
+    free(y);
+    free(A);
+}
+
+static int pure_neumann(const struct smac_ctx* c) {
+    for (int d = 0; d < 3; d++) {
+        if (c->per[d]) continue;
+        for (int s = 0; s < 2; s++) {
+            int bt = c->bctype[c->side_bcid[2 * d + s]];
+            if (bt == SMAC_BC_OUTLET || bt == SMAC_BC_INFLOW) return 0;
+        }
+    }
+    return 1;
+}
+
+static int mg_setup(struct smac_ctx* c, int L) {
+    if (c->mg_ready && c->nlev == L) return SMAC_OK;
+    mg_free(c);
+    c->nlev = L;
+    for (int l = 0; l < L; l++) {
+        level* V = &c->lev[l];
+        for (int d = 0; d < 3; d++) {
+            V->n[d] = c->n[d] >> l;
+            V->p[d] = V->n[d] + 3;
+            double h = c->d[d] * (double)(1 << l);
+            V->inv_d2[d] = 1.0 / (h * h);
+        }
+        V->tot = (size_t)V->p[0] * V->p[1] * V->p[2];
+        V->e = (double*)calloc(V->tot, sizeof(double));
+        V->f = (double*)calloc(V->tot, sizeof(double));
+        V->res = (double*)calloc(V->tot, sizeof(double));
+        if (l == 0) {
+            for (int d = 0; d < 3; d++) V->b[d] = c->beta[d];
+        } else {
+            for (int d = 0; d < 3; d++) V->b[d] = (double*)calloc(V->tot, sizeof(double));
+            mg_coarsen_beta(c, &c->lev[l - 1], V);
+        }
+    }
+    level* C = &c->lev[L - 1];
+    int n = C->n[0] * C->n[1] * C->n[2];
+    c->cinv = (double*)malloc((size_t)n * n * sizeof(double));
+    c->cinv_n = n;
+    /* no Dirichlet (outlet) faces are supported, so A is always singular */
+    dense_pinv(c, C, c->cinv, 1);
+    c->mg_ready = 1;
+    return SMAC_OK;
+}
+
+/* RBGS half-sweep of colour col ((i+j+k)&1 == col) on level V: e = (f + S)/diag.
+ * zero_start: e is known to be zero everywhere (first half-sweep). */
+static void rbgs(const struct smac_ctx* c, level* V, int col) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= V->n[2]; k++)
+        for (int j = 1; j <= V->n[1]; j++)
+            for (int i = 1 + ((col + j + k + 1) & 1); i <= V->n[0]; i += 2) {
+                double a6[6], diag;
+                lcoef(c, V, i, j, k, a6, &diag);
+                double S = lnbsum(c, V, V->e, i, j, k, a6);
+                V->e[LIDX(V, i, j, k)] = (V->f[LIDX(V, i, j, k)] + S) / diag;
+            }
+}
+static void residual(const struct smac_ctx* c, level* V) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= V->n[2]; k++)
+        for (int j = 1; j <= V->n[1]; j++)
+            for (int i = 1; i <= V->n[0]; i++) {
+                double a6[6], diag;
+                lcoef(c, V, i, j, k, a6, &diag);
+                double S = lnbsum(c, V, V->e, i, j, k, a6);
+                size_t id = LIDX(V, i, j, k);
+                V->res[id] = V->f[id] - (diag * V->e[id] - S);
+            }
+}
+/* 1-D restriction weights of coarse index I: fine indices and weights, the
+ * transpose of cell-centred linear prolongation (walls fold the mirror weight) */
+static int rweights(const struct smac_ctx* c, const level* C, int d, int I, int idx[4], double w[4]) {
+    int nc = C->n[d], nf = 2 * nc;
+    int m = 0;
+    int f0 = 2 * I - 2, f1 = 2 * I - 1, f2 = 2 * I, f3 = 2 * I + 1;
+    if (c->per[d]) {
+        idx[0] = wrap(f0, nf); w[0] = 0.25;
+        idx[1] = f1; w[1] = 0.75;
+        idx[2] = f2; w[2] = 0.75;
+        idx[3] = wrap(f3, nf); w[3] = 0.25;
+        return 4;
+    }
+    if (I > 1) { idx[m] = f0; w[m] = 0.25; m++; }
+    idx[m] = f1; w[m] = (I == 1) ? 1.0 : 0.75; m++;
+    idx[m] = f2; w[m] = (I == nc) ? 1.0 : 0.75; m++;
+    if (I < nc) { idx[m] = f3; w[m] = 0.25; m++; }
+    return m;
+}
+static void restrict_res(const struct smac_ctx* c, const level* F, level* C) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int K = 1; K <= C->n[2]; K++)
+        for (int J = 1; J <= C->n[1]; J++)
+            for (int I = 1; I <= C->n[0]; I++) {
+                int ix[4], iy[4], iz[4];
+                double wx[4], wy[4], wz[4];
+                int mx = rweights(c, C, 0, I, ix, wx);
+                int my = rweights(c, C, 1, J, iy, wy);
+                int mz = rweights(c, C, 2, K, iz, wz);
+                double acc = 0.0;
+                for (int a = 0; a < mz; a++)
+                    for (int b = 0; b < my; b++)
+                        for (int g = 0; g < mx; g++) {
+                            double w = wz[a] * wy[b] * wx[g] * 0.125;
+                            acc += w * F->res[LIDX(F, ix[g], iy[b], iz[a])];
+                        }
+                C->f[LIDX(C, I, J, K)] = acc;
+            }
+}
+/* fine index i: parent (i+1)/2 weight 0.75, neighbour parent-1 (odd i) or
+ * parent+1 (even i) weight 0.25; walls mirror the neighbour onto the parent */
+static inline void pweights(const struct smac_ctx* c, const level* C, int d, int i, int idx[2], double w[2]) {
+    int par = (i + 1) >> 1;
+    int nb = (i & 1) ? par - 1 : par + 1;
+    if (c->per[d]) nb = wrap(nb, C->n[d]);
+    else if (nb < 1 || nb > C->n[d]) nb = par;
+    idx[0] = par; w[0] = 0.75;
+    idx[1] = nb; w[1] = 0.25;
+}
+static void prolong_add(const struct smac_ctx* c, const level* C, level* F) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= F->n[2]; k++)
+        for (int j = 1; j <= F->n[1]; j++)
+            for (int i = 1; i <= F->n[0]; i++) {
+                int ix[2], iy[2], iz[2];
+                double wx[2], wy[2], wz[2];
+                pweights(c, C, 0, i, ix, wx);
+                pweights(c, C, 1, j, iy, wy);
+                pweights(c, C, 2, k, iz, wz);
+                double acc = 0.0;
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++)
+                        for (int g = 0; g < 2; g++) {
+                            double w = wz[a] * wy[b] * wx[g];
+                            acc += w * C->e[LIDX(C, ix[g], iy[b], iz[a])];
+                        }
+                F->e[LIDX(F, i, j, k)] += acc;
+            }
+}
+static void coarse_solve(const struct smac_ctx* c, level* V) {
+    int n = c->cinv_n;
+    double* rv = (double*)malloc((size_t)n * sizeof(double));
+    for (int k = 1; k <= V->n[2]; k++)
+        for (int j = 1; j <= V->n[1]; j++)
+            for (int i = 1; i <= V->n[0]; i++)
+                rv[((k - 1) * V->n[1] + (j - 1)) * V->n[0] + (i - 1)] = V->f[LIDX(V, i, j, k)];
+#pragma omp parallel for schedule(static)
+    for (int row = 0; row < n; row++) {
+        double acc = 0.0;
+        const double* a = c->cinv + (size_t)row * n;
+        for (int col = 0; col < n; col++) acc += a[col] * rv[col];
+        int i = row % V->n[0] + 1, j = (row / V->n[0]) % V->n[1] + 1, k = row / (V->n[0] * V->n[1]) + 1;
+        V->e[LIDX(V, i, j, k)] = acc;
+    }
+    free(rv);
+}
+static void zero_level(level* V) { memset(V->e, 0, V->tot * sizeof(double)); }
+
+static void vcycle(const struct smac_ctx* c, int l, int nu1, int nu2) {
+    level* V = (level*)&c->lev[l];
+    if (l == c->nlev - 1) {
+        coarse_solve(c, V);
+        return;
+    }
+    zero_level(V);
+    for (int s = 0; s < nu1; s++) { rbgs(c, V, 0); rbgs(c, V, 1); }
+    residual(c, V);
+    level* C = (level*)&c->lev[l + 1];
+    restrict_res(c, V, C);
+    vcycle(c, l + 1, nu1, nu2);
+    prolong_add(c, C, V);
+    for (int s = 0; s < nu2; s++) { rbgs(c, V, 1); rbgs(c, V, 0); }
+}
+
+/* z = M^{-1} r */
+static void precond(struct smac_ctx* c, const smac_solver_opts* o) {
+    int nx = c->n[0], ny = c->n[1], nz = c->n[2];
+    if (o->precond == SMAC_PRECOND_JACOBI) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    size_t id = IDX(c, i, j, k);
+                    c->z[id] = (1. / c->Adiag[id]) * c->r[id];
+                }
+        return;
+    }
+    level* V = &c->lev[0];
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) V->f[LIDX(V, i, j, k)] = c->r[IDX(c, i, j, k)];
+    vcycle(c, 0, o->nu1, o->nu2);
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) c->z[IDX(c, i, j, k)] = V->e[LIDX(V, i, j, k)];
+}
+
+/* --------------------------------------------------------------- the PCG */
+static void add_scalar_interior(struct smac_ctx* c, double* q, double a, double sum) {
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                size_t id = IDX(c, i, j, k);
+                q[id] += a * sum;
+            }
+}
+
+void smac_default_opts(smac_solver_opts* o) {
+    if (!o) return;
+    o->precond = SMAC_PRECOND_MG;
+    o->precision = SMAC_PREC_FP64;
+    o->stop_rule = SMAC_STOP_REL_L2;
+    o->tol = 1e-8;
+    o->check_every = 1;
+    o->max_iter = 2000;
+    o->mg_levels = 0;
+    o->nu1 = 2;
+    o->nu2 = 2;
+    o->flexible = 0;
+}
+
+int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st) {
+    if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
+    smac_solver_opts o;
+    if (opts) o = *opts; else smac_default_opts(&o);
+    if (o.check_every < 1) o.check_every = 1;
+    if (o.precision != SMAC_PREC_FP64) return SMAC_ERR_UNSUPPORTED; /* the oracle is FP64 */
+    if (!c->props_ready) smac_update_properties(c, 1);
+    int nx = c->n[0], ny = c->n[1], nz = c->n[2];
+    double Ncell = (double)nx * (double)ny * (double)nz;
+    int pure = pure_neumann(c);
+    double inv_dt = 1. / dt;
+
+    make_rhs(c, inv_dt);
+    build_A(c);
+    int L = 1;
+    if (o.precond == SMAC_PRECOND_MG) {
+        L = o.mg_levels > 0 ? o.mg_levels : mg_auto_levels(c);
+        if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
+        mg_setup(c, L);
+    }
+    /* dp = 0 on entry (correct_vof_velocity clears it), so A dp = 0 and
+     * r = b - mean(b) (pure Neumann) or r = b. */
+    double mean_b = 0.0;
+    if (pure) mean_b = csum(c, RED_SUM, c->rhs, NULL) / Ncell;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                size_t id = IDX(c, i, j, k);
+                c->dp[id] = 0.0;
+                c->r[id] = c->rhs[id] - mean_b;
+            }
+    double bb = csum(c, RED_SQ, c->r, NULL);
+    double bnorm = sqrt(bb);
+    double size_inv = 1. / Ncell;
+    if (pure) add_scalar_interior(c, c->r, -1. * size_inv, csum(c, RED_SUM, c->r, NULL));
+    precond(c, &o);
+    if (pure) add_scalar_interior(c, c->z, -1. * size_inv, csum(c, RED_SUM, c->z, NULL));
+    memcpy(c->dvec, c->z, c->tot * sizeof(double));
+    double rz = csum(c, RED_DOT, c->r, c->z);
+
+    int iter = 0, conv = 0;
+    double rr = bb, linf = 0.0;
+    if (bb == 0.0) conv = 1;
+    for (iter = 0; iter < o.max_iter && !conv; ++iter) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) c->Ad[IDX(c, i, j, k)] = apply_A(c, c->dvec, i, j, k);
+        double pAp = csum(c, RED_DOT, c->dvec, c->Ad);
+        double alpha = rz / pAp;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    size_t id = IDX(c, i, j, k);
+                    c->dp[id] += alpha * c->dvec[id];
+                    c->r[id] -= alpha * c->Ad[id];
+                }
+        precond(c, &o);
+        double rz_new = csum(c, RED_DOT, c->r, c->z);
+        double zAd = 0.0;
+        if (o.flexible) zAd = csum(c, RED_DOT, c->z, c->Ad);
+        if (iter % o.check_every == 0) {
+            if (o.stop_rule == SMAC_STOP_REF_LINF_DT) {
+                linf = cmaxabs(c, c->r) * dt;
+                if (linf < o.tol) { iter++; conv = 1; break; }
+            } else {
+                rr = csum(c, RED_SQ, c->r, NULL);
+                if (sqrt(rr) < o.tol * bnorm) { iter++; conv = 1; break; }
+            }
+        }
+        double beta = o.flexible ? (-alpha * zAd) / rz : rz_new / rz;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    size_t id = IDX(c, i, j, k);
+                    c->dvec[id] = c->z[id] + beta * c->dvec[id];
+                }
+        rz = rz_new;
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->iters = iter;
+        st->converged = conv;
+        st->pure_neumann = pure;
+        st->mg_levels = o.precond == SMAC_PRECOND_MG ? L : 0;
+        rr = csum(c, RED_SQ, c->r, NULL);
+        st->bnorm = bnorm;
+        st->rel_res = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
+        st->linf_res_dt = cmaxabs(c, c->r) * dt;
+    }
+    return SMAC_OK;
+}
+
+/* k_correct_vof_velocity + memset dp (CFD/G_SMACSolver.cu:1072-1142,1269-1275) */
+int smac_correct(smac_ctx* c, double dt) {
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    for (int a = 0; a < 3; a++) {
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; d++) { lo[d] = 1; hi[d] = c->n[d]; }
+        lo[a] = face_lo(c, a);
+        hi[a] = face_hi(c, a);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    ivec q = iv(i, j, k);
+                    size_t id = IX(c, q);
+                    ivec cm = wr(c, add(q, a, -1));
+                    double g = (c->dp[id] - c->dp[IX(c, cm)]) * c->inv_d[a];
+                    c->vel[a][id] = c->vst[a][id] + c->inv_rhof[a][id] * dt * (0.0 - g);
+                }
+    }
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                size_t id = IDX(c, i, j, k);
+                c->p_[id] += c->dp[id];
+            }
+    memset(c->dp, 0, c->tot * sizeof(double));
+    return SMAC_OK;
+}
+
+int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac_solve_stats* st) {
+    int rc;
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    else if (c->fl.rho0 != c->fl.rho1 || c->fl.mu0 != c->fl.mu1) smac_update_properties(c, 0);
+    if ((rc = smac_predictor(c, dt))) return rc;
+    if ((rc = smac_pressure_solve(c, dt, o, st))) return rc;
+    return smac_correct(c, dt);
+}
+
+/* k_calc_cfl + calc_cfl (CFD/G_SMACSolver.cu:1365-1430) */
+int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    double mc = 0.0, mv = 0.0;
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                ivec q = iv(i, j, k);
+                double m[3];
+                for (int a = 0; a < 3; a++) {
+                    double vp = fabs(c->vel[a][IX(c, wr(c, add(q, a, 1)))]);
+                    double vm = fabs(c->vel[a][IX(c, q)]);
+                    m[a] = vp > vm ? vp : vm;
+                }
+                double cf = dt * (m[0] * c->inv_d[0] + m[1] * c->inv_d[1] + m[2] * c->inv_d[2]);
+                size_t id = IX(c, q);
+                double cvv = c->inv_rho[id] * c->mu[id] * dt * (c->inv_d2[0] + c->inv_d2[1] + c->inv_d2[2]);
+                if (cf > mc) mc = cf;
+                if (cvv > mv) mv = cvv;
+            }
+    if (cc) *cc = mc;
+    if (cv) *cv = mv;
+    return SMAC_OK;
+}
+
+/* --------------------------------------------------------- lifecycle, IO */
+static int shape_of(const struct smac_ctx* c, int field, int sh[3]) {
+    sh[0] = c->n[0] + 2; sh[1] = c->n[1] + 2; sh[2] = c->n[2] + 2;
+    switch (field) {
+        case SMAC_FIELD_U: case SMAC_FIELD_USTAR: sh[0]++; return 0;
+        case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: sh[1]++; return 1;
+        case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: sh[2]++; return 2;
+        default: return -1;
+    }
+}
+static double* field_ptr(struct smac_ctx* c, int field) {
+    switch (field) {
+        case SMAC_FIELD_U: return c->vel[0];
+        case SMAC_FIELD_V: return c->vel[1];
+        case SMAC_FIELD_W: return c->vel[2];
+        case SMAC_FIELD_USTAR: return c->vst[0];
+        case SMAC_FIELD_VSTAR: return c->vst[1];
+        case SMAC_FIELD_WSTAR: return c->vst[2];
+        case SMAC_FIELD_P: return c->p_;
+        case SMAC_FIELD_DP: return c->dp;
+        case SMAC_FIELD_RHS: return c->rhs;
+        case SMAC_FIELD_ALPHA: return c->alpha;
+        default: return NULL;
+    }
+}
+size_t smac_field_count(const smac_ctx* c, int32_t field) {
+    if (!c || field < 0 || field >= SMAC_FIELD_COUNT) return 0;
+    int sh[3];
+    shape_of(c, field, sh);
+    return (size_t)sh[0] * sh[1] * sh[2];
+}
+
+/* Interior entries are taken from the host buffer; on download, periodic
+ * images (ghost cells, duplicate faces) are filled from their sources. */
+int smac_upload(smac_ctx* c, int32_t field, const double* h, size_t count) {
+    if (!c || !h || field < 0 || field >= SMAC_FIELD_COUNT) return SMAC_ERR_INVALID_ARG;
+    if (count != smac_field_count(c, field)) return SMAC_ERR_INVALID_ARG;
+    int sh[3];
+    shape_of(c, field, sh);
+    double* dst = field_ptr(c, field);
+    for (int k = 0; k < sh[2]; k++)
+        for (int j = 0; j < sh[1]; j++)
+            for (int i = 0; i < sh[0]; i++)
+                dst[IDX(c, i, j, k)] = h[((size_t)k * sh[1] + j) * sh[0] + i];
+    if (field == SMAC_FIELD_ALPHA) c->props_ready = 0;
+    return SMAC_OK;
+}
+int smac_download(smac_ctx* c, int32_t field, double* h, size_t count) {
+    if (!c || !h || field < 0 || field >= SMAC_FIELD_COUNT) return SMAC_ERR_INVALID_ARG;
+    if (count != smac_field_count(c, field)) return SMAC_ERR_INVALID_ARG;
+    int sh[3];
+    int a = shape_of(c, field, sh);
+    const double* src = field_ptr(c, field);
+    for (int k = 0; k < sh[2]; k++)
+        for (int j = 0; j < sh[1]; j++)
+            for (int i = 0; i < sh[0]; i++) {
+                int q[3] = {i, j, k};
+                for (int d = 0; d < 3; d++) {
+                    if (!c->per[d]) continue;
+                    /* cells: 0 -> n, n+1 -> 1; faces along a: 0 -> n, n+1 -> 1, n+2 -> 2 */
+                    q[d] = wrap(q[d], c->n[d]);
+                    if (d == a && q[d] > c->n[d]) q[d] -= c->n[d];
+                }
+                h[((size_t)k * sh[1] + j) * sh[0] + i] = src[IDX(c, q[0], q[1], q[2])];
+            }
+    return SMAC_OK;
+}
+int smac_upload_device(smac_ctx* c, int32_t f, const double* d, size_t n) {
+    (void)c; (void)f; (void)d; (void)n;
+    return SMAC_ERR_UNSUPPORTED;
+}
+int smac_download_device(smac_ctx* c, int32_t f, double* d, size_t n) {
+    (void)c; (void)f; (void)d; (void)n;
+    return SMAC_ERR_UNSUPPORTED;
+}
+
+int smac_set_fluid(smac_ctx* c, const smac_fluid_desc* f) {
+    if (!c || !f) return SMAC_ERR_INVALID_ARG;
+    c->fl = *f;
+    c->props_ready = 0;
+    return SMAC_OK;
+}
+
+int smac_create(const smac_grid_desc* g, const smac_fluid_desc* f, smac_ctx** out) {
+    if (!g || !out) return SMAC_ERR_INVALID_ARG;
+    *out = NULL;
+    int n[3] = {g->nx, g->ny, g->nz};
+    double sz[3] = {g->sizex, g->sizey, g->sizez};
+    for (int d = 0; d < 3; d++) {
+        if (n[d] < 2 || !(sz[d] > 0)) return SMAC_ERR_INVALID_GRID;
+        if (g->periodic[d] && n[d] < 4) return SMAC_ERR_INVALID_GRID;
+    }
+    if (g->num_bc < 0 || g->num_bc > 256 || (g->num_bc > 0 && !g->bc_table)) return SMAC_ERR_INVALID_ARG;
+    struct smac_ctx* c = (struct smac_ctx*)calloc(1, sizeof(*c));
+    for (int d = 0; d < 3; d++) {
+        c->n[d] = n[d];
+        c->p[d] = n[d] + 3;
+        c->per[d] = g->periodic[d] ? 1 : 0;
+        c->d[d] = sz[d] / (double)n[d];
+        c->inv_d[d] = 1. / c->d[d];
+        c->inv_d2[d] = 1. / (c->d[d] * c->d[d]);
+    }
+    memcpy(c->side_bcid, g->side_bcid, 6);
+    c->nbc = g->num_bc;
+    for (int b = 0; b < g->num_bc; b++) {
+        c->bctype[b] = g->bc_table[b].type;
+        c->bcv[b][0] = g->bc_table[b].vx;
+        c->bcv[b][1] = g->bc_table[b].vy;
+        c->bcv[b][2] = g->bc_table[b].vz;
+    }
+    for (int s = 0; s < 6; s++) {
+        if (c->per[s / 2]) continue;
+        if (c->bctype[c->side_bcid[s]] == SMAC_BC_OUTLET || c->bctype[c->side_bcid[s]] == SMAC_BC_OUTFLOW) {
+            free(c);
+            return SMAC_ERR_UNSUPPORTED;
+        }
+    }
+    if (f) c->fl = *f;
+    else { c->fl.rho0 = c->fl.rho1 = 1.0; c->fl.mu0 = c->fl.mu1 = 0.0; }
+    c->tot = (size_t)c->p[0] * c->p[1] * c->p[2];
+    double** arrs[] = {&c->vel[0], &c->vel[1], &c->vel[2], &c->vst[0], &c->vst[1], &c->vst[2], &c->p_, &c->dp,
+                       &c->rhs, &c->alpha, &c->rho, &c->rho_old, &c->inv_rho, &c->mu, &c->beta[0], &c->beta[1],
+                       &c->beta[2], &c->inv_rhof[0], &c->inv_rhof[1], &c->inv_rhof[2], &c->muf[0], &c->muf[1],
+                       &c->muf[2], &c->mf[0], &c->mf[1], &c->mf[2], &c->rhof[0], &c->rhof[1], &c->rhof[2],
+                       &c->r, &c->z, &c->dvec, &c->Ad, &c->Adiag, &c->Aoff[0], &c->Aoff[1], &c->Aoff[2],
+                       &c->Aoff[3], &c->Aoff[4], &c->Aoff[5]};
+    for (size_t t = 0; t < sizeof(arrs) / sizeof(arrs[0]); t++) {
+        *arrs[t] = (double*)calloc(c->tot, sizeof(double));
+        if (!*arrs[t]) { smac_destroy(c); return SMAC_ERR_OOM; }
+    }
+    *out = c;
+    return SMAC_OK;
+}
+
+void smac_destroy(smac_ctx* c) {
+    if (!c) return;
+    mg_free(c);
+    double* arrs[] = {c->vel[0], c->vel[1], c->vel[2], c->vst[0], c->vst[1], c->vst[2], c->p_, c->dp, c->rhs,
+                      c->alpha, c->rho, c->rho_old, c->inv_rho, c->mu, c->beta[0], c->beta[1], c->beta[2],
+                      c->inv_rhof[0], c->inv_rhof[1], c->inv_rhof[2], c->muf[0], c->muf[1], c->muf[2], c->mf[0],
+                      c->mf[1], c->mf[2], c->rhof[0], c->rhof[1], c->rhof[2], c->r, c->z, c->dvec, c->Ad,
+                      c->Adiag, c->Aoff[0], c->Aoff[1], c->Aoff[2], c->Aoff[3], c->Aoff[4], c->Aoff[5]};
+    for (size_t t = 0; t < sizeof(arrs) / sizeof(arrs[0]); t++) free(arrs[t]);
+    free(c);
+}
+
+int smac_set_profiling(smac_ctx* c, int32_t on) { (void)c; (void)on; return SMAC_OK; }
+int smac_kernel_time(smac_ctx* c, const char* name, int64_t* n, double* ms) {
+    (void)c; (void)name;
+    if (n) *n = 0;
+    if (ms) *ms = 0;
+    return SMAC_ERR_INVALID_ARG;
+}
+int smac_sync(smac_ctx* c) { (void)c; return SMAC_OK; }
+void* smac_stream(smac_ctx* c) { (void)c; return NULL; }
+const char* smac_backend_name(void) { return "oracle-cpu-fp64"; }
+const char* smac_strerror(int s) {
+    switch (s) {
+        case SMAC_OK: return "ok";
+        case SMAC_ERR_INVALID_ARG: return "invalid argument";
+        case SMAC_ERR_INVALID_GRID: return "grid sizes not supported by the multigrid hierarchy";
+        case SMAC_ERR_HIP: return "HIP runtime error";
+        case SMAC_ERR_NO_DEVICE: return "no device";
+        case SMAC_ERR_OOM: return "out of memory";
+        case SMAC_ERR_NAN: return "NaN in solve";
+        case SMAC_ERR_UNSUPPORTED: return "unsupported option";
+        default: return "unknown error";
+    }
+}
+
+/* ------------------------------------------------------------ test hooks */
+/* Apply the assembled Poisson operator to a cell field x (reference layout),
+ * for operator property tests (symmetry, null space, eigenfunctions). */
+int smac_oracle_apply_A(smac_ctx* c, const double* x_ref, double* y_ref) {
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    build_A(c);
+    size_t cnt = smac_field_count(c, SMAC_FIELD_P);
+    smac_upload(c, SMAC_FIELD_DP, x_ref, cnt);
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) c->Ad[IDX(c, i, j, k)] = apply_A(c, c->dp, i, j, k);
+    double* save = c->rhs;
+    c->rhs = c->Ad;
+    smac_download(c, SMAC_FIELD_RHS, y_ref, cnt);
+    c->rhs = save;
+    memset(c->dp, 0, c->tot * sizeof(double));
+    return SMAC_OK;
+}
+/* Apply the multigrid preconditioner z = M r (reference layout). */
+int smac_oracle_apply_mg(smac_ctx* c, const smac_solver_opts* o, const double* r_ref, double* z_ref) {
+    if (!c || !o) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    build_A(c);
+    int L = o->mg_levels > 0 ? o->mg_levels : mg_auto_levels(c);
+    if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
+    mg_setup(c, L);
+    size_t cnt = smac_field_count(c, SMAC_FIELD_P);
+    double* save = c->rhs;
+    c->rhs = c->r;
+    smac_upload(c, SMAC_FIELD_RHS, r_ref, cnt);
+    c->rhs = save;
+    precond(c, o);
+    save = c->rhs;
+    c->rhs = c->z;
+    smac_download(c, SMAC_FIELD_RHS, z_ref, cnt);
+    c->rhs = save;
+    return SMAC_OK;
+}
+/* Canonical sum of an interior cell field (reference layout). */
+double smac_oracle_csum(smac_ctx* c, const double* x_ref) {
+    size_t cnt = smac_field_count(c, SMAC_FIELD_P);
+    double* save = c->rhs;
+    c->rhs = c->Ad;
+    smac_upload(c, SMAC_FIELD_RHS, x_ref, cnt);
+    c->rhs = save;
+    return csum(c, RED_SUM, c->Ad, NULL);
+}
+int smac_oracle_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

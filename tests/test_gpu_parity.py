@@ -1,0 +1,133 @@
+"""GPU (HIP, gfx950) vs CPU FP64 oracle: the FP64 form of the SMAC step.
+
+Bar: max|a-b|/max|b| <= 1e-10 on u, v, w and p after N
+steps on identical seeded inputs (p compared modulo its mean). Both sides
+share the arithmetic contract (no FMA contraction, canonical reduction
+order), so in practice the fields agree exactly.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err, rel_err_mod_mean
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _pair(hip, oracle_mod, desc):
+    return hip.SMACContext(**desc), oracle_mod.OracleContext(**desc)
+
+
+def _upload_both(a, b, fields):
+    for name, arr in fields.items():
+        a.upload(name, arr)
+        b.upload(name, arr)
+
+
+def _compare(a, b, n, fields=("u", "v", "w"), tol=TOL):
+    out = {}
+    for f in fields:
+        out[f] = rel_err(a.download(f), b.download(f))
+    out["p"] = rel_err_mod_mean(a.download("p"), b.download("p"), n)
+    for f, e in out.items():
+        assert e <= tol, (f, e, out)
+    return out
+
+
+def test_predictor_cavity(hip, oracle_mod):
+    from mydemcfd_amd import cases
+
+    print("case: predictor cavity 12x10x8", flush=True)
+    desc = cases.cavity_desc(8)
+    desc["n"] = (12, 10, 8)
+    g, o = _pair(hip, oracle_mod, desc)
+    rng = np.random.default_rng(1)
+    fields = {f: rng.standard_normal(g.shape(f)) for f in ("u", "v", "w", "p")}
+    _upload_both(g, o, fields)
+    g.predictor(2e-3)
+    o.predictor(2e-3)
+    for f in ("ustar", "vstar", "wstar"):
+        assert rel_err(g.download(f), o.download(f)) == 0.0, f
+
+
+def test_predictor_periodic(hip, oracle_mod):
+    from mydemcfd_amd import cases
+
+    print("case: predictor periodic 16x8x12", flush=True)
+    desc = cases.periodic_desc((16, 8, 12))
+    g, o = _pair(hip, oracle_mod, desc)
+    rng = np.random.default_rng(2)
+    fields = {f: rng.standard_normal(g.shape(f)) for f in ("u", "v", "w", "p")}
+    _upload_both(g, o, fields)
+    g.predictor(1e-3)
+    o.predictor(1e-3)
+    for f in ("ustar", "vstar", "wstar"):
+        assert rel_err(g.download(f), o.download(f)) == 0.0, f
+
+
+@pytest.mark.parametrize("precond,stop", [("jacobi", "rel_l2"), ("jacobi", "ref_linf_dt"), ("mg", "rel_l2")])
+def test_cavity_steps(hip, oracle_mod, precond, stop):
+    from mydemcfd_amd import cases
+
+    print(f"case: cavity 16^3 x5 {precond} {stop}", flush=True)
+    desc = cases.cavity_desc(16)
+    g, o = _pair(hip, oracle_mod, desc)
+    tol = 1e-8 if stop == "rel_l2" else 1e-9
+    ce = 1 if precond == "mg" else 10
+    for s in range(5):
+        sg = g.step(1e-3, precond=precond, stop_rule=stop, tol=tol, check_every=ce)
+        so = o.step(1e-3, precond=precond, stop_rule=stop, tol=tol, check_every=ce)
+        assert sg.iters == so.iters and sg.converged == so.converged, (s, sg.as_dict(), so.as_dict())
+    _compare(g, o, desc["n"])
+
+
+def test_periodic_tg_mg(hip, oracle_mod):
+    from mydemcfd_amd import cases
+
+    print("case: TG periodic 32^3 x4 mg", flush=True)
+    n = 32
+    desc = cases.periodic_desc(n)
+    g, o = _pair(hip, oracle_mod, desc)
+    u, v, w = cases.tg_fields(n, pert="white")
+    _upload_both(g, o, {"u": u, "v": v, "w": w})
+    dt = cases.cfl_dt(o, 0.3)
+    assert abs(cases.cfl_dt(g, 0.3) - dt) == 0.0
+    for s in range(4):
+        sg = g.step(dt, precond="mg", tol=1e-8)
+        so = o.step(dt, precond="mg", tol=1e-8)
+        assert sg.iters == so.iters, (s, sg.as_dict(), so.as_dict())
+        assert sg.converged
+    _compare(g, o, (n, n, n))
+
+
+def test_cfl(hip, oracle_mod):
+    from mydemcfd_amd import cases
+
+    print("case: cfl periodic 16^3", flush=True)
+    desc = cases.periodic_desc(16, re=50.0)
+    g, o = _pair(hip, oracle_mod, desc)
+    u, v, w = cases.tg_fields(16, pert="white", pert_rms=1e-2)
+    _upload_both(g, o, {"u": u, "v": v, "w": w})
+    assert g.calc_cfl(1e-2) == o.calc_cfl(1e-2)
+
+
+@pytest.mark.timeout(600)
+def test_c1_cavity_64_100_steps(hip, oracle_mod):
+    """Config 1: 64^3 lid-driven cavity, Re=100, dt=1e-3, 100 SMAC steps,
+    Jacobi-PCG (pure Neumann) to 1e-8 relative residual."""
+    from mydemcfd_amd import cases
+
+    print("case: C1 cavity 64^3 x100 jacobi rel_l2 1e-8", flush=True)
+    desc = cases.cavity_desc(64, re=100.0)
+    g, o = _pair(hip, oracle_mod, desc)
+    iters = []
+    for s in range(100):
+        sg = g.step(1e-3, precond="jacobi", stop_rule="rel_l2", tol=1e-8)
+        so = o.step(1e-3, precond="jacobi", stop_rule="rel_l2", tol=1e-8)
+        assert sg.iters == so.iters, (s, sg.iters, so.iters)
+        iters.append(sg.iters)
+        if s % 10 == 0:
+            print(f"C1 step {s} iters {sg.iters}", flush=True)
+    errs = _compare(g, o, desc["n"])
+    print("C1 errors", errs, "iters first/last", iters[0], iters[-1], flush=True)
