@@ -1,0 +1,241 @@
+"""Benchmark: SMAC projection step, cell-steps/s at 512^3 with MG-PCG to 1e-8.
+
+Workload (one GPU): triply periodic [0,2pi)^3 box, 512^3 MAC grid, Taylor-Green
+velocity plus a seeded random-phase solenoidal k^-5/3 perturbation (|k| <= 32,
+5% of the kinetic energy), Re = 1600, dt fixed from CFL 0.3 of the initial
+field. One step = predictor -> RHS -> MG-PCG (RBGS V(2,2) preconditioner,
+||r||_2/||b||_2 < 1e-8) -> correction. Inputs are resident in HBM before the
+timed region; each form is timed over --steps steps after --warmup steps.
+
+Prints ONE JSON line. `roofline` is for the dominant kernel (largest total
+device time in the timed region, HIP events on the context stream), with
+algorithmic bytes per launch from the byte model below. `cpu_baseline` is the
+CPU FP64 oracle (a restatement of the reference algorithm; the reference's own
+CPU SMACSolver has no step) timed on this host on a bounded sample.
+
+Canonical byte model (bytes per interior cell, FP64 word = 8 B; W_s = storage
+width of the V-cycle):
+  step   B_step = 160 + (17.43*W_s + 32) + N_it * B_it,
+         B_it   = 104 + 17.43*W_s (+16 when the PCG is flexible)
+  kernels (per launch, per cell of the level it runs on):
+         spmv_dot 16, update 48, dot2 16 (+8 flexible), xpay 24,
+         mg_rbgs_l0 1.5*W_s (a half sweep), mg_residual_l0 3*W_s,
+         mg_restrict_l0 1.125*W_s, mg_prolong_l0 2.125*W_s, mg_fill_l0 W_s,
+         predict 56, rhs 32, correct 56, add_dp 32
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width
+    "spmv_dot": lambda W, flex: 16.0,
+    "update": lambda W, flex: 48.0,
+    "dot2": lambda W, flex: 16.0 + (8.0 if flex else 0.0),
+    "xpay": lambda W, flex: 24.0,
+    "mg_rbgs_l0": lambda W, flex: 1.5 * W,
+    "mg_residual_l0": lambda W, flex: 3.0 * W,
+    "mg_restrict_l0": lambda W, flex: 1.125 * W,
+    "mg_prolong_l0": lambda W, flex: 2.125 * W,
+    "mg_fill_l0": lambda W, flex: 1.0 * W,
+    "predict": lambda W, flex: 56.0,
+    "rhs": lambda W, flex: 32.0,
+    "correct": lambda W, flex: 56.0,
+    "add_dp": lambda W, flex: 32.0,
+}
+
+FORMS = {
+    "fp64": dict(precision="fp64", W=8.0, flexible=0),
+    "mixed_fp32": dict(precision="fp32", W=4.0, flexible=1),
+    "mixed_bf16": dict(precision="bf16", W=2.0, flexible=1),
+}
+
+
+def step_bytes(W, n_it_per_step, flexible):
+    b_it = 104.0 + 17.43 * W + (16.0 if flexible else 0.0)
+    return 160.0 + (17.43 * W + 32.0) + n_it_per_step * b_it
+
+
+def run_form(name, n, steps, warmup, fields, re, cfl, tol, levels):
+    import torch
+
+    import mydemcfd_amd
+    from mydemcfd_amd import cases
+
+    spec = FORMS[name]
+    desc = cases.periodic_desc(n, re=re)
+    ctx = mydemcfd_amd.SMACContext(**desc)
+    for fname, t in zip(("u", "v", "w"), fields):
+        ctx.upload_device(fname, t.data_ptr(), t.numel())
+    ctx.sync()
+    dt = cases.cfl_dt(ctx, cfl)
+    opts = ctx.opts(precond="mg", precision=spec["precision"], stop_rule="rel_l2", tol=tol,
+                    flexible=spec["flexible"], mg_levels=levels)
+    for _ in range(warmup):
+        st = ctx.step(dt, opts)
+        if not st.converged:
+            raise RuntimeError(f"{name}: warmup step did not converge: {st.as_dict()}")
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = []
+    res = []
+    for _ in range(steps):
+        st = ctx.step(dt, opts)
+        iters.append(st.iters)
+        res.append(st.rel_res)
+        if not st.converged:
+            raise RuntimeError(f"{name}: step did not converge: {st.as_dict()}")
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    kt = {}
+    for k in ctx.kernel_names():
+        cnt, ms = ctx.kernel_time(k)
+        kt[k] = (cnt, ms)
+    ctx.set_profiling(False)
+    out = {"form": name, "dt": dt, "wall_s": wall, "ms_per_step": 1e3 * wall / steps,
+           "cell_steps_per_s": n ** 3 * steps / wall, "iters": iters, "rel_res_max": max(res),
+           "kernels": kt, "W": spec["W"], "flexible": spec["flexible"]}
+    n_it = sum(iters) / len(iters)
+    bstep = step_bytes(spec["W"], n_it, spec["flexible"])
+    out["bytes_per_cell_step"] = bstep
+    out["step_model_gbps"] = bstep * n ** 3 * steps / wall / 1e9
+    out["step_roofline_frac"] = out["step_model_gbps"] / HBM_PEAK_GBS
+    # fields for the error report of the mixed forms
+    out["_ctx"] = ctx
+    return out
+
+
+def dominant_roofline(form, n):
+    best = None
+    for k, (cnt, ms) in form["kernels"].items():
+        if cnt == 0 or k not in KERNEL_BYTES:
+            continue
+        if best is None or ms > best[2]:
+            best = (k, cnt, ms)
+    if best is None:
+        return None
+    k, cnt, ms = best
+    per_launch = KERNEL_BYTES[k](form["W"], form["flexible"]) * n ** 3
+    avg_s = ms / cnt / 1e3
+    achieved = per_launch / avg_s / 1e9
+    return {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "launches": cnt,
+            "avg_us": round(1e6 * avg_s, 2), "alg_bytes_per_launch": per_launch}
+
+
+def cpu_baseline(n_cpu, re, cfl, tol):
+    """CPU FP64 oracle: one SMAC step of the same workload type at n_cpu^3."""
+    import numpy as np
+
+    from mydemcfd_amd import cases
+    from oracle import oracle
+
+    u, v, w = cases.tg_fields(n_cpu, seed=0x5EED0003, pert="k53")
+    ctx = oracle.OracleContext(**cases.periodic_desc(n_cpu, re=re))
+    for f, a in zip(("u", "v", "w"), (u, v, w)):
+        ctx.upload(f, a)
+    dt = cases.cfl_dt(ctx, cfl)
+    opts = ctx.opts(precond="mg", stop_rule="rel_l2", tol=tol)
+    ctx.apply_mg(np.zeros(ctx.shape("p")))  # multigrid setup (dense coarse inverse) outside the sample
+    t0 = time.perf_counter()
+    st = ctx.step(dt, opts)
+    t1 = time.perf_counter()
+    del np
+    return {"value": n_cpu ** 3 / (t1 - t0), "unit": "cell-steps/s", "cores": oracle.threads(), "kind": "port",
+            "sample": f"1 step, {n_cpu}^3 periodic TG+k^-5/3, FP64 RBGS-MG-PCG to 1e-8 "
+                      f"({st.iters} iterations, {t1 - t0:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--forms", default="fp64")
+    ap.add_argument("--headline", default=None, help="form reported as value (default: first form)")
+    ap.add_argument("--re", type=float, default=1600.0)
+    ap.add_argument("--cfl", type=float, default=0.3)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--levels", type=int, default=0)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is described "
+              f"in DESIGN.md and is not built.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import mydemcfd_amd
+    from mydemcfd_amd import cases
+
+    mydemcfd_amd.load()
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    headline = args.headline or forms[0]
+    t_ic = time.perf_counter()
+    fields = cases.tg_fields_torch(args.n, device="cuda")
+    torch.cuda.synchronize()
+    t_ic = time.perf_counter() - t_ic
+    results = {}
+    for f in forms:
+        results[f] = run_form(f, args.n, args.steps, args.warmup, fields, args.re, args.cfl, args.tol, args.levels)
+        print(f"# {f}: {results[f]['ms_per_step']:.2f} ms/step, iters {results[f]['iters']}", file=sys.stderr,
+              flush=True)
+    del fields
+    torch.cuda.empty_cache()
+    h = results[headline]
+    line = {
+        "metric": "cell-steps/s at 512^3 (MG-PCG to 1e-8 resid) + % HBM roofline, mixed vs FP64",
+        "value": h["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "mixed_fp32": "f32+f64", "mixed_bf16": "bf16+f32+f64"}[headline],
+        "data": "synthetic",
+        "config": {"workload": f"periodic TG + k^-5/3 (k<=32, 5% energy) {args.n}^3, Re={args.re:g}, "
+                               f"CFL {args.cfl:g}, RBGS-MG-PCG V(2,2) to rel-res {args.tol:g}, form {headline}",
+                   "n": args.n, "form": headline},
+        "roofline": dominant_roofline(h, args.n),
+        "step_model_gbps": h["step_model_gbps"],
+        "step_roofline_frac": h["step_roofline_frac"],
+        "bytes_per_cell_step": h["bytes_per_cell_step"],
+        "iters_per_step": h["iters"],
+        "forms": {f: {"cell_steps_per_s": r["cell_steps_per_s"], "ms_per_step": r["ms_per_step"],
+                      "iters": r["iters"], "step_model_gbps": r["step_model_gbps"],
+                      "step_roofline_frac": r["step_roofline_frac"], "rel_res_max": r["rel_res_max"],
+                      "kernels_ms": {k: round(v[1], 3) for k, v in r["kernels"].items()}}
+                  for f, r in results.items()},
+        "ic_seconds": t_ic,
+    }
+    for r in results.values():
+        r.pop("_ctx", None)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.re, args.cfl, args.tol)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -196,6 +196,10 @@ int smac_set_profiling(smac_ctx* ctx, int32_t on);
 /* Accumulated count and device milliseconds of kernel `name` since profiling was
  * switched on. Returns SMAC_ERR_INVALID_ARG for an unknown name. */
 int smac_kernel_time(smac_ctx* ctx, const char* name, int64_t* launches, double* ms);
+/* Comma-separated names of the kernels timed so far (buf of len bytes). */
+int smac_kernel_names(smac_ctx* ctx, char* buf, size_t len);
+/* Zero the accumulated kernel times. */
+int smac_reset_kernel_times(smac_ctx* ctx);
 /* Synchronize the context stream. */
 int smac_sync(smac_ctx* ctx);
 /* HIP stream of the context (hipStream_t), for callers that order work with it. */

@@ -90,6 +90,8 @@ SIGNATURES = {
     "smac_default_opts": (None, [C.POINTER(SolverOpts)]),
     "smac_set_profiling": (C.c_int, [_P, C.c_int32]),
     "smac_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), _D]),
+    "smac_kernel_names": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
+    "smac_reset_kernel_times": (C.c_int, [_P]),
     "smac_sync": (C.c_int, [_P]),
     "smac_stream": (_P, [_P]),
     "smac_strerror": (C.c_char_p, [C.c_int]),
@@ -245,6 +247,15 @@ class Context:
         n, ms = C.c_int64(), C.c_double()
         self._check(self.lib.smac_kernel_time(self.h, name.encode(), C.byref(n), C.byref(ms)), "smac_kernel_time")
         return n.value, ms.value
+
+    def kernel_names(self):
+        buf = C.create_string_buffer(8192)
+        self._check(self.lib.smac_kernel_names(self.h, buf, len(buf)), "smac_kernel_names")
+        v = buf.value.decode()
+        return [x for x in v.split(",") if x]
+
+    def reset_kernel_times(self):
+        self._check(self.lib.smac_reset_kernel_times(self.h), "smac_reset_kernel_times")
 
     def sync(self):
         self._check(self.lib.smac_sync(self.h), "smac_sync")

@@ -133,3 +133,58 @@ def cfl_dt(ctx, cfl_target, dt_probe=1.0, eps=1e-12):
     (CFD/CFDTime.cpp:22-48) with the clamps removed."""
     cc, cv = ctx.calc_cfl(dt_probe)
     return dt_probe * cfl_target / (max(cc, cv) + eps)
+
+
+# ----------------------------------------------------------------- torch (GPU) generators
+def tg_fields_torch(n, seed=0x5EED0003, kmax=32, slope=-5.0 / 3.0, energy_frac=0.05, device="cuda",
+                    length=2.0 * math.pi):
+    """Config-3 field on the GPU: Taylor-Green plus a random-phase solenoidal
+    k^slope perturbation (|k| <= kmax) holding energy_frac of the total kinetic
+    energy, generated with torch.fft on `device`. Returns reference-layout
+    float64 tensors (u, v, w), contiguous, on `device`."""
+    import torch
+
+    f64 = torch.float64
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    k1 = torch.fft.fftfreq(n, d=1.0 / n, device=device, dtype=f64)
+    kr = torch.fft.rfftfreq(n, d=1.0 / n, device=device, dtype=f64)
+    kk = torch.sqrt(kr[None, None, :] ** 2 + k1[None, :, None] ** 2 + k1[:, None, None] ** 2)
+    amp = torch.where((kk >= 1.0) & (kk <= kmax), torch.clamp(kk, min=1.0) ** ((slope - 4.0) / 2.0),
+                      torch.zeros_like(kk))
+    del kk
+    A = []
+    for _ in range(3):
+        ph = torch.rand(amp.shape, generator=gen, device=device, dtype=f64) * (2.0 * math.pi)
+        mag = torch.randn(amp.shape, generator=gen, device=device, dtype=f64) * amp
+        spec = torch.complex(mag * torch.cos(ph), mag * torch.sin(ph))
+        del ph, mag
+        A.append(torch.fft.irfftn(spec, s=(n, n, n)))
+        del spec
+    del amp
+    h = length / n
+    r = torch.roll
+    Ax, Ay, Az = A
+    pu = (r(Az, -1, 1) - Az) / h - (r(Ay, -1, 0) - Ay) / h
+    pv = (r(Ax, -1, 0) - Ax) / h - (r(Az, -1, 2) - Az) / h
+    pw = (r(Ay, -1, 2) - Ay) / h - (r(Ax, -1, 1) - Ax) / h
+    del A, Ax, Ay, Az
+    xf = torch.arange(n, device=device, dtype=f64) * h
+    xc = (torch.arange(n, device=device, dtype=f64) + 0.5) * h
+    u = torch.sin(xf)[None, None, :] * torch.cos(xc)[None, :, None] * torch.cos(xc)[:, None, None]
+    v = -torch.cos(xc)[None, None, :] * torch.sin(xf)[None, :, None] * torch.cos(xc)[:, None, None]
+    e_tg = 0.5 * float(torch.mean(u * u) + torch.mean(v * v))
+    e_p = 0.5 * float(torch.mean(pu * pu) + torch.mean(pv * pv) + torch.mean(pw * pw))
+    s = math.sqrt(energy_frac / (1.0 - energy_frac) * e_tg / e_p)
+    u = u + s * pu
+    v = v + s * pv
+    w = s * pw
+    del pu, pv, pw
+    out = []
+    for a0, fid in ((u, _abi.FIELD_U), (v, _abi.FIELD_V), (w, _abi.FIELD_W)):
+        shape = _abi.field_shape((n, n, n), fid)
+        iz = (torch.arange(shape[0], device=device) - 1) % n
+        iy = (torch.arange(shape[1], device=device) - 1) % n
+        ix = (torch.arange(shape[2], device=device) - 1) % n
+        out.append(a0.index_select(0, iz).index_select(1, iy).index_select(2, ix).contiguous())
+    return tuple(out)

@@ -63,7 +63,70 @@ struct smac_ctx {
   bool prof = false;
   hipEvent_t ev[8];
   bool ev_ok = false;
+  // per-kernel profiling: event pairs recorded around launches, resolved lazily
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+  struct Rec {
+    int name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::map<std::string, int> name_id;
+  std::vector<std::string> names;
+  std::vector<KStat> totals;
 };
+
+static hipEvent_t pool_event(smac_ctx* c) {
+  if (c->pool_used == c->pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->pool.push_back(e);
+  }
+  return c->pool[c->pool_used++];
+}
+static void prof_flush(smac_ctx* c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->st);
+  for (auto& r : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->totals[r.name].n += 1;
+      c->totals[r.name].ms += (double)ms;
+    }
+  }
+  c->pending.clear();
+  c->pool_used = 0;
+}
+// returns a record index, or -1 when profiling is off
+static int kbeg(smac_ctx* c, const char* name) {
+  if (!c->prof) return -1;
+  if (c->pool_used + 2 > 65536) prof_flush(c);
+  auto it = c->name_id.find(name);
+  int id;
+  if (it == c->name_id.end()) {
+    id = (int)c->names.size();
+    c->name_id[name] = id;
+    c->names.push_back(name);
+    c->totals.push_back(KStat());
+  } else {
+    id = it->second;
+  }
+  hipEvent_t a = pool_event(c), b = pool_event(c);
+  if (!a || !b) return -1;
+  (void)hipEventRecord(a, c->st);
+  c->pending.push_back({id, a, b});
+  return (int)c->pending.size() - 1;
+}
+static void kend(smac_ctx* c, int rec) {
+  if (rec < 0) return;
+  (void)hipEventRecord(c->pending[rec].b, c->st);
+}
+#define KT(c, name, launch)      \
+  do {                           \
+    int r_ = kbeg((c), (name));  \
+    launch;                      \
+    kend((c), r_);               \
+  } while (0)
 
 #define HIPOK(x)                         \
   do {                                   \
@@ -391,7 +454,7 @@ extern "C" int smac_predictor(smac_ctx* c, double dt) {
   if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
   Vel V;
   for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
-  launch_predict(c->g, V, c->p, c->us, dt, c->st);
+  KT(c, "predict", launch_predict(c->g, V, c->p, c->us, dt, c->st));
   HIPOK(hipGetLastError());
   return SMAC_OK;
 }
@@ -543,29 +606,35 @@ static int mg_setup(smac_ctx* c, int L) {
 
 static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   Level& V = c->lev[l];
+  const bool fine = l == 0;
   if (l == (int)c->lev.size() - 1) {
-    launch_mg_coarse(V.lv, c->cinv, V.f, V.e, c->st);
+    KT(c, "mg_coarse_solve", launch_mg_coarse(V.lv, c->cinv, V.f, V.e, c->st));
     return;
   }
-  launch_fill(V.lv, V.e, 0.0, c->st);
+  // the first red half sweep starts from e = 0 and writes every red cell; the
+  // black half sweep that follows reads only red cells, so no fill is needed
+  if (nu1 == 0) KT(c, fine ? "mg_fill_l0" : "mg_coarse_levels", launch_fill(V.lv, V.e, 0.0, c->st));
   for (int s = 0; s < nu1; s++) {
-    launch_mg_rbgs(V.lv, V.e, V.f, 0, c->st);
-    launch_mg_rbgs(V.lv, V.e, V.f, 1, c->st);
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 0, s == 0, c->st));
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 1, 0, c->st));
   }
-  launch_mg_residual(V.lv, V.e, V.f, V.res, c->st);
   Level& C = c->lev[l + 1];
-  launch_mg_restrict(V.lv, C.lv, V.res, C.f, c->st);
+  KT(c, fine ? "mg_restrict_l0" : "mg_coarse_levels",
+     launch_mg_restrict(V.lv, C.lv, V.e, V.f, V.res, C.f, c->st));
   vcycle(c, l + 1, nu1, nu2);
-  launch_mg_prolong(C.lv, V.lv, C.e, V.e, c->st);
+  // trilinear prolongation: with post-smoothing the black cells are recomputed
+  // before they are read, so only the red cells take the correction
+  KT(c, fine ? "mg_prolong_l0" : "mg_coarse_levels",
+     launch_mg_prolong(C.lv, V.lv, C.e, V.e, nu2 > 0 ? 0 : -1, c->st));
   for (int s = 0; s < nu2; s++) {
-    launch_mg_rbgs(V.lv, V.e, V.f, 1, c->st);
-    launch_mg_rbgs(V.lv, V.e, V.f, 0, c->st);
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 1, 0, c->st));
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 0, 0, c->st));
   }
 }
 
 static void precond(smac_ctx* c, const smac_solver_opts& o) {
   if (o.precond == SMAC_PRECOND_JACOBI) {
-    launch_jacobi(c->g, c->r, c->z, c->st);
+    KT(c, "jacobi", launch_jacobi(c->g, c->r, c->z, c->st));
     return;
   }
   vcycle(c, 0, o.nu1, o.nu2);
@@ -626,7 +695,7 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
   hipStream_t s = c->st;
   double** P = c->part;
 
-  launch_rhs(g, c->us, c->rhs, inv_dt, P[0], s);
+  KT(c, "rhs", launch_rhs(g, c->us, c->rhs, inv_dt, P[0], s));
   if (pure) {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_SUMB);
@@ -669,7 +738,7 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
   if (bb == 0.0) conv = 1;
   bool have_update = false;
   for (iter = 0; iter < o.max_iter && !conv; ++iter) {
-    launch_spmv_dot(g, c->dv, c->Ad, P[0], s);
+    KT(c, "spmv_dot", launch_spmv_dot(g, c->dv, c->Ad, P[0], s));
     {
       FinalizeArgs fa = fin(c);
       fin_add(fa, P[0], 0, S_PAP);
@@ -679,10 +748,10 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
       fa.sb = S_PAP;
       launch_finalize(fa, c->S, s);
     }
-    launch_update(g, c->dp, c->dv, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], s);
+    KT(c, "update", launch_update(g, c->dp, c->dv, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], s));
     have_update = true;
     precond(c, o);
-    launch_dot2(g, c->r, c->z, c->Ad, o.flexible, P[2], P[3], s);
+    KT(c, "dot2", launch_dot2(g, c->r, c->z, c->Ad, o.flexible, P[2], P[3], s));
     bool check = (iter % o.check_every) == 0;
     {
       FinalizeArgs fa = fin(c);
@@ -728,7 +797,7 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
         }
       }
     }
-    launch_xpay(g, c->z, c->dv, c->S, S_BETA, s);
+    KT(c, "xpay", launch_xpay(g, c->z, c->dv, c->S, S_BETA, s));
     launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }
   HIPOK(hipGetLastError());
@@ -761,8 +830,8 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
 extern "C" int smac_correct(smac_ctx* c, double dt) {
   if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
   if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
-  launch_correct(c->g, c->us, c->dp, c->u, dt, c->st);
-  launch_add_dp(c->g, c->p, c->dp, c->st);
+  KT(c, "correct", launch_correct(c->g, c->us, c->dp, c->u, dt, c->st));
+  KT(c, "add_dp", launch_add_dp(c->g, c->p, c->dp, c->st));
   HIPOK(hipGetLastError());
   return SMAC_OK;
 }
@@ -813,11 +882,34 @@ extern "C" int smac_set_profiling(smac_ctx* c, int32_t on) {
   return SMAC_OK;
 }
 extern "C" int smac_kernel_time(smac_ctx* c, const char* name, int64_t* n, double* ms) {
-  (void)c;
-  (void)name;
-  if (n) *n = 0;
-  if (ms) *ms = 0.0;
-  return SMAC_ERR_INVALID_ARG;
+  if (!c || !name) return SMAC_ERR_INVALID_ARG;
+  prof_flush(c);
+  auto it = c->name_id.find(name);
+  if (it == c->name_id.end()) {
+    if (n) *n = 0;
+    if (ms) *ms = 0.0;
+    return SMAC_ERR_INVALID_ARG;
+  }
+  if (n) *n = c->totals[it->second].n;
+  if (ms) *ms = c->totals[it->second].ms;
+  return SMAC_OK;
+}
+extern "C" int smac_kernel_names(smac_ctx* c, char* buf, size_t len) {
+  if (!c || !buf || len == 0) return SMAC_ERR_INVALID_ARG;
+  std::string all;
+  for (size_t i = 0; i < c->names.size(); i++) {
+    if (i) all += ",";
+    all += c->names[i];
+  }
+  if (all.size() + 1 > len) return SMAC_ERR_INVALID_ARG;
+  memcpy(buf, all.c_str(), all.size() + 1);
+  return SMAC_OK;
+}
+extern "C" int smac_reset_kernel_times(smac_ctx* c) {
+  if (!c) return SMAC_ERR_INVALID_ARG;
+  prof_flush(c);
+  for (auto& t : c->totals) t = KStat();
+  return SMAC_OK;
 }
 extern "C" int smac_sync(smac_ctx* c) {
   if (!c) return SMAC_ERR_INVALID_ARG;

@@ -57,6 +57,12 @@ __host__ __device__ __forceinline__ size_t idx(const Geo& g, int i, int j, int k
 }
 __host__ __device__ __forceinline__ size_t IX(const Geo& g, I3 a) { return idx(g, a.v[0], a.v[1], a.v[2]); }
 
+// Row-split position of natural x-index i (0..nx+1) inside its row: even i
+// first (positions 1..nx/2), then odd i. Ghost i = 0 maps to slot 0.
+__host__ __device__ __forceinline__ int spos(int i, int nx) {
+  return (i & 1) ? (nx >> 1) + ((i + 1) >> 1) : (i >> 1);
+}
+
 // Analytic face type of a box without internal walls
 // (SMACSolver::set_face_type, CFD/SMACSolver.cpp:104-202); q already wrapped.
 __device__ __forceinline__ int face_type(const Geo& g, int a, I3 q) {
@@ -84,6 +90,29 @@ __device__ __forceinline__ int face_slot(const Geo& g, int a, I3 q) {
   return 6;
 }
 
+// bc table lookups by slot with a select chain (a dynamic index into the
+// by-value kernel argument would force a private scratch copy)
+__device__ __forceinline__ int btype_of(const Geo& g, int s) {
+  int t = g.btype[6];
+  t = s == 0 ? g.btype[0] : t;
+  t = s == 1 ? g.btype[1] : t;
+  t = s == 2 ? g.btype[2] : t;
+  t = s == 3 ? g.btype[3] : t;
+  t = s == 4 ? g.btype[4] : t;
+  t = s == 5 ? g.btype[5] : t;
+  return t;
+}
+__device__ __forceinline__ double bv_of(const Geo& g, int s, int a) {
+  double v = g.bv[6][a];
+  v = s == 0 ? g.bv[0][a] : v;
+  v = s == 1 ? g.bv[1][a] : v;
+  v = s == 2 ? g.bv[2][a] : v;
+  v = s == 3 ? g.bv[3][a] : v;
+  v = s == 4 ? g.bv[4][a] : v;
+  v = s == 5 ? g.bv[5][a] : v;
+  return v;
+}
+
 // Velocity arrays of the three components.
 struct Vel {
   const double* c[3];
@@ -96,8 +125,8 @@ __device__ __forceinline__ double get_face(const Geo& g, const Vel& V, int a, I3
   if (t == F_INTERIOR) return V.c[a][IX(g, q)];
   if (t == F_BOUNDARY) {
     int s = face_slot(g, a, q);
-    int bt = g.btype[s];
-    if (bt == BC_INFLOW) return (1. - 0.0) * g.bv[s][a];
+    int bt = btype_of(g, s);
+    if (bt == BC_INFLOW) return (1. - 0.0) * bv_of(g, s, a);
     if (bt == BC_NOSLIP) return 0.;
     if (bt == BC_OUTLET) return V.c[a][IX(g, q)];
     if (bt == BC_SLIP && a != 0) return V.c[a][IX(g, q)];
@@ -116,16 +145,16 @@ __device__ __forceinline__ double get_dir(const Geo& g, const Vel& V, int a, int
   double vw = 0.0;
   int count = 0;
   if (face_type(g, b, qf) == F_BOUNDARY) {
-    vw += g.bv[face_slot(g, b, qf)][a];
+    vw += bv_of(g, face_slot(g, b, qf), a);
     count++;
   }
   I3 qf2 = wr(g, sh(qf, a, -1));
   if (face_type(g, b, qf2) == F_BOUNDARY) {
-    vw += g.bv[face_slot(g, b, qf2)][a];
+    vw += bv_of(g, face_slot(g, b, qf2), a);
     count++;
   }
   if (count > 0) vw /= (double)count;
-  int bt = g.btype[face_slot(g, b, qf)];
+  int bt = btype_of(g, face_slot(g, b, qf));
   if (bt == BC_SLIP || bt == BC_OUTLET) return vin;
   return 2.0 * vw - vin;
 }

@@ -53,10 +53,12 @@ void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
 void launch_correct(const Geo& g, double* const us[3], const double* dp, double* u[3], double dt, hipStream_t st);
 void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st);
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, double* p0, double* p1, hipStream_t st);
-void launch_mg_rbgs(const Lv& L, double* e, const double* f, int col, hipStream_t st);
+void launch_mg_rbgs(const Lv& L, double* e, const double* f, int col, int zero_start, hipStream_t st);
 void launch_mg_residual(const Lv& L, const double* e, const double* f, double* res, hipStream_t st);
-void launch_mg_restrict(const Lv& F, const Lv& C, const double* res, double* fc, hipStream_t st);
-void launch_mg_prolong(const Lv& C, const Lv& F, const double* ec, double* ef, hipStream_t st);
+// residual of level F restricted to the rhs fc of level C (T: scratch of F's size)
+void launch_mg_restrict(const Lv& F, const Lv& C, const double* e, const double* f, double* T, double* fc,
+                        hipStream_t st);
+void launch_mg_prolong(const Lv& C, const Lv& F, const double* ec, double* ef, int col, hipStream_t st);
 void launch_mg_coarse(const Lv& C, const double* ainv, const double* f, double* e, hipStream_t st);
 void launch_fill(const Lv& L, double* x, double v, hipStream_t st);
 

@@ -794,6 +794,8 @@ static int rweights(const struct smac_ctx* c, const level* C, int d, int I, int 
     if (I < nc) { idx[m] = f3; w[m] = 0.25; m++; }
     return m;
 }
+/* coarse f = (1/8) sum_z wz sum_y wy sum_x wx res over the folded 4x4x4 fine
+ * neighbourhood (the transpose of trilinear prolongation, scaled by 1/8) */
 static void restrict_res(const struct smac_ctx* c, const level* F, level* C) {
 #pragma omp parallel for collapse(2) schedule(static)
     for (int K = 1; K <= C->n[2]; K++)
@@ -804,14 +806,18 @@ static void restrict_res(const struct smac_ctx* c, const level* F, level* C) {
                 int mx = rweights(c, C, 0, I, ix, wx);
                 int my = rweights(c, C, 1, J, iy, wy);
                 int mz = rweights(c, C, 2, K, iz, wz);
-                double acc = 0.0;
-                for (int a = 0; a < mz; a++)
-                    for (int b = 0; b < my; b++)
-                        for (int g = 0; g < mx; g++) {
-                            double w = wz[a] * wy[b] * wx[g] * 0.125;
-                            acc += w * F->res[LIDX(F, ix[g], iy[b], iz[a])];
-                        }
-                C->f[LIDX(C, I, J, K)] = acc;
+                /* separable: x innermost, then y, then z, then the 1/8 */
+                double tz = 0.0;
+                for (int a = 0; a < mz; a++) {
+                    double ty = 0.0;
+                    for (int b = 0; b < my; b++) {
+                        double tx = 0.0;
+                        for (int g = 0; g < mx; g++) tx += wx[g] * F->res[LIDX(F, ix[g], iy[b], iz[a])];
+                        ty += wy[b] * tx;
+                    }
+                    tz += wz[a] * ty;
+                }
+                C->f[LIDX(C, I, J, K)] = 0.125 * tz;
             }
 }
 /* fine index i: parent (i+1)/2 weight 0.75, neighbour parent-1 (odd i) or
@@ -1239,6 +1245,13 @@ int smac_kernel_time(smac_ctx* c, const char* name, int64_t* n, double* ms) {
     if (ms) *ms = 0;
     return SMAC_ERR_INVALID_ARG;
 }
+int smac_kernel_names(smac_ctx* c, char* buf, size_t len) {
+    (void)c;
+    if (!buf || len == 0) return SMAC_ERR_INVALID_ARG;
+    buf[0] = 0;
+    return SMAC_OK;
+}
+int smac_reset_kernel_times(smac_ctx* c) { (void)c; return SMAC_OK; }
 int smac_sync(smac_ctx* c) { (void)c; return SMAC_OK; }
 void* smac_stream(smac_ctx* c) { (void)c; return NULL; }
 const char* smac_backend_name(void) { return "oracle-cpu-fp64"; }
