@@ -120,6 +120,29 @@ def run_form(name, n, steps, warmup, fields, re, cfl, tol, levels):
     return out
 
 
+def field_errors(ctx_a, ctx_b, n):
+    """max|a-b|/max|b| per field on the GPU (p modulo its interior mean)."""
+    import torch
+
+    from mydemcfd_amd import _abi
+
+    out = {}
+    for f in ("u", "v", "w", "p"):
+        shape = _abi.field_shape((n, n, n), _abi.FIELD_NAMES[f])
+        ta = torch.empty(shape, dtype=torch.float64, device="cuda")
+        tb = torch.empty_like(ta)
+        ctx_a.download_device(f, ta.data_ptr(), ta.numel())
+        ctx_b.download_device(f, tb.data_ptr(), tb.numel())
+        ia = ta[1:n + 1, 1:n + 1, 1:n + 1]
+        ib = tb[1:n + 1, 1:n + 1, 1:n + 1]
+        if f == "p":
+            ia = ia - ia.mean()
+            ib = ib - ib.mean()
+        out[f] = float((ia - ib).abs().max() / ib.abs().max())
+        del ta, tb
+    return out
+
+
 def dominant_roofline(form, n):
     best = None
     for k, (cnt, ms) in form["kernels"].items():
@@ -167,7 +190,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--forms", default="fp64")
+    ap.add_argument("--forms", default="mixed_fp32,mixed_bf16,fp64")
     ap.add_argument("--headline", default=None, help="form reported as value (default: first form)")
     ap.add_argument("--re", type=float, default=1600.0)
     ap.add_argument("--cfl", type=float, default=0.3)
@@ -228,6 +251,13 @@ def main():
                   for f, r in results.items()},
         "ic_seconds": t_ic,
     }
+    if "fp64" in results:
+        for f, r in results.items():
+            if f != "fp64":
+                line["forms"][f]["error_vs_fp64"] = field_errors(r["_ctx"], results["fp64"]["_ctx"], args.n)
+        if headline != "fp64":
+            line["error_vs_fp64"] = line["forms"][headline]["error_vs_fp64"]
+            line["speedup_vs_fp64"] = results[headline]["cell_steps_per_s"] / results["fp64"]["cell_steps_per_s"]
     for r in results.values():
         r.pop("_ctx", None)
     if not args.no_cpu:

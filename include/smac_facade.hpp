@@ -1,0 +1,164 @@
+// smac_facade.hpp -- header-only C++ facade over the C-ABI (smac_c_api.h) that
+// keeps the call surface of the reference's GPU solver classes, so a driver
+// shaped like CFD/main.cu:225-317 compiles against it:
+//
+//   G_SMACSolver g;                     // CFD/G_SMACSolver.h:9-125
+//   g.set_calc_properties(...);         // G_SMACSolver.cu:19-46
+//   g.solver_malloc(bc, fluid);         // G_SMACSolver.cu:1451-1496 (+ host set-up of face/cell types)
+//   G_PCGSolver pcg; G_GMGSolver gmg;   // pressure_solver/G_PCGSolver.h, G_GMGSolver.h
+//   gmg.initialize(g, levels); pcg.set_gmg(gmg);
+//   pcg.set_solver(PURENEUMANN_GMG_PCG);                  // G_PCGSolver.cu:592-613
+//   g.pressure_solver_ = &pcg;
+//   loop: g.pressure_solver_->tol_ = ...; g.calc_cfl(); g.get_vof_vstar_rhouu_consistent(solv);
+//         g.update_vstar_boundary(); g.solve_poisson(); g.correct_vof_velocity(solv);
+//
+// Errors: the reference returns void and aborts; the facade throws smac_error
+// carrying the C status code.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "smac_c_api.h"
+
+namespace smac_facade {
+
+// PCG_TYPES of CFD/Enums.h:28-33
+enum PCG_TYPES : unsigned char { STANDARD_PCG, PURENEUMANN_STANDARD_PCG, GMG_PCG, PURENEUMANN_GMG_PCG };
+
+struct smac_error : std::runtime_error {
+  int status;
+  smac_error(int s, const std::string& what)
+      : std::runtime_error(what + ": " + smac_strerror(s)), status(s) {}
+};
+inline void check(int s, const char* what) {
+  if (s != SMAC_OK) throw smac_error(s, what);
+}
+
+// The host-side parameters the reference passes by value as SMACSolver (dt, gravity).
+struct SMACSolver {
+  double dt_ = 0.001;
+  double inv_dt_ = 1000.0;
+  double gx_ = 0.0, gy_ = 0.0, gz_ = 0.0;
+};
+
+struct G_SMACSolver;
+
+// CFD/pressure_solver/G_PressureSolverBase.h:24-42
+struct G_PressureSolverBase {
+  virtual ~G_PressureSolverBase() {}
+  virtual void solve(G_SMACSolver& solv) = 0;
+  double tol_ = 1e-8;
+};
+
+// CFD/pressure_solver/G_GMGSolver.h:18-40 (here: RBGS V-cycle levels)
+struct G_GMGSolver {
+  int num_levels_ = 0;  // 0 = automatic
+  int nu1_ = 2, nu2_ = 2;
+  void initialize(G_SMACSolver&, int num_levels) { num_levels_ = num_levels; }
+  void free_levels() {}
+};
+
+struct G_SMACSolver {
+  smac_ctx* ctx_ = nullptr;
+  smac_grid_desc grid_{};
+  std::vector<smac_bc_entry> bc_;
+  smac_fluid_desc fluid_{1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double dt_ = 0.001, inv_dt_ = 1000.0;
+  double rho0_ = 1.0, rho1_ = 1.0, mu0_ = 0.0, mu1_ = 0.0;
+  G_PressureSolverBase* pressure_solver_ = nullptr;  // non-owning, as the reference
+  smac_solve_stats last_stats_{};
+
+  G_SMACSolver() = default;
+  G_SMACSolver(const G_SMACSolver&) = delete;
+  G_SMACSolver& operator=(const G_SMACSolver&) = delete;
+  ~G_SMACSolver() { solver_free(); }
+
+  void set_calc_properties(double ox, double oy, double oz, double sx, double sy, double sz, int nx, int ny,
+                           int nz) {
+    grid_.nx = nx;
+    grid_.ny = ny;
+    grid_.nz = nz;
+    grid_.sizex = sx;
+    grid_.sizey = sy;
+    grid_.sizez = sz;
+    grid_.originx = ox;
+    grid_.originy = oy;
+    grid_.originz = oz;
+  }
+  // periodic directions (an extension) and the per-side bc ids / bc table
+  // (StaggeredGrid::set_boundary_id, SimulationSetup::apply_boundary_conditions)
+  void set_boundaries(const int periodic[3], const unsigned char side_bcid[6], const std::vector<smac_bc_entry>& bc) {
+    for (int d = 0; d < 3; d++) grid_.periodic[d] = periodic[d];
+    for (int s = 0; s < 6; s++) grid_.side_bcid[s] = side_bcid[s];
+    bc_ = bc;
+  }
+  void set_fluid(double rho0, double rho1, double mu0, double mu1, double gx, double gy, double gz) {
+    rho0_ = rho0;
+    rho1_ = rho1;
+    mu0_ = mu0;
+    mu1_ = mu1;
+    fluid_ = smac_fluid_desc{rho0, rho1, mu0, mu1, gx, gy, gz};
+    if (ctx_) check(smac_set_fluid(ctx_, &fluid_), "smac_set_fluid");
+  }
+  void solver_malloc() {
+    solver_free();
+    grid_.num_bc = (int)bc_.size();
+    grid_.bc_table = bc_.data();
+    check(smac_create(&grid_, &fluid_, &ctx_), "smac_create");
+  }
+  void solver_free() {
+    if (ctx_) smac_destroy(ctx_);
+    ctx_ = nullptr;
+  }
+  // cpuTogpu / gpuTocpu for one field in the reference layout
+  void upload(int field, const std::vector<double>& h) {
+    check(smac_upload(ctx_, field, h.data(), h.size()), "smac_upload");
+  }
+  std::vector<double> download(int field) {
+    std::vector<double> h(smac_field_count(ctx_, field));
+    check(smac_download(ctx_, field, h.data(), h.size()), "smac_download");
+    return h;
+  }
+  double calc_cfl() {
+    double cc = 0, cv = 0;
+    check(smac_calc_cfl(ctx_, dt_, &cc, &cv), "smac_calc_cfl");
+    return cc > cv ? cc : cv;
+  }
+  void get_vof_vstar_rhouu_consistent(const SMACSolver& solv) {
+    dt_ = solv.dt_;
+    inv_dt_ = solv.inv_dt_;
+    check(smac_predictor(ctx_, solv.dt_), "smac_predictor");
+  }
+  void update_vstar_boundary() {}  // fused into smac_predictor
+  void solve_poisson() {
+    if (!pressure_solver_) throw smac_error(SMAC_ERR_INVALID_ARG, "solve_poisson without pressure_solver_");
+    pressure_solver_->solve(*this);
+  }
+  void correct_vof_velocity(const SMACSolver& solv) { check(smac_correct(ctx_, solv.dt_), "smac_correct"); }
+  void update_boundary_ghost(const SMACSolver&) {}  // not needed by the next step for the built BCs
+};
+
+// CFD/pressure_solver/G_PCGSolver.h:17-37
+struct G_PCGSolver : G_PressureSolverBase {
+  PCG_TYPES type_ = GMG_PCG;
+  G_GMGSolver* gmg_ = nullptr;
+  smac_solver_opts opts_{};
+  G_PCGSolver() { smac_default_opts(&opts_); }
+  void set_gmg(G_GMGSolver& gmg) { gmg_ = &gmg; }
+  void set_solver(PCG_TYPES t) { type_ = t; }
+  void solve(G_SMACSolver& s) override {
+    smac_solver_opts o = opts_;
+    o.tol = tol_;
+    o.precond = (type_ == GMG_PCG || type_ == PURENEUMANN_GMG_PCG) ? SMAC_PRECOND_MG : SMAC_PRECOND_JACOBI;
+    if (gmg_) {
+      o.mg_levels = gmg_->num_levels_;
+      o.nu1 = gmg_->nu1_;
+      o.nu2 = gmg_->nu2_;
+    }
+    check(smac_pressure_solve(s.ctx_, s.dt_, &o, &s.last_stats_), "smac_pressure_solve");
+  }
+};
+
+}  // namespace smac_facade

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OUT = os.path.join(LIBDIR, "libsmac_hip.so")
-SOURCES = ["smac_kernels.hip", "smac_capi.hip"]
+SOURCES = ["smac_kernels.hip", "smac_mg.hip", "smac_capi.hip"]
 ARCH = os.environ.get("SMAC_OFFLOAD_ARCH", "gfx950")
 
 
@@ -50,7 +50,26 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, OUT)
+    build_facade_driver(verbose)
     return OUT
+
+
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "cavity_facade.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "build", "cavity_facade")
+
+
+def build_facade_driver(verbose: bool = False) -> str:
+    """The C++ driver of tests/cpp (reference-shaped loop over the C++ facade)."""
+    if not os.path.exists(DRIVER_SRC):
+        return ""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o", DRIVER,
+           "-L", LIBDIR, "-lsmac_hip", "-Wl,-rpath," + LIBDIR]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return DRIVER
 
 
 if __name__ == "__main__":

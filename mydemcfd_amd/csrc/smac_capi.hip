@@ -22,12 +22,14 @@ using namespace smac;
 
 namespace {
 
+// One multigrid level: e (correction) and f (rhs) in the storage type of the
+// hierarchy's precision, tb (restriction buffer) in its compute type.
 struct Level {
   Lv lv;
-  double* e = nullptr;
-  double* f = nullptr;
-  double* res = nullptr;
-  bool owned = false;
+  void* e = nullptr;
+  void* f = nullptr;
+  void* tb = nullptr;
+  bool own_e = false, own_f = false, own_tb = false;
 };
 
 struct KStat {
@@ -60,6 +62,7 @@ struct smac_ctx {
   double* cinv = nullptr;
   int cinv_n = 0;
   int mg_L = 0;
+  int mg_prec = -1;
   bool prof = false;
   hipEvent_t ev[8];
   bool ev_ok = false;
@@ -137,6 +140,14 @@ static void kend(smac_ctx* c, int rec) {
     }                                    \
   } while (0)
 
+static int balloc(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    return SMAC_ERR_OOM;
+  }
+  if (hipMemset(*p, 0, bytes) != hipSuccess) return SMAC_ERR_HIP;
+  return SMAC_OK;
+}
 static int dalloc(double** p, size_t n) {
   if (hipMalloc((void**)p, n * sizeof(double)) != hipSuccess) {
     *p = nullptr;
@@ -263,13 +274,13 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
 }
 
 static void mg_free(smac_ctx* c) {
-  for (auto& L : c->lev)
-    if (L.owned) {
-      (void)hipFree(L.e);
-      (void)hipFree(L.f);
-      (void)hipFree(L.res);
-    }
+  for (auto& L : c->lev) {
+    if (L.own_e) (void)hipFree(L.e);
+    if (L.own_f) (void)hipFree(L.f);
+    if (L.own_tb) (void)hipFree(L.tb);
+  }
   c->lev.clear();
+  c->mg_prec = -1;
   if (c->cinv) (void)hipFree(c->cinv);
   c->cinv = nullptr;
   c->cinv_n = 0;
@@ -563,10 +574,14 @@ static void dense_pinv(const Lv& L, double* out, int singular) {
   }
 }
 
-static int mg_setup(smac_ctx* c, int L) {
-  if (c->mg_L == L && !c->lev.empty() && c->lev[0].lv.beta0 == c->g.beta0) return SMAC_OK;
+static size_t prec_bytes(int prec) { return prec == SMAC_PREC_FP64 ? 8 : (prec == SMAC_PREC_MIXED_FP32 ? 4 : 2); }
+static size_t prec_cbytes(int prec) { return prec == SMAC_PREC_FP64 ? 8 : 4; }
+
+static int mg_setup(smac_ctx* c, int L, int prec) {
+  if (c->mg_L == L && c->mg_prec == prec && !c->lev.empty() && c->lev[0].lv.beta0 == c->g.beta0) return SMAC_OK;
   mg_free(c);
   c->lev.resize(L);
+  const size_t sb = prec_bytes(prec), cb = prec_cbytes(prec);
   for (int l = 0; l < L; l++) {
     Level& V = c->lev[l];
     for (int d = 0; d < 3; d++) {
@@ -578,19 +593,26 @@ static int mg_setup(smac_ctx* c, int L) {
     V.lv.px = V.lv.n[0] + 3;
     V.lv.py = V.lv.n[1] + 3;
     V.lv.beta0 = c->g.beta0;
-    if (l == 0) {
+    size_t t = (size_t)V.lv.px * V.lv.py * (size_t)(V.lv.n[2] + 3);
+    int rc = SMAC_OK;
+    if (l == 0 && prec == SMAC_PREC_FP64) {
+      // the FP64 fine level works on the PCG vectors themselves
       V.e = c->z;
       V.f = c->r;
-      V.res = c->res0;
-      V.owned = false;
+      V.tb = c->res0;
     } else {
-      size_t t = (size_t)V.lv.px * V.lv.py * (size_t)(V.lv.n[2] + 3);
-      int rc = dalloc(&V.e, t);
-      if (!rc) rc = dalloc(&V.f, t);
-      if (!rc) rc = dalloc(&V.res, t);
-      V.owned = true;
-      if (rc) return rc;
+      rc = balloc(&V.e, t * sb);
+      V.own_e = rc == SMAC_OK;
+      if (!rc) {
+        rc = balloc(&V.f, t * sb);
+        V.own_f = rc == SMAC_OK;
+      }
+      if (!rc) {
+        rc = balloc(&V.tb, t * cb);
+        V.own_tb = rc == SMAC_OK;
+      }
     }
+    if (rc) return rc;
   }
   const Lv& C = c->lev[L - 1].lv;
   int n = C.n[0] * C.n[1] * C.n[2];
@@ -601,43 +623,40 @@ static int mg_setup(smac_ctx* c, int L) {
   HIPOK(hipMemcpy(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
   c->cinv_n = n;
   c->mg_L = L;
+  c->mg_prec = prec;
   return SMAC_OK;
 }
 
+template <typename T>
 static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
+  using Cc = typename St<T>::C;
   Level& V = c->lev[l];
+  T* e = (T*)V.e;
+  T* f = (T*)V.f;
   const bool fine = l == 0;
   if (l == (int)c->lev.size() - 1) {
-    KT(c, "mg_coarse_solve", launch_mg_coarse(V.lv, c->cinv, V.f, V.e, c->st));
+    KT(c, "mg_coarse_solve", launch_mg_coarse<T>(V.lv, c->cinv, f, e, c->st));
     return;
   }
   // the first red half sweep starts from e = 0 and writes every red cell; the
   // black half sweep that follows reads only red cells, so no fill is needed
-  if (nu1 == 0) KT(c, fine ? "mg_fill_l0" : "mg_coarse_levels", launch_fill(V.lv, V.e, 0.0, c->st));
+  if (nu1 == 0) KT(c, fine ? "mg_fill_l0" : "mg_coarse_levels", launch_fill<T>(V.lv, e, c->st));
   for (int s = 0; s < nu1; s++) {
-    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 0, s == 0, c->st));
-    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 1, 0, c->st));
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 0, s == 0, c->st));
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
   }
   Level& C = c->lev[l + 1];
   KT(c, fine ? "mg_restrict_l0" : "mg_coarse_levels",
-     launch_mg_restrict(V.lv, C.lv, V.e, V.f, V.res, C.f, c->st));
-  vcycle(c, l + 1, nu1, nu2);
+     launch_mg_restrict<T>(V.lv, C.lv, e, f, (Cc*)V.tb, (T*)C.f, c->st));
+  vcycle<T>(c, l + 1, nu1, nu2);
   // trilinear prolongation: with post-smoothing the black cells are recomputed
   // before they are read, so only the red cells take the correction
   KT(c, fine ? "mg_prolong_l0" : "mg_coarse_levels",
-     launch_mg_prolong(C.lv, V.lv, C.e, V.e, nu2 > 0 ? 0 : -1, c->st));
+     launch_mg_prolong<T>(C.lv, V.lv, (const T*)C.e, e, nu2 > 0 ? 0 : -1, c->st));
   for (int s = 0; s < nu2; s++) {
-    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 1, 0, c->st));
-    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs(V.lv, V.e, V.f, 0, 0, c->st));
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
+    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st));
   }
-}
-
-static void precond(smac_ctx* c, const smac_solver_opts& o) {
-  if (o.precond == SMAC_PRECOND_JACOBI) {
-    KT(c, "jacobi", launch_jacobi(c->g, c->r, c->z, c->st));
-    return;
-  }
-  vcycle(c, 0, o.nu1, o.nu2);
 }
 
 static FinalizeArgs fin(const smac_ctx* c) {
@@ -672,26 +691,27 @@ extern "C" void smac_default_opts(smac_solver_opts* o) {
   o->flexible = 0;
 }
 
-extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st) {
-  if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
-  smac_solver_opts o;
-  if (opts) o = *opts;
-  else smac_default_opts(&o);
-  if (o.check_every < 1) o.check_every = 1;
-  if (o.precision != SMAC_PREC_FP64) return SMAC_ERR_UNSUPPORTED;
-  if (o.precond != SMAC_PRECOND_JACOBI && o.precond != SMAC_PRECOND_MG) return SMAC_ERR_INVALID_ARG;
-  if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
+// The PCG of G_PCGSolver::solve_* (CFD/pressure_solver/G_PCGSolver.cu:307-922)
+// with storage T for the preconditioner vectors: z and the multigrid levels in
+// T, r, d, Ad and dp in FP64. For T = double z is c->z and the fine level works
+// on r directly; otherwise the fine-level rhs f0 = (T) r is refreshed by the
+// update kernel.
+template <typename T>
+static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve_stats* st, int L) {
   const Geo& g = c->g;
   double Ncell = (double)c->n[0] * (double)c->n[1] * (double)c->n[2];
   int pure = pure_neumann(c);
   double inv_dt = 1. / dt;
-  int L = 1;
-  if (o.precond == SMAC_PRECOND_MG) {
-    L = o.mg_levels > 0 ? o.mg_levels : mg_auto_levels(c);
-    if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
-    int rc = mg_setup(c, L);
-    if (rc) return rc;
-  }
+  const bool mg = o.precond == SMAC_PRECOND_MG;
+  T* z = mg ? (T*)c->lev[0].e : (T*)c->z;
+  T* f0 = (mg && (void*)c->lev[0].f != (void*)c->r) ? (T*)c->lev[0].f : nullptr;
+  auto precond = [&]() {
+    if (!mg) {
+      KT(c, "jacobi", launch_jacobi(c->g, c->r, c->z, c->st));
+      return;
+    }
+    vcycle<T>(c, 0, o.nu1, o.nu2);
+  };
   hipStream_t s = c->st;
   double** P = c->part;
 
@@ -713,16 +733,17 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
     launch_finalize(fa, c->S, s);
   }
   double size_inv = 1. / Ncell;
-  if (pure) launch_add_scalar(g, c->r, -1. * size_inv, c->S, S_SUMR, s);
-  precond(c, o);
+  if (pure) launch_add_scalar<double>(g, c->r, -1. * size_inv, c->S, S_SUMR, s);
+  if (f0) launch_cvt<T>(g, c->r, f0, s);
+  precond();
   if (pure) {
-    launch_sum(g, c->z, P[0], s);
+    launch_sum<T>(g, z, P[0], s);
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_SUMZ);
     launch_finalize(fa, c->S, s);
-    launch_add_scalar(g, c->z, -1. * size_inv, c->S, S_SUMZ, s);
+    launch_add_scalar<T>(g, z, -1. * size_inv, c->S, S_SUMZ, s);
   }
-  launch_copy_dot(g, c->z, c->dv, c->r, P[0], s);
+  launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s);
   {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_RZ);
@@ -748,10 +769,10 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
       fa.sb = S_PAP;
       launch_finalize(fa, c->S, s);
     }
-    KT(c, "update", launch_update(g, c->dp, c->dv, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], s));
+    KT(c, "update", launch_update<T>(g, c->dp, c->dv, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
     have_update = true;
-    precond(c, o);
-    KT(c, "dot2", launch_dot2(g, c->r, c->z, c->Ad, o.flexible, P[2], P[3], s));
+    precond();
+    KT(c, "dot2", launch_dot2<T>(g, c->r, z, c->Ad, o.flexible, P[2], P[3], s));
     bool check = (iter % o.check_every) == 0;
     {
       FinalizeArgs fa = fin(c);
@@ -797,7 +818,7 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
         }
       }
     }
-    KT(c, "xpay", launch_xpay(g, c->z, c->dv, c->S, S_BETA, s));
+    KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->S, S_BETA, s));
     launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }
   HIPOK(hipGetLastError());
@@ -806,7 +827,7 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
     st->iters = iter;
     st->converged = conv;
     st->pure_neumann = pure;
-    st->mg_levels = o.precond == SMAC_PRECOND_MG ? L : 0;
+    st->mg_levels = mg ? L : 0;
     st->bnorm = bnorm;
     if (have_update) {
       FinalizeArgs fa = fin(c);
@@ -825,6 +846,31 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
   }
   if (!(bb == bb) || !(rr == rr)) return SMAC_ERR_NAN;
   return SMAC_OK;
+}
+
+extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st) {
+  if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
+  smac_solver_opts o;
+  if (opts) o = *opts;
+  else smac_default_opts(&o);
+  if (o.check_every < 1) o.check_every = 1;
+  if (o.precond != SMAC_PRECOND_JACOBI && o.precond != SMAC_PRECOND_MG) return SMAC_ERR_INVALID_ARG;
+  if (o.precision < SMAC_PREC_FP64 || o.precision > SMAC_PREC_MIXED_BF16) return SMAC_ERR_INVALID_ARG;
+  if (o.precond == SMAC_PRECOND_JACOBI && o.precision != SMAC_PREC_FP64) return SMAC_ERR_UNSUPPORTED;
+  if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
+  if (c->n[0] > 4000) return SMAC_ERR_INVALID_GRID;  // restriction row buffer in LDS
+  int L = 1;
+  if (o.precond == SMAC_PRECOND_MG) {
+    L = o.mg_levels > 0 ? o.mg_levels : mg_auto_levels(c);
+    if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
+    int rc = mg_setup(c, L, o.precision);
+    if (rc) return rc;
+  }
+  switch (o.precision) {
+    case SMAC_PREC_MIXED_FP32: return solve_t<float>(c, dt, o, st, L);
+    case SMAC_PREC_MIXED_BF16: return solve_t<bf16s>(c, dt, o, st, L);
+    default: return solve_t<double>(c, dt, o, st, L);
+  }
 }
 
 extern "C" int smac_correct(smac_ctx* c, double dt) {

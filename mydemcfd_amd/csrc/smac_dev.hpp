@@ -198,6 +198,45 @@ __device__ __forceinline__ Coef coef_uniform(const int n[3], const int per[3], c
   return c;
 }
 
+// Storage traits of the multigrid vectors: storage type T, compute type C.
+// bf16 is kept as raw 16-bit words (upper half of an IEEE float), rounded to
+// nearest-even on store.
+struct bf16s {
+  uint16_t u;
+};
+template <typename T>
+struct St;
+template <>
+struct St<double> {
+  using C = double;
+  static __device__ __forceinline__ double ld(const double* p, size_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(double* p, size_t i, double v) { p[i] = v; }
+};
+template <>
+struct St<float> {
+  using C = float;
+  static __device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
+};
+template <>
+struct St<bf16s> {
+  using C = float;
+  static __device__ __forceinline__ float ld(const bf16s* p, size_t i) {
+    return __uint_as_float(((unsigned)p[i].u) << 16);
+  }
+  static __device__ __forceinline__ void st(bf16s* p, size_t i, float v) {
+    unsigned u = __float_as_uint(v);
+    unsigned r;
+    if ((u & 0x7f800000u) == 0x7f800000u) {
+      r = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);  // inf stays inf, NaN stays NaN
+    } else {
+      u += 0x7fffu + ((u >> 16) & 1u);
+      r = u >> 16;
+    }
+    p[i].u = (uint16_t)r;
+  }
+};
+
 // Canonical wave64 halving tree: lane l += lane l+s, s = 32..1; lane 0 holds
 // the result. Identical association to the CPU oracle's tree().
 __device__ __forceinline__ double wave_tree_sum(double v) {

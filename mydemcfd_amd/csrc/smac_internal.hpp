@@ -16,6 +16,28 @@ struct Lv {
   double beta0;
 };
 
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// works on a contiguous slab of rows and the y/z stencil neighbours of a row
+// sit in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int q = nb >> 3;
+  if (b >= (q << 3)) return b;
+  return (b & 7) * q + (b >> 3);
+}
+
+#define ROW_SETUP_N(NX, NY, NZ, PX, PY)                                        \
+  const int lane = threadIdx.x & 63;                                          \
+  const int row = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)(threadIdx.x >> 6); \
+  if (row >= (NY) * (NZ)) return;                                             \
+  const int j = row % (NY) + 1;                                               \
+  const int k = row / (NY) + 1;                                               \
+  const size_t rb = ((size_t)k * (size_t)(PY) + (size_t)j) * (size_t)(PX);    \
+  (void)rb;
+
+#define ROW_SETUP ROW_SETUP_N(g.n[0], g.n[1], g.n[2], g.px, g.py)
+#define LROW_SETUP ROW_SETUP_N(L.n[0], L.n[1], L.n[2], L.px, L.py)
+
+
 enum { OP_NONE = 0, OP_MEAN = 1, OP_DIV = 2, OP_BETA_PR = 3 };
 
 struct FinalizeArgs {
@@ -38,28 +60,41 @@ void launch_predict(const Geo& g, const Vel& V, const double* p, double* us[3], 
 void launch_rhs(const Geo& g, double* const us[3], double* rhs, double inv_dt, double* part0, hipStream_t st);
 void launch_init_r(const Geo& g, const double* rhs, double* r, double* dp, const double* S, int slot_mean,
                    double* p0, double* p1, hipStream_t st);
-void launch_add_scalar(const Geo& g, double* x, double a, const double* S, int slot, hipStream_t st);
-void launch_sum(const Geo& g, const double* x, double* p0, hipStream_t st);
+template <typename Z>
+void launch_add_scalar(const Geo& g, Z* x, double a, const double* S, int slot, hipStream_t st);
+template <typename Z>
+void launch_sum(const Geo& g, const Z* x, double* p0, hipStream_t st);
 void launch_jacobi(const Geo& g, const double* r, double* z, hipStream_t st);
-void launch_copy_dot(const Geo& g, const double* z, double* d, const double* r, double* p0, hipStream_t st);
+template <typename Z>
+void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st);
 void launch_spmv_dot(const Geo& g, const double* d, double* Ad, double* p0, hipStream_t st);
+template <typename F>
 void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,
-                   int slot_alpha, double* p0, double* p1, hipStream_t st);
-void launch_dot2(const Geo& g, const double* r, const double* z, const double* Ad, int flexible, double* p0,
+                   int slot_alpha, double* p0, double* p1, F* f0, hipStream_t st);
+template <typename Z>
+void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, int flexible, double* p0,
                  double* p1, hipStream_t st);
-void launch_xpay(const Geo& g, const double* z, double* d, const double* S, int slot_beta, hipStream_t st);
+template <typename Z>
+void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st);
+template <typename F>
+void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
 void launch_correct(const Geo& g, double* const us[3], const double* dp, double* u[3], double dt, hipStream_t st);
 void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st);
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, double* p0, double* p1, hipStream_t st);
-void launch_mg_rbgs(const Lv& L, double* e, const double* f, int col, int zero_start, hipStream_t st);
-void launch_mg_residual(const Lv& L, const double* e, const double* f, double* res, hipStream_t st);
-// residual of level F restricted to the rhs fc of level C (T: scratch of F's size)
-void launch_mg_restrict(const Lv& F, const Lv& C, const double* e, const double* f, double* T, double* fc,
+// multigrid (smac_mg.hip), storage T in {double, float, bf16s}; the restriction
+// buffer tb has the compute type of T
+template <typename T>
+void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st);
+template <typename T>
+void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
                         hipStream_t st);
-void launch_mg_prolong(const Lv& C, const Lv& F, const double* ec, double* ef, int col, hipStream_t st);
-void launch_mg_coarse(const Lv& C, const double* ainv, const double* f, double* e, hipStream_t st);
-void launch_fill(const Lv& L, double* x, double v, hipStream_t st);
+template <typename T>
+void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hipStream_t st);
+template <typename T>
+void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st);
+template <typename T>
+void launch_fill(const Lv& L, T* x, hipStream_t st);
 
 }  // namespace smac

@@ -7,8 +7,7 @@
 //   PCG vectors      k_init_r, k_add_scalar, k_jacobi, k_copy_dot, k_sum,
 //                    k_spmv_dot, k_update, k_dot2, k_xpay
 //   reductions       k_finalize        canonical 1024-lane tree + scalar ops
-//   multigrid        k_mg_rbgs, k_mg_residual, k_mg_restrict, k_mg_prolong,
-//                    k_mg_coarse, k_fill
+//   (multigrid kernels: smac_mg.hip)
 //   correction       k_correct, k_add_dp
 //
 // Row kernels: one wave64 per grid row (j,k); lane l owns i = 1+l, 1+l+64, ...
@@ -211,26 +210,6 @@ __global__ __launch_bounds__(256) void k_star_bc(Geo g, double* us0, double* us1
 }
 
 // ------------------------------------------------------------- row kernels
-// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
-// works on a contiguous slab of rows and the y/z stencil neighbours of a row
-// sit in the same L2.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  const int q = nb >> 3;
-  if (b >= (q << 3)) return b;
-  return (b & 7) * q + (b >> 3);
-}
-
-#define ROW_SETUP_N(NX, NY, NZ, PX, PY)                                        \
-  const int lane = threadIdx.x & 63;                                          \
-  const int row = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)(threadIdx.x >> 6); \
-  if (row >= (NY) * (NZ)) return;                                             \
-  const int j = row % (NY) + 1;                                               \
-  const int k = row / (NY) + 1;                                               \
-  const size_t rb = ((size_t)k * (size_t)(PY) + (size_t)j) * (size_t)(PX);    \
-  (void)rb;
-
-#define ROW_SETUP ROW_SETUP_N(g.n[0], g.n[1], g.n[2], g.px, g.py)
-#define LROW_SETUP ROW_SETUP_N(L.n[0], L.n[1], L.n[2], L.px, L.py)
 
 // base_k_make_poisson_rhs with eps = 1, no surface tension
 // (CFD/pressure_solver/G_PressureSolverBase.cu:49-118). part0 <- row sums of b.
@@ -276,18 +255,21 @@ __global__ __launch_bounds__(256) void k_init_r(Geo g, const double* __restrict_
 }
 
 // x += a * S[slot] on interior cells (subtract_cell_mean, G_PressureSolverBase.cu:142-155)
-__global__ __launch_bounds__(256) void k_add_scalar(Geo g, double* __restrict__ x, double a,
+template <typename Z>
+__global__ __launch_bounds__(256) void k_add_scalar(Geo g, Z* __restrict__ x, double a,
                                                     const double* __restrict__ S, int slot) {
   ROW_SETUP
   const double s = S[slot];
-  for (int p = 1 + lane; p <= g.n[0]; p += 64) x[rb + p] += a * s;
+  for (int p = 1 + lane; p <= g.n[0]; p += 64)
+    St<Z>::st(x, rb + p, (typename St<Z>::C)((double)St<Z>::ld(x, rb + p) + a * s));
 }
 
 // part0 <- row sums of x (split layout, canonical natural order)
-__global__ __launch_bounds__(256) void k_sum(Geo g, const double* __restrict__ x, double* __restrict__ part0) {
+template <typename Z>
+__global__ __launch_bounds__(256) void k_sum(Geo g, const Z* __restrict__ x, double* __restrict__ part0) {
   ROW_SETUP
   double acc = 0.0;
-  for (int i = 1 + lane; i <= g.n[0]; i += 64) acc += x[rb + spos(i, g.n[0])];
+  for (int i = 1 + lane; i <= g.n[0]; i += 64) acc += (double)St<Z>::ld(x, rb + spos(i, g.n[0]));
   acc = wave_tree_sum(acc);
   if (lane == 0) part0[row] = acc;
 }
@@ -303,13 +285,14 @@ __global__ __launch_bounds__(256) void k_jacobi(Geo g, const double* __restrict_
 }
 
 // d = z; part0 <- sum r*z
-__global__ __launch_bounds__(256) void k_copy_dot(Geo g, const double* __restrict__ z, double* __restrict__ d,
+template <typename Z>
+__global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z, double* __restrict__ d,
                                                   const double* __restrict__ r, double* __restrict__ part0) {
   ROW_SETUP
   double acc = 0.0;
   for (int i = 1 + lane; i <= g.n[0]; i += 64) {
     const size_t o = rb + spos(i, g.n[0]);
-    double zz = z[o];
+    double zz = (double)St<Z>::ld(z, o);
     d[o] = zz;
     acc += r[o] * zz;
   }
@@ -342,10 +325,14 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, const double* __restric
 
 // dp += alpha d (dp natural); r -= alpha Ad (k_update_p_r, G_PCGSolver.cu:66-85);
 // part0 <- sum r^2, part1 <- max |r|
+// F: multigrid storage; when f0 is given it receives (F) r (the fine-level rhs
+// of the next V-cycle in the mixed forms)
+template <typename F>
 __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, const double* __restrict__ d,
                                                 double* __restrict__ r, const double* __restrict__ Ad,
                                                 const double* __restrict__ S, int slot_alpha,
-                                                double* __restrict__ part0, double* __restrict__ part1) {
+                                                double* __restrict__ part0, double* __restrict__ part1,
+                                                F* __restrict__ f0) {
   ROW_SETUP
   const double alpha = S[slot_alpha];
   double a0 = 0.0, mx = 0.0;
@@ -354,6 +341,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, 
     dp[rb + i] += alpha * d[o];
     double rn = r[o] - alpha * Ad[o];
     r[o] = rn;
+    if (f0) St<F>::st(f0, o, (typename St<F>::C)rn);
     a0 += rn * rn;
     double ar = fabs(rn);
     if (ar > mx || ar != ar) mx = ar;
@@ -367,14 +355,15 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, 
 }
 
 // part0 <- sum r*z; part1 <- sum z*Ad (flexible only)
-__global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ r, const double* __restrict__ z,
+template <typename Z>
+__global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ r, const Z* __restrict__ z,
                                               const double* __restrict__ Ad, int flexible,
                                               double* __restrict__ part0, double* __restrict__ part1) {
   ROW_SETUP
   double a0 = 0.0, a1 = 0.0;
   for (int i = 1 + lane; i <= g.n[0]; i += 64) {
     const size_t o = rb + spos(i, g.n[0]);
-    double zz = z[o];
+    double zz = (double)St<Z>::ld(z, o);
     a0 += r[o] * zz;
     if (flexible) a1 += zz * Ad[o];
   }
@@ -387,11 +376,19 @@ __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ 
 }
 
 // d = z + beta d (k_update_dir, G_PCGSolver.cu:276-288)
-__global__ __launch_bounds__(256) void k_xpay(Geo g, const double* __restrict__ z, double* __restrict__ d,
+template <typename Z>
+__global__ __launch_bounds__(256) void k_xpay(Geo g, const Z* __restrict__ z, double* __restrict__ d,
                                               const double* __restrict__ S, int slot_beta) {
   ROW_SETUP
   const double beta = S[slot_beta];
-  for (int p = 1 + lane; p <= g.n[0]; p += 64) d[rb + p] = z[rb + p] + beta * d[rb + p];
+  for (int p = 1 + lane; p <= g.n[0]; p += 64) d[rb + p] = (double)St<Z>::ld(z, rb + p) + beta * d[rb + p];
+}
+
+// f0 = (F) r on interior cells (fine-level rhs of the mixed V-cycle)
+template <typename F>
+__global__ __launch_bounds__(256) void k_cvt(Geo g, const double* __restrict__ r, F* __restrict__ f0) {
+  ROW_SETUP
+  for (int p = 1 + lane; p <= g.n[0]; p += 64) St<F>::st(f0, rb + p, (typename St<F>::C)r[rb + p]);
 }
 
 // --------------------------------------------------------------- finalize
@@ -404,12 +401,28 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs fa, double* __re
     const double* part = fa.part[m];
     const int kind = fa.kind[m];
     double acc = 0.0;
-    if (kind == 0) {
-      for (int r = t; r < fa.nrows; r += 1024) acc += part[r];
-    } else {
-      for (int r = t; r < fa.nrows; r += 1024) {
-        double v = part[r];
-        if (v > acc || v != v) acc = v;
+    // lane t visits rows t, t+1024, ... in order; loads are issued 16 at a
+    // time so their latencies overlap (the summation order is unchanged)
+    int r = t;
+    for (; r + 15 * 1024 < fa.nrows; r += 16 * 1024) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = part[r + u * 1024];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        if (kind == 0) {
+          acc += v[u];
+        } else if (v[u] > acc || v[u] != v[u]) {
+          acc = v[u];
+        }
+      }
+    }
+    for (; r < fa.nrows; r += 1024) {
+      double v = part[r];
+      if (kind == 0) {
+        acc += v;
+      } else if (v > acc || v != v) {
+        acc = v;
       }
     }
     lds[t] = acc;
@@ -506,221 +519,8 @@ __global__ __launch_bounds__(256) void k_cfl(Geo g, Vel V, double dt, double vis
   }
 }
 
-// ------------------------------------------------------------- multigrid
-// Level geometry with uniform coefficients beta0/h_l^2: the coarse face
-// coefficient is the mean of the four fine faces covering it, which keeps
-// beta0, and walls stay 0. All level arrays use the row-split layout.
-__device__ __forceinline__ size_t lrow(const Lv& L, int j, int k) {
-  return ((size_t)k * (size_t)L.py + (size_t)j) * (size_t)L.px;
-}
-__device__ __forceinline__ int LW(const Lv& L, int d, int i) { return L.per[d] ? wrapi(i, L.n[d]) : i; }
-
-// Red-black Gauss-Seidel half sweep of colour col ((i+j+k)&1 == col) on the
-// active half row: e = (f + S)/diag. zero_start: e is zero on entry, so the
-// half sweep is e = f/diag without reading neighbours.
-__global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, double* __restrict__ e, const double* __restrict__ f, int col,
-                                                 int zero_start) {
-  LROW_SETUP
-  const int nx = L.n[0], nE = nx >> 1;
-  const int par = (col + j + k) & 1;  // parity of the active i
-  const int cnt = par ? nx - nE : nE;
-  const int off = par ? nE : 0;
-  const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
-  const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
-  for (int h = 1 + lane; h <= cnt; h += 64) {
-    const int i = par ? 2 * h - 1 : 2 * h;
-    const int p = off + h;
-    Coef c = coef_uniform(L.n, L.per, L.inv_d2, L.beta0, i, j, k);
-    if (zero_start) {
-      e[rb + p] = f[rb + p] / c.diag;
-    } else {
-      double xp = e[rb + spos(LW(L, 0, i + 1), nx)], xm = e[rb + spos(LW(L, 0, i - 1), nx)];
-      double yp = e[ypr + p], ym = e[ymr + p], zp = e[zpr + p], zm = e[zmr + p];
-      double S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
-      e[rb + p] = (f[rb + p] + S) / c.diag;
-    }
-  }
-}
-
-// res = f - (diag e - S) on every cell of the level
-__global__ __launch_bounds__(256) void k_mg_residual(Lv L, const double* __restrict__ e, const double* __restrict__ f,
-                                                     double* __restrict__ res) {
-  LROW_SETUP
-  const int nx = L.n[0], nE = nx >> 1;
-  const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
-  const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
-  for (int p = 1 + lane; p <= nx; p += 64) {
-    const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
-    Coef c = coef_uniform(L.n, L.per, L.inv_d2, L.beta0, i, j, k);
-    double xp = e[rb + spos(LW(L, 0, i + 1), nx)], xm = e[rb + spos(LW(L, 0, i - 1), nx)];
-    double yp = e[ypr + p], ym = e[ymr + p], zp = e[zpr + p], zm = e[zmr + p];
-    double S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
-    res[rb + p] = f[rb + p] - (c.diag * e[rb + p] - S);
-  }
-}
-
-// 1-D restriction stencil of coarse index I (transpose of cell-centred linear
-// prolongation; at a wall the mirrored weight folds onto the boundary cell)
-__device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], double w[4]) {
-  int nf = 2 * nc;
-  if (per) {
-    ix[0] = wrapi(2 * I - 2, nf); w[0] = 0.25;
-    ix[1] = 2 * I - 1; w[1] = 0.75;
-    ix[2] = 2 * I; w[2] = 0.75;
-    ix[3] = wrapi(2 * I + 1, nf); w[3] = 0.25;
-    return 4;
-  }
-  int m = 0;
-  if (I > 1) { ix[m] = 2 * I - 2; w[m] = 0.25; m++; }
-  ix[m] = 2 * I - 1; w[m] = (I == 1) ? 1.0 : 0.75; m++;
-  ix[m] = 2 * I; w[m] = (I == nc) ? 1.0 : 0.75; m++;
-  if (I < nc) { ix[m] = 2 * I + 1; w[m] = 0.25; m++; }
-  return m;
-}
-
-// Restriction, separable: coarse f = (1/8) sum_z wz sum_y wy sum_x wx res
-// over the folded 4x4x4 fine neighbourhood. Pass 1 (k_mg_resid_rx) forms the
-// residual of one fine row in LDS and restricts it along x into T (coarse x,
-// fine y and z, natural order, pitch C.n[0]+3). Pass 2 (k_mg_ryz) restricts
-// T along y and z. T lives in the fine level's residual buffer.
-__global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv C, const double* __restrict__ e,
-                                                     const double* __restrict__ f, double* __restrict__ T) {
-  extern __shared__ double sm[];
-  const Lv& L = F;
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int row = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + w;
-  const bool valid = row < L.n[1] * L.n[2];
-  const int nx = L.n[0], nE = nx >> 1;
-  double* buf = sm + (size_t)w * (size_t)(nx + 2);
-  int j = 1, k = 1;
-  if (valid) {
-    j = row % L.n[1] + 1;
-    k = row / L.n[1] + 1;
-    const size_t rb = lrow(L, j, k);
-    const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
-    const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
-    for (int p = 1 + lane; p <= nx; p += 64) {
-      const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
-      Coef c = coef_uniform(L.n, L.per, L.inv_d2, L.beta0, i, j, k);
-      double xp = e[rb + spos(LW(L, 0, i + 1), nx)], xm = e[rb + spos(LW(L, 0, i - 1), nx)];
-      double yp = e[ypr + p], ym = e[ymr + p], zp = e[zpr + p], zm = e[zmr + p];
-      double S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
-      buf[i] = f[rb + p] - (c.diag * e[rb + p] - S);
-    }
-  }
-  __syncthreads();
-  if (!valid) return;
-  const size_t tb = ((size_t)k * (size_t)L.py + (size_t)j) * (size_t)(C.n[0] + 3);
-  for (int I = 1 + lane; I <= C.n[0]; I += 64) {
-    int ix[4];
-    double wx[4];
-    int mx = rweights(C.per[0], C.n[0], I, ix, wx);
-    double tx = 0.0;
-    for (int c = 0; c < mx; c++) tx += wx[c] * buf[ix[c]];
-    T[tb + I] = tx;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_mg_ryz(Lv F, Lv C, const double* __restrict__ T, double* __restrict__ fc) {
-  const int I = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int J = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int K = 1 + (int)blockIdx.z;
-  if (I > C.n[0] || J > C.n[1] || K > C.n[2]) return;
-  int iy[4], iz[4];
-  double wy[4], wz[4];
-  int my = rweights(C.per[1], C.n[1], J, iy, wy);
-  int mz = rweights(C.per[2], C.n[2], K, iz, wz);
-  const size_t pxT = (size_t)(C.n[0] + 3);
-  double tz = 0.0;
-  for (int a = 0; a < mz; a++) {
-    double ty = 0.0;
-    for (int b = 0; b < my; b++) ty += wy[b] * T[((size_t)iz[a] * (size_t)F.py + (size_t)iy[b]) * pxT + I];
-    tz += wz[a] * ty;
-  }
-  fc[lrow(C, J, K) + spos(I, C.n[0])] = 0.125 * tz;
-}
-
-__device__ __forceinline__ void pweights(int per, int nc, int i, int ix[2], double w[2]) {
-  int par = (i + 1) >> 1;
-  int nb = (i & 1) ? par - 1 : par + 1;
-  if (per) nb = wrapi(nb, nc);
-  else if (nb < 1 || nb > nc) nb = par;
-  ix[0] = par; w[0] = 0.75;
-  ix[1] = nb; w[1] = 0.25;
-}
-
-// e_f += sum over 8 coarse cells of (wz*wy*wx) e_c (cell-centred trilinear), on
-// the fine cells of colour col only: the other colour is overwritten by the
-// next half sweep without being read.
-__global__ __launch_bounds__(256) void k_mg_prolong(Lv C, Lv F, const double* __restrict__ ec, double* __restrict__ ef,
-                                                    int col) {
-  const Lv& L = F;
-  LROW_SETUP
-  const int nx = L.n[0], nE = nx >> 1;
-  // col >= 0: the active half row of that colour; col < 0: the whole row
-  const int par = (col + j + k) & 1;
-  const int cnt = col < 0 ? nx : (par ? nx - nE : nE);
-  const int off = (col < 0 || !par) ? 0 : nE;
-  int iy[2], iz[2];
-  double wy[2], wz[2];
-  pweights(C.per[1], C.n[1], j, iy, wy);
-  pweights(C.per[2], C.n[2], k, iz, wz);
-  size_t cr[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) cr[a][b] = lrow(C, iy[b], iz[a]);
-  for (int h = 1 + lane; h <= cnt; h += 64) {
-    const int pp = off + h;
-    const int i = pp <= nE ? 2 * pp : 2 * (pp - nE) - 1;
-    int ix[2];
-    double wx[2];
-    pweights(C.per[0], C.n[0], i, ix, wx);
-    const int c0 = spos(ix[0], C.n[0]), c1 = spos(ix[1], C.n[0]);
-    double acc = 0.0;
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++) {
-        double w0 = wz[a] * wy[b] * wx[0];
-        acc += w0 * ec[cr[a][b] + c0];
-        double w1 = wz[a] * wy[b] * wx[1];
-        acc += w1 * ec[cr[a][b] + c1];
-      }
-    ef[rb + pp] += acc;
-  }
-}
-
-// coarsest level: e = Ainv f with the dense pseudo-inverse (rows in natural
-// x-fastest order), sequential accumulation per row
-__global__ __launch_bounds__(256) void k_mg_coarse(Lv C, const double* __restrict__ ainv, const double* __restrict__ f,
-                                                   double* __restrict__ e) {
-  const int n = C.n[0] * C.n[1] * C.n[2];
-  const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (row >= n) return;
-  const double* a = ainv + (size_t)row * (size_t)n;
-  double acc = 0.0;
-  for (int col = 0; col < n; col++) {
-    int ci = col % C.n[0] + 1, cj = (col / C.n[0]) % C.n[1] + 1, ck = col / (C.n[0] * C.n[1]) + 1;
-    acc += a[col] * f[lrow(C, cj, ck) + spos(ci, C.n[0])];
-  }
-  int i = row % C.n[0] + 1, j = (row / C.n[0]) % C.n[1] + 1, k = row / (C.n[0] * C.n[1]) + 1;
-  e[lrow(C, j, k) + spos(i, C.n[0])] = acc;
-}
-
-// x = v on interior cells of a level (a fill kernel, safe under graph capture)
-__global__ __launch_bounds__(256) void k_fill(Lv L, double* __restrict__ x, double v) {
-  LROW_SETUP
-  for (int p = 1 + lane; p <= L.n[0]; p += 64) x[rb + p] = v;
-}
-
 // ---------------------------------------------------------------- launchers
 static inline dim3 rows_grid(const Geo& g) { return dim3((unsigned)((g.n[1] * g.n[2] + 3) / 4)); }
-static inline dim3 lrows_grid(const Lv& L) { return dim3((unsigned)((L.n[1] * L.n[2] + 3) / 4)); }
-static inline dim3 cell_grid(const Lv& L) {
-  return dim3((unsigned)((L.n[0] + 63) / 64), (unsigned)((L.n[1] + 3) / 4), (unsigned)L.n[2]);
-}
 static const dim3 ROWB(256), CELLB(64, 4, 1);
 
 void launch_predict(const Geo& g, const Vel& V, const double* p, double* us[3], double dt, hipStream_t st) {
@@ -755,32 +555,55 @@ void launch_init_r(const Geo& g, const double* rhs, double* r, double* dp, const
                    double* p0, double* p1, hipStream_t st) {
   hipLaunchKernelGGL(k_init_r, rows_grid(g), ROWB, 0, st, g, rhs, r, dp, S, slot_mean, p0, p1);
 }
-void launch_add_scalar(const Geo& g, double* x, double a, const double* S, int slot, hipStream_t st) {
-  hipLaunchKernelGGL(k_add_scalar, rows_grid(g), ROWB, 0, st, g, x, a, S, slot);
+template <typename Z>
+void launch_add_scalar(const Geo& g, Z* x, double a, const double* S, int slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_scalar<Z>, rows_grid(g), ROWB, 0, st, g, x, a, S, slot);
 }
-void launch_sum(const Geo& g, const double* x, double* p0, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum, rows_grid(g), ROWB, 0, st, g, x, p0);
+template <typename Z>
+void launch_sum(const Geo& g, const Z* x, double* p0, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum<Z>, rows_grid(g), ROWB, 0, st, g, x, p0);
 }
 void launch_jacobi(const Geo& g, const double* r, double* z, hipStream_t st) {
   hipLaunchKernelGGL(k_jacobi, rows_grid(g), ROWB, 0, st, g, r, z);
 }
-void launch_copy_dot(const Geo& g, const double* z, double* d, const double* r, double* p0, hipStream_t st) {
-  hipLaunchKernelGGL(k_copy_dot, rows_grid(g), ROWB, 0, st, g, z, d, r, p0);
+template <typename Z>
+void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0);
 }
 void launch_spmv_dot(const Geo& g, const double* d, double* Ad, double* p0, hipStream_t st) {
   hipLaunchKernelGGL(k_spmv_dot, rows_grid(g), ROWB, 0, st, g, d, Ad, p0);
 }
+template <typename F>
 void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,
-                   int slot_alpha, double* p0, double* p1, hipStream_t st) {
-  hipLaunchKernelGGL(k_update, rows_grid(g), ROWB, 0, st, g, dp, d, r, Ad, S, slot_alpha, p0, p1);
+                   int slot_alpha, double* p0, double* p1, F* f0, hipStream_t st) {
+  hipLaunchKernelGGL(k_update<F>, rows_grid(g), ROWB, 0, st, g, dp, d, r, Ad, S, slot_alpha, p0, p1, f0);
 }
-void launch_dot2(const Geo& g, const double* r, const double* z, const double* Ad, int flexible, double* p0,
+template <typename Z>
+void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, int flexible, double* p0,
                  double* p1, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot2, rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1);
+  hipLaunchKernelGGL(k_dot2<Z>, rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1);
 }
-void launch_xpay(const Geo& g, const double* z, double* d, const double* S, int slot_beta, hipStream_t st) {
-  hipLaunchKernelGGL(k_xpay, rows_grid(g), ROWB, 0, st, g, z, d, S, slot_beta);
+template <typename Z>
+void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st) {
+  hipLaunchKernelGGL(k_xpay<Z>, rows_grid(g), ROWB, 0, st, g, z, d, S, slot_beta);
 }
+template <typename F>
+void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st) {
+  hipLaunchKernelGGL(k_cvt<F>, rows_grid(g), ROWB, 0, st, g, r, f0);
+}
+#define SMAC_INST_Z(Z)                                                                                     \
+  template void launch_add_scalar<Z>(const Geo&, Z*, double, const double*, int, hipStream_t);            \
+  template void launch_sum<Z>(const Geo&, const Z*, double*, hipStream_t);                                \
+  template void launch_copy_dot<Z>(const Geo&, const Z*, double*, const double*, double*, hipStream_t);   \
+  template void launch_update<Z>(const Geo&, double*, const double*, double*, const double*, const double*, \
+                                 int, double*, double*, Z*, hipStream_t);                                 \
+  template void launch_dot2<Z>(const Geo&, const double*, const Z*, const double*, int, double*, double*,  \
+                               hipStream_t);                                                              \
+  template void launch_xpay<Z>(const Geo&, const Z*, double*, const double*, int, hipStream_t);           \
+  template void launch_cvt<Z>(const Geo&, const double*, Z*, hipStream_t);
+SMAC_INST_Z(double)
+SMAC_INST_Z(float)
+SMAC_INST_Z(bf16s)
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fa, S);
 }
@@ -804,28 +627,6 @@ void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st) {
 }
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, double* p0, double* p1, hipStream_t st) {
   hipLaunchKernelGGL(k_cfl, rows_grid(g), ROWB, 0, st, g, V, dt, visc, p0, p1);
-}
-void launch_mg_rbgs(const Lv& L, double* e, const double* f, int col, int zero_start, hipStream_t st) {
-  hipLaunchKernelGGL(k_mg_rbgs, lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start);
-}
-void launch_mg_residual(const Lv& L, const double* e, const double* f, double* res, hipStream_t st) {
-  hipLaunchKernelGGL(k_mg_residual, lrows_grid(L), ROWB, 0, st, L, e, f, res);
-}
-void launch_mg_restrict(const Lv& F, const Lv& C, const double* e, const double* f, double* T, double* fc,
-                        hipStream_t st) {
-  size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(double);
-  hipLaunchKernelGGL(k_mg_resid_rx, lrows_grid(F), ROWB, shm, st, F, C, e, f, T);
-  hipLaunchKernelGGL(k_mg_ryz, cell_grid(C), CELLB, 0, st, F, C, T, fc);
-}
-void launch_mg_prolong(const Lv& C, const Lv& F, const double* ec, double* ef, int col, hipStream_t st) {
-  hipLaunchKernelGGL(k_mg_prolong, lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col);
-}
-void launch_mg_coarse(const Lv& C, const double* ainv, const double* f, double* e, hipStream_t st) {
-  int n = C.n[0] * C.n[1] * C.n[2];
-  hipLaunchKernelGGL(k_mg_coarse, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C, ainv, f, e);
-}
-void launch_fill(const Lv& L, double* x, double v, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill, lrows_grid(L), ROWB, 0, st, L, x, v);
 }
 
 }  // namespace smac

@@ -1,0 +1,276 @@
+// smac_mg.hip -- geometric multigrid V-cycle kernels (RBGS smoother,
+// separable restriction, trilinear prolongation, dense coarsest solve).
+//
+// Templated on the storage type T of the level vectors: double (the FP64
+// form, bit-identical to the CPU restatement), float (MIXED_FP32) and bf16s
+// (MIXED_BF16, 16-bit storage with FP32 arithmetic). The coarsest solve always
+// accumulates in FP64. All level arrays use the row-split layout (even i, then
+// odd i, per row) so that a half sweep streams contiguous half rows.
+//
+// Coefficients are uniform per level: beta0/h_l^2 on interior faces, zero across
+// walls (the coarse face coefficient is the mean of the four fine faces covering
+// it, which keeps beta0 and keeps walls at zero).
+#include "smac_internal.hpp"
+
+namespace smac {
+
+__device__ __forceinline__ size_t lrow(const Lv& L, int j, int k) {
+  return ((size_t)k * (size_t)L.py + (size_t)j) * (size_t)L.px;
+}
+__device__ __forceinline__ int LW(const Lv& L, int d, int i) { return L.per[d] ? wrapi(i, L.n[d]) : i; }
+
+template <typename C>
+struct CoefC {
+  C a[6];
+  C diag;
+};
+template <typename C>
+__device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k) {
+  Coef c = coef_uniform(L.n, L.per, L.inv_d2, L.beta0, i, j, k);
+  CoefC<C> r;
+#pragma unroll
+  for (int m = 0; m < 6; m++) r.a[m] = (C)c.a[m];
+  r.diag = (C)c.diag;
+  return r;
+}
+
+// Red-black Gauss-Seidel half sweep of colour col ((i+j+k)&1 == col) on the
+// active half row: e = (f + S)/diag with S = sum of a_nb e_nb in the order
+// xp, xm, yp, ym, zp, zm. zero_start: e is zero on entry, so e = f/diag.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
+                                                 int zero_start) {
+  using C = typename St<T>::C;
+  LROW_SETUP
+  const int nx = L.n[0], nE = nx >> 1;
+  const int par = (col + j + k) & 1;  // parity of the active i
+  const int cnt = par ? nx - nE : nE;
+  const int off = par ? nE : 0;
+  const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
+  const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+  for (int h = 1 + lane; h <= cnt; h += 64) {
+    const int i = par ? 2 * h - 1 : 2 * h;
+    const int p = off + h;
+    CoefC<C> c = level_coef<C>(L, i, j, k);
+    if (zero_start) {
+      St<T>::st(e, rb + p, St<T>::ld(f, rb + p) / c.diag);
+    } else {
+      C xp = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx)), xm = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
+      C yp = St<T>::ld(e, ypr + p), ym = St<T>::ld(e, ymr + p);
+      C zp = St<T>::ld(e, zpr + p), zm = St<T>::ld(e, zmr + p);
+      C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
+      St<T>::st(e, rb + p, (St<T>::ld(f, rb + p) + S) / c.diag);
+    }
+  }
+}
+
+// 1-D restriction stencil of coarse index I (transpose of cell-centred linear
+// prolongation; at a wall the mirrored weight folds onto the boundary cell)
+__device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], double w[4]) {
+  int nf = 2 * nc;
+  if (per) {
+    ix[0] = wrapi(2 * I - 2, nf); w[0] = 0.25;
+    ix[1] = 2 * I - 1; w[1] = 0.75;
+    ix[2] = 2 * I; w[2] = 0.75;
+    ix[3] = wrapi(2 * I + 1, nf); w[3] = 0.25;
+    return 4;
+  }
+  int m = 0;
+  if (I > 1) { ix[m] = 2 * I - 2; w[m] = 0.25; m++; }
+  ix[m] = 2 * I - 1; w[m] = (I == 1) ? 1.0 : 0.75; m++;
+  ix[m] = 2 * I; w[m] = (I == nc) ? 1.0 : 0.75; m++;
+  if (I < nc) { ix[m] = 2 * I + 1; w[m] = 0.25; m++; }
+  return m;
+}
+
+// Restriction, separable: coarse f = (1/8) sum_z wz sum_y wy sum_x wx res over
+// the folded 4x4x4 fine neighbourhood. Pass 1 forms the residual
+// res = f - (diag e - S) of one fine row in LDS and restricts it along x into
+// tb (coarse x, fine y and z, natural order, pitch C.n[0]+3). Pass 2 restricts
+// tb along y and z.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __restrict__ e, const T* __restrict__ f,
+                                                     typename St<T>::C* __restrict__ tb) {
+  using C = typename St<T>::C;
+  extern __shared__ unsigned char sm_raw[];
+  C* sm = reinterpret_cast<C*>(sm_raw);
+  const Lv& L = F;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int row = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + w;
+  const bool valid = row < L.n[1] * L.n[2];
+  const int nx = L.n[0], nE = nx >> 1;
+  C* buf = sm + (size_t)w * (size_t)(nx + 2);
+  int j = 1, k = 1;
+  if (valid) {
+    j = row % L.n[1] + 1;
+    k = row / L.n[1] + 1;
+    const size_t rb = lrow(L, j, k);
+    const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
+    const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+    for (int p = 1 + lane; p <= nx; p += 64) {
+      const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
+      CoefC<C> c = level_coef<C>(L, i, j, k);
+      C xp = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx)), xm = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
+      C yp = St<T>::ld(e, ypr + p), ym = St<T>::ld(e, ymr + p);
+      C zp = St<T>::ld(e, zpr + p), zm = St<T>::ld(e, zmr + p);
+      C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
+      buf[i] = St<T>::ld(f, rb + p) - (c.diag * St<T>::ld(e, rb + p) - S);
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const size_t tr = ((size_t)k * (size_t)L.py + (size_t)j) * (size_t)(Cl.n[0] + 3);
+  for (int I = 1 + lane; I <= Cl.n[0]; I += 64) {
+    int ix[4];
+    double wx[4];
+    int mx = rweights(Cl.per[0], Cl.n[0], I, ix, wx);
+    C tx = 0;
+    for (int c = 0; c < mx; c++) tx += (C)wx[c] * buf[ix[c]];
+    tb[tr + I] = tx;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_ryz(Lv F, Lv Cl, const typename St<T>::C* __restrict__ tb,
+                                                T* __restrict__ fc) {
+  using C = typename St<T>::C;
+  const int I = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int J = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int K = 1 + (int)blockIdx.z;
+  if (I > Cl.n[0] || J > Cl.n[1] || K > Cl.n[2]) return;
+  int iy[4], iz[4];
+  double wy[4], wz[4];
+  int my = rweights(Cl.per[1], Cl.n[1], J, iy, wy);
+  int mz = rweights(Cl.per[2], Cl.n[2], K, iz, wz);
+  const size_t pxT = (size_t)(Cl.n[0] + 3);
+  C tz = 0;
+  for (int a = 0; a < mz; a++) {
+    C ty = 0;
+    for (int b = 0; b < my; b++) ty += (C)wy[b] * tb[((size_t)iz[a] * (size_t)F.py + (size_t)iy[b]) * pxT + I];
+    tz += (C)wz[a] * ty;
+  }
+  St<T>::st(fc, lrow(Cl, J, K) + spos(I, Cl.n[0]), (C)0.125 * tz);
+}
+
+__device__ __forceinline__ void pweights(int per, int nc, int i, int ix[2], double w[2]) {
+  int par = (i + 1) >> 1;
+  int nb = (i & 1) ? par - 1 : par + 1;
+  if (per) nb = wrapi(nb, nc);
+  else if (nb < 1 || nb > nc) nb = par;
+  ix[0] = par; w[0] = 0.75;
+  ix[1] = nb; w[1] = 0.25;
+}
+
+// e_f += sum over 8 coarse cells of (wz*wy*wx) e_c (cell-centred trilinear).
+// col >= 0: only the fine cells of that colour (the other colour is
+// recomputed by the next half sweep without being read); col < 0: all cells.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_prolong(Lv Cl, Lv F, const T* __restrict__ ec, T* __restrict__ ef,
+                                                    int col) {
+  using C = typename St<T>::C;
+  const Lv& L = F;
+  LROW_SETUP
+  const int nx = L.n[0], nE = nx >> 1;
+  const int par = (col + j + k) & 1;
+  const int cnt = col < 0 ? nx : (par ? nx - nE : nE);
+  const int off = (col < 0 || !par) ? 0 : nE;
+  int iy[2], iz[2];
+  double wy[2], wz[2];
+  pweights(Cl.per[1], Cl.n[1], j, iy, wy);
+  pweights(Cl.per[2], Cl.n[2], k, iz, wz);
+  size_t cr[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) cr[a][b] = lrow(Cl, iy[b], iz[a]);
+  for (int h = 1 + lane; h <= cnt; h += 64) {
+    const int pp = off + h;
+    const int i = pp <= nE ? 2 * pp : 2 * (pp - nE) - 1;
+    int ix[2];
+    double wx[2];
+    pweights(Cl.per[0], Cl.n[0], i, ix, wx);
+    const int c0 = spos(ix[0], Cl.n[0]), c1 = spos(ix[1], Cl.n[0]);
+    C acc = 0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        C w0 = (C)(wz[a] * wy[b] * wx[0]);
+        acc += w0 * St<T>::ld(ec, cr[a][b] + c0);
+        C w1 = (C)(wz[a] * wy[b] * wx[1]);
+        acc += w1 * St<T>::ld(ec, cr[a][b] + c1);
+      }
+    St<T>::st(ef, rb + pp, St<T>::ld(ef, rb + pp) + acc);
+  }
+}
+
+// coarsest level: e = Ainv f with the dense pseudo-inverse (rows in natural
+// x-fastest order), FP64 sequential accumulation per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_mg_coarse(Lv Cl, const double* __restrict__ ainv, const T* __restrict__ f,
+                                                   T* __restrict__ e) {
+  const int n = Cl.n[0] * Cl.n[1] * Cl.n[2];
+  const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (row >= n) return;
+  const double* a = ainv + (size_t)row * (size_t)n;
+  double acc = 0.0;
+  for (int col = 0; col < n; col++) {
+    int ci = col % Cl.n[0] + 1, cj = (col / Cl.n[0]) % Cl.n[1] + 1, ck = col / (Cl.n[0] * Cl.n[1]) + 1;
+    acc += a[col] * (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
+  }
+  int i = row % Cl.n[0] + 1, j = (row / Cl.n[0]) % Cl.n[1] + 1, k = row / (Cl.n[0] * Cl.n[1]) + 1;
+  St<T>::st(e, lrow(Cl, j, k) + spos(i, Cl.n[0]), (typename St<T>::C)acc);
+}
+
+// x = 0 on interior cells of a level (a fill kernel, safe under graph capture)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(Lv L, T* __restrict__ x) {
+  LROW_SETUP
+  for (int p = 1 + lane; p <= L.n[0]; p += 64) St<T>::st(x, rb + p, (typename St<T>::C)0);
+}
+
+// ---------------------------------------------------------------- launchers
+static inline dim3 lrows_grid(const Lv& L) { return dim3((unsigned)((L.n[1] * L.n[2] + 3) / 4)); }
+static inline dim3 cell_grid(const Lv& L) {
+  return dim3((unsigned)((L.n[0] + 63) / 64), (unsigned)((L.n[1] + 3) / 4), (unsigned)L.n[2]);
+}
+static const dim3 ROWB(256), CELLB(64, 4, 1);
+
+template <typename T>
+void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
+  hipLaunchKernelGGL(k_mg_rbgs<T>, lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start);
+}
+template <typename T>
+void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
+                        hipStream_t st) {
+  size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
+  hipLaunchKernelGGL(k_mg_resid_rx<T>, lrows_grid(F), ROWB, shm, st, F, C, e, f, tb);
+  hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
+}
+template <typename T>
+void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hipStream_t st) {
+  hipLaunchKernelGGL(k_mg_prolong<T>, lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col);
+}
+template <typename T>
+void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st) {
+  int n = C.n[0] * C.n[1] * C.n[2];
+  hipLaunchKernelGGL(k_mg_coarse<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C, ainv, f, e);
+}
+template <typename T>
+void launch_fill(const Lv& L, T* x, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill<T>, lrows_grid(L), ROWB, 0, st, L, x);
+}
+
+#define SMAC_INST_MG(T)                                                                                 \
+  template void launch_mg_rbgs<T>(const Lv&, T*, const T*, int, int, hipStream_t);                     \
+  template void launch_mg_restrict<T>(const Lv&, const Lv&, const T*, const T*, typename St<T>::C*, T*, \
+                                      hipStream_t);                                                    \
+  template void launch_mg_prolong<T>(const Lv&, const Lv&, const T*, T*, int, hipStream_t);            \
+  template void launch_mg_coarse<T>(const Lv&, const double*, const T*, T*, hipStream_t);              \
+  template void launch_fill<T>(const Lv&, T*, hipStream_t);
+SMAC_INST_MG(double)
+SMAC_INST_MG(float)
+SMAC_INST_MG(bf16s)
+
+}  // namespace smac

@@ -131,3 +131,82 @@ def test_c1_cavity_64_100_steps(hip, oracle_mod):
             print(f"C1 step {s} iters {sg.iters}", flush=True)
     errs = _compare(g, o, desc["n"])
     print("C1 errors", errs, "iters first/last", iters[0], iters[-1], flush=True)
+
+
+# ------------------------------------------------------------ mixed precision
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("kind", ["periodic", "cavity"])
+def test_mixed_precision_vs_fp64_oracle(hip, oracle_mod, precision, kind):
+    """Mixed forms (V-cycle in FP32, or BF16 storage with FP32 arithmetic,
+    inside a flexible FP64 PCG; FP64 residual, dots and coarsest solve) solve to
+    the same 1e-8 relative residual; their fields differ from the FP64 oracle
+    only at that tolerance. The error is printed and bounded."""
+    from mydemcfd_amd import cases
+
+    n = 32
+    if kind == "periodic":
+        desc = cases.periodic_desc(n)
+    else:
+        desc = cases.cavity_desc(n, re=100.0)
+    print(f"case: mixed {precision} {kind} {n}^3 x4", flush=True)
+    g, o = _pair(hip, oracle_mod, desc)
+    if kind == "periodic":
+        u, v, w = cases.tg_fields(n, pert="white")
+        _upload_both(g, o, {"u": u, "v": v, "w": w})
+        dt = cases.cfl_dt(o, 0.3)
+    else:
+        dt = 2e-3
+    for s in range(4):
+        sg = g.step(dt, precond="mg", precision=precision, flexible=1, tol=1e-8)
+        so = o.step(dt, precond="mg", tol=1e-8)
+        assert sg.converged and sg.rel_res < 1e-8, sg.as_dict()
+        assert sg.iters <= so.iters + 6, (sg.iters, so.iters)
+    errs = {f: rel_err(g.download(f), o.download(f)) for f in ("u", "v", "w")}
+    errs["p"] = rel_err_mod_mean(g.download("p"), o.download("p"), desc["n"])
+    print(f"mixed {precision} {kind}: error vs FP64 oracle {errs}, iters {sg.iters} vs {so.iters}", flush=True)
+    for f, e in errs.items():
+        assert e < 1e-6, (f, e)
+
+
+def test_gpu_failure_handling(hip):
+    from mydemcfd_amd import _abi, cases
+
+    print("case: failure handling", flush=True)
+    c = hip.SMACContext(**cases.cavity_desc(8))
+    st = c.step(1e-3, precond="jacobi", tol=0.0, max_iter=3)
+    assert st.converged == 0 and st.iters == 3
+    with pytest.raises(_abi.SmacError) as e:
+        c.step(1e-3, precond="mg", mg_levels=5)
+    assert e.value.status == _abi.SMAC_ERR_INVALID_GRID
+    with pytest.raises(_abi.SmacError) as e:
+        c.step(1e-3, precond="jacobi", precision="fp32")
+    assert e.value.status == _abi.SMAC_ERR_UNSUPPORTED
+    with pytest.raises(_abi.SmacError) as e:
+        hip.SMACContext(**cases.periodic_desc(3))
+    assert e.value.status == _abi.SMAC_ERR_INVALID_GRID
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "gmg"])
+def test_cpp_facade_driver(hip, oracle_mod, tmp_path, precond):
+    """A C++ driver shaped like CFD/main.cu:225-317 (tests/cpp/cavity_facade.cpp),
+    built against include/smac_facade.hpp, runs the cavity through the C-ABI;
+    its fields equal the oracle's."""
+    import os
+    import subprocess
+
+    from mydemcfd_amd import build, cases
+
+    if not os.path.exists(build.DRIVER):
+        pytest.skip("C++ facade driver not built")
+    print(f"case: C++ facade cavity 16^3 x5 {precond}", flush=True)
+    args = [build.DRIVER, "16", "5", str(tmp_path)] + (["gmg"] if precond == "gmg" else [])
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    o = oracle_mod.OracleContext(**cases.cavity_desc(16, re=100.0))
+    for _ in range(5):
+        o.step(1e-3, precond="mg" if precond == "gmg" else "jacobi", tol=1e-8)
+    for f in ("u", "v", "w", "p"):
+        a = np.fromfile(os.path.join(tmp_path, f + ".f64"), dtype=np.float64).reshape(o.shape(f))
+        b = o.download(f)
+        err = rel_err_mod_mean(a, b, (16, 16, 16)) if f == "p" else rel_err(a, b)
+        assert err <= TOL, (f, err)

@@ -1,0 +1,351 @@
+"""Pins of the CPU FP64 oracle (no GPU).
+
+The reference holds no tests, golden vectors or fixtures for its CFD path, so
+the oracle is pinned by what the mathematics fixes: closed forms (discrete
+eigenfunctions, Taylor-Green energy decay and initial pressure), invariants
+(null space, symmetry, exact discrete divergence), special cases that reduce to
+a one-line formula (uniform flow, shear flow, linear pressure), an independent
+numpy MUSCL-minmod restatement and an independent scipy sparse solve
+(tests/golden/). Each check is chosen so that a dropped term, a sign error or
+an off-by-one index fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+from mydemcfd_amd import _abi, cases
+
+SHAPES = {"u": 0, "v": 1, "w": 2}
+
+
+def interior(a, n):
+    return a[1:n[2] + 1, 1:n[1] + 1, 1:n[0] + 1]
+
+
+def face_interior(a, n, axis):
+    """(z, y, x) interior faces 1..N of a periodic face array."""
+    return a[1:n[2] + 1, 1:n[1] + 1, 1:n[0] + 1]
+
+
+# ------------------------------------------------------------------ predictor
+def test_predictor_zero_state(oracle_mod):
+    for desc in (cases.cavity_desc(8, lid=0.0), cases.periodic_desc(8)):
+        c = oracle_mod.OracleContext(**desc)
+        c.predictor(1e-3)
+        for f in ("ustar", "vstar", "wstar"):
+            assert np.max(np.abs(c.download(f))) == 0.0
+
+
+def test_predictor_uniform_flow_periodic(oracle_mod):
+    """Uniform velocity, zero pressure: advection and stress vanish, u* = u."""
+    n = 8
+    c = oracle_mod.OracleContext(**cases.periodic_desc(n, re=10.0))
+    vals = {"u": 0.7, "v": -0.3, "w": 0.2}
+    for f, v in vals.items():
+        c.upload(f, np.full(c.shape(f), v))
+    c.predictor(0.01)
+    for f, v in vals.items():
+        us = c.download(f + "star")
+        assert np.max(np.abs(interior(us, (n, n, n)) - v)) < 1e-15
+
+
+def test_predictor_shear_flow_periodic(oracle_mod):
+    """u = f(y), v = w = 0: advection vanishes and the stress reduces to
+    nu * second difference in y, so u* = u + dt*nu*D2y(u), v* = w* = 0."""
+    n = 16
+    nu = 0.05
+    desc = cases.periodic_desc(n, re=1.0 / nu)
+    c = oracle_mod.OracleContext(**desc)
+    h = 2 * math.pi / n
+    yc = (np.arange(n) + 0.5) * h
+    prof = np.sin(yc) + 0.3 * np.cos(3 * yc)
+    u0 = np.broadcast_to(prof[None, :, None], (n, n, n)).copy()
+    c.upload("u", cases._embed_periodic(u0, _abi.FIELD_U))
+    dt = 1e-2
+    c.predictor(dt)
+    us = interior(c.download("ustar"), (n, n, n))
+    d2 = (np.roll(prof, -1) - 2 * prof + np.roll(prof, 1)) / h ** 2
+    expect = u0 + dt * nu * d2[None, :, None]
+    assert np.max(np.abs(us - expect)) < 1e-14
+    assert np.max(np.abs(c.download("vstar"))) == 0.0
+    assert np.max(np.abs(c.download("wstar"))) == 0.0
+
+
+def test_predictor_linear_pressure_walls(oracle_mod):
+    """At rest with p = a*x + b*z in a closed box: u* = -dt*beta*a on the
+    interior x-faces, w* = -dt*beta*b on interior z-faces, wall faces 0."""
+    n = (8, 6, 5)
+    desc = cases.cavity_desc(8)
+    desc["n"] = n
+    desc["bc_table"] = ((_abi.BC_NOSLIP, 0, 0, 0),) * 3
+    rho = 2.0
+    desc["rho"] = (rho, rho)
+    c = oracle_mod.OracleContext(**desc)
+    sh = c.shape("p")
+    h = np.array([1.0 / n[0], 1.0 / n[1], 1.0 / n[2]])
+    xc = (np.arange(sh[2]) - 0.5) * h[0]
+    zc = (np.arange(sh[0]) - 0.5) * h[2]
+    a, b = 3.0, -1.5
+    p = a * xc[None, None, :] + b * zc[:, None, None] + 0.0 * np.zeros(sh)
+    c.upload("p", p)
+    dt = 1e-3
+    c.predictor(dt)
+    us = c.download("ustar")
+    ws = c.download("wstar")
+    inner_u = us[1:n[2] + 1, 1:n[1] + 1, 2:n[0] + 1]
+    inner_w = ws[2:n[2] + 1, 1:n[1] + 1, 1:n[0] + 1]
+    assert np.max(np.abs(inner_u - (-dt * a / rho))) < 1e-13
+    assert np.max(np.abs(inner_w - (-dt * b / rho))) < 1e-13
+    assert np.max(np.abs(us[1:n[2] + 1, 1:n[1] + 1, [1, n[0] + 1]])) == 0.0
+    assert np.max(np.abs(c.download("vstar"))) < 1e-18
+
+
+def _muscl_minmod_flux_1d(u, dx):
+    """Independent numpy restatement of the x-momentum convection for a
+    1-D periodic field u(x) (face values), reference G_SMACSolver.cu:186-263:
+    face flux at cell i (between faces i and i+1): m = (u_i + u_{i+1})/2,
+    upwind face up = i if m > 0 else i+1, value u_up + sgn(m)/2*minmod(
+    u_{up+1}-u_up, u_up-u_{up-1}); returns -(M_{i} - M_{i-1})/dx per face."""
+    n = u.size
+    up1 = np.roll(u, -1)
+    m = 0.5 * (u + up1)  # flux at the cell right of face i
+    dp = np.roll(u, -1) - u
+    dm = u - np.roll(u, 1)
+    prod = dp * dm
+    mm_face = np.where(prod > 0, np.where(np.abs(dp) < np.abs(dm), dp, dm), 0.0)  # slope at face i
+    sgn = np.where(m < 0, -1.0, 1.0)
+    val = np.where(m > 0, u + sgn * 0.5 * mm_face, up1 + sgn * 0.5 * np.roll(mm_face, -1))
+    M = m * val  # flux through cell i (right of face i)
+    return -(M - np.roll(M, 1)) / dx
+
+
+def test_predictor_muscl_minmod_1d(oracle_mod):
+    """u = u(x) only, inviscid: u* = u + dt*(-d(uu)/dx) with the MUSCL-minmod
+    flux, compared with the independent numpy restatement above."""
+    n = 32
+    c = oracle_mod.OracleContext(**cases.periodic_desc(n, re=1e300))
+    c.set_fluid(rho=(1.0, 1.0), mu=(0.0, 0.0))
+    rng = np.random.default_rng(7)
+    prof = np.sin(2 * math.pi * np.arange(n) / n) + 0.4 * rng.standard_normal(n)
+    u0 = np.broadcast_to(prof[None, None, :], (n, n, n)).copy()
+    c.upload("u", cases._embed_periodic(u0, _abi.FIELD_U))
+    dt = 1e-3
+    c.predictor(dt)
+    us = interior(c.download("ustar"), (n, n, n))[3, 5, :]
+    dx = 2 * math.pi / n
+    expect = prof + dt * _muscl_minmod_flux_1d(prof, dx)
+    assert np.max(np.abs(us - expect)) < 1e-14
+
+
+# ---------------------------------------------------------- Poisson operator
+def test_operator_eigenfunction_periodic(oracle_mod):
+    """A cos x cos y cos z = lambda cos x cos y cos z with
+    lambda = sum_d (4/h^2) sin^2(h/2) (beta = 1/rho = 1)."""
+    n = 16
+    c = oracle_mod.OracleContext(**cases.periodic_desc(n))
+    h = 2 * math.pi / n
+    xc = (np.arange(n + 2) - 0.5) * h
+    p = np.cos(xc)[None, None, :] * np.cos(xc)[None, :, None] * np.cos(xc)[:, None, None]
+    Ap = c.apply_A(p)
+    lam = 3 * (4 / h ** 2) * math.sin(h / 2) ** 2
+    assert np.max(np.abs(interior(Ap, (n, n, n)) - lam * interior(p, (n, n, n)))) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["periodic", "walls"])
+def test_operator_nullspace_and_symmetry(oracle_mod, kind):
+    if kind == "periodic":
+        desc = cases.periodic_desc((8, 6, 10))
+    else:
+        desc = cases.cavity_desc(8)
+        desc["n"] = (7, 6, 5)
+    c = oracle_mod.OracleContext(**desc)
+    n = desc["n"]
+    sh = c.shape("p")
+    one = np.zeros(sh)
+    one[1:n[2] + 1, 1:n[1] + 1, 1:n[0] + 1] = 1.0
+    assert np.max(np.abs(interior(c.apply_A(one), n))) < 1e-10
+    rng = np.random.default_rng(3)
+    x = np.zeros(sh)
+    y = np.zeros(sh)
+    interior(x, n)[...] = rng.standard_normal(interior(x, n).shape)
+    interior(y, n)[...] = rng.standard_normal(interior(y, n).shape)
+    Ax, Ay = c.apply_A(x), c.apply_A(y)
+    l, r = np.sum(interior(Ax, n) * interior(y, n)), np.sum(interior(x, n) * interior(Ay, n))
+    assert abs(l - r) <= 1e-12 * max(abs(l), 1.0)
+    assert np.sum(interior(Ax, n) * interior(x, n)) > 0
+
+
+@pytest.mark.parametrize("kind", ["periodic", "walls"])
+def test_vcycle_symmetric_positive(oracle_mod, kind):
+    """The RBGS V(2,2) preconditioner is a symmetric positive operator on the
+    mean-free subspace (pre = reverse of post, R = P^T/8, symmetric coarse solve)."""
+    if kind == "periodic":
+        desc = cases.periodic_desc(16)
+    else:
+        desc = cases.cavity_desc(16)
+    c = oracle_mod.OracleContext(**desc)
+    n = desc["n"]
+    rng = np.random.default_rng(11)
+    sh = c.shape("p")
+    xs = []
+    for _ in range(2):
+        v = np.zeros(sh)
+        w = rng.standard_normal((n[2], n[1], n[0]))
+        interior(v, n)[...] = w - w.mean()
+        xs.append(v)
+    Mx = interior(c.apply_mg(xs[0], precond="mg"), n)
+    My = interior(c.apply_mg(xs[1], precond="mg"), n)
+    x, y = interior(xs[0], n), interior(xs[1], n)
+    l, r = np.sum(Mx * y), np.sum(x * My)
+    assert abs(l - r) <= 1e-10 * np.sqrt(np.sum(Mx * Mx) * np.sum(y * y))
+    assert np.sum(Mx * x) > 0 and np.sum(My * y) > 0
+
+
+def test_vcycle_reduces_error(oracle_mod):
+    """One V-cycle as a stationary solver reduces the error by a large factor."""
+    n = 32
+    c = oracle_mod.OracleContext(**cases.periodic_desc(n))
+    rng = np.random.default_rng(5)
+    sh = c.shape("p")
+    e = np.zeros(sh)
+    w = rng.standard_normal((n, n, n))
+    interior(e, (n,) * 3)[...] = w - w.mean()
+    b = c.apply_A(e)
+    z = c.apply_mg(b, precond="mg")
+    err0 = interior(e, (n,) * 3)
+    err1 = err0 - (interior(z, (n,) * 3) - interior(z, (n,) * 3).mean())
+    Ae1 = interior(c.apply_A(np.pad(err1, 1)), (n,) * 3)
+    Ae0 = interior(b, (n,) * 3)
+    factor = math.sqrt(np.sum(err1 * Ae1) / np.sum(err0 * Ae0))  # energy norm
+    assert factor < 0.2, factor
+
+
+def test_canonical_sum_matches_exact_sum(oracle_mod):
+    """The canonical reduction is an ordinary sum: exact on integer data."""
+    c = oracle_mod.OracleContext(**cases.periodic_desc((70, 9, 5)))
+    rng = np.random.default_rng(1)
+    x = np.zeros(c.shape("p"))
+    interior(x, (70, 9, 5))[...] = rng.integers(-1000, 1000, size=(5, 9, 70)).astype(float)
+    assert c.csum(x) == float(np.sum(interior(x, (70, 9, 5))))
+
+
+# --------------------------------------------------------- solve / project
+@pytest.mark.parametrize("fixture", ["poisson_periodic_8.npz", "poisson_walls_8x6x4.npz"])
+def test_pressure_solve_matches_scipy(oracle_mod, fixture):
+    """Pressure solve vs an independent scipy sparse direct solve of the same
+    discrete Poisson problem (tests/golden/make_golden.py)."""
+    import os
+
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", fixture))
+    n = tuple(int(v) for v in z["n"])
+    periodic = tuple(int(v) for v in z["periodic"])
+    size = tuple(float(v) for v in z["size"])
+    desc = cases.periodic_desc(n) if all(periodic) else cases.cavity_desc(8)
+    desc.update(n=n, size=size, periodic=periodic)
+    desc["bc_table"] = ((_abi.BC_NOSLIP, 0, 0, 0),) * 3
+    for precond in ("jacobi", "mg"):
+        c = oracle_mod.OracleContext(**desc)
+        c.upload("ustar", z["ustar"])
+        c.upload("vstar", z["vstar"])
+        c.upload("wstar", z["wstar"])
+        st = c.pressure_solve(float(z["dt"]), precond=precond, tol=1e-12, max_iter=5000)
+        assert st.converged
+        dp = interior(c.download("dp"), n)
+        ref = z["dp"]
+        assert rel_err(dp - dp.mean(), ref - ref.mean()) < 1e-8, precond
+
+
+def test_projection_divergence_free(oracle_mod):
+    """After a step the discrete divergence of u is at the solver tolerance."""
+    n = 16
+    c = oracle_mod.OracleContext(**cases.cavity_desc(n, re=100.0))
+    for _ in range(3):
+        st = c.step(1e-2, precond="mg", tol=1e-10)
+    u, v, w = c.download("u"), c.download("v"), c.download("w")
+    h = 1.0 / n
+    div = ((u[1:-1, 1:-1, 2:-1] - u[1:-1, 1:-1, 1:-2]) / h + (v[1:-1, 2:-1, 1:-1] - v[1:-1, 1:-2, 1:-1]) / h
+           + (w[2:-1, 1:-1, 1:-1] - w[1:-2, 1:-1, 1:-1]) / h)
+    umax = max(np.max(np.abs(u)), 1e-30)
+    assert np.max(np.abs(div)) * h / umax < 1e-8
+    assert st.converged
+
+
+def test_discrete_curl_is_divergence_free():
+    n = 12
+    u, v, w = cases.white_solenoidal(n, 3, 1.0)
+    h = 2 * math.pi / n
+    div = (np.roll(u, -1, 2) - u) / h + (np.roll(v, -1, 1) - v) / h + (np.roll(w, -1, 0) - w) / h
+    assert np.max(np.abs(div)) < 1e-12
+
+
+# ----------------------------------------------------- Taylor-Green closed forms
+def test_taylor_green_initial_energy():
+    u, v, w = cases.taylor_green(32)
+    e = 0.5 * (np.mean(u * u) + np.mean(v * v) + np.mean(w * w))
+    assert abs(e - 0.125) < 1e-12
+
+
+def _tg_first_step(oracle_mod, n, re=100.0, dt=1e-3):
+    c = oracle_mod.OracleContext(**cases.periodic_desc(n, re=re))
+    u, v, w = cases.tg_fields(n, pert=None)
+    for f, a in zip("uvw", (u, v, w)):
+        c.upload(f, a)
+
+    def energy():
+        uu, vv, ww = (interior(c.download(f), (n,) * 3) for f in "uvw")
+        return 0.5 * (np.mean(uu * uu) + np.mean(vv * vv) + np.mean(ww * ww))
+
+    e0 = energy()
+    c.step(dt, precond="mg", tol=1e-12)
+    rate = (energy() - e0) / dt
+    h = 2 * math.pi / n
+    xc = (np.arange(n) + 0.5) * h
+    X, Y, Z = xc[None, None, :], xc[None, :, None], xc[:, None, None]
+    pe = (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2) / 16.0
+    p = interior(c.download("p"), (n,) * 3)
+    return rate, rel_err(p - p.mean(), pe - pe.mean())
+
+
+def test_taylor_green_decay_rate_and_pressure(oracle_mod):
+    """Taylor-Green at t = 0: dE/dt = -nu <|omega|^2> = -(3/4) nu, and the first
+    step from rest pressure recovers p = (1/16)(cos 2x + cos 2y)(cos 2z + 2) up
+    to a constant. Both errors shrink at second order (16^3 -> 32^3)."""
+    nu = 1.0 / 100.0
+    r16, p16 = _tg_first_step(oracle_mod, 16)
+    r32, p32 = _tg_first_step(oracle_mod, 32)
+    e16 = abs(r16 / (-0.75 * nu) - 1.0)
+    e32 = abs(r32 / (-0.75 * nu) - 1.0)
+    assert e32 < 0.2 and e16 / e32 > 3.0, (e16, e32)
+    assert p32 < 0.025 and p16 / p32 > 2.8, (p16, p32)
+
+
+# ------------------------------------------------------------ failure handling
+def test_nonconvergence_is_reported(oracle_mod):
+    c = oracle_mod.OracleContext(**cases.cavity_desc(8))
+    st = c.step(1e-3, precond="jacobi", tol=0.0, max_iter=3)
+    assert st.converged == 0 and st.iters == 3
+
+
+def test_invalid_levels_rejected(oracle_mod):
+    c = oracle_mod.OracleContext(**cases.cavity_desc(8))
+    with pytest.raises(_abi.SmacError) as e:
+        c.step(1e-3, precond="mg", mg_levels=5)
+    assert e.value.status == _abi.SMAC_ERR_INVALID_GRID
+
+
+def test_invalid_grid_and_args(oracle_mod):
+    with pytest.raises(_abi.SmacError) as e:
+        oracle_mod.OracleContext(**cases.periodic_desc(3))
+    assert e.value.status == _abi.SMAC_ERR_INVALID_GRID
+    c = oracle_mod.OracleContext(**cases.cavity_desc(8))
+    with pytest.raises(ValueError):
+        c.upload("u", np.zeros((3, 3, 3)))
+    desc = cases.cavity_desc(8)
+    desc["bc_table"] = ((_abi.BC_OUTLET, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0))
+    desc["side_bcid"] = (0, 1, 1, 1, 1, 1)
+    with pytest.raises(_abi.SmacError) as e:
+        oracle_mod.OracleContext(**desc)
+    assert e.value.status == _abi.SMAC_ERR_UNSUPPORTED
