@@ -210,3 +210,22 @@ def test_cpp_facade_driver(hip, oracle_mod, tmp_path, precond):
         b = o.download(f)
         err = rel_err_mod_mean(a, b, (16, 16, 16)) if f == "p" else rel_err(a, b)
         assert err <= TOL, (f, err)
+
+
+def test_periodic_mg_wide_rows(hip, oracle_mod):
+    """Anisotropic periodic box whose rows take the 2-values-per-lane path of
+    the tiled smoother (nx = 256 -> 128 cells per half row)."""
+    from mydemcfd_amd import cases
+
+    print("case: TG-like periodic 256x16x8 x3 mg", flush=True)
+    n = (256, 16, 8)
+    desc = cases.periodic_desc(n)
+    g, o = _pair(hip, oracle_mod, desc)
+    rng = np.random.default_rng(21)
+    fields = {f: 0.1 * rng.standard_normal(g.shape(f)) for f in ("u", "v", "w")}
+    _upload_both(g, o, fields)
+    for s in range(3):
+        sg = g.step(2e-3, precond="mg", tol=1e-8)
+        so = o.step(2e-3, precond="mg", tol=1e-8)
+        assert sg.iters == so.iters, (s, sg.as_dict(), so.as_dict())
+    _compare(g, o, n)
