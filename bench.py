@@ -40,6 +40,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width
     "spmv_dot": lambda W, flex: 16.0,
+    "xpay_spmv": lambda W, flex: 24.0 + W,
     "update": lambda W, flex: 48.0,
     "dot2": lambda W, flex: 16.0 + (8.0 if flex else 0.0),
     "xpay": lambda W, flex: 24.0,

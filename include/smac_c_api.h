@@ -72,7 +72,11 @@ enum smac_field {
 /* PCG_TYPES (CFD/Enums.h:28-33) plus the north-star multigrid. */
 enum smac_precond {
     SMAC_PRECOND_JACOBI = 0,     /* STANDARD_PCG / PURENEUMANN_STANDARD_PCG  */
-    SMAC_PRECOND_MG = 1          /* RBGS V-cycle geometric multigrid          */
+    SMAC_PRECOND_MG = 1,         /* RBGS V-cycle geometric multigrid          */
+    SMAC_PRECOND_REF_GMG = 2     /* GMG_PCG / PURENEUMANN_GMG_PCG: the reference's
+                                    weighted-Jacobi V-cycle with mean restriction
+                                    and constant prolongation
+                                    (pressure_solver/G_GMGSolver.cu:10-147,925-998) */
 };
 
 enum smac_precision {
@@ -119,9 +123,15 @@ typedef struct smac_solver_opts {
     double tol;                  /* G_PressureSolverBase::tol_                */
     int32_t check_every;         /* convergence check cadence (ref: 10 Jacobi, 5 GMG) */
     int32_t max_iter;            /* ref: 2000                                 */
-    int32_t mg_levels;           /* 0 = automatic                             */
+    int32_t mg_levels;           /* MG: 0 = automatic. REF_GMG: the requested
+                                    num_levels of G_GMGSolver::initialize, 0 = 4
+                                    (CFD/config.json "gmg_levels")            */
     int32_t nu1, nu2;            /* pre/post RBGS sweeps per level            */
     int32_t flexible;            /* 1 = Polak-Ribiere beta (auto-on for mixed) */
+    int32_t ref_sweeps;          /* REF_GMG weighted-Jacobi sweeps before and
+                                    after the coarse correction, num_iter_fine_ (4) */
+    int32_t ref_coarse_sweeps;   /* REF_GMG sweeps on the coarsest level,
+                                    num_iter_coarse_ (30, G_GMGSolver.h:21-22) */
 } smac_solver_opts;
 
 typedef struct smac_solve_stats {

@@ -52,9 +52,16 @@ struct G_PressureSolverBase {
   double tol_ = 1e-8;
 };
 
-// CFD/pressure_solver/G_GMGSolver.h:18-40 (here: RBGS V-cycle levels)
+// CFD/pressure_solver/G_GMGSolver.h:18-40. By default the V-cycle is the
+// reference's own (weighted Jacobi, num_iter_fine_ / num_iter_coarse_ sweeps,
+// same iterates); rbgs_ = true selects the red-black Gauss-Seidel V(nu1_,nu2_)
+// multigrid instead (fewer PCG iterations, same solution to the tolerance).
 struct G_GMGSolver {
-  int num_levels_ = 0;  // 0 = automatic
+  int num_levels_ = 4;  // requested levels (config "gmg_levels"); rbgs_: 0 = automatic
+  int min_size_ = 6;    // fixed by the reference algorithm, informational
+  int num_iter_coarse_ = 30;
+  int num_iter_fine_ = 4;
+  bool rbgs_ = false;
   int nu1_ = 2, nu2_ = 2;
   void initialize(G_SMACSolver&, int num_levels) { num_levels_ = num_levels; }
   void free_levels() {}
@@ -140,22 +147,37 @@ struct G_SMACSolver {
   void update_boundary_ghost(const SMACSolver&) {}  // not needed by the next step for the built BCs
 };
 
-// CFD/pressure_solver/G_PCGSolver.h:17-37
+// CFD/pressure_solver/G_PCGSolver.h:17-37. The stopping rule is the
+// reference's: max|r|*dt < tol_, checked every 10 (Jacobi) or 5 (GMG)
+// iterations, at most 2000 (G_PCGSolver.cu:318-327,612-625). The pure-Neumann
+// treatment follows the boundary conditions, as SimulationConfig::is_pure_neumann;
+// the PURENEUMANN_* and plain variants select the same solver here.
 struct G_PCGSolver : G_PressureSolverBase {
   PCG_TYPES type_ = GMG_PCG;
   G_GMGSolver* gmg_ = nullptr;
   smac_solver_opts opts_{};
-  G_PCGSolver() { smac_default_opts(&opts_); }
+  G_PCGSolver() {
+    smac_default_opts(&opts_);
+    opts_.stop_rule = SMAC_STOP_REF_LINF_DT;
+    opts_.max_iter = 2000;
+  }
   void set_gmg(G_GMGSolver& gmg) { gmg_ = &gmg; }
   void set_solver(PCG_TYPES t) { type_ = t; }
   void solve(G_SMACSolver& s) override {
     smac_solver_opts o = opts_;
     o.tol = tol_;
-    o.precond = (type_ == GMG_PCG || type_ == PURENEUMANN_GMG_PCG) ? SMAC_PRECOND_MG : SMAC_PRECOND_JACOBI;
-    if (gmg_) {
+    const bool gmg = type_ == GMG_PCG || type_ == PURENEUMANN_GMG_PCG;
+    o.precond = SMAC_PRECOND_JACOBI;
+    o.check_every = 10;
+    if (gmg) {
+      if (!gmg_) throw smac_error(SMAC_ERR_INVALID_ARG, "GMG_PCG without set_gmg");
+      o.check_every = 5;
+      o.precond = gmg_->rbgs_ ? SMAC_PRECOND_MG : SMAC_PRECOND_REF_GMG;
       o.mg_levels = gmg_->num_levels_;
       o.nu1 = gmg_->nu1_;
       o.nu2 = gmg_->nu2_;
+      o.ref_sweeps = gmg_->num_iter_fine_;
+      o.ref_coarse_sweeps = gmg_->num_iter_coarse_;
     }
     check(smac_pressure_solve(s.ctx_, s.dt_, &o, &s.last_stats_), "smac_pressure_solve");
   }

@@ -29,7 +29,7 @@ FIELD_U, FIELD_V, FIELD_W, FIELD_USTAR, FIELD_VSTAR, FIELD_WSTAR, FIELD_P, FIELD
 FIELD_NAMES = {"u": FIELD_U, "v": FIELD_V, "w": FIELD_W, "ustar": FIELD_USTAR, "vstar": FIELD_VSTAR,
                "wstar": FIELD_WSTAR, "p": FIELD_P, "dp": FIELD_DP, "rhs": FIELD_RHS, "alpha": FIELD_ALPHA}
 
-PRECOND_JACOBI, PRECOND_MG = 0, 1
+PRECOND_JACOBI, PRECOND_MG, PRECOND_REF_GMG = 0, 1, 2
 PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16 = 0, 1, 2
 STOP_REF_LINF_DT, STOP_REL_L2 = 0, 1
 
@@ -54,7 +54,8 @@ class FluidDesc(C.Structure):
 class SolverOpts(C.Structure):
     _fields_ = [("precond", C.c_int32), ("precision", C.c_int32), ("stop_rule", C.c_int32),
                 ("tol", C.c_double), ("check_every", C.c_int32), ("max_iter", C.c_int32),
-                ("mg_levels", C.c_int32), ("nu1", C.c_int32), ("nu2", C.c_int32), ("flexible", C.c_int32)]
+                ("mg_levels", C.c_int32), ("nu1", C.c_int32), ("nu2", C.c_int32), ("flexible", C.c_int32),
+                ("ref_sweeps", C.c_int32), ("ref_coarse_sweeps", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -119,7 +120,7 @@ def default_opts(lib, **kw) -> SolverOpts:
     lib.smac_default_opts(C.byref(o))
     for k, v in kw.items():
         if k == "precond" and isinstance(v, str):
-            v = {"jacobi": PRECOND_JACOBI, "mg": PRECOND_MG}[v]
+            v = {"jacobi": PRECOND_JACOBI, "mg": PRECOND_MG, "ref_gmg": PRECOND_REF_GMG}[v]
         if k == "precision" and isinstance(v, str):
             v = {"fp64": PREC_FP64, "fp32": PREC_MIXED_FP32, "bf16": PREC_MIXED_BF16}[v]
         if k == "stop_rule" and isinstance(v, str):

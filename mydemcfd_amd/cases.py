@@ -24,7 +24,8 @@ def cavity_desc(n=64, re=100.0, lid=1.0):
     """Context kwargs for the lid-driven cavity: bc id 1 = no-slip at rest on
     five sides, bc id 2 = no-slip moving with u=lid on the y-max side."""
     nu = lid * 1.0 / re
-    return dict(n=(n, n, n), size=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), periodic=(0, 0, 0),
+    nn = tuple(n) if isinstance(n, (tuple, list)) else (n, n, n)
+    return dict(n=nn, size=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), periodic=(0, 0, 0),
                 side_bcid=(1, 1, 1, 2, 1, 1),
                 bc_table=((_abi.BC_NOSLIP, 0.0, 0.0, 0.0), (_abi.BC_NOSLIP, 0.0, 0.0, 0.0),
                           (_abi.BC_NOSLIP, lid, 0.0, 0.0)),

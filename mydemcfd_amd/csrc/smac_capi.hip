@@ -13,6 +13,7 @@
 
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/smac_c_api.h"
@@ -30,6 +31,15 @@ struct Level {
   void* f = nullptr;
   void* tb = nullptr;
   bool own_e = false, own_f = false, own_tb = false;
+};
+
+// One level of the reference's multigrid: q ping-pongs with qt (ghosts stay
+// zero), rhs from the restriction, face coefficients b[3].
+struct RgLevel {
+  RgLv lv;
+  size_t tot = 0;
+  double *q = nullptr, *qt = nullptr, *rhs = nullptr;
+  double* b[3] = {nullptr, nullptr, nullptr};
 };
 
 struct KStat {
@@ -53,7 +63,7 @@ struct smac_ctx {
   double* u[3] = {nullptr, nullptr, nullptr};
   double* us[3] = {nullptr, nullptr, nullptr};
   double *p = nullptr, *dp = nullptr, *rhs = nullptr, *alpha = nullptr;
-  double *r = nullptr, *z = nullptr, *dv = nullptr, *Ad = nullptr, *res0 = nullptr;
+  double *r = nullptr, *z = nullptr, *dv = nullptr, *dv2 = nullptr, *Ad = nullptr, *res0 = nullptr;
   double* part[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int nrows = 0;
   double* S = nullptr;
@@ -63,6 +73,10 @@ struct smac_ctx {
   int cinv_n = 0;
   int mg_L = 0;
   int mg_prec = -1;
+  std::vector<RgLevel> rg;
+  int rg_req = 0;
+  double rg_beta0 = 0.0;
+  double* rgt = nullptr;  // fine ping-pong partner of z in the reference V-cycle
   bool prof = false;
   hipEvent_t ev[8];
   bool ev_ok = false;
@@ -254,7 +268,7 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
   int rc = SMAC_OK;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) rc = SMAC_ERR_HIP;
   double** arrs[] = {&c->u[0], &c->u[1], &c->u[2], &c->us[0], &c->us[1], &c->us[2], &c->p, &c->dp, &c->rhs,
-                     &c->r, &c->z, &c->dv, &c->Ad, &c->res0};
+                     &c->r, &c->z, &c->dv, &c->dv2, &c->Ad, &c->res0};
   for (auto a : arrs)
     if (rc == SMAC_OK) rc = dalloc(a, c->tot);
   for (int m = 0; m < 5 && rc == SMAC_OK; m++) rc = dalloc(&c->part[m], (size_t)c->nrows);
@@ -287,12 +301,25 @@ static void mg_free(smac_ctx* c) {
   c->mg_L = 0;
 }
 
+static void rg_free(smac_ctx* c) {
+  for (auto& L : c->rg) {
+    double* arrs[] = {L.q, L.qt, L.rhs, L.b[0], L.b[1], L.b[2]};
+    for (auto a : arrs)
+      if (a) (void)hipFree(a);
+  }
+  c->rg.clear();
+  c->rg_req = 0;
+  if (c->rgt) (void)hipFree(c->rgt);
+  c->rgt = nullptr;
+}
+
 extern "C" void smac_destroy(smac_ctx* c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
   mg_free(c);
+  rg_free(c);
   double* arrs[] = {c->u[0], c->u[1], c->u[2], c->us[0], c->us[1], c->us[2], c->p, c->dp, c->rhs, c->alpha,
-                    c->r, c->z, c->dv, c->Ad, c->res0, c->S};
+                    c->r, c->z, c->dv, c->dv2, c->Ad, c->res0, c->S};
   for (auto a : arrs)
     if (a) (void)hipFree(a);
   for (int m = 0; m < 5; m++)
@@ -659,6 +686,111 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   }
 }
 
+// G_GMGSolver::initialize + create_coeffs (CFD/pressure_solver/G_GMGSolver.cu:10-147,913-922).
+// Returns the level count or a negative status: an odd size in a coarsened
+// direction (the reference aborts) or a single level (the reference then
+// prolongs through levels_[-1], G_GMGSolver.cu:971).
+static int rg_setup(smac_ctx* c, int req) {
+  if (req < 1 || req > 16) return SMAC_ERR_INVALID_ARG;
+  if (!c->rg.empty() && c->rg_req == req && c->rg_beta0 == c->g.beta0) return (int)c->rg.size();
+  rg_free(c);
+  std::vector<RgLv> lv;
+  int prev[3] = {c->n[0], c->n[1], c->n[2]};
+  double pinv[3] = {c->g.inv_d2[0], c->g.inv_d2[1], c->g.inv_d2[2]};
+  for (int l = 0; l < req; l++) {
+    RgLv V;
+    memset(&V, 0, sizeof(V));
+    int any = 0;
+    for (int d = 0; d < 3; d++) {
+      V.co[d] = !(prev[d] / 2 < 6);
+      any |= V.co[d];
+    }
+    if (l > 0 && !any) break;
+    for (int d = 0; d < 3; d++)
+      if (V.co[d] && (prev[d] & 1)) return SMAC_ERR_INVALID_GRID;
+    for (int d = 0; d < 3; d++) {
+      V.n[d] = V.co[d] ? prev[d] / 2 : prev[d];
+      V.inv_d2[d] = V.co[d] ? pinv[d] * 0.25 : pinv[d];
+      V.per[d] = c->per[d];
+      prev[d] = V.n[d];
+      pinv[d] = V.inv_d2[d];
+    }
+    V.px = V.n[0] + 3;
+    V.py = V.n[1] + 3;
+    lv.push_back(V);
+  }
+  if (lv.size() < 2) return SMAC_ERR_INVALID_GRID;
+  c->rg.resize(lv.size());
+  int rc = dalloc(&c->rgt, c->tot);
+  for (size_t l = 0; l < lv.size() && !rc; l++) {
+    RgLevel& V = c->rg[l];
+    V.lv = lv[l];
+    V.tot = (size_t)V.lv.px * V.lv.py * (size_t)(V.lv.n[2] + 3);
+    double** arrs[] = {&V.q, &V.qt, &V.rhs, &V.b[0], &V.b[1], &V.b[2]};
+    for (auto a : arrs)
+      if (!rc) rc = dalloc(a, V.tot);
+  }
+  if (rc) {
+    rg_free(c);
+    return rc;
+  }
+  // face coefficients: constant while the density is (uniform grids)
+  for (size_t l = 0; l < c->rg.size(); l++)
+    for (int d = 0; d < 3; d++)
+      launch_rg_beta(c->g, c->rg[l].lv, l ? c->rg[l - 1].lv : c->rg[l].lv, l ? c->rg[l - 1].b[d] : nullptr, d,
+                     c->rg[l].b[d], c->st);
+  HIPOK(hipGetLastError());
+  c->rg_req = req;
+  c->rg_beta0 = c->g.beta0;
+  return (int)c->rg.size();
+}
+
+// v_cycle_as_preconditioner (CFD/pressure_solver/G_GMGSolver.cu:925-998): z = M^{-1} r
+static void rg_vcycle(smac_ctx* c, const smac_solver_opts& o) {
+  const Geo& g = c->g;
+  hipStream_t s = c->st;
+  const int nf = o.ref_sweeps, nc = o.ref_coarse_sweeps, L = (int)c->rg.size();
+  double* x = c->z;
+  double* t = c->rgt;
+  for (auto& V : c->rg) KT(c, "rg_levels", launch_rg_zero(V.q, V.tot, s));
+  if (nf == 0) KT(c, "rg_fine", launch_rg_zero(x, c->tot, s));
+  for (int it = 0; it < nf; it++) {
+    KT(c, "rg_fine_jacobi", launch_rg_fine_jacobi(g, x, t, c->r, it == 0, s));
+    std::swap(x, t);
+  }
+  KT(c, "rg_fine_restrict", launch_rg_fine_res_restrict(g, x, c->r, c->rg[0].lv, c->rg[0].rhs, s));
+  for (int l = 0; l < L - 1; l++) {
+    RgLevel& V = c->rg[l];
+    for (int it = 0; it < nf; it++) {
+      KT(c, "rg_levels", launch_rg_jacobi(V.lv, V.b, V.q, V.qt, V.rhs, s));
+      std::swap(V.q, V.qt);
+    }
+    KT(c, "rg_levels", launch_rg_res_restrict(V.lv, V.b, V.q, V.rhs, c->rg[l + 1].lv, c->rg[l + 1].rhs, s));
+  }
+  {
+    RgLevel& V = c->rg[L - 1];
+    for (int it = 0; it < nc; it++) {
+      KT(c, "rg_levels", launch_rg_jacobi(V.lv, V.b, V.q, V.qt, V.rhs, s));
+      std::swap(V.q, V.qt);
+    }
+    KT(c, "rg_levels", launch_rg_prolong(V.lv, V.q, c->rg[L - 2].lv, c->rg[L - 2].q, s));
+  }
+  for (int l = L - 2; l >= 0; l--) {
+    RgLevel& V = c->rg[l];
+    for (int it = 0; it < nf; it++) {
+      KT(c, "rg_levels", launch_rg_jacobi(V.lv, V.b, V.q, V.qt, V.rhs, s));
+      std::swap(V.q, V.qt);
+    }
+    if (l != 0) KT(c, "rg_levels", launch_rg_prolong(V.lv, V.q, c->rg[l - 1].lv, c->rg[l - 1].q, s));
+    else KT(c, "rg_fine_prolong", launch_rg_prolong_fine(V.lv, V.q, g, x, s));
+  }
+  for (int it = 0; it < nf; it++) {
+    KT(c, "rg_fine_jacobi", launch_rg_fine_jacobi(g, x, t, c->r, 0, s));
+    std::swap(x, t);
+  }
+  // 2*nf sweeps: the result is back in z
+}
+
 static FinalizeArgs fin(const smac_ctx* c) {
   FinalizeArgs fa;
   memset(&fa, 0, sizeof(fa));
@@ -689,6 +821,8 @@ extern "C" void smac_default_opts(smac_solver_opts* o) {
   o->nu1 = 2;
   o->nu2 = 2;
   o->flexible = 0;
+  o->ref_sweeps = 4;
+  o->ref_coarse_sweeps = 30;
 }
 
 // The PCG of G_PCGSolver::solve_* (CFD/pressure_solver/G_PCGSolver.cu:307-922)
@@ -703,9 +837,14 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   int pure = pure_neumann(c);
   double inv_dt = 1. / dt;
   const bool mg = o.precond == SMAC_PRECOND_MG;
+  const bool refg = o.precond == SMAC_PRECOND_REF_GMG;  // FP64 only (checked by the caller)
   T* z = mg ? (T*)c->lev[0].e : (T*)c->z;
   T* f0 = (mg && (void*)c->lev[0].f != (void*)c->r) ? (T*)c->lev[0].f : nullptr;
   auto precond = [&]() {
+    if (refg) {
+      rg_vcycle(c, o);
+      return;
+    }
     if (!mg) {
       KT(c, "jacobi", launch_jacobi(c->g, c->r, c->z, c->st));
       return;
@@ -743,7 +882,13 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     launch_finalize(fa, c->S, s);
     launch_add_scalar<T>(g, z, -1. * size_inv, c->S, S_SUMZ, s);
   }
-  launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s);
+  // fused direction update + SpMV (2.5-D tiled) when the grid fits; the
+  // direction then lives in a ping-pong pair (dcur is the current direction)
+  const bool fused = xpay_spmv_fits(g);
+  double* dcur = c->dv;
+  double* dnxt = c->dv2;
+  if (fused) launch_dot2<T>(g, c->r, z, c->Ad, 0, P[0], P[1], s);
+  else launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s);
   {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_RZ);
@@ -759,7 +904,12 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   if (bb == 0.0) conv = 1;
   bool have_update = false;
   for (iter = 0; iter < o.max_iter && !conv; ++iter) {
-    KT(c, "spmv_dot", launch_spmv_dot(g, c->dv, c->Ad, P[0], s));
+    if (fused) {
+      KT(c, "xpay_spmv", launch_xpay_spmv<T>(g, z, dcur, dnxt, c->Ad, c->S, iter == 0 ? -1 : S_BETA, P[0], s));
+      std::swap(dcur, dnxt);
+    } else {
+      KT(c, "spmv_dot", launch_spmv_dot(g, c->dv, c->Ad, P[0], s));
+    }
     {
       FinalizeArgs fa = fin(c);
       fin_add(fa, P[0], 0, S_PAP);
@@ -769,8 +919,17 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       fa.sb = S_PAP;
       launch_finalize(fa, c->S, s);
     }
-    KT(c, "update", launch_update<T>(g, c->dp, c->dv, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
+    KT(c, "update", launch_update<T>(g, c->dp, dcur, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
     have_update = true;
+    if (refg && pure) {
+      // solve_pureneumann_gmgpcg removes the mean of r after every update
+      // (G_PCGSolver.cu:866); the check norms are taken after it
+      launch_sum<double>(g, c->r, P[2], s);
+      FinalizeArgs fa = fin(c);
+      fin_add(fa, P[2], 0, S_SUMR);
+      launch_finalize(fa, c->S, s);
+      KT(c, "add_scalar_norms", launch_add_scalar_norms(g, c->r, -1. * size_inv, c->S, S_SUMR, P[0], P[1], s));
+    }
     precond();
     KT(c, "dot2", launch_dot2<T>(g, c->r, z, c->Ad, o.flexible, P[2], P[3], s));
     bool check = (iter % o.check_every) == 0;
@@ -818,7 +977,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
         }
       }
     }
-    KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->S, S_BETA, s));
+    if (!fused) KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->S, S_BETA, s));
     launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }
   HIPOK(hipGetLastError());
@@ -827,7 +986,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     st->iters = iter;
     st->converged = conv;
     st->pure_neumann = pure;
-    st->mg_levels = mg ? L : 0;
+    st->mg_levels = (mg || refg) ? L : 0;
     st->bnorm = bnorm;
     if (have_update) {
       FinalizeArgs fa = fin(c);
@@ -854,9 +1013,12 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
   if (opts) o = *opts;
   else smac_default_opts(&o);
   if (o.check_every < 1) o.check_every = 1;
-  if (o.precond != SMAC_PRECOND_JACOBI && o.precond != SMAC_PRECOND_MG) return SMAC_ERR_INVALID_ARG;
+  if (o.precond != SMAC_PRECOND_JACOBI && o.precond != SMAC_PRECOND_MG && o.precond != SMAC_PRECOND_REF_GMG)
+    return SMAC_ERR_INVALID_ARG;
   if (o.precision < SMAC_PREC_FP64 || o.precision > SMAC_PREC_MIXED_BF16) return SMAC_ERR_INVALID_ARG;
-  if (o.precond == SMAC_PRECOND_JACOBI && o.precision != SMAC_PREC_FP64) return SMAC_ERR_UNSUPPORTED;
+  if (o.precond != SMAC_PRECOND_MG && o.precision != SMAC_PREC_FP64) return SMAC_ERR_UNSUPPORTED;
+  if (o.precond == SMAC_PRECOND_REF_GMG && (o.ref_sweeps < 0 || o.ref_coarse_sweeps < 0))
+    return SMAC_ERR_INVALID_ARG;
   if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
   if (c->n[0] > 4000) return SMAC_ERR_INVALID_GRID;  // restriction row buffer in LDS
   int L = 1;
@@ -865,6 +1027,9 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
     if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
     int rc = mg_setup(c, L, o.precision);
     if (rc) return rc;
+  } else if (o.precond == SMAC_PRECOND_REF_GMG) {
+    L = rg_setup(c, o.mg_levels > 0 ? o.mg_levels : 4);
+    if (L < 0) return L;
   }
   switch (o.precision) {
     case SMAC_PREC_MIXED_FP32: return solve_t<float>(c, dt, o, st, L);

@@ -16,6 +16,15 @@ struct Lv {
   double beta0;
 };
 
+// level of the reference's multigrid (natural layout, pitches n+3)
+struct RgLv {
+  int n[3];
+  int co[3];   // coarsened from the level above (or the grid) in this direction
+  int per[3];
+  int px, py;
+  double inv_d2[3];
+};
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
 // works on a contiguous slab of rows and the y/z stencil neighbours of a row
 // sit in the same L2.
@@ -78,6 +87,12 @@ template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st);
 template <typename F>
 void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st);
+bool xpay_spmv_fits(const Geo& g);
+// fused d_new = z + beta d_old (beta_slot < 0: d_new = z), Ad = A d_new, row
+// partials of d_new.Ad; returns false when the grid does not fit the tiling
+template <typename Z>
+bool launch_xpay_spmv(const Geo& g, const Z* z, const double* dold, double* dnew, double* Ad, const double* S,
+                      int beta_slot, double* p0, hipStream_t st);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
 void launch_correct(const Geo& g, double* const us[3], const double* dp, double* u[3], double dt, hipStream_t st);
@@ -96,5 +111,23 @@ template <typename T>
 void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st);
 template <typename T>
 void launch_fill(const Lv& L, T* x, hipStream_t st);
+
+// the reference's multigrid (smac_refgmg.hip); fb == nullptr: coefficients
+// of level 0 from the grid
+void launch_rg_beta(const Geo& g, const RgLv& C, const RgLv& F, const double* fb, int d, double* cb,
+                    hipStream_t st);
+void launch_rg_jacobi(const RgLv& L, double* const b[3], const double* q, double* qt, const double* rhs,
+                      hipStream_t st);
+void launch_rg_res_restrict(const RgLv& F, double* const b[3], const double* q, const double* rhs, const RgLv& C,
+                            double* crhs, hipStream_t st);
+void launch_rg_prolong(const RgLv& C, const double* cq, const RgLv& F, double* fq, hipStream_t st);
+void launch_rg_fine_jacobi(const Geo& g, const double* x, double* out, const double* r, int zero_start,
+                           hipStream_t st);
+void launch_rg_fine_res_restrict(const Geo& g, const double* x, const double* r, const RgLv& C, double* crhs,
+                                 hipStream_t st);
+void launch_rg_prolong_fine(const RgLv& C, const double* cq, const Geo& g, double* x, hipStream_t st);
+void launch_rg_zero(double* x, size_t n, hipStream_t st);
+void launch_add_scalar_norms(const Geo& g, double* r, double a, const double* S, int slot, double* p0, double* p1,
+                             hipStream_t st);
 
 }  // namespace smac

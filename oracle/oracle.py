@@ -41,6 +41,9 @@ def lib():
         l.smac_oracle_apply_mg.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), _abi._D, _abi._D]
         l.smac_oracle_csum.restype = C.c_double
         l.smac_oracle_csum.argtypes = [C.c_void_p, _abi._D]
+        l.smac_oracle_rg_beta.restype = C.c_int
+        l.smac_oracle_rg_beta.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), C.c_int, C.c_int,
+                                          C.POINTER(C.c_int * 3), _abi._D, C.c_size_t]
         l.smac_oracle_threads.restype = C.c_int
         l.smac_oracle_threads.argtypes = []
         _lib = l
@@ -67,6 +70,22 @@ class OracleContext(_abi.Context):
         self._check(self.lib.smac_oracle_apply_mg(self.h, C.byref(o), r.ctypes.data_as(_abi._D),
                                                   z.ctypes.data_as(_abi._D)), "apply_mg")
         return z
+
+    def rg_beta(self, level, axis, **kw):
+        """Face coefficients of one axis on a level of the reference GMG, as a
+        (n2+3, n1+3, n0+3) array (index = the reference's face/cell index)."""
+        o = self.opts(precond="ref_gmg", **kw)
+        dims = (C.c_int * 3)()
+        L = self.lib.smac_oracle_rg_beta(self.h, C.byref(o), level, axis, C.byref(dims), None, 0)
+        if L < 0:
+            self._check(L, "rg_beta")
+        n = [dims[0], dims[1], dims[2]]
+        out = np.zeros((n[2] + 3, n[1] + 3, n[0] + 3))
+        rc = self.lib.smac_oracle_rg_beta(self.h, C.byref(o), level, axis, C.byref(dims),
+                                          out.ctypes.data_as(_abi._D), out.size)
+        if rc < 0:
+            self._check(rc, "rg_beta")
+        return L, tuple(n), out
 
     def csum(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

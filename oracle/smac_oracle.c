@@ -77,6 +77,9 @@ struct smac_ctx {
     int cinv_n;
     int mg_ready;
     int props_ready;
+    /* the reference's GMG (REF_GMG) */
+    int nrg, rg_req;
+    struct rglevel* rg;
 };
 
 /* ---------------------------------------------------------------- indexing */
@@ -885,6 +888,288 @@ static void vcycle(const struct smac_ctx* c, int l, int nu1, int nu2) {
     for (int s = 0; s < nu2; s++) { rbgs(c, V, 1); rbgs(c, V, 0); }
 }
 
+/* ------------------------------------------------ the reference's GMG
+ * G_GMGSolver (CFD/pressure_solver/G_GMGSolver.cu, G_GMGSolver.h): weighted
+ * Jacobi (omega = 0.6) V-cycle, mean restriction over the 2x2x2 children,
+ * constant prolongation, coarse face coefficients = mean of the fine face
+ * coefficients of the 2x2x2 block (two faces along the face's own axis),
+ * coarse ghosts held at zero. Level 0 is already coarse (N/2). Directions
+ * whose size halved would drop below min_size_ = 6 stop coarsening. Periodic
+ * directions (absent in the reference) wrap face and cell indices. */
+#define RG_MIN_SIZE 6        /* G_GMGSolver.h:20 */
+#define RG_MAX_LEVELS 16     /* G_GMGSolver.h:2  */
+#define RG_EPS 1.E-12        /* CFD/hardCodedParameters.h:7 */
+
+typedef struct rglevel {
+    int n[3], p[3], coarse[3];
+    size_t tot;
+    double inv_d2[3];
+    double *q, *qt, *rhs, *res, *b[3];
+} rglevel;
+
+static inline size_t RIDX(const rglevel* L, int i, int j, int k) {
+    return ((size_t)k * (size_t)L->p[1] + (size_t)j) * (size_t)L->p[0] + (size_t)i;
+}
+
+static void rg_free(struct smac_ctx* c) {
+    if (c->rg) {
+        for (int l = 0; l < c->nrg; l++) {
+            rglevel* L = &c->rg[l];
+            free(L->q); free(L->qt); free(L->rhs); free(L->res);
+            free(L->b[0]); free(L->b[1]); free(L->b[2]);
+        }
+        free(c->rg);
+    }
+    c->rg = NULL;
+    c->nrg = 0;
+    c->rg_req = 0;
+}
+
+/* G_GMGSolver::initialize (G_GMGSolver.cu:10-147). Returns the level count,
+ * or a negative status: odd size in a coarsened direction (the reference
+ * aborts), or a single level (the reference then prolongs through
+ * levels_[-1], G_GMGSolver.cu:971). */
+static int rg_setup(struct smac_ctx* c, int req) {
+    if (req < 1 || req > RG_MAX_LEVELS) return SMAC_ERR_INVALID_ARG;
+    if (c->rg && c->rg_req == req) return c->nrg;
+    rg_free(c);
+    rglevel tmp[RG_MAX_LEVELS];
+    memset(tmp, 0, sizeof(tmp));
+    int prev[3] = {c->n[0], c->n[1], c->n[2]};
+    double pinv[3] = {c->inv_d2[0], c->inv_d2[1], c->inv_d2[2]};
+    int L = 0;
+    for (int l = 0; l < req; l++) {
+        rglevel* V = &tmp[l];
+        int any = 0;
+        for (int d = 0; d < 3; d++) {
+            V->coarse[d] = !(prev[d] / 2 < RG_MIN_SIZE);
+            any |= V->coarse[d];
+        }
+        if (l > 0 && !any) break;
+        for (int d = 0; d < 3; d++)
+            if (V->coarse[d] && (prev[d] & 1)) return SMAC_ERR_INVALID_GRID;
+        for (int d = 0; d < 3; d++) {
+            V->n[d] = V->coarse[d] ? prev[d] / 2 : prev[d];
+            V->inv_d2[d] = V->coarse[d] ? pinv[d] * 0.25 : pinv[d];
+            V->p[d] = V->n[d] + 3;
+            prev[d] = V->n[d];
+            pinv[d] = V->inv_d2[d];
+        }
+        L++;
+    }
+    if (L < 2) return SMAC_ERR_INVALID_GRID;
+    c->rg = (rglevel*)calloc((size_t)L, sizeof(rglevel));
+    for (int l = 0; l < L; l++) {
+        rglevel* V = &c->rg[l];
+        *V = tmp[l];
+        V->tot = (size_t)V->p[0] * V->p[1] * V->p[2];
+        V->q = (double*)calloc(V->tot, sizeof(double));
+        V->qt = (double*)calloc(V->tot, sizeof(double));
+        V->rhs = (double*)calloc(V->tot, sizeof(double));
+        V->res = (double*)calloc(V->tot, sizeof(double));
+        for (int d = 0; d < 3; d++) V->b[d] = (double*)calloc(V->tot, sizeof(double));
+    }
+    c->nrg = L;
+    c->rg_req = req;
+    return L;
+}
+
+/* face coefficient of axis d at face index (i,j,k) of the level below
+ * (lv < 0: the grid's f_b*). Indices past the last face of a wall direction
+ * are ghost faces and read 0 (F_GHOST, SMACSolver.cpp:358-360); periodic
+ * directions wrap. */
+static double rg_fine_beta(const struct smac_ctx* c, int lv, int d, int q[3]) {
+    int n[3];
+    for (int e = 0; e < 3; e++) n[e] = lv < 0 ? c->n[e] : c->rg[lv].n[e];
+    int w[3] = {q[0], q[1], q[2]};
+    if (c->per[d]) w[d] = wrap(w[d], n[d]);
+    else if (w[d] > n[d] + 1) return 0.0;
+    if (lv < 0) return c->beta[d][IDX(c, w[0], w[1], w[2])];
+    return c->rg[lv].b[d][RIDX(&c->rg[lv], w[0], w[1], w[2])];
+}
+
+/* k_create_level0_coeffs / k_create_level_coeffs (G_GMGSolver.cu:499-644) */
+static void rg_coeffs(struct smac_ctx* c) {
+    for (int l = 0; l < c->nrg; l++) {
+        rglevel* V = &c->rg[l];
+        for (int d = 0; d < 3; d++) {
+            int hi[3] = {V->n[0], V->n[1], V->n[2]};
+            hi[d] += 1;
+            for (int k = 1; k <= hi[2]; k++)
+                for (int j = 1; j <= hi[1]; j++)
+                    for (int i = 1; i <= hi[0]; i++) {
+                        int q[3] = {i, j, k}, B[3], R[3];
+                        for (int e = 0; e < 3; e++) {
+                            B[e] = V->coarse[e] ? 2 * q[e] - 1 : q[e];
+                            R[e] = V->coarse[e] ? 2 : 1;
+                        }
+                        double count = 0., tmp = 0.;
+                        for (int iz = 0; iz < R[2]; iz++)
+                            for (int iy = 0; iy < R[1]; iy++)
+                                for (int ix = 0; ix < R[0]; ix++) {
+                                    int f[3] = {B[0] + ix, B[1] + iy, B[2] + iz};
+                                    tmp += rg_fine_beta(c, l - 1, d, f);
+                                    count += 1.0;
+                                }
+                        V->b[d][RIDX(V, i, j, k)] = tmp / count;
+                    }
+        }
+    }
+}
+
+/* Level coefficients of k_create_level_A (G_GMGSolver.cu:646-685). */
+static inline void rg_coef(const rglevel* V, int i, int j, int k, double a6[6], double* diag, double* inv) {
+    a6[0] = V->b[0][RIDX(V, i + 1, j, k)] * V->inv_d2[0];
+    a6[1] = V->b[0][RIDX(V, i, j, k)] * V->inv_d2[0];
+    a6[2] = V->b[1][RIDX(V, i, j + 1, k)] * V->inv_d2[1];
+    a6[3] = V->b[1][RIDX(V, i, j, k)] * V->inv_d2[1];
+    a6[4] = V->b[2][RIDX(V, i, j, k + 1)] * V->inv_d2[2];
+    a6[5] = V->b[2][RIDX(V, i, j, k)] * V->inv_d2[2];
+    *diag = (a6[0] + a6[1] + a6[2] + a6[3] + a6[4] + a6[5]);
+    *inv = fabs(*diag) < RG_EPS ? 0. : 1. / *diag;
+}
+/* neighbour of cell (i,j,k) along d by s on a level: ghosts are zero on
+ * walls, periodic directions wrap */
+static inline double rg_nb(const struct smac_ctx* c, const rglevel* V, const double* x, int i, int j, int k, int d,
+                           int s) {
+    int q[3] = {i, j, k};
+    q[d] += s;
+    if (c->per[d]) q[d] = wrap(q[d], V->n[d]);
+    return x[RIDX(V, q[0], q[1], q[2])];
+}
+static inline double rg_nbsum(const struct smac_ctx* c, const rglevel* V, const double* x, int i, int j, int k,
+                              const double a6[6]) {
+    return a6[0] * rg_nb(c, V, x, i, j, k, 0, 1) + a6[1] * rg_nb(c, V, x, i, j, k, 0, -1) +
+           a6[2] * rg_nb(c, V, x, i, j, k, 1, 1) + a6[3] * rg_nb(c, V, x, i, j, k, 1, -1) +
+           a6[4] * rg_nb(c, V, x, i, j, k, 2, 1) + a6[5] * rg_nb(c, V, x, i, j, k, 2, -1);
+}
+
+/* k_jacobi_level_iteration (G_GMGSolver.cu:692-717) + the pointer swap */
+static void rg_jacobi(const struct smac_ctx* c, rglevel* V) {
+    const double omega = 0.6;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= V->n[2]; k++)
+        for (int j = 1; j <= V->n[1]; j++)
+            for (int i = 1; i <= V->n[0]; i++) {
+                double a6[6], diag, inv;
+                rg_coef(V, i, j, k, a6, &diag, &inv);
+                double tmp_p = rg_nbsum(c, V, V->q, i, j, k, a6);
+                size_t id = RIDX(V, i, j, k);
+                V->qt[id] = (1.0 - omega) * V->q[id] + omega * (tmp_p + V->rhs[id]) * inv;
+            }
+    double* t = V->q; V->q = V->qt; V->qt = t;
+}
+/* k_get_res_levels (G_GMGSolver.cu:152-188) */
+static void rg_residual(const struct smac_ctx* c, rglevel* V) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= V->n[2]; k++)
+        for (int j = 1; j <= V->n[1]; j++)
+            for (int i = 1; i <= V->n[0]; i++) {
+                double a6[6], diag, inv;
+                rg_coef(V, i, j, k, a6, &diag, &inv);
+                size_t id = RIDX(V, i, j, k);
+                double Ap = V->q[id] * diag - rg_nbsum(c, V, V->q, i, j, k, a6);
+                V->res[id] = V->rhs[id] - Ap;
+            }
+}
+/* k_restrict_grid_to_level0 / k_restrict_level_to_level (G_GMGSolver.cu:416-493):
+ * C->rhs = mean of the children of `res` (fine box of pitch fp) */
+static void rg_restrict(const rglevel* C, const double* res, const int fp[3], int fine_is_grid,
+                        const struct smac_ctx* c) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= C->n[2]; k++)
+        for (int j = 1; j <= C->n[1]; j++)
+            for (int i = 1; i <= C->n[0]; i++) {
+                int q[3] = {i, j, k}, B[3], R[3];
+                for (int e = 0; e < 3; e++) {
+                    B[e] = C->coarse[e] ? 2 * q[e] - 1 : q[e];
+                    R[e] = C->coarse[e] ? 2 : 1;
+                }
+                double count = 0., tmp = 0.;
+                for (int iz = 0; iz < R[2]; iz++)
+                    for (int iy = 0; iy < R[1]; iy++)
+                        for (int ix = 0; ix < R[0]; ix++) {
+                            size_t id = fine_is_grid ? IDX(c, B[0] + ix, B[1] + iy, B[2] + iz)
+                                                     : ((size_t)(B[2] + iz) * fp[1] + (B[1] + iy)) * fp[0] + (B[0] + ix);
+                            tmp += res[id];
+                            count += 1.0;
+                        }
+                C->rhs[RIDX(C, i, j, k)] = tmp / count;
+            }
+}
+/* k_propagate_level_to_level / k_propagate_level0_to_array (G_GMGSolver.cu:285-389) */
+static void rg_prolong(const rglevel* C, double* fq, const int fp[3], int fine_is_grid, const struct smac_ctx* c) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= C->n[2]; k++)
+        for (int j = 1; j <= C->n[1]; j++)
+            for (int i = 1; i <= C->n[0]; i++) {
+                int q[3] = {i, j, k}, B[3], R[3];
+                for (int e = 0; e < 3; e++) {
+                    B[e] = C->coarse[e] ? 2 * q[e] - 1 : q[e];
+                    R[e] = C->coarse[e] ? 2 : 1;
+                }
+                double prop = C->q[RIDX(C, i, j, k)];
+                for (int iz = 0; iz < R[2]; iz++)
+                    for (int iy = 0; iy < R[1]; iy++)
+                        for (int ix = 0; ix < R[0]; ix++) {
+                            size_t id = fine_is_grid ? IDX(c, B[0] + ix, B[1] + iy, B[2] + iz)
+                                                     : ((size_t)(B[2] + iz) * fp[1] + (B[1] + iy)) * fp[0] + (B[0] + ix);
+                            fq[id] += prop;
+                        }
+            }
+}
+/* k_jacobi_iteration_general (G_GMGSolver.cu:719-746) on the grid's operator:
+ * out = (1-omega) x + omega (sum a_nb x_nb + r) / Adiag */
+static void rg_fine_jacobi(const struct smac_ctx* c, const double* x, double* out, const double* r) {
+    const double omega = 0.6;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                size_t id = IDX(c, i, j, k);
+                double xp = x[IDX(c, W(c, 0, i + 1), j, k)], xm = x[IDX(c, W(c, 0, i - 1), j, k)];
+                double yp = x[IDX(c, i, W(c, 1, j + 1), k)], ym = x[IDX(c, i, W(c, 1, j - 1), k)];
+                double zp = x[IDX(c, i, j, W(c, 2, k + 1))], zm = x[IDX(c, i, j, W(c, 2, k - 1))];
+                double tmp_p = c->Aoff[0][id] * xp + c->Aoff[1][id] * xm + c->Aoff[2][id] * yp +
+                               c->Aoff[3][id] * ym + c->Aoff[4][id] * zp + c->Aoff[5][id] * zm;
+                out[id] = (1.0 - omega) * x[id] + omega * (tmp_p + r[id]) * (1. / c->Adiag[id]);
+            }
+}
+
+/* v_cycle_as_preconditioner (G_GMGSolver.cu:925-998): z = M^{-1} r */
+static void rg_vcycle(struct smac_ctx* c, const smac_solver_opts* o) {
+    int nf = o->ref_sweeps, nc = o->ref_coarse_sweeps, L = c->nrg;
+    double* buf = (double*)calloc(c->tot, sizeof(double));
+    double* t = buf;
+    double* x = c->z;
+    memset(x, 0, c->tot * sizeof(double));
+    for (int l = 0; l < L; l++) memset(c->rg[l].q, 0, c->rg[l].tot * sizeof(double));
+    for (int s = 0; s < nf; s++) { rg_fine_jacobi(c, x, t, c->r); double* u = x; x = t; t = u; }
+    /* k_get_res_general: residual of the grid's operator, into t */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) t[IDX(c, i, j, k)] = c->r[IDX(c, i, j, k)] - apply_A(c, x, i, j, k);
+    rg_restrict(&c->rg[0], t, c->p, 1, c);
+    for (int l = 0; l < L - 1; l++) {
+        rglevel* V = &c->rg[l];
+        for (int s = 0; s < nf; s++) rg_jacobi(c, V);
+        rg_residual(c, V);
+        rg_restrict(&c->rg[l + 1], V->res, V->p, 0, c);
+    }
+    for (int s = 0; s < nc; s++) rg_jacobi(c, &c->rg[L - 1]);
+    rg_prolong(&c->rg[L - 1], c->rg[L - 2].q, c->rg[L - 2].p, 0, c);
+    for (int l = L - 2; l >= 0; l--) {
+        rglevel* V = &c->rg[l];
+        for (int s = 0; s < nf; s++) rg_jacobi(c, V);
+        if (l != 0) rg_prolong(V, c->rg[l - 1].q, c->rg[l - 1].p, 0, c);
+        else rg_prolong(V, x, c->p, 1, c);
+    }
+    for (int s = 0; s < nf; s++) { rg_fine_jacobi(c, x, t, c->r); double* u = x; x = t; t = u; }
+    if (x != c->z) memcpy(c->z, x, c->tot * sizeof(double));
+    free(buf);
+}
+
 /* z = M^{-1} r */
 static void precond(struct smac_ctx* c, const smac_solver_opts* o) {
     int nx = c->n[0], ny = c->n[1], nz = c->n[2];
@@ -896,6 +1181,10 @@ static void precond(struct smac_ctx* c, const smac_solver_opts* o) {
                     size_t id = IDX(c, i, j, k);
                     c->z[id] = (1. / c->Adiag[id]) * c->r[id];
                 }
+        return;
+    }
+    if (o->precond == SMAC_PRECOND_REF_GMG) {
+        rg_vcycle(c, o);
         return;
     }
     level* V = &c->lev[0];
@@ -930,6 +1219,8 @@ void smac_default_opts(smac_solver_opts* o) {
     o->nu1 = 2;
     o->nu2 = 2;
     o->flexible = 0;
+    o->ref_sweeps = 4;
+    o->ref_coarse_sweeps = 30;
 }
 
 int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st) {
@@ -951,6 +1242,13 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
         L = o.mg_levels > 0 ? o.mg_levels : mg_auto_levels(c);
         if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
         mg_setup(c, L);
+    } else if (o.precond == SMAC_PRECOND_REF_GMG) {
+        if (o.ref_sweeps < 0 || o.ref_coarse_sweeps < 0) return SMAC_ERR_INVALID_ARG;
+        L = rg_setup(c, o.mg_levels > 0 ? o.mg_levels : 4);
+        if (L < 0) return L;
+        rg_coeffs(c); /* create_coeffs on every solve (G_PCGSolver.cu:805) */
+    } else if (o.precond != SMAC_PRECOND_JACOBI) {
+        return SMAC_ERR_INVALID_ARG;
     }
     /* dp = 0 on entry (correct_vof_velocity clears it), so A dp = 0 and
      * r = b - mean(b) (pure Neumann) or r = b. */
@@ -990,6 +1288,10 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
                     c->dp[id] += alpha * c->dvec[id];
                     c->r[id] -= alpha * c->Ad[id];
                 }
+        /* solve_pureneumann_gmgpcg removes the mean of r after every update
+         * (G_PCGSolver.cu:866) */
+        if (pure && o.precond == SMAC_PRECOND_REF_GMG)
+            add_scalar_interior(c, c->r, -1. * size_inv, csum(c, RED_SUM, c->r, NULL));
         precond(c, &o);
         double rz_new = csum(c, RED_DOT, c->r, c->z);
         double zAd = 0.0;
@@ -1018,7 +1320,7 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
         st->iters = iter;
         st->converged = conv;
         st->pure_neumann = pure;
-        st->mg_levels = o.precond == SMAC_PRECOND_MG ? L : 0;
+        st->mg_levels = o.precond != SMAC_PRECOND_JACOBI ? L : 0;
         rr = csum(c, RED_SQ, c->r, NULL);
         st->bnorm = bnorm;
         st->rel_res = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
@@ -1229,6 +1531,7 @@ int smac_create(const smac_grid_desc* g, const smac_fluid_desc* f, smac_ctx** ou
 void smac_destroy(smac_ctx* c) {
     if (!c) return;
     mg_free(c);
+    rg_free(c);
     double* arrs[] = {c->vel[0], c->vel[1], c->vel[2], c->vst[0], c->vst[1], c->vst[2], c->p_, c->dp, c->rhs,
                       c->alpha, c->rho, c->rho_old, c->inv_rho, c->mu, c->beta[0], c->beta[1], c->beta[2],
                       c->inv_rhof[0], c->inv_rhof[1], c->inv_rhof[2], c->muf[0], c->muf[1], c->muf[2], c->mf[0],
@@ -1293,9 +1596,15 @@ int smac_oracle_apply_mg(smac_ctx* c, const smac_solver_opts* o, const double* r
     if (!c || !o) return SMAC_ERR_INVALID_ARG;
     if (!c->props_ready) smac_update_properties(c, 1);
     build_A(c);
-    int L = o->mg_levels > 0 ? o->mg_levels : mg_auto_levels(c);
-    if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
-    mg_setup(c, L);
+    if (o->precond == SMAC_PRECOND_REF_GMG) {
+        int L = rg_setup(c, o->mg_levels > 0 ? o->mg_levels : 4);
+        if (L < 0) return L;
+        rg_coeffs(c);
+    } else {
+        int L = o->mg_levels > 0 ? o->mg_levels : mg_auto_levels(c);
+        if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
+        mg_setup(c, L);
+    }
     size_t cnt = smac_field_count(c, SMAC_FIELD_P);
     double* save = c->rhs;
     c->rhs = c->r;
@@ -1307,6 +1616,22 @@ int smac_oracle_apply_mg(smac_ctx* c, const smac_solver_opts* o, const double* r
     smac_download(c, SMAC_FIELD_RHS, z_ref, cnt);
     c->rhs = save;
     return SMAC_OK;
+}
+/* Face coefficients of axis `axis` on level `level` of the reference GMG:
+ * dims <- the level's cell counts; out (if cap is large enough) <- the
+ * (n0+3)(n1+3)(n2+3) box, x fastest. Returns the level count or a status. */
+int smac_oracle_rg_beta(smac_ctx* c, const smac_solver_opts* o, int level, int axis, int dims[3], double* out,
+                        size_t cap) {
+    if (!c || !o || axis < 0 || axis > 2) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    int L = rg_setup(c, o->mg_levels > 0 ? o->mg_levels : 4);
+    if (L < 0) return L;
+    if (level < 0 || level >= L) return SMAC_ERR_INVALID_ARG;
+    rg_coeffs(c);
+    const rglevel* V = &c->rg[level];
+    for (int d = 0; d < 3; d++) dims[d] = V->n[d];
+    if (out && cap >= V->tot) memcpy(out, V->b[axis], V->tot * sizeof(double));
+    return L;
 }
 /* Canonical sum of an interior cell field (reference layout). */
 double smac_oracle_csum(smac_ctx* c, const double* x_ref) {

@@ -1,7 +1,8 @@
 // A driver shaped like the reference time loop (CFD/main.cu:225-317), built
-// against include/smac_facade.hpp: lid-driven cavity, Jacobi or GMG PCG,
-// pressure tolerance set per step through pressure_solver_->tol_.
-// Usage: cavity_facade N STEPS OUTDIR [gmg]
+// against include/smac_facade.hpp: lid-driven cavity, Jacobi PCG, the
+// reference's GMG-PCG, or RBGS-multigrid PCG; pressure tolerance set per step
+// through pressure_solver_->tol_.
+// Usage: cavity_facade N STEPS OUTDIR [jacobi|gmg|rbgs]
 // Writes u, v, w, p (reference layout, little-endian float64) to OUTDIR.
 #include <cstdio>
 #include <cstdlib>
@@ -21,13 +22,14 @@ static void dump(G_SMACSolver& g, int field, const std::string& path) {
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: %s N STEPS OUTDIR [gmg]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s N STEPS OUTDIR [jacobi|gmg|rbgs]\n", argv[0]);
     return 2;
   }
   const int n = std::atoi(argv[1]);
   const int steps = std::atoi(argv[2]);
   const std::string out = argv[3];
-  const bool use_gmg = argc > 4 && std::string(argv[4]) == "gmg";
+  const std::string mode = argc > 4 ? argv[4] : "jacobi";
+  const bool use_gmg = mode == "gmg" || mode == "rbgs";
   try {
     G_SMACSolver g;
     g.set_calc_properties(0.0, 0.0, 0.0, 1.0, 1.0, 1.0, n, n, n);
@@ -40,7 +42,8 @@ int main(int argc, char** argv) {
 
     G_PCGSolver pcg;
     G_GMGSolver gmg;
-    gmg.initialize(g, 0);
+    gmg.initialize(g, 4);  // config.json "gmg_levels"
+    gmg.rbgs_ = mode == "rbgs";
     pcg.set_gmg(gmg);
     pcg.set_solver(use_gmg ? PURENEUMANN_GMG_PCG : PURENEUMANN_STANDARD_PCG);
     g.pressure_solver_ = &pcg;

@@ -186,11 +186,12 @@ def test_gpu_failure_handling(hip):
     assert e.value.status == _abi.SMAC_ERR_INVALID_GRID
 
 
-@pytest.mark.parametrize("precond", ["jacobi", "gmg"])
-def test_cpp_facade_driver(hip, oracle_mod, tmp_path, precond):
+@pytest.mark.parametrize("mode", ["jacobi", "gmg", "rbgs"])
+def test_cpp_facade_driver(hip, oracle_mod, tmp_path, mode):
     """A C++ driver shaped like CFD/main.cu:225-317 (tests/cpp/cavity_facade.cpp),
-    built against include/smac_facade.hpp, runs the cavity through the C-ABI;
-    its fields equal the oracle's."""
+    built against include/smac_facade.hpp, runs the cavity through the C-ABI
+    with the reference's stopping rule (max|r|*dt < tol_, every 10 / 5
+    iterations); its fields equal the oracle's."""
     import os
     import subprocess
 
@@ -198,17 +199,20 @@ def test_cpp_facade_driver(hip, oracle_mod, tmp_path, precond):
 
     if not os.path.exists(build.DRIVER):
         pytest.skip("C++ facade driver not built")
-    print(f"case: C++ facade cavity 16^3 x5 {precond}", flush=True)
-    args = [build.DRIVER, "16", "5", str(tmp_path)] + (["gmg"] if precond == "gmg" else [])
+    n = 24
+    print(f"case: C++ facade cavity {n}^3 x5 {mode}", flush=True)
+    args = [build.DRIVER, str(n), "5", str(tmp_path), mode]
     out = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
-    o = oracle_mod.OracleContext(**cases.cavity_desc(16, re=100.0))
+    o = oracle_mod.OracleContext(**cases.cavity_desc(n, re=100.0))
+    precond = {"jacobi": "jacobi", "gmg": "ref_gmg", "rbgs": "mg"}[mode]
     for _ in range(5):
-        o.step(1e-3, precond="mg" if precond == "gmg" else "jacobi", tol=1e-8)
+        o.step(1e-3, precond=precond, stop_rule="ref_linf_dt", tol=1e-8,
+               check_every=10 if mode == "jacobi" else 5, mg_levels=0 if mode == "jacobi" else 4)
     for f in ("u", "v", "w", "p"):
         a = np.fromfile(os.path.join(tmp_path, f + ".f64"), dtype=np.float64).reshape(o.shape(f))
         b = o.download(f)
-        err = rel_err_mod_mean(a, b, (16, 16, 16)) if f == "p" else rel_err(a, b)
+        err = rel_err_mod_mean(a, b, (n, n, n)) if f == "p" else rel_err(a, b)
         assert err <= TOL, (f, err)
 
 
@@ -229,3 +233,64 @@ def test_periodic_mg_wide_rows(hip, oracle_mod):
         so = o.step(2e-3, precond="mg", tol=1e-8)
         assert sg.iters == so.iters, (s, sg.as_dict(), so.as_dict())
     _compare(g, o, n)
+
+
+def test_cavity_mg_tiled(hip, oracle_mod):
+    """Walled box whose fine levels take the 2.5-D tiled smoother and the
+    tiled residual/restriction (nx >= 32, ny % 8 == 0) and whose coarse levels
+    take the row kernels."""
+    from mydemcfd_amd import cases
+
+    print("case: cavity 64x32x24 x3 mg", flush=True)
+    n = (64, 32, 24)
+    desc = cases.cavity_desc(n, re=100.0)
+    g, o = _pair(hip, oracle_mod, desc)
+    for s in range(3):
+        sg = g.step(2e-3, precond="mg", tol=1e-8)
+        so = o.step(2e-3, precond="mg", tol=1e-8)
+        assert sg.iters == so.iters and sg.mg_levels == so.mg_levels, (s, sg.as_dict(), so.as_dict())
+    _compare(g, o, n)
+
+
+@pytest.mark.parametrize("case", ["cavity24", "walls48x24x12", "periodic32"])
+def test_ref_gmg_steps(hip, oracle_mod, case):
+    """The reference's GMG-PCG (weighted-Jacobi V-cycle, mean restriction,
+    constant prolongation, r mean removed each iteration, max|r|*dt stop
+    every 5 iterations): iterations and fields equal the oracle's."""
+    from mydemcfd_amd import cases
+
+    print(f"case: ref_gmg {case} x4", flush=True)
+    if case == "periodic32":
+        n = (32, 32, 32)
+        desc = cases.periodic_desc(n)
+    else:
+        n = (24, 24, 24) if case == "cavity24" else (48, 24, 12)
+        desc = cases.cavity_desc(n, re=100.0)
+    g, o = _pair(hip, oracle_mod, desc)
+    if case == "periodic32":
+        u, v, w = cases.tg_fields(32, pert="white")
+        _upload_both(g, o, {"u": u, "v": v, "w": w})
+        dt = cases.cfl_dt(o, 0.3)
+    else:
+        dt = 1e-3
+    kw = dict(precond="ref_gmg", stop_rule="ref_linf_dt", tol=1e-8, check_every=5)
+    for s in range(4):
+        sg = g.step(dt, **kw)
+        so = o.step(dt, **kw)
+        assert sg.iters == so.iters and sg.mg_levels == so.mg_levels, (s, sg.as_dict(), so.as_dict())
+        assert sg.converged
+    _compare(g, o, n)
+
+
+def test_ref_gmg_rejects(hip):
+    from mydemcfd_amd import _abi, cases
+
+    print("case: ref_gmg invalid grids and precision", flush=True)
+    c = hip.SMACContext(**cases.cavity_desc(16))
+    with pytest.raises(_abi.SmacError) as e:
+        c.step(1e-3, precond="ref_gmg")
+    assert e.value.status == _abi.SMAC_ERR_INVALID_GRID
+    c = hip.SMACContext(**cases.cavity_desc(24))
+    with pytest.raises(_abi.SmacError) as e:
+        c.step(1e-3, precond="ref_gmg", precision="fp32")
+    assert e.value.status == _abi.SMAC_ERR_UNSUPPORTED
