@@ -17,11 +17,17 @@ Canonical byte model (bytes per interior cell, FP64 word = 8 B; W_s = storage
 width of the V-cycle):
   step   B_step = 160 + (17.43*W_s + 32) + N_it * B_it,
          B_it   = 104 + 17.43*W_s (+16 when the PCG is flexible)
-  kernels (per launch, per cell of the level it runs on):
-         spmv_dot 16, update 48, dot2 16 (+8 flexible), xpay 24,
-         mg_rbgs_l0 1.5*W_s (a half sweep), mg_residual_l0 3*W_s,
-         mg_restrict_l0 1.125*W_s, mg_prolong_l0 2.125*W_s, mg_fill_l0 W_s,
-         predict 56, rhs 32, correct 56, add_dp 32
+  kernels (per launch, per cell of the level it runs on; W_z = width of z,
+  8 in FP64, W_s in the mixed forms):
+         xpay_spmv 24 + W_z (d_new = z + beta d_old, Ad = A d_new, d_new.Ad),
+         spmv_dot 16, update 48 (+W_s: the mixed forms also write (T) r),
+         dot2 8 + W_z (+8 flexible), xpay 16 + W_z,
+         mg_rbgs_l0 1.5*W_s (a half sweep), mg_restrict_l0 2.125*W_s
+         (residual + restriction), mg_prolong_l0 1.125*W_s (red cells),
+         mg_fill_l0 W_s, predict 56, rhs 32, correct 56, add_dp 32
+
+`--copy-calib BYTES` instead times a plain device copy of BYTES bytes (the
+calibration of the FETCH_SIZE / WRITE_SIZE counters) and prints its rate.
 """
 from __future__ import annotations
 
@@ -38,16 +44,15 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width
+KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (flex: the mixed forms)
     "spmv_dot": lambda W, flex: 16.0,
     "xpay_spmv": lambda W, flex: 24.0 + W,
-    "update": lambda W, flex: 48.0,
-    "dot2": lambda W, flex: 16.0 + (8.0 if flex else 0.0),
-    "xpay": lambda W, flex: 24.0,
+    "update": lambda W, flex: 48.0 + (W if flex else 0.0),
+    "dot2": lambda W, flex: 8.0 + W + (8.0 if flex else 0.0),
+    "xpay": lambda W, flex: 16.0 + W,
     "mg_rbgs_l0": lambda W, flex: 1.5 * W,
-    "mg_residual_l0": lambda W, flex: 3.0 * W,
-    "mg_restrict_l0": lambda W, flex: 1.125 * W,
-    "mg_prolong_l0": lambda W, flex: 2.125 * W,
+    "mg_restrict_l0": lambda W, flex: 2.125 * W,
+    "mg_prolong_l0": lambda W, flex: 1.125 * W,
     "mg_fill_l0": lambda W, flex: 1.0 * W,
     "predict": lambda W, flex: 56.0,
     "rhs": lambda W, flex: 32.0,
@@ -144,6 +149,24 @@ def field_errors(ctx_a, ctx_b, n):
     return out
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+
+
+def measured_traffic(n, form, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (scaled by the plain-copy calibration), from the table in
+    profiles/hbm_traffic.md: rows | n | form | kernel | ... | bytes per launch |."""
+    try:
+        with open(TRAFFIC_TABLE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 4 and cells[0] == str(n) and cells[1] == form and cells[2] == kernel:
+                    return int(float(cells[-1]))
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def dominant_roofline(form, n):
     best = None
     for k, (cnt, ms) in form["kernels"].items():
@@ -158,7 +181,8 @@ def dominant_roofline(form, n):
     avg_s = ms / cnt / 1e3
     achieved = per_launch / avg_s / 1e9
     return {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "launches": cnt,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, form["form"], k),
+            "launches": cnt,
             "avg_us": round(1e6 * avg_s, 2), "alg_bytes_per_launch": per_launch}
 
 
@@ -185,6 +209,23 @@ def cpu_baseline(n_cpu, re, cfl, tol):
                       f"({st.iters} iterations, {t1 - t0:.1f} s)"}
 
 
+def copy_calibration(nbytes, reps=4):
+    """A plain device-to-device copy of nbytes (reads nbytes, writes nbytes),
+    the reference point for the HBM byte counters."""
+    import torch
+
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes": nbytes, "launches": reps + 1, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,12 +240,17 @@ def main():
     ap.add_argument("--levels", type=int, default=0)
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is described "
               f"in DESIGN.md and is not built.", file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
+        return
 
     import mydemcfd_amd
     from mydemcfd_amd import cases

@@ -189,3 +189,29 @@ def tg_fields_torch(n, seed=0x5EED0003, kmax=32, slope=-5.0 / 3.0, energy_frac=0
         ix = (torch.arange(shape[2], device=device) - 1) % n
         out.append(a0.index_select(0, iz).index_select(1, iy).index_select(2, ix).contiguous())
     return tuple(out)
+
+
+def dam_break_desc(n=(256, 128, 128), length=(2.0, 1.0, 1.0), rho=(1.0, 1000.0), mu=(1e-5, 1e-3),
+                   g=(0.0, 0.0, -9.81)):
+    """Context kwargs for the two-phase dam break: a closed box of no-slip
+    walls, phase 0 (light, alpha = 0) and phase 1 (heavy, alpha = 1) with
+    densities rho and viscosities mu, gravity along -z. The extents default to
+    [0,2]x[0,1]x[0,1] with h = 1/128 at 256x128x128."""
+    return dict(n=tuple(n), size=tuple(float(v) for v in length), origin=(0.0, 0.0, 0.0), periodic=(0, 0, 0),
+                side_bcid=(1, 1, 1, 1, 1, 1), bc_table=((_abi.BC_NOSLIP, 0.0, 0.0, 0.0), (_abi.BC_NOSLIP, 0.0, 0.0, 0.0)),
+                rho=tuple(rho), mu=tuple(mu), g=tuple(g))
+
+
+def dam_break_alpha(n=(256, 128, 128), length=(2.0, 1.0, 1.0), x_front=0.25, height=0.5):
+    """alpha (reference cell layout, (z, y, x) with ghosts) = 1 in the column
+    x < x_front, z < height (cell centres), 0 elsewhere; ghosts copy the
+    nearest interior cell."""
+    nx, ny, nz = n
+    hx, hz = length[0] / nx, length[2] / nz
+    xc = (np.arange(nx + 2) - 0.5) * hx
+    zc = (np.arange(nz + 2) - 0.5) * hz
+    xc[0], xc[-1] = xc[1], xc[-2]
+    zc[0], zc[-1] = zc[1], zc[-2]
+    col = (xc[None, :] < x_front) & (zc[:, None] < height)  # (z, x)
+    a = np.repeat(col[:, None, :], ny + 2, axis=1).astype(np.float64)
+    return a

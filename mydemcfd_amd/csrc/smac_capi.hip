@@ -76,6 +76,15 @@ struct smac_ctx {
   std::vector<RgLevel> rg;
   int rg_req = 0;
   double rg_beta0 = 0.0;
+  uint64_t rg_ver = ~0ull;
+  // two-phase properties (allocated on first use; natural layout)
+  double *rho = nullptr, *rho_old = nullptr, *inv_rho = nullptr, *mu = nullptr;
+  double* beta[3] = {nullptr, nullptr, nullptr};
+  double* muf[3] = {nullptr, nullptr, nullptr};
+  bool props_ready = false;
+  uint64_t props_ver = 1;     // bumped whenever the face coefficients may change
+  uint64_t mg_ver = ~0ull;    // props_ver the multigrid hierarchy was built for
+  std::vector<double*> mg_b;  // coarse-level face coefficients, 3 per level >= 1
   double* rgt = nullptr;  // fine ping-pong partner of z in the reference V-cycle
   bool prof = false;
   hipEvent_t ev[8];
@@ -197,6 +206,27 @@ static void set_props(smac_ctx* c) {
   c->g.g[2] = c->fl.gz;
 }
 
+// two-phase grid (rho0 != rho1, mu0 != mu1 or a nonzero alpha field): the
+// properties live in arrays built from alpha
+static bool var_props(const smac_ctx* c) { return !uniform_props(c); }
+
+static Props props_of(const smac_ctx* c) {
+  Props P;
+  P.mu = c->mu;
+  P.rho_old = c->rho_old;
+  P.inv_rho = c->inv_rho;
+  for (int d = 0; d < 3; d++) {
+    P.beta[d] = c->beta[d];
+    P.muf[d] = c->muf[d];
+  }
+  return P;
+}
+static BetaF beta_of(const smac_ctx* c) {
+  BetaF b;
+  for (int d = 0; d < 3; d++) b.b[d] = c->beta[d];
+  return b;
+}
+
 // ------------------------------------------------------------------ profiling
 static void prof_begin(smac_ctx* c, int slot) {
   if (c->prof && c->ev_ok) (void)hipEventRecord(c->ev[slot], c->st);
@@ -288,6 +318,10 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
 }
 
 static void mg_free(smac_ctx* c) {
+  for (auto b : c->mg_b)
+    if (b) (void)hipFree(b);
+  c->mg_b.clear();
+  c->mg_ver = ~0ull;
   for (auto& L : c->lev) {
     if (L.own_e) (void)hipFree(L.e);
     if (L.own_f) (void)hipFree(L.f);
@@ -318,8 +352,10 @@ extern "C" void smac_destroy(smac_ctx* c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   mg_free(c);
   rg_free(c);
-  double* arrs[] = {c->u[0], c->u[1], c->u[2], c->us[0], c->us[1], c->us[2], c->p, c->dp, c->rhs, c->alpha,
-                    c->r, c->z, c->dv, c->dv2, c->Ad, c->res0, c->S};
+  double* arrs[] = {c->u[0],   c->u[1],    c->u[2],    c->us[0],   c->us[1],  c->us[2],  c->p,      c->dp,
+                    c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->dv2,    c->Ad,     c->res0,
+                    c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
+                    c->muf[0], c->muf[1],  c->muf[2]};
   for (auto a : arrs)
     if (a) (void)hipFree(a);
   for (int m = 0; m < 5; m++)
@@ -335,6 +371,8 @@ extern "C" int smac_set_fluid(smac_ctx* c, const smac_fluid_desc* fl) {
   if (!c || !fl) return SMAC_ERR_INVALID_ARG;
   c->fl = *fl;
   set_props(c);
+  c->props_ready = false;
+  c->props_ver++;
   return SMAC_OK;
 }
 
@@ -438,6 +476,8 @@ extern "C" int smac_upload(smac_ctx* c, int32_t f, const double* h, size_t count
     bool nz = false;
     for (size_t t = 0; t < count && !nz; t++) nz = h[t] != 0.0;
     c->alpha_nonzero = nz;
+    c->props_ready = false;
+    c->props_ver++;
   }
   return copy_field(c, f, *slot, h, true, hipMemcpyHostToDevice, nullptr, nullptr);
 }
@@ -472,27 +512,49 @@ extern "C" int smac_download_device(smac_ctx* c, int32_t f, double* ddst, size_t
 }
 
 // ------------------------------------------------------------------ the step
+// update_properties_by_alpha + update_boundary_faces
+// (CFD/G_SMACSolver_VoF.cu:1544-1566,1944-1950). initial = 0 keeps the
+// previous density as rho_old (the rho/rho_old swap of k_swap_rho).
 extern "C" int smac_update_properties(smac_ctx* c, int32_t initial) {
-  (void)initial;
   if (!c) return SMAC_ERR_INVALID_ARG;
-  if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
   set_props(c);
+  if (!var_props(c)) {
+    c->props_ready = true;
+    return SMAC_OK;
+  }
+  int rc = SMAC_OK;
+  double** arrs[] = {&c->rho,    &c->rho_old, &c->inv_rho, &c->mu,     &c->beta[0],
+                     &c->beta[1], &c->beta[2], &c->muf[0],  &c->muf[1], &c->muf[2]};
+  for (auto a : arrs)
+    if (!rc && !*a) rc = dalloc(a, c->tot);
+  if (!rc && !c->alpha) rc = dalloc(&c->alpha, c->tot);
+  if (rc) return rc;
+  const bool init = initial || !c->props_ready;
+  if (!init) std::swap(c->rho, c->rho_old);
+  KT(c, "props", launch_props(c->g, c->alpha, c->rho, c->rho_old, c->inv_rho, c->mu, c->beta, c->muf, c->fl.rho0,
+                              c->fl.rho1, c->fl.mu0, c->fl.mu1, init ? 1 : 0, c->st));
+  HIPOK(hipGetLastError());
+  c->props_ready = true;
   return SMAC_OK;
 }
+
+static int ensure_props(smac_ctx* c) { return c->props_ready ? SMAC_OK : smac_update_properties(c, 1); }
 
 extern "C" int smac_compute_mass_flux(smac_ctx* c, double dt) {
   (void)dt;
   if (!c) return SMAC_ERR_INVALID_ARG;
-  // uniform properties: mf = rho0*u_f is formed inside the predictor
-  return uniform_props(c) ? SMAC_OK : SMAC_ERR_UNSUPPORTED;
+  // without a VoF flux mf = rho0*u_f, formed inside the predictor
+  return SMAC_OK;
 }
 
 extern "C" int smac_predictor(smac_ctx* c, double dt) {
   if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
-  if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
+  int rc = ensure_props(c);
+  if (rc) return rc;
   Vel V;
   for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
-  KT(c, "predict", launch_predict(c->g, V, c->p, c->us, dt, c->st));
+  const Props P = props_of(c);
+  KT(c, "predict", launch_predict(c->g, var_props(c) ? &P : nullptr, V, c->p, c->us, dt, c->st));
   HIPOK(hipGetLastError());
   return SMAC_OK;
 }
@@ -527,12 +589,24 @@ static int mg_check_levels(const smac_ctx* c, int L) {
 }
 
 // level operator of a uniform-coefficient level (same rule as coef_uniform)
-static void host_coef(const Lv& L, int i, int j, int k, double a6[6], double* diag) {
+// Level coefficients on the host (uniform, or from host copies hb[3] of the
+// level's face coefficients): the rule of level_coef.
+static void host_coef(const Lv& L, const std::vector<double>* hb, int i, int j, int k, double a6[6],
+                      double* diag) {
   int q[3] = {i, j, k};
   for (int d = 0; d < 3; d++) {
     bool wp = !L.per[d] && (q[d] + 1 == 1 || q[d] + 1 == L.n[d] + 1);
     bool wm = !L.per[d] && (q[d] == 1 || q[d] == L.n[d] + 1);
-    double bp = wp ? 0.0 : L.beta0, bm = wm ? 0.0 : L.beta0;
+    double bp, bm;
+    if (hb) {
+      int qp[3] = {i, j, k};
+      qp[d] = L.per[d] ? wrapi(q[d] + 1, L.n[d]) : q[d] + 1;
+      bp = hb[d][((size_t)qp[2] * L.py + qp[1]) * L.px + qp[0]];
+      bm = hb[d][((size_t)k * L.py + j) * L.px + i];
+    } else {
+      bp = wp ? 0.0 : L.beta0;
+      bm = wm ? 0.0 : L.beta0;
+    }
     a6[2 * d] = bp * L.inv_d2[d];
     a6[2 * d + 1] = bm * L.inv_d2[d];
   }
@@ -547,7 +621,7 @@ static void host_coef(const Lv& L, int i, int j, int k, double a6[6], double* di
 
 // Dense (pseudo-)inverse of the coarsest operator: Cholesky of A + c*J (c =
 // A00/n when A is singular), column-by-column inverse, minus J/(c n^2).
-static void dense_pinv(const Lv& L, double* out, int singular) {
+static void dense_pinv(const Lv& L, const std::vector<double>* hb, double* out, int singular) {
   int n = L.n[0] * L.n[1] * L.n[2];
   std::vector<double> A((size_t)n * n, 0.0);
   for (int k = 1; k <= L.n[2]; k++)
@@ -555,7 +629,7 @@ static void dense_pinv(const Lv& L, double* out, int singular) {
       for (int i = 1; i <= L.n[0]; i++) {
         int row = ((k - 1) * L.n[1] + (j - 1)) * L.n[0] + (i - 1);
         double a6[6], diag;
-        host_coef(L, i, j, k, a6, &diag);
+        host_coef(L, hb, i, j, k, a6, &diag);
         A[(size_t)row * n + row] += diag;
         int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
         for (int m = 0; m < 6; m++) {
@@ -604,13 +678,50 @@ static void dense_pinv(const Lv& L, double* out, int singular) {
 static size_t prec_bytes(int prec) { return prec == SMAC_PREC_FP64 ? 8 : (prec == SMAC_PREC_MIXED_FP32 ? 4 : 2); }
 static size_t prec_cbytes(int prec) { return prec == SMAC_PREC_FP64 ? 8 : 4; }
 
+// Coarse face coefficients (two-phase grids) and the dense coarsest inverse.
+static int mg_coeffs(smac_ctx* c) {
+  const int L = (int)c->lev.size();
+  const bool var = c->lev[0].lv.b[0] != nullptr;
+  for (int l = 1; l < L && var; l++)
+    for (int d = 0; d < 3; d++)
+      launch_mg_coarsen_beta(c->lev[l - 1].lv, c->lev[l].lv, d, (double*)c->lev[l].lv.b[d], c->st);
+  HIPOK(hipGetLastError());
+  const Lv& C = c->lev[L - 1].lv;
+  std::vector<double> hb[3];
+  if (var) {
+    const size_t t = (size_t)C.px * C.py * (size_t)(C.n[2] + 3);
+    for (int d = 0; d < 3; d++) {
+      hb[d].resize(t);
+      HIPOK(hipMemcpyAsync(hb[d].data(), C.b[d], t * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    }
+    HIPOK(hipStreamSynchronize(c->st));
+  }
+  int n = C.n[0] * C.n[1] * C.n[2];
+  std::vector<double> h((size_t)n * n);
+  dense_pinv(C, var ? hb : nullptr, h.data(), 1);
+  if (!c->cinv) {
+    int rc = dalloc(&c->cinv, (size_t)n * n);
+    if (rc) return rc;
+  }
+  HIPOK(hipMemcpy(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+  c->cinv_n = n;
+  c->mg_ver = c->props_ver;
+  return SMAC_OK;
+}
+
 static int mg_setup(smac_ctx* c, int L, int prec) {
-  if (c->mg_L == L && c->mg_prec == prec && !c->lev.empty() && c->lev[0].lv.beta0 == c->g.beta0) return SMAC_OK;
+  const bool var = var_props(c);
+  if (c->mg_L == L && c->mg_prec == prec && !c->lev.empty() && c->lev[0].lv.beta0 == c->g.beta0 &&
+      (c->lev[0].lv.b[0] != nullptr) == var) {
+    if (!var || c->mg_ver == c->props_ver) return SMAC_OK;
+    return mg_coeffs(c);  // same hierarchy, new coefficients
+  }
   mg_free(c);
   c->lev.resize(L);
   const size_t sb = prec_bytes(prec), cb = prec_cbytes(prec);
   for (int l = 0; l < L; l++) {
     Level& V = c->lev[l];
+    memset(&V.lv, 0, sizeof(V.lv));
     for (int d = 0; d < 3; d++) {
       V.lv.n[d] = c->n[d] >> l;
       V.lv.per[d] = c->per[d];
@@ -639,19 +750,25 @@ static int mg_setup(smac_ctx* c, int L, int prec) {
         V.own_tb = rc == SMAC_OK;
       }
     }
+    if (var) {
+      for (int d = 0; d < 3 && !rc; d++) {
+        if (l == 0) {
+          V.lv.b[d] = c->beta[d];
+        } else {
+          double* b = nullptr;
+          rc = dalloc(&b, t);
+          if (!rc) {
+            c->mg_b.push_back(b);
+            V.lv.b[d] = b;
+          }
+        }
+      }
+    }
     if (rc) return rc;
   }
-  const Lv& C = c->lev[L - 1].lv;
-  int n = C.n[0] * C.n[1] * C.n[2];
-  std::vector<double> h((size_t)n * n);
-  dense_pinv(C, h.data(), 1);
-  int rc = dalloc(&c->cinv, (size_t)n * n);
-  if (rc) return rc;
-  HIPOK(hipMemcpy(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
-  c->cinv_n = n;
   c->mg_L = L;
   c->mg_prec = prec;
-  return SMAC_OK;
+  return mg_coeffs(c);
 }
 
 template <typename T>
@@ -692,7 +809,8 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
 // prolongs through levels_[-1], G_GMGSolver.cu:971).
 static int rg_setup(smac_ctx* c, int req) {
   if (req < 1 || req > 16) return SMAC_ERR_INVALID_ARG;
-  if (!c->rg.empty() && c->rg_req == req && c->rg_beta0 == c->g.beta0) return (int)c->rg.size();
+  const uint64_t ver = var_props(c) ? c->props_ver : 0;
+  if (!c->rg.empty() && c->rg_req == req && c->rg_beta0 == c->g.beta0 && c->rg_ver == ver) return (int)c->rg.size();
   rg_free(c);
   std::vector<RgLv> lv;
   int prev[3] = {c->n[0], c->n[1], c->n[2]};
@@ -734,14 +852,16 @@ static int rg_setup(smac_ctx* c, int req) {
     rg_free(c);
     return rc;
   }
-  // face coefficients: constant while the density is (uniform grids)
+  // face coefficients (create_coeffs); rebuilt when the grid's coefficients change
+  const BetaF gb = beta_of(c);
   for (size_t l = 0; l < c->rg.size(); l++)
     for (int d = 0; d < 3; d++)
-      launch_rg_beta(c->g, c->rg[l].lv, l ? c->rg[l - 1].lv : c->rg[l].lv, l ? c->rg[l - 1].b[d] : nullptr, d,
-                     c->rg[l].b[d], c->st);
+      launch_rg_beta(c->g, ver ? &gb : nullptr, c->rg[l].lv, l ? c->rg[l - 1].lv : c->rg[l].lv,
+                     l ? c->rg[l - 1].b[d] : nullptr, d, c->rg[l].b[d], c->st);
   HIPOK(hipGetLastError());
   c->rg_req = req;
   c->rg_beta0 = c->g.beta0;
+  c->rg_ver = ver;
   return (int)c->rg.size();
 }
 
@@ -750,15 +870,17 @@ static void rg_vcycle(smac_ctx* c, const smac_solver_opts& o) {
   const Geo& g = c->g;
   hipStream_t s = c->st;
   const int nf = o.ref_sweeps, nc = o.ref_coarse_sweeps, L = (int)c->rg.size();
+  const BetaF gb = beta_of(c);
+  const BetaF* gbp = var_props(c) ? &gb : nullptr;
   double* x = c->z;
   double* t = c->rgt;
   for (auto& V : c->rg) KT(c, "rg_levels", launch_rg_zero(V.q, V.tot, s));
   if (nf == 0) KT(c, "rg_fine", launch_rg_zero(x, c->tot, s));
   for (int it = 0; it < nf; it++) {
-    KT(c, "rg_fine_jacobi", launch_rg_fine_jacobi(g, x, t, c->r, it == 0, s));
+    KT(c, "rg_fine_jacobi", launch_rg_fine_jacobi(g, gbp, x, t, c->r, it == 0, s));
     std::swap(x, t);
   }
-  KT(c, "rg_fine_restrict", launch_rg_fine_res_restrict(g, x, c->r, c->rg[0].lv, c->rg[0].rhs, s));
+  KT(c, "rg_fine_restrict", launch_rg_fine_res_restrict(g, gbp, x, c->r, c->rg[0].lv, c->rg[0].rhs, s));
   for (int l = 0; l < L - 1; l++) {
     RgLevel& V = c->rg[l];
     for (int it = 0; it < nf; it++) {
@@ -785,7 +907,7 @@ static void rg_vcycle(smac_ctx* c, const smac_solver_opts& o) {
     else KT(c, "rg_fine_prolong", launch_rg_prolong_fine(V.lv, V.q, g, x, s));
   }
   for (int it = 0; it < nf; it++) {
-    KT(c, "rg_fine_jacobi", launch_rg_fine_jacobi(g, x, t, c->r, 0, s));
+    KT(c, "rg_fine_jacobi", launch_rg_fine_jacobi(g, gbp, x, t, c->r, 0, s));
     std::swap(x, t);
   }
   // 2*nf sweeps: the result is back in z
@@ -840,13 +962,16 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   const bool refg = o.precond == SMAC_PRECOND_REF_GMG;  // FP64 only (checked by the caller)
   T* z = mg ? (T*)c->lev[0].e : (T*)c->z;
   T* f0 = (mg && (void*)c->lev[0].f != (void*)c->r) ? (T*)c->lev[0].f : nullptr;
+  const bool var = var_props(c);
+  const BetaF bf = beta_of(c);
+  const BetaF* bfp = var ? &bf : nullptr;
   auto precond = [&]() {
     if (refg) {
       rg_vcycle(c, o);
       return;
     }
     if (!mg) {
-      KT(c, "jacobi", launch_jacobi(c->g, c->r, c->z, c->st));
+      KT(c, "jacobi", launch_jacobi(c->g, bfp, c->r, c->z, c->st));
       return;
     }
     vcycle<T>(c, 0, o.nu1, o.nu2);
@@ -884,7 +1009,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   }
   // fused direction update + SpMV (2.5-D tiled) when the grid fits; the
   // direction then lives in a ping-pong pair (dcur is the current direction)
-  const bool fused = xpay_spmv_fits(g);
+  const bool fused = !var && xpay_spmv_fits(g);
   double* dcur = c->dv;
   double* dnxt = c->dv2;
   if (fused) launch_dot2<T>(g, c->r, z, c->Ad, 0, P[0], P[1], s);
@@ -908,7 +1033,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       KT(c, "xpay_spmv", launch_xpay_spmv<T>(g, z, dcur, dnxt, c->Ad, c->S, iter == 0 ? -1 : S_BETA, P[0], s));
       std::swap(dcur, dnxt);
     } else {
-      KT(c, "spmv_dot", launch_spmv_dot(g, c->dv, c->Ad, P[0], s));
+      KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
     }
     {
       FinalizeArgs fa = fin(c);
@@ -1019,8 +1144,11 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
   if (o.precond != SMAC_PRECOND_MG && o.precision != SMAC_PREC_FP64) return SMAC_ERR_UNSUPPORTED;
   if (o.precond == SMAC_PRECOND_REF_GMG && (o.ref_sweeps < 0 || o.ref_coarse_sweeps < 0))
     return SMAC_ERR_INVALID_ARG;
-  if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
   if (c->n[0] > 4000) return SMAC_ERR_INVALID_GRID;  // restriction row buffer in LDS
+  {
+    int rc = ensure_props(c);
+    if (rc) return rc;
+  }
   int L = 1;
   if (o.precond == SMAC_PRECOND_MG) {
     L = o.mg_levels > 0 ? o.mg_levels : mg_auto_levels(c);
@@ -1040,8 +1168,10 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
 
 extern "C" int smac_correct(smac_ctx* c, double dt) {
   if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
-  if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
-  KT(c, "correct", launch_correct(c->g, c->us, c->dp, c->u, dt, c->st));
+  int rc = ensure_props(c);
+  if (rc) return rc;
+  const BetaF bf = beta_of(c);
+  KT(c, "correct", launch_correct(c->g, var_props(c) ? &bf : nullptr, c->us, c->dp, c->u, dt, c->st));
   KT(c, "add_dp", launch_add_dp(c->g, c->p, c->dp, c->st));
   HIPOK(hipGetLastError());
   return SMAC_OK;
@@ -1052,7 +1182,13 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
   smac_solve_stats local;
   smac_solve_stats* sp = st ? st : &local;
   prof_begin(c, 0);
-  int rc = smac_predictor(c, dt);
+  // properties: built once, then renewed every step of a two-phase grid
+  // (the order of the reference loop, CFD/main.cu:270-317)
+  int rc = SMAC_OK;
+  if (!c->props_ready) rc = smac_update_properties(c, 1);
+  else if (var_props(c) && (c->fl.rho0 != c->fl.rho1 || c->fl.mu0 != c->fl.mu1)) rc = smac_update_properties(c, 0);
+  if (rc) return rc;
+  rc = smac_predictor(c, dt);
   if (rc) return rc;
   double t_pred = prof_end(c, 0);
   prof_begin(c, 2);
@@ -1070,17 +1206,19 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
 
 extern "C" int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
   if (!c) return SMAC_ERR_INVALID_ARG;
-  if (!uniform_props(c)) return SMAC_ERR_UNSUPPORTED;
+  int rc = ensure_props(c);
+  if (rc) return rc;
   Vel V;
   for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
   double visc = (1. / c->fl.rho0) * c->fl.mu0 * dt * (c->g.inv_d2[0] + c->g.inv_d2[1] + c->g.inv_d2[2]);
-  launch_cfl(c->g, V, dt, visc, c->part[0], c->part[1], c->st);
+  const bool var = var_props(c);
+  launch_cfl(c->g, V, dt, visc, var ? c->inv_rho : nullptr, var ? c->mu : nullptr, c->part[0], c->part[1], c->st);
   FinalizeArgs fa = fin(c);
   fin_add(fa, c->part[0], 1, S_CFLC);
   fin_add(fa, c->part[1], 1, S_CFLV);
   launch_finalize(fa, c->S, c->st);
   HIPOK(hipGetLastError());
-  int rc = read_scalars(c);
+  rc = read_scalars(c);
   if (rc) return rc;
   if (cc) *cc = c->Sh[S_CFLC];
   if (cv) *cv = c->Sh[S_CFLV];

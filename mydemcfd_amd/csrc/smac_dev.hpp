@@ -198,6 +198,69 @@ __device__ __forceinline__ Coef coef_uniform(const int n[3], const int per[3], c
   return c;
 }
 
+// Variable-coefficient Poisson operator of a two-phase grid: the same rule on
+// the face coefficients beta = 1/rho_f (natural layout; 0 on wall and ghost
+// faces) of k_build_vof_poisson_A.
+__device__ __forceinline__ Coef coef_var(const Geo& g, const double* const b[3], int i, int j, int k) {
+  Coef c;
+  int q[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    int qp[3] = {i, j, k};
+    qp[d] = W(g, d, q[d] + 1);
+    c.a[2 * d] = b[d][idx(g, qp[0], qp[1], qp[2])] * g.inv_d2[d];
+    c.a[2 * d + 1] = b[d][idx(g, i, j, k)] * g.inv_d2[d];
+  }
+  c.diag = c.a[0] + c.a[1] + c.a[2] + c.a[3] + c.a[4] + c.a[5];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (wall_face(g.per[d], q[d] + 1, g.n[d])) c.a[2 * d] = 0.;
+    if (wall_face(g.per[d], q[d], g.n[d])) c.a[2 * d + 1] = 0.;
+  }
+  return c;
+}
+
+struct BetaF {
+  const double* b[3];
+};
+
+// Cell and face properties of a two-phase grid (natural layout):
+// mu, rho_old, 1/rho on cells (ghosts filled), beta = 1/rho_f and mu_f on faces
+// (k_update_*_properties_by_alpha, CFD/G_SMACSolver_VoF.cu:1244-1535).
+struct Props {
+  const double* mu;
+  const double* rho_old;
+  const double* inv_rho;
+  const double* beta[3];
+  const double* muf[3];
+};
+
+// Property policies of the predictor and the correction. UniP: one phase,
+// constants from Geo (mu_f = 0.5(mu0+mu0), rho_f = 0.5(rho0+rho0), as the
+// face formulas give them); VarP: the arrays of Props.
+struct UniP {
+  static __device__ __forceinline__ double mu(const Geo& g, const Props&, I3) { return g.mu0; }
+  static __device__ __forceinline__ double mu_edge(const Geo& g, const Props&, int, I3, I3) {
+    return 0.5 * (g.mu0 + g.mu0);
+  }
+  static __device__ __forceinline__ double beta(const Geo& g, const Props&, int, I3) { return g.beta0; }
+  static __device__ __forceinline__ double rho_old_f(const Geo& g, const Props&, I3, I3) {
+    return 0.5 * (g.rho0 + g.rho0);
+  }
+};
+struct VarP {
+  static __device__ __forceinline__ double mu(const Geo& g, const Props& P, I3 c) { return P.mu[IX(g, c)]; }
+  static __device__ __forceinline__ double mu_edge(const Geo& g, const Props& P, int b, I3 f1, I3 f2) {
+    return 0.5 * (P.muf[b][IX(g, f1)] + P.muf[b][IX(g, f2)]);
+  }
+  static __device__ __forceinline__ double beta(const Geo& g, const Props& P, int a, I3 q) {
+    return P.beta[a][IX(g, q)];
+  }
+  static __device__ __forceinline__ double rho_old_f(const Geo& g, const Props& P, I3 q, I3 cm) {
+    return 0.5 * (P.rho_old[IX(g, q)] + P.rho_old[IX(g, cm)]);
+  }
+};
+
 // Storage traits of the multigrid vectors: storage type T, compute type C.
 // bf16 is kept as raw 16-bit words (upper half of an IEEE float), rounded to
 // nearest-even on store.

@@ -7,13 +7,15 @@
 
 namespace smac {
 
-// multigrid level geometry (uniform coefficients)
+// multigrid level geometry; b[0..2] == nullptr: uniform coefficient beta0,
+// else the level's face coefficients (natural layout, pitches px, py)
 struct Lv {
   int n[3];
   int px, py;
   int per[3];
   double inv_d2[3];
   double beta0;
+  const double* b[3];
 };
 
 // level of the reference's multigrid (natural layout, pitches n+3)
@@ -65,7 +67,9 @@ enum {
   S_RR, S_LINF, S_ZAD, S_CFLC, S_CFLV, S_COUNT = 32
 };
 
-void launch_predict(const Geo& g, const Vel& V, const double* p, double* us[3], double dt, hipStream_t st);
+// P == nullptr: one-phase (uniform) properties from Geo
+void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
+                    hipStream_t st);
 void launch_rhs(const Geo& g, double* const us[3], double* rhs, double inv_dt, double* part0, hipStream_t st);
 void launch_init_r(const Geo& g, const double* rhs, double* r, double* dp, const double* S, int slot_mean,
                    double* p0, double* p1, hipStream_t st);
@@ -73,10 +77,11 @@ template <typename Z>
 void launch_add_scalar(const Geo& g, Z* x, double a, const double* S, int slot, hipStream_t st);
 template <typename Z>
 void launch_sum(const Geo& g, const Z* x, double* p0, hipStream_t st);
-void launch_jacobi(const Geo& g, const double* r, double* z, hipStream_t st);
+// bf == nullptr: one-phase operator (beta0); else face coefficients bf->b
+void launch_jacobi(const Geo& g, const BetaF* bf, const double* r, double* z, hipStream_t st);
 template <typename Z>
 void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st);
-void launch_spmv_dot(const Geo& g, const double* d, double* Ad, double* p0, hipStream_t st);
+void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st);
 template <typename F>
 void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,
                    int slot_alpha, double* p0, double* p1, F* f0, hipStream_t st);
@@ -95,9 +100,15 @@ bool launch_xpay_spmv(const Geo& g, const Z* z, const double* dold, double* dnew
                       int beta_slot, double* p0, hipStream_t st);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
-void launch_correct(const Geo& g, double* const us[3], const double* dp, double* u[3], double dt, hipStream_t st);
+void launch_correct(const Geo& g, const BetaF* bf, double* const us[3], const double* dp, double* u[3], double dt,
+                    hipStream_t st);
+void launch_props(const Geo& g, const double* alpha, double* rho, double* rho_old, double* inv_rho, double* mu,
+                  double* beta[3], double* muf[3], double rho0, double rho1, double mu0, double mu1, int initial,
+                  hipStream_t st);
 void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st);
-void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, double* p0, double* p1, hipStream_t st);
+// inv_rho == nullptr: one-phase viscous CFL `visc`; else (mu/rho) dt sum 1/h^2 per cell
+void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, const double* inv_rho, const double* mu,
+                double* p0, double* p1, hipStream_t st);
 // multigrid (smac_mg.hip), storage T in {double, float, bf16s}; the restriction
 // buffer tb has the compute type of T
 template <typename T>
@@ -111,20 +122,23 @@ template <typename T>
 void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st);
 template <typename T>
 void launch_fill(const Lv& L, T* x, hipStream_t st);
+// coarse face coefficients of axis d on level C from level F: mean of the four
+// fine faces covering each coarse face, ((b00 + b10) + (b01 + b11)) / 4
+void launch_mg_coarsen_beta(const Lv& F, const Lv& C, int d, double* cb, hipStream_t st);
 
 // the reference's multigrid (smac_refgmg.hip); fb == nullptr: coefficients
 // of level 0 from the grid
-void launch_rg_beta(const Geo& g, const RgLv& C, const RgLv& F, const double* fb, int d, double* cb,
-                    hipStream_t st);
+void launch_rg_beta(const Geo& g, const BetaF* gb, const RgLv& C, const RgLv& F, const double* fb, int d,
+                    double* cb, hipStream_t st);
 void launch_rg_jacobi(const RgLv& L, double* const b[3], const double* q, double* qt, const double* rhs,
                       hipStream_t st);
 void launch_rg_res_restrict(const RgLv& F, double* const b[3], const double* q, const double* rhs, const RgLv& C,
                             double* crhs, hipStream_t st);
 void launch_rg_prolong(const RgLv& C, const double* cq, const RgLv& F, double* fq, hipStream_t st);
-void launch_rg_fine_jacobi(const Geo& g, const double* x, double* out, const double* r, int zero_start,
-                           hipStream_t st);
-void launch_rg_fine_res_restrict(const Geo& g, const double* x, const double* r, const RgLv& C, double* crhs,
-                                 hipStream_t st);
+void launch_rg_fine_jacobi(const Geo& g, const BetaF* gb, const double* x, double* out, const double* r,
+                           int zero_start, hipStream_t st);
+void launch_rg_fine_res_restrict(const Geo& g, const BetaF* gb, const double* x, const double* r, const RgLv& C,
+                                 double* crhs, hipStream_t st);
 void launch_rg_prolong_fine(const RgLv& C, const double* cq, const Geo& g, double* x, hipStream_t st);
 void launch_rg_zero(double* x, size_t n, hipStream_t st);
 void launch_add_scalar_norms(const Geo& g, double* r, double a, const double* S, int slot, double* p0, double* p1,

@@ -53,17 +53,18 @@ struct PeriodicBC {
 };
 
 // Cross-direction viscous term of component A along axis B (B != A).
-template <class BC, int A, int B>
-__device__ __forceinline__ double visc_cross(const Geo& g, const Vel& V, I3 q, double v0, double mu0) {
+// Edge viscosities are the mean of the two B-face viscosities around the edge.
+template <class BC, class PR, int A, int B>
+__device__ __forceinline__ double visc_cross(const Geo& g, const Props& P, const Vel& V, I3 q, double v0) {
   double vbm = BC::dir(g, V, A, B, q, -1);
   double vbp = BC::dir(g, V, A, B, q, +1);
   double wpp = BC::face(g, V, B, sh(q, B, +1));
   double wmp = BC::face(g, V, B, sh(sh(q, B, +1), A, -1));
   double wp0 = BC::face(g, V, B, q);
   double wm0 = BC::face(g, V, B, sh(q, A, -1));
-  double mu_p = 0.5 * (mu0 + mu0);
+  double mu_p = PR::mu_edge(g, P, B, wr(g, sh(q, B, +1)), wr(g, sh(sh(q, B, +1), A, -1)));
   double t_p = mu_p * ((wpp - wmp) * g.inv_d[A] + (vbp - v0) * g.inv_d[B]);
-  double mu_m = 0.5 * (mu0 + mu0);
+  double mu_m = PR::mu_edge(g, P, B, wr(g, q), wr(g, sh(q, A, -1)));
   double t_m = mu_m * ((wp0 - wm0) * g.inv_d[A] + (v0 - vbm) * g.inv_d[B]);
   return (t_p - t_m) * g.inv_d[B];
 }
@@ -139,39 +140,40 @@ __device__ __forceinline__ double conv_term(const Geo& g, const Vel& V, I3 q, do
 }
 
 // One component of k_get_vof_vstar_rhouu_consistent_{x,y,z}
-// (CFD/G_SMACSolver.cu:139-392, 441-662, 763-1003) with uniform properties:
-// mass flux mf = rho0*u_f (CFD/G_SMACSolver_VoF.cu:1708-1790), mu and face mu
-// = mu0, beta = beta0, rho_old = rho0. Terms are accumulated in the
+// (CFD/G_SMACSolver.cu:139-392, 441-662, 763-1003). Mass flux mf = rho0*u_f
+// (CFD/G_SMACSolver_VoF.cu:1708-1790 with no VoF flux); mu, face mu, beta and
+// rho_old from the property policy PR. Terms are accumulated in the
 // reference's order: own-direction stress, cross stresses in increasing
 // axis order, convection along x, y, z, old pressure gradient.
-template <class BC, int A>
-__device__ double predict_face(const Geo& g, const Vel& V, const double* __restrict__ p, I3 q, double dt) {
+template <class BC, class PR, int A>
+__device__ double predict_face(const Geo& g, const Props& P, const Vel& V, const double* __restrict__ p, I3 q,
+                               double dt) {
   constexpr int B1 = A == 0 ? 1 : 0;
   constexpr int B2 = A == 2 ? 1 : 2;
-  const double rho0 = g.rho0, mu0 = g.mu0;
+  const double rho0 = g.rho0;
   double v0 = V.c[A][IX(g, q)];
   double vp1 = BC::face(g, V, A, sh(q, A, +1));
   double vm1 = BC::face(g, V, A, sh(q, A, -1));
   I3 cm = wr(g, sh(q, A, -1));
 
-  double tau_p = mu0 * (vp1 - v0);
-  double tau_m = mu0 * (v0 - vm1);
+  double tau_p = PR::mu(g, P, q) * (vp1 - v0);
+  double tau_m = PR::mu(g, P, cm) * (v0 - vm1);
   double tmp = 2. * (tau_p - tau_m) * g.inv_d2[A];
-  tmp += visc_cross<BC, A, B1>(g, V, q, v0, mu0);
-  tmp += visc_cross<BC, A, B2>(g, V, q, v0, mu0);
+  tmp += visc_cross<BC, PR, A, B1>(g, P, V, q, v0);
+  tmp += visc_cross<BC, PR, A, B2>(g, P, V, q, v0);
   tmp -= conv_term<BC, A, 0>(g, V, q, v0, vp1, vm1, rho0);
   tmp -= conv_term<BC, A, 1>(g, V, q, v0, vp1, vm1, rho0);
   tmp -= conv_term<BC, A, 2>(g, V, q, v0, vp1, vm1, rho0);
 
-  double beta = g.beta0;
-  double rho_old_f = 0.5 * (rho0 + rho0);
+  double beta = PR::beta(g, P, A, q);
+  double rho_old_f = PR::rho_old_f(g, P, q, cm);
   tmp -= (p[IX(g, q)] - p[IX(g, cm)]) * g.inv_d[A];
   return beta * (v0 * rho_old_f) + dt * (beta * tmp + g.g[A]);
 }
 
 // grid: x = face index along i (block 64), y = j (block 4), z = k
-template <class BC, int A>
-__global__ __launch_bounds__(256) void k_predict(Geo g, Vel V, const double* __restrict__ p,
+template <class BC, class PR, int A>
+__global__ __launch_bounds__(256) void k_predict(Geo g, Props P, Vel V, const double* __restrict__ p,
                                                  double* __restrict__ ustar, double dt) {
   int lo[3] = {1, 1, 1}, hi[3] = {g.n[0], g.n[1], g.n[2]};
   lo[A] = g.per[A] ? 1 : 2;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, Vel V, const double* __r
   int k = lo[2] + (int)blockIdx.z;
   if (i > hi[0] || j > hi[1] || k > hi[2]) return;
   I3 q = mk(i, j, k);
-  ustar[IX(g, q)] = predict_face<BC, A>(g, V, p, q, dt);
+  ustar[IX(g, q)] = predict_face<BC, PR, A>(g, P, V, p, q, dt);
 }
 
 // All three components in one pass: the thread of cell (i,j,k) computes u* on
@@ -204,9 +206,10 @@ __global__ __launch_bounds__(256) void k_predict_fused(Geo g, Vel V, const doubl
     const bool zi = g.per[2] || k >= 2;
     I3 q = mk(i, j, k);
     const size_t id = IX(g, q);
-    if (xi) us0[id] = predict_face<BC, 0>(g, V, p, q, dt);
-    if (yi) us1[id] = predict_face<BC, 1>(g, V, p, q, dt);
-    if (zi) us2[id] = predict_face<BC, 2>(g, V, p, q, dt);
+    const Props P{};
+    if (xi) us0[id] = predict_face<BC, UniP, 0>(g, P, V, p, q, dt);
+    if (yi) us1[id] = predict_face<BC, UniP, 1>(g, P, V, p, q, dt);
+    if (zi) us2[id] = predict_face<BC, UniP, 2>(g, P, V, p, q, dt);
   }
 }
 
@@ -302,10 +305,12 @@ __global__ __launch_bounds__(256) void k_sum(Geo g, const Z* __restrict__ x, dou
 }
 
 // Jacobi preconditioner z = (1/Adiag) r (G_PCGSolver.cu:270)
-__global__ __launch_bounds__(256) void k_jacobi(Geo g, const double* __restrict__ r, double* __restrict__ z) {
+template <bool VAR>
+__global__ __launch_bounds__(256) void k_jacobi(Geo g, BetaF bf, const double* __restrict__ r,
+                                                double* __restrict__ z) {
   ROW_SETUP
   for (int i = 1 + lane; i <= g.n[0]; i += 64) {
-    Coef c = coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+    Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
     const size_t o = rb + spos(i, g.n[0]);
     z[o] = (1. / c.diag) * r[o];
   }
@@ -328,15 +333,16 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z
 }
 
 // Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124)
-__global__ __launch_bounds__(256) void k_spmv_dot(Geo g, const double* __restrict__ d, double* __restrict__ Ad,
-                                                  double* __restrict__ part0) {
+template <bool VAR>
+__global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double* __restrict__ d,
+                                                  double* __restrict__ Ad, double* __restrict__ part0) {
   ROW_SETUP
   const int nx = g.n[0];
   const size_t ypr = idx(g, 0, W(g, 1, j + 1), k), ymr = idx(g, 0, W(g, 1, j - 1), k);
   const size_t zpr = idx(g, 0, j, W(g, 2, k + 1)), zmr = idx(g, 0, j, W(g, 2, k - 1));
   double acc = 0.0;
   for (int i = 1 + lane; i <= nx; i += 64) {
-    Coef c = coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+    Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
     const int p = spos(i, nx);
     double xc = d[rb + p];
     double xp = d[rb + spos(W(g, 0, i + 1), nx)], xm = d[rb + spos(W(g, 0, i - 1), nx)];
@@ -599,7 +605,8 @@ __global__ void k_scalar_copy(double* S, int dst, int src) { S[dst] = S[src]; }
 // ------------------------------------------------------------- correction
 // k_correct_vof_velocity (CFD/G_SMACSolver.cu:1072-1142), component A (dp natural)
 template <int A>
-__global__ __launch_bounds__(256) void k_correct(Geo g, const double* __restrict__ us, const double* __restrict__ dp,
+__global__ __launch_bounds__(256) void k_correct(Geo g, const double* __restrict__ beta,
+                                                 const double* __restrict__ us, const double* __restrict__ dp,
                                                  double* __restrict__ u, double dt) {
   int lo[3] = {1, 1, 1}, hi[3] = {g.n[0], g.n[1], g.n[2]};
   lo[A] = g.per[A] ? 1 : 2;
@@ -611,7 +618,8 @@ __global__ __launch_bounds__(256) void k_correct(Geo g, const double* __restrict
   size_t id = IX(g, q);
   I3 cm = wr(g, sh(q, A, -1));
   double gr = (dp[id] - dp[IX(g, cm)]) * g.inv_d[A];
-  u[id] = us[id] + g.beta0 * dt * (0.0 - gr);
+  const double ir = beta ? beta[id] : g.beta0;  // 1/rho_f on interior faces
+  u[id] = us[id] + ir * dt * (0.0 - gr);
 }
 
 // p += dp; dp = 0 on interior cells
@@ -626,12 +634,17 @@ __global__ __launch_bounds__(256) void k_add_dp(Geo g, double* __restrict__ p, d
 // ---------------------------------------------------------------- CFL
 // k_calc_cfl (CFD/G_SMACSolver.cu:1365-1407): part0 <- row max of the convective
 // CFL, part1 <- row max of the viscous CFL (uniform).
-__global__ __launch_bounds__(256) void k_cfl(Geo g, Vel V, double dt, double visc,
-                                             double* __restrict__ part0, double* __restrict__ part1) {
+__global__ __launch_bounds__(256) void k_cfl(Geo g, Vel V, double dt, double visc, const double* __restrict__ inv_rho,
+                                             const double* __restrict__ mu, double* __restrict__ part0,
+                                             double* __restrict__ part1) {
   ROW_SETUP
   const size_t yp = idx(g, 0, W(g, 1, j + 1), k), zp = idx(g, 0, j, W(g, 2, k + 1));
-  double m0 = 0.0;
+  double m0 = 0.0, m1 = 0.0;
   for (int i = 1 + lane; i <= g.n[0]; i += 64) {
+    if (inv_rho) {  // two-phase: (mu/rho) dt sum 1/h^2 per cell
+      double cvv = inv_rho[rb + i] * mu[rb + i] * dt * (g.inv_d2[0] + g.inv_d2[1] + g.inv_d2[2]);
+      if (cvv > m1 || cvv != cvv) m1 = cvv;
+    }
     double a = fabs(V.c[0][rb + W(g, 0, i + 1)]), b = fabs(V.c[0][rb + i]);
     double mxv = a > b ? a : b;
     a = fabs(V.c[1][yp + i]);
@@ -644,9 +657,10 @@ __global__ __launch_bounds__(256) void k_cfl(Geo g, Vel V, double dt, double vis
     if (cf > m0 || cf != cf) m0 = cf;
   }
   m0 = wave_max(m0);
+  m1 = wave_max(m1);
   if (lane == 0) {
     part0[row] = m0;
-    part1[row] = visc;
+    part1[row] = inv_rho ? m1 : visc;
   }
 }
 
@@ -654,24 +668,110 @@ __global__ __launch_bounds__(256) void k_cfl(Geo g, Vel V, double dt, double vis
 static inline dim3 rows_grid(const Geo& g) { return dim3((unsigned)((g.n[1] * g.n[2] + 3) / 4)); }
 static const dim3 ROWB(256), CELLB(64, 4, 1);
 
-void launch_predict(const Geo& g, const Vel& V, const double* p, double* us[3], double dt, hipStream_t st) {
-  const int KC = 4;
-  dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4), (unsigned)((g.n[2] + KC - 1) / KC));
-  if (g.per[0] && g.per[1] && g.per[2]) {
-    hipLaunchKernelGGL(k_predict_fused<PeriodicBC>, grid, CELLB, 0, st, g, V, p, us[0], us[1], us[2], dt, KC);
-    return;
+// ------------------------------------------------- two-phase properties
+// update_properties_by_alpha (CFD/G_SMACSolver_VoF.cu:1374-1407) over the
+// whole cell box: a ghost cell takes the values of its wrapped (periodic) or
+// clamped (wall) interior cell (k_update_cell_boundary_properties_by_alpha,
+// 1244-1307), so it is computed from that cell's alpha.
+__global__ __launch_bounds__(256) void k_props_cells(Geo g, const double* __restrict__ alpha,
+                                                     double* __restrict__ rho, double* __restrict__ rho_old,
+                                                     double* __restrict__ inv_rho, double* __restrict__ mu,
+                                                     double rho0, double rho1, double mu0, double mu1, int initial) {
+  const int i = (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int k = (int)blockIdx.z;
+  if (i > g.n[0] + 1 || j > g.n[1] + 1) return;
+  int q[3] = {i, j, k}, sq[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    int v = q[d];
+    if (g.per[d]) v = wrapi(v, g.n[d]);
+    else v = v < 1 ? 1 : (v > g.n[d] ? g.n[d] : v);
+    sq[d] = v;
   }
-  // walls: one kernel per component (the fused wall kernel exceeds the register file)
+  const double al = alpha[idx(g, sq[0], sq[1], sq[2])];
+  const size_t id = idx(g, i, j, k);
+  const double r = (1. - al) * rho0 + al * rho1;
+  rho[id] = r;
+  inv_rho[id] = 1. / r;
+  mu[id] = (1. - al) * mu0 + al * mu1;
+  if (initial) rho_old[id] = r;
+}
+
+// face properties of axis A over the whole face box: interior faces
+// rho_f = (rho_L + rho_R)/2, beta = 1/rho_f, mu_f = (mu_L + mu_R)/2
+// (k_update_{x,y,z}_face_properties_by_alpha, CFD/G_SMACSolver_VoF.cu:1409-1535);
+// wall faces mu_f = inner mu, beta = 0 (update_boundary_faces, 1798-1941);
+// ghost faces beta = 0.
+template <int A>
+__global__ __launch_bounds__(256) void k_props_faces(Geo g, const double* __restrict__ rho,
+                                                     const double* __restrict__ mu, double* __restrict__ beta,
+                                                     double* __restrict__ muf) {
+  const int i = (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int k = (int)blockIdx.z;
+  if (i > g.n[0] + 2 || j > g.n[1] + 2) return;
+  const I3 q = mk(i, j, k);
+  const size_t id = IX(g, q);
+  const int t = face_type(g, A, q);
+  if (t == F_INTERIOR) {
+    const size_t cm = IX(g, wr(g, sh(q, A, -1))), cp = IX(g, wr(g, q));
+    const double rf = 0.5 * (rho[cp] + rho[cm]);
+    beta[id] = 1. / rf;
+    muf[id] = 0.5 * (mu[cp] + mu[cm]);
+  } else if (t == F_BOUNDARY) {
+    const int inner = q.v[A] == 1 ? 0 : -1;
+    muf[id] = mu[IX(g, sh(q, A, inner))];
+    beta[id] = 0.;
+  } else {
+    beta[id] = 0.;
+  }
+}
+
+void launch_props(const Geo& g, const double* alpha, double* rho, double* rho_old, double* inv_rho, double* mu,
+                  double* beta[3], double* muf[3], double rho0, double rho1, double mu0, double mu1, int initial,
+                  hipStream_t st) {
+  dim3 cg((unsigned)((g.n[0] + 2 + 63) / 64), (unsigned)((g.n[1] + 2 + 3) / 4), (unsigned)(g.n[2] + 2));
+  hipLaunchKernelGGL(k_props_cells, cg, CELLB, 0, st, g, alpha, rho, rho_old, inv_rho, mu, rho0, rho1, mu0, mu1,
+                     initial);
+  dim3 fg((unsigned)((g.n[0] + 3 + 63) / 64), (unsigned)((g.n[1] + 3 + 3) / 4), (unsigned)(g.n[2] + 3));
+  hipLaunchKernelGGL(k_props_faces<0>, fg, CELLB, 0, st, g, rho, mu, beta[0], muf[0]);
+  hipLaunchKernelGGL(k_props_faces<1>, fg, CELLB, 0, st, g, rho, mu, beta[1], muf[1]);
+  hipLaunchKernelGGL(k_props_faces<2>, fg, CELLB, 0, st, g, rho, mu, beta[2], muf[2]);
+}
+
+template <class BC, class PR>
+static void predict_components(const Geo& g, const Props& P, const Vel& V, const double* p, double* us[3],
+                               double dt, hipStream_t st) {
   for (int a = 0; a < 3; a++) {
     int lo = g.per[a] ? 1 : 2;
     int nf[3] = {g.n[0], g.n[1], g.n[2]};
     nf[a] = g.n[a] - lo + 1;
     if (nf[0] <= 0 || nf[1] <= 0 || nf[2] <= 0) continue;
     dim3 cg((unsigned)((nf[0] + 63) / 64), (unsigned)((nf[1] + 3) / 4), (unsigned)nf[2]);
-    if (a == 0) hipLaunchKernelGGL((k_predict<WallBC, 0>), cg, CELLB, 0, st, g, V, p, us[0], dt);
-    if (a == 1) hipLaunchKernelGGL((k_predict<WallBC, 1>), cg, CELLB, 0, st, g, V, p, us[1], dt);
-    if (a == 2) hipLaunchKernelGGL((k_predict<WallBC, 2>), cg, CELLB, 0, st, g, V, p, us[2], dt);
+    if (a == 0) hipLaunchKernelGGL((k_predict<BC, PR, 0>), cg, CELLB, 0, st, g, P, V, p, us[0], dt);
+    if (a == 1) hipLaunchKernelGGL((k_predict<BC, PR, 1>), cg, CELLB, 0, st, g, P, V, p, us[1], dt);
+    if (a == 2) hipLaunchKernelGGL((k_predict<BC, PR, 2>), cg, CELLB, 0, st, g, P, V, p, us[2], dt);
   }
+}
+
+void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
+                    hipStream_t st) {
+  const int KC = 4;
+  const bool allper = g.per[0] && g.per[1] && g.per[2];
+  if (P) {
+    // two-phase grids: one kernel per component, properties from the arrays
+    if (allper) predict_components<PeriodicBC, VarP>(g, *P, V, p, us, dt, st);
+    else predict_components<WallBC, VarP>(g, *P, V, p, us, dt, st);
+  } else if (allper) {
+    dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4), (unsigned)((g.n[2] + KC - 1) / KC));
+    hipLaunchKernelGGL(k_predict_fused<PeriodicBC>, grid, CELLB, 0, st, g, V, p, us[0], us[1], us[2], dt, KC);
+    return;
+  } else {
+    // walls: one kernel per component (the fused wall kernel exceeds the register file)
+    predict_components<WallBC, UniP>(g, Props{}, V, p, us, dt, st);
+  }
+  if (allper) return;
   int m = g.n[0] > g.n[1] ? g.n[0] : g.n[1];
   m = m > g.n[2] ? m : g.n[2];
   dim3 sgrid((unsigned)((m + 63) / 64), (unsigned)((m + 3) / 4), 6);
@@ -693,15 +793,17 @@ template <typename Z>
 void launch_sum(const Geo& g, const Z* x, double* p0, hipStream_t st) {
   hipLaunchKernelGGL(k_sum<Z>, rows_grid(g), ROWB, 0, st, g, x, p0);
 }
-void launch_jacobi(const Geo& g, const double* r, double* z, hipStream_t st) {
-  hipLaunchKernelGGL(k_jacobi, rows_grid(g), ROWB, 0, st, g, r, z);
+void launch_jacobi(const Geo& g, const BetaF* bf, const double* r, double* z, hipStream_t st) {
+  if (bf) hipLaunchKernelGGL(k_jacobi<true>, rows_grid(g), ROWB, 0, st, g, *bf, r, z);
+  else hipLaunchKernelGGL(k_jacobi<false>, rows_grid(g), ROWB, 0, st, g, BetaF{}, r, z);
 }
 template <typename Z>
 void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0);
 }
-void launch_spmv_dot(const Geo& g, const double* d, double* Ad, double* p0, hipStream_t st) {
-  hipLaunchKernelGGL(k_spmv_dot, rows_grid(g), ROWB, 0, st, g, d, Ad, p0);
+void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st) {
+  if (bf) hipLaunchKernelGGL(k_spmv_dot<true>, rows_grid(g), ROWB, 0, st, g, *bf, d, Ad, p0);
+  else hipLaunchKernelGGL(k_spmv_dot<false>, rows_grid(g), ROWB, 0, st, g, BetaF{}, d, Ad, p0);
 }
 template <typename F>
 void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,
@@ -763,23 +865,25 @@ void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st) {
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st) {
   hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(1), 0, st, S, dst, src);
 }
-void launch_correct(const Geo& g, double* const us[3], const double* dp, double* u[3], double dt, hipStream_t st) {
+void launch_correct(const Geo& g, const BetaF* bf, double* const us[3], const double* dp, double* u[3], double dt,
+                    hipStream_t st) {
   for (int a = 0; a < 3; a++) {
     int lo = g.per[a] ? 1 : 2;
     int nf[3] = {g.n[0], g.n[1], g.n[2]};
     nf[a] = g.n[a] - lo + 1;
     if (nf[0] <= 0 || nf[1] <= 0 || nf[2] <= 0) continue;
     dim3 grid((unsigned)((nf[0] + 63) / 64), (unsigned)((nf[1] + 3) / 4), (unsigned)nf[2]);
-    if (a == 0) hipLaunchKernelGGL(k_correct<0>, grid, CELLB, 0, st, g, us[0], dp, u[0], dt);
-    if (a == 1) hipLaunchKernelGGL(k_correct<1>, grid, CELLB, 0, st, g, us[1], dp, u[1], dt);
-    if (a == 2) hipLaunchKernelGGL(k_correct<2>, grid, CELLB, 0, st, g, us[2], dp, u[2], dt);
+    if (a == 0) hipLaunchKernelGGL(k_correct<0>, grid, CELLB, 0, st, g, bf ? bf->b[0] : nullptr, us[0], dp, u[0], dt);
+    if (a == 1) hipLaunchKernelGGL(k_correct<1>, grid, CELLB, 0, st, g, bf ? bf->b[1] : nullptr, us[1], dp, u[1], dt);
+    if (a == 2) hipLaunchKernelGGL(k_correct<2>, grid, CELLB, 0, st, g, bf ? bf->b[2] : nullptr, us[2], dp, u[2], dt);
   }
 }
 void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st) {
   hipLaunchKernelGGL(k_add_dp, rows_grid(g), ROWB, 0, st, g, p, dp);
 }
-void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, double* p0, double* p1, hipStream_t st) {
-  hipLaunchKernelGGL(k_cfl, rows_grid(g), ROWB, 0, st, g, V, dt, visc, p0, p1);
+void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, const double* inv_rho, const double* mu,
+                double* p0, double* p1, hipStream_t st) {
+  hipLaunchKernelGGL(k_cfl, rows_grid(g), ROWB, 0, st, g, V, dt, visc, inv_rho, mu, p0, p1);
 }
 
 }  // namespace smac

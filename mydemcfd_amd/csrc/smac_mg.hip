@@ -26,7 +26,27 @@ struct CoefC {
 };
 template <typename C>
 __device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k) {
-  Coef c = coef_uniform(L.n, L.per, L.inv_d2, L.beta0, i, j, k);
+  Coef c;
+  if (L.b[0]) {
+    // variable coefficients: a = beta/h^2 per face, diag = sum of all six,
+    // off-diagonals zero across walls
+    const int q[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      int qp[3] = {i, j, k};
+      qp[d] = L.per[d] ? wrapi(q[d] + 1, L.n[d]) : q[d] + 1;
+      c.a[2 * d] = L.b[d][lrow(L, qp[1], qp[2]) + qp[0]] * L.inv_d2[d];
+      c.a[2 * d + 1] = L.b[d][lrow(L, j, k) + i] * L.inv_d2[d];
+    }
+    c.diag = c.a[0] + c.a[1] + c.a[2] + c.a[3] + c.a[4] + c.a[5];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (wall_face(L.per[d], q[d] + 1, L.n[d])) c.a[2 * d] = 0.;
+      if (wall_face(L.per[d], q[d], L.n[d])) c.a[2 * d + 1] = 0.;
+    }
+  } else {
+    c = coef_uniform(L.n, L.per, L.inv_d2, L.beta0, i, j, k);
+  }
   CoefC<C> r;
 #pragma unroll
   for (int m = 0; m < 6; m++) r.a[m] = (C)c.a[m];
@@ -444,7 +464,7 @@ static void rbgs25(const Lv& L, T* e, const T* f, int col, hipStream_t st) {
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
   const int nE = L.n[0] / 2;
-  const bool tiled = !zero_start && (L.n[0] % 2 == 0) && (L.n[1] % 8 == 0) && L.n[1] >= 8 && L.n[2] >= 2 &&
+  const bool tiled = !zero_start && !L.b[0] && (L.n[0] % 2 == 0) && (L.n[1] % 8 == 0) && L.n[1] >= 8 && L.n[2] >= 2 &&
                      nE >= 16 && nE <= 512;
   if (tiled) {
     if (nE <= 64) return rbgs25<T, 1>(L, e, f, col, st);
@@ -468,7 +488,7 @@ template <typename T>
 void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
                         hipStream_t st) {
   const int nx = F.n[0];
-  const bool tiled = nx % 2 == 0 && F.n[1] % 8 == 0 && F.n[1] >= 8 && F.n[2] >= 2 && nx >= 32 && nx <= 512;
+  const bool tiled = !F.b[0] && nx % 2 == 0 && F.n[1] % 8 == 0 && F.n[1] >= 8 && F.n[2] >= 2 && nx >= 32 && nx <= 512;
   if (tiled) {
     if (nx <= 64) resid_rx25<T, 1>(F, C, e, f, tb, st);
     else if (nx <= 128) resid_rx25<T, 2>(F, C, e, f, tb, st);
@@ -489,6 +509,45 @@ void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStre
   int n = C.n[0] * C.n[1] * C.n[2];
   hipLaunchKernelGGL(k_mg_coarse<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C, ainv, f, e);
 }
+// coarse face coefficient of axis d (mg_coarsen_beta of the oracle): faces
+// off the level's face range are 0
+__global__ __launch_bounds__(256) void k_mg_coarsen_beta(Lv F, Lv C, int d, double* __restrict__ cb) {
+  const int I = (int)(blockIdx.x * 64 + threadIdx.x);
+  const int J = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int K = (int)blockIdx.z;
+  if (I >= C.px || J >= C.py) return;
+  const int q[3] = {I, J, K};
+  const size_t id = lrow(C, J, K) + I;
+  bool ok = true;
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    const int hi = C.n[e] + (e == d ? 1 : 0);
+    if (q[e] < 1 || q[e] > hi) ok = false;
+  }
+  if (!ok) {
+    cb[id] = 0.;
+    return;
+  }
+  const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+  int fq[3];
+  fq[d] = 2 * q[d] - 1;
+  double v[2][2];
+#pragma unroll
+  for (int s2 = 0; s2 < 2; s2++)
+#pragma unroll
+    for (int s1 = 0; s1 < 2; s1++) {
+      fq[t1] = 2 * q[t1] - 1 + s1;
+      fq[t2] = 2 * q[t2] - 1 + s2;
+      v[s2][s1] = F.b[d][lrow(F, fq[1], fq[2]) + fq[0]];
+    }
+  cb[id] = ((v[0][0] + v[0][1]) + (v[1][0] + v[1][1])) * 0.25;
+}
+
+void launch_mg_coarsen_beta(const Lv& F, const Lv& C, int d, double* cb, hipStream_t st) {
+  dim3 g((unsigned)((C.px + 63) / 64), (unsigned)((C.py + 3) / 4), (unsigned)(C.n[2] + 3));
+  hipLaunchKernelGGL(k_mg_coarsen_beta, g, dim3(64, 4), 0, st, F, C, d, cb);
+}
+
 template <typename T>
 void launch_fill(const Lv& L, T* x, hipStream_t st) {
   hipLaunchKernelGGL(k_fill<T>, lrows_grid(L), ROWB, 0, st, L, x);

@@ -59,8 +59,8 @@ __device__ __forceinline__ double grid_beta(const Geo& g, int d, const int f[3])
 
 // coarse face coefficients of axis d on level C (k_create_level0_coeffs when
 // fb == nullptr, else k_create_level_coeffs from level F)
-__global__ __launch_bounds__(RGB) void k_rg_beta(Geo g, RgLv C, RgLv F, const double* __restrict__ fb, int d,
-                                                 double* __restrict__ cb) {
+__global__ __launch_bounds__(RGB) void k_rg_beta(Geo g, BetaF gb, RgLv C, RgLv F, const double* __restrict__ fb,
+                                                 int d, double* __restrict__ cb) {
   int hi[3] = {C.n[0], C.n[1], C.n[2]};
   hi[d] += 1;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,7 +84,12 @@ __global__ __launch_bounds__(RGB) void k_rg_beta(Geo g, RgLv C, RgLv F, const do
         int f[3] = {B[0] + ix, B[1] + iy, B[2] + iz};
         double v;
         if (fb == nullptr) {
-          v = grid_beta(g, d, f);
+          if (gb.b[0]) {  // the grid's face coefficients (two-phase grids)
+            if (g.per[d]) f[d] = rwrap(f[d], g.n[d]);
+            v = f[d] > g.n[d] + 1 ? 0.0 : gb.b[d][idx(g, f[0], f[1], f[2])];
+          } else {
+            v = grid_beta(g, d, f);
+          }
         } else {
           if (F.per[d]) f[d] = rwrap(f[d], F.n[d]);
           v = f[d] > F.n[d] + 1 ? 0.0 : fb[ridx(F, f[0], f[1], f[2])];
@@ -190,7 +195,7 @@ __global__ __launch_bounds__(RGB) void k_rg_prolong(RgLv C, const double* __rest
 // k_jacobi_iteration_general (G_GMGSolver.cu:719-746) with the grid operator
 // of k_build_vof_poisson_A: out = (1-omega) x + omega (sum a_nb x_nb + r) / Adiag.
 // zero_start: x is known to be zero (the first sweep of a V-cycle).
-__global__ __launch_bounds__(256) void k_rg_fine_jacobi(Geo g, const double* __restrict__ x,
+__global__ __launch_bounds__(256) void k_rg_fine_jacobi(Geo g, BetaF gb, const double* __restrict__ x,
                                                         double* __restrict__ out, const double* __restrict__ r,
                                                         int zero_start) {
   ROW_SETUP
@@ -199,7 +204,7 @@ __global__ __launch_bounds__(256) void k_rg_fine_jacobi(Geo g, const double* __r
   const size_t zpr = idx(g, 0, j, W(g, 2, k + 1)), zmr = idx(g, 0, j, W(g, 2, k - 1));
   const double omega = RG_OMEGA;
   for (int i = 1 + lane; i <= nx; i += 64) {
-    const Coef c = coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+    const Coef c = gb.b[0] ? coef_var(g, gb.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
     const int p = spos(i, nx);
     double xc = 0.0, xp = 0.0, xm = 0.0, yp = 0.0, ym = 0.0, zp = 0.0, zm = 0.0;
     if (!zero_start) {
@@ -217,7 +222,7 @@ __global__ __launch_bounds__(256) void k_rg_fine_jacobi(Geo g, const double* __r
 }
 
 // k_get_res_general + k_restrict_grid_to_level0 (G_GMGSolver.cu:191-226,416-455)
-__global__ __launch_bounds__(RGB) void k_rg_fine_res_restrict(Geo g, const double* __restrict__ x,
+__global__ __launch_bounds__(RGB) void k_rg_fine_res_restrict(Geo g, BetaF gb, const double* __restrict__ x,
                                                               const double* __restrict__ r, RgLv C,
                                                               double* __restrict__ crhs) {
   int i, j, k;
@@ -235,7 +240,8 @@ __global__ __launch_bounds__(RGB) void k_rg_fine_res_restrict(Geo g, const doubl
     for (int iy = 0; iy < R[1]; iy++)
       for (int ix = 0; ix < R[0]; ix++) {
         const int fi = B[0] + ix, fj = B[1] + iy, fk = B[2] + iz;
-        const Coef c = coef_uniform(g.n, g.per, g.inv_d2, g.beta0, fi, fj, fk);
+        const Coef c =
+            gb.b[0] ? coef_var(g, gb.b, fi, fj, fk) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, fi, fj, fk);
         const size_t rb = idx(g, 0, fj, fk);
         const int p = spos(fi, nx);
         const double xc = x[rb + p];
@@ -307,10 +313,11 @@ dim3 rows_grid_g(const Geo& g) { return dim3((unsigned)((g.n[1] * g.n[2] + 3) / 
 
 }  // namespace
 
-void launch_rg_beta(const Geo& g, const RgLv& C, const RgLv& F, const double* fb, int d, double* cb,
-                    hipStream_t st) {
+void launch_rg_beta(const Geo& g, const BetaF* gb, const RgLv& C, const RgLv& F, const double* fb, int d,
+                    double* cb, hipStream_t st) {
   size_t n = (size_t)(C.n[0] + (d == 0)) * (size_t)(C.n[1] + (d == 1)) * (size_t)(C.n[2] + (d == 2));
-  hipLaunchKernelGGL(k_rg_beta, dim3((unsigned)((n + RGB - 1) / RGB)), dim3(RGB), 0, st, g, C, F, fb, d, cb);
+  hipLaunchKernelGGL(k_rg_beta, dim3((unsigned)((n + RGB - 1) / RGB)), dim3(RGB), 0, st, g, gb ? *gb : BetaF{}, C,
+                     F, fb, d, cb);
 }
 void launch_rg_jacobi(const RgLv& L, double* const b[3], const double* q, double* qt, const double* rhs,
                       hipStream_t st) {
@@ -323,13 +330,15 @@ void launch_rg_res_restrict(const RgLv& F, double* const b[3], const double* q, 
 void launch_rg_prolong(const RgLv& C, const double* cq, const RgLv& F, double* fq, hipStream_t st) {
   hipLaunchKernelGGL(k_rg_prolong, cells_grid(C), dim3(RGB), 0, st, C, cq, F, fq);
 }
-void launch_rg_fine_jacobi(const Geo& g, const double* x, double* out, const double* r, int zero_start,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(k_rg_fine_jacobi, rows_grid_g(g), dim3(256), 0, st, g, x, out, r, zero_start);
+void launch_rg_fine_jacobi(const Geo& g, const BetaF* gb, const double* x, double* out, const double* r,
+                           int zero_start, hipStream_t st) {
+  hipLaunchKernelGGL(k_rg_fine_jacobi, rows_grid_g(g), dim3(256), 0, st, g, gb ? *gb : BetaF{}, x, out, r,
+                     zero_start);
 }
-void launch_rg_fine_res_restrict(const Geo& g, const double* x, const double* r, const RgLv& C, double* crhs,
-                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_rg_fine_res_restrict, cells_grid(C), dim3(RGB), 0, st, g, x, r, C, crhs);
+void launch_rg_fine_res_restrict(const Geo& g, const BetaF* gb, const double* x, const double* r, const RgLv& C,
+                                 double* crhs, hipStream_t st) {
+  hipLaunchKernelGGL(k_rg_fine_res_restrict, cells_grid(C), dim3(RGB), 0, st, g, gb ? *gb : BetaF{}, x, r, C,
+                     crhs);
 }
 void launch_rg_prolong_fine(const RgLv& C, const double* cq, const Geo& g, double* x, hipStream_t st) {
   hipLaunchKernelGGL(k_rg_prolong_fine, cells_grid(C), dim3(RGB), 0, st, C, cq, g, x);

@@ -458,3 +458,107 @@ def test_ref_gmg_pcg_solves_the_same_system(oracle_mod):
         for a, b in zip(out[precond][:3], out["jacobi"][:3]):
             assert np.max(np.abs(a - b)) < 1e-9 * scale, precond
         assert rel_err_mod_mean(out[precond][3], out["jacobi"][3], n) < 1e-9, precond
+
+
+# ------------------------------------------------- two-phase (variable rho)
+def _stratified(oracle_mod, n=(8, 8, 16), rho=(1.0, 1000.0), mu=(1e-5, 1e-3)):
+    """Closed box, heavy phase in the lower half (alpha depends on z only)."""
+    desc = cases.dam_break_desc(n, length=(0.5, 0.5, 1.0), rho=rho, mu=mu)
+    c = oracle_mod.OracleContext(**desc)
+    a = np.zeros(c.shape("alpha"))
+    a[: n[2] // 2 + 1] = 1.0  # z cells 0..n/2 (ghost 0 included)
+    c.upload("alpha", a)
+    return c, desc
+
+
+def test_two_phase_hydrostatic_balance(oracle_mod):
+    """Fluid at rest under gravity with density varying along z only: the
+    predictor gives u* = dt g on the interior z-faces, and the projection with
+    beta = 1/rho_f must cancel it exactly, leaving u = 0 and the hydrostatic
+    pressure p_k - p_(k-1) = rho_f g h, rho_f = (rho_k + rho_(k-1))/2."""
+    n = (8, 8, 16)
+    c, desc = _stratified(oracle_mod, n)
+    dt = 1e-3
+    st = c.step(dt, precond="mg", tol=1e-13, max_iter=500)
+    assert st.converged
+    gz = desc["g"][2]
+    for f in ("u", "v", "w"):
+        assert np.max(np.abs(c.download(f))) < 1e-10 * dt * abs(gz), f
+    p = interior(c.download("p"), n)
+    h = 1.0 / n[2]
+    rho = np.where(np.arange(1, n[2] + 1) <= n[2] // 2, desc["rho"][1], desc["rho"][0])
+    rho_f = 0.5 * (rho[1:] + rho[:-1])
+    dpz = p[1:] - p[:-1]                         # (nz-1, ny, nx)
+    expect = (rho_f * gz * h)[:, None, None]
+    assert np.max(np.abs(dpz - expect)) < 1e-8 * np.max(np.abs(expect))
+
+
+def test_two_phase_operator_symmetric_nullspace(oracle_mod):
+    """The variable-coefficient operator keeps A 1 = 0 and symmetry."""
+    n = (8, 8, 16)
+    c, _ = _stratified(oracle_mod, n)
+    c.predictor(1e-3)
+    one = np.zeros(c.shape("p"))
+    interior(one, n)[...] = 1.0
+    assert np.max(np.abs(interior(c.apply_A(one), n))) < 1e-9
+    rng = np.random.default_rng(3)
+    x = np.zeros(c.shape("p"))
+    y = np.zeros(c.shape("p"))
+    interior(x, n)[...] = rng.standard_normal(n[::-1])
+    interior(y, n)[...] = rng.standard_normal(n[::-1])
+    Ax, Ay = interior(c.apply_A(x), n), interior(c.apply_A(y), n)
+    l, r = np.sum(Ax * interior(y, n)), np.sum(interior(x, n) * Ay)
+    assert abs(l - r) <= 1e-12 * np.sqrt(np.sum(Ax * Ax) * np.sum(interior(y, n) ** 2))
+    # the heavy phase (beta = 1/1000) gives the small coefficients
+    e = np.zeros(c.shape("p"))
+    e[2, 4, 4] = 1.0  # heavy cell (z = 2)
+    d_heavy = interior(c.apply_A(e), n)[1, 3, 3]
+    e[:] = 0.0
+    e[12, 4, 4] = 1.0  # light cell
+    d_light = interior(c.apply_A(e), n)[11, 3, 3]
+    h2 = (0.5 / 8) ** -2
+    assert d_heavy == pytest.approx(6 * h2 / 1000.0, rel=1e-12)
+    assert d_light == pytest.approx(6 * h2 / 1.0, rel=1e-12)
+
+
+@pytest.mark.parametrize("precond", ["mg", "ref_gmg"])
+def test_two_phase_vcycle_symmetric_positive(oracle_mod, precond):
+    n = (32, 16, 32)
+    desc = cases.dam_break_desc(n, length=(1.0, 0.5, 1.0))
+    c = oracle_mod.OracleContext(**desc)
+    c.upload("alpha", cases.dam_break_alpha(n, length=(1.0, 0.5, 1.0)))
+    c.predictor(1e-3)
+    rng = np.random.default_rng(13)
+    xs = []
+    for _ in range(2):
+        v = np.zeros(c.shape("p"))
+        w = rng.standard_normal(n[::-1])
+        interior(v, n)[...] = w - w.mean()
+        xs.append(v)
+    kw = dict(precond=precond, mg_levels=2) if precond == "ref_gmg" else dict(precond=precond)
+    Mx = interior(c.apply_mg(xs[0], **kw), n)
+    My = interior(c.apply_mg(xs[1], **kw), n)
+    x, y = interior(xs[0], n), interior(xs[1], n)
+    l, r = np.sum(Mx * y), np.sum(x * My)
+    assert abs(l - r) <= 1e-10 * np.sqrt(np.sum(Mx * Mx) * np.sum(y * y))
+    assert np.sum(Mx * x) > 0 and np.sum(My * y) > 0
+
+
+def test_two_phase_pcg_variants_agree(oracle_mod):
+    """Jacobi-, RBGS-multigrid- and reference-GMG-preconditioned CG on the
+    dam-break column give the same velocities and pressure to the tolerance."""
+    n = (32, 16, 16)
+    desc = cases.dam_break_desc(n)
+    out = {}
+    for precond in ("jacobi", "mg", "ref_gmg"):
+        c = oracle_mod.OracleContext(**desc)
+        c.upload("alpha", cases.dam_break_alpha(n))
+        for _ in range(2):
+            st = c.step(1e-3, precond=precond, tol=1e-13, max_iter=5000)
+            assert st.converged, precond
+        out[precond] = [c.download(f) for f in ("u", "v", "w", "p")]
+    scale = max(np.max(np.abs(out["jacobi"][i])) for i in range(3))
+    for precond in ("mg", "ref_gmg"):
+        for a, b in zip(out[precond][:3], out["jacobi"][:3]):
+            assert np.max(np.abs(a - b)) < 1e-8 * scale, precond
+        assert rel_err_mod_mean(out[precond][3], out["jacobi"][3], n) < 1e-8, precond
