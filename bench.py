@@ -243,8 +243,8 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded design is described "
-              f"in DESIGN.md and is not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU (z-slab) decomposition "
+              f"is a design, not built here.", file=sys.stderr)
         sys.exit(2)
     import torch
 

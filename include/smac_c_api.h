@@ -69,7 +69,7 @@ enum smac_field {
     SMAC_FIELD_COUNT = 10
 };
 
-/* PCG_TYPES (CFD/Enums.h:28-33) plus the north-star multigrid. */
+/* PCG_TYPES (CFD/Enums.h:28-33) plus the red-black Gauss-Seidel multigrid. */
 enum smac_precond {
     SMAC_PRECOND_JACOBI = 0,     /* STANDARD_PCG / PURENEUMANN_STANDARD_PCG  */
     SMAC_PRECOND_MG = 1,         /* RBGS V-cycle geometric multigrid          */
@@ -171,6 +171,7 @@ int smac_download_device(smac_ctx* ctx, int32_t field, double* dev, size_t count
 /* Element count of a field in the reference layout. */
 size_t smac_field_count(const smac_ctx* ctx, int32_t field);
 
+
 /* ---- the step ---------------------------------------------------------------
  * Property assembly from alpha: G_SMACSolver::update_properties_by_alpha +
  * update_boundary_faces (CFD/G_SMACSolver_VoF.cu:1544-1566,1944-1950), the t=0
@@ -199,7 +200,22 @@ int smac_step(smac_ctx* ctx, double dt, const smac_solver_opts* opts, smac_solve
  * CFL numbers for time step dt. */
 int smac_calc_cfl(smac_ctx* ctx, double dt, double* cfl_conv, double* cfl_visc);
 
-/* Fill opts with the defaults of the north-star solver (MG, FP64, REL_L2, 1e-8). */
+/* Adjoint (vector-Jacobian product) of one smac_step with time step dt taken
+ * from the velocity u_n (three device arrays, x/y/z faces, reference layout):
+ * on entry ubar[3] (faces) and pbar (cells), device arrays in the reference
+ * layout, hold the cotangents of the step's output (u^{n+1}, p^{n+1}); on
+ * return they hold the cotangents of its input (u^n, p^n). The predictor is
+ * transposed at u_n (upwind side and minmod branch of the forward), the
+ * Poisson solve is repeated with opts (the operator is self-adjoint) and its
+ * statistics go to stats. Entries that are not state (ghosts, periodic images)
+ * are ignored on entry and zero on return. The context's velocity is
+ * overwritten with u_n; two-phase properties are those of the context.
+ * No counterpart in the reference (it has no adjoint); it serves gradient-based
+ * callers through a torch.autograd.Function (mydemcfd_amd/autograd.py). */
+int smac_adjoint_step(smac_ctx* ctx, double dt, const smac_solver_opts* opts, const double* const u_n[3],
+                      double* const ubar[3], double* pbar, smac_solve_stats* stats);
+
+/* Fill opts with the defaults (RBGS multigrid, FP64, REL_L2, 1e-8). */
 void smac_default_opts(smac_solver_opts* opts);
 /* Device-time profiling of the step stages and of named kernels (hipEvents). */
 int smac_set_profiling(smac_ctx* ctx, int32_t on);

@@ -88,6 +88,8 @@ SIGNATURES = {
     "smac_correct": (C.c_int, [_P, C.c_double]),
     "smac_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "smac_calc_cfl": (C.c_int, [_P, C.c_double, _D, _D]),
+    "smac_adjoint_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(_P * 3), C.POINTER(_P * 3), _P,
+                                    C.POINTER(SolveStats)]),
     "smac_default_opts": (None, [C.POINTER(SolverOpts)]),
     "smac_set_profiling": (C.c_int, [_P, C.c_int32]),
     "smac_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), _D]),
@@ -234,6 +236,18 @@ class Context:
         o = opts if opts is not None else default_opts(self.lib, **kw)
         st = SolveStats()
         self._check(self.lib.smac_step(self.h, dt, C.byref(o), C.byref(st)), "smac_step")
+        return st
+
+    def adjoint_step(self, dt, u_n, ubar, pbar, opts=None, **kw):
+        """Adjoint of one step from the state u_n (three device pointers, the
+        reference layout): ubar (three device pointers) and pbar (one) hold the
+        cotangents of the step's output on entry and of its input on return."""
+        o = opts if opts is not None else default_opts(self.lib, **kw)
+        st = SolveStats()
+        un = (_P * 3)(*[int(x) for x in u_n])
+        ub = (_P * 3)(*[int(x) for x in ubar])
+        self._check(self.lib.smac_adjoint_step(self.h, dt, C.byref(o), C.byref(un), C.byref(ub), int(pbar),
+                                               C.byref(st)), "smac_adjoint_step")
         return st
 
     def calc_cfl(self, dt):

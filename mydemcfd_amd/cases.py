@@ -215,3 +215,28 @@ def dam_break_alpha(n=(256, 128, 128), length=(2.0, 1.0, 1.0), x_front=0.25, hei
     col = (xc[None, :] < x_front) & (zc[:, None] < height)  # (z, x)
     a = np.repeat(col[:, None, :], ny + 2, axis=1).astype(np.float64)
     return a
+
+
+def adjoint_fields(n, seed=0x5EED0004, kmax=32):
+    """Random-phase solenoidal k^-5/3 field with unit RMS on the periodic
+    [0,2pi)^3 box (|k| <= kmax), reference-layout (u, v, w): the initial
+    velocity of the gradient checks (Re = 500, dt = 2e-3)."""
+    u, v, w = spectral_solenoidal(n, seed, kmax=min(kmax, n // 2))
+    return (_embed_periodic(u, _abi.FIELD_U), _embed_periodic(v, _abi.FIELD_V), _embed_periodic(w, _abi.FIELD_W))
+
+
+def state_mask(n, periodic, field):
+    """Boolean mask (reference layout) of the entries of `field` that are state:
+    interior cells, faces 1..N+1 along a wall axis of their own direction and
+    1..N along a periodic one; ghosts and periodic images are not state."""
+    fid = _abi.FIELD_NAMES[field] if isinstance(field, str) else field
+    shape = _abi.field_shape(n, fid)
+    axis = {_abi.FIELD_U: 0, _abi.FIELD_V: 1, _abi.FIELD_W: 2, _abi.FIELD_USTAR: 0, _abi.FIELD_VSTAR: 1,
+            _abi.FIELD_WSTAR: 2}.get(fid, -1)
+    sl = []
+    for d in (2, 1, 0):  # numpy axes z, y, x
+        hi = n[d] + (1 if (d == axis and not periodic[d]) else 0)
+        sl.append(slice(1, hi + 1))
+    m = np.zeros(shape, dtype=bool)
+    m[tuple(sl)] = True
+    return m

@@ -85,6 +85,9 @@ struct smac_ctx {
   uint64_t props_ver = 1;     // bumped whenever the face coefficients may change
   uint64_t mg_ver = ~0ull;    // props_ver the multigrid hierarchy was built for
   std::vector<double*> mg_b;  // coarse-level face coefficients, 3 per level >= 1
+  // adjoint work arrays (box layout, allocated on first use): cotangents of
+  // u^{n+1}, u*, u^n (3 faces each), of p^{n+1} and of p^n
+  double* adj[11] = {nullptr};
   double* rgt = nullptr;  // fine ping-pong partner of z in the reference V-cycle
   bool prof = false;
   hipEvent_t ev[8];
@@ -360,6 +363,8 @@ extern "C" void smac_destroy(smac_ctx* c) {
     if (a) (void)hipFree(a);
   for (int m = 0; m < 5; m++)
     if (c->part[m]) (void)hipFree(c->part[m]);
+  for (int m = 0; m < 11; m++)
+    if (c->adj[m]) (void)hipFree(c->adj[m]);
   if (c->Sh) (void)hipHostFree(c->Sh);
   if (c->ev_ok)
     for (int e = 0; e < 8; e++) (void)hipEventDestroy(c->ev[e]);
@@ -953,7 +958,7 @@ extern "C" void smac_default_opts(smac_solver_opts* o) {
 // on r directly; otherwise the fine-level rhs f0 = (T) r is refreshed by the
 // update kernel.
 template <typename T>
-static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve_stats* st, int L) {
+static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve_stats* st, int L, bool rhs_given) {
   const Geo& g = c->g;
   double Ncell = (double)c->n[0] * (double)c->n[1] * (double)c->n[2];
   int pure = pure_neumann(c);
@@ -979,7 +984,8 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   hipStream_t s = c->st;
   double** P = c->part;
 
-  KT(c, "rhs", launch_rhs(g, c->us, c->rhs, inv_dt, P[0], s));
+  if (rhs_given) launch_rowsum_nat(g, c->rhs, P[0], s);  // b supplied in c->rhs (adjoint solve)
+  else KT(c, "rhs", launch_rhs(g, c->us, c->rhs, inv_dt, P[0], s));
   if (pure) {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_SUMB);
@@ -1132,7 +1138,15 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   return SMAC_OK;
 }
 
+static int pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st,
+                          bool rhs_given);
+
 extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st) {
+  return pressure_solve(c, dt, opts, st, false);
+}
+
+static int pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st,
+                          bool rhs_given) {
   if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
   smac_solver_opts o;
   if (opts) o = *opts;
@@ -1160,9 +1174,9 @@ extern "C" int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opt
     if (L < 0) return L;
   }
   switch (o.precision) {
-    case SMAC_PREC_MIXED_FP32: return solve_t<float>(c, dt, o, st, L);
-    case SMAC_PREC_MIXED_BF16: return solve_t<bf16s>(c, dt, o, st, L);
-    default: return solve_t<double>(c, dt, o, st, L);
+    case SMAC_PREC_MIXED_FP32: return solve_t<float>(c, dt, o, st, L, rhs_given);
+    case SMAC_PREC_MIXED_BF16: return solve_t<bf16s>(c, dt, o, st, L, rhs_given);
+    default: return solve_t<double>(c, dt, o, st, L, rhs_given);
   }
 }
 
@@ -1202,6 +1216,58 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
   sp->ms_solve = t_solve;
   sp->ms_correct = t_corr;
   return rc;
+}
+
+// Adjoint of one smac_step (see smac_adjoint.hip for the derivation).
+extern "C" int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts* opts, const double* const u_n[3],
+                                 double* const ubar[3], double* pbar, smac_solve_stats* st) {
+  if (!c || !(dt > 0) || !u_n || !ubar || !pbar) return SMAC_ERR_INVALID_ARG;
+  for (int a = 0; a < 3; a++)
+    if (!u_n[a] || !ubar[a]) return SMAC_ERR_INVALID_ARG;
+  int rc = ensure_props(c);
+  if (rc) return rc;
+  for (int m = 0; m < 11 && !rc; m++)
+    if (!c->adj[m]) rc = dalloc(&c->adj[m], c->tot);
+  if (rc) return rc;
+  double* ubn[3] = {c->adj[0], c->adj[1], c->adj[2]};
+  double* ubs[3] = {c->adj[3], c->adj[4], c->adj[5]};
+  double* un[3] = {c->adj[6], c->adj[7], c->adj[8]};
+  double* pb = c->adj[9];
+  double* pbo = c->adj[10];
+  const Geo& g = c->g;
+  hipStream_t s = c->st;
+  // the state of the step and the cotangents of its output, in the box layout
+  for (int a = 0; a < 3; a++) {
+    rc = copy_field(c, SMAC_FIELD_U + a, c->u[a], u_n[a], true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+    if (!rc) rc = copy_field(c, SMAC_FIELD_U + a, ubn[a], ubar[a], true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  rc = copy_field(c, SMAC_FIELD_P, pb, pbar, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+  if (rc) return rc;
+  const bool var = var_props(c);
+  const BetaF bf = beta_of(c);
+  const BetaF* bfp = var ? &bf : nullptr;
+  // cotangent of dp: pb' + dt D_i(beta ub'), the right-hand side of the adjoint solve
+  KT(c, "adj_div", launch_adj_div(g, bfp, ubn, pb, c->rhs, dt, s));
+  launch_rg_zero(c->dp, c->tot, s);
+  rc = pressure_solve(c, dt, opts, st, true);
+  if (rc) return rc;
+  // ub* = ub' + (1/dt) G bb on interior faces; wall faces pass ub' to u^n
+  KT(c, "adj_ustar", launch_adj_ustar(g, ubn, c->dp, ubs, un, 1. / dt, s));
+  // pb = pb' + dt D_i(beta ub*) (the old-pressure gradient of the predictor)
+  KT(c, "adj_div", launch_adj_div(g, bfp, ubs, pb, pbo, dt, s));
+  // ub += (dF/du)^T ub*
+  Vel V;
+  for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
+  const Props P = props_of(c);
+  KT(c, "predict_vjp", launch_predict_vjp(g, var ? &P : nullptr, V, ubs, un, dt, s));
+  launch_rg_zero(c->dp, c->tot, s);
+  HIPOK(hipGetLastError());
+  for (int a = 0; a < 3; a++) {
+    rc = copy_field(c, SMAC_FIELD_U + a, nullptr, nullptr, false, hipMemcpyDeviceToDevice, ubar[a], un[a]);
+    if (rc) return rc;
+  }
+  return copy_field(c, SMAC_FIELD_P, nullptr, nullptr, false, hipMemcpyDeviceToDevice, pbar, pbo);
 }
 
 extern "C" int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {

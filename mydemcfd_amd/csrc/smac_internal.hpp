@@ -144,4 +144,13 @@ void launch_rg_zero(double* x, size_t n, hipStream_t st);
 void launch_add_scalar_norms(const Geo& g, double* r, double a, const double* S, int slot, double* p0, double* p1,
                              hipStream_t st);
 
+// adjoint (smac_adjoint.hip)
+void launch_predict_vjp(const Geo& g, const Props* P, const Vel& V, double* const wbar[3], double* ub[3], double dt,
+                        hipStream_t st);
+void launch_adj_div(const Geo& g, const BetaF* bf, double* const ub[3], const double* pin, double* out, double s,
+                    hipStream_t st);
+void launch_adj_ustar(const Geo& g, double* const ubn[3], const double* bb, double* ubs[3], double* un[3],
+                      double inv_dt, hipStream_t st);
+void launch_rowsum_nat(const Geo& g, const double* x, double* p0, hipStream_t st);
+
 }  // namespace smac

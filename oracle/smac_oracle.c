@@ -1368,6 +1368,14 @@ int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac_solve_stat
     return smac_correct(c, dt);
 }
 
+/* The oracle restates the reference, which has no adjoint: the adjoint is
+ * pinned by central finite differences of the forward step instead. */
+int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts* o, const double* const u_n[3],
+                      double* const ubar[3], double* pbar, smac_solve_stats* st) {
+    (void)c; (void)dt; (void)o; (void)u_n; (void)ubar; (void)pbar; (void)st;
+    return SMAC_ERR_UNSUPPORTED;
+}
+
 /* k_calc_cfl + calc_cfl (CFD/G_SMACSolver.cu:1365-1430) */
 int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
     if (!c) return SMAC_ERR_INVALID_ARG;

@@ -1,0 +1,207 @@
+"""Adjoint of the SMAC step (smac_adjoint_step, mydemcfd_amd/autograd.py).
+
+The reference has no adjoint, so the oracle here is the mathematics:
+  * the transpose identity of one step, <w, dF(u)[d]> = <dF(u)^T w, d>, with the
+    left side from central differences of the forward step (FP64, pressure
+    solved to 1e-13);
+  * the gradient of the final kinetic energy of a rollout against central
+    finite differences along seeded random directions;
+  * the mixed-precision adjoint against the FP64 adjoint.
+Finite differences are only meaningful where the forward is smooth: the
+upwind side and the minmod branch are piecewise choices, so the perturbation
+eps is small enough that (for the seeded states) no face changes branch.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(precond="mg", stop_rule="rel_l2", tol=1e-13, max_iter=400)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _state(ctx, fields, device="cuda"):
+    torch = _torch()
+    out = {}
+    for f in ("u", "v", "w", "p"):
+        a = fields.get(f)
+        if a is None:
+            a = np.zeros(ctx.shape(f))
+        out[f] = torch.tensor(a, dtype=torch.float64, device=device).contiguous()
+    return out
+
+
+def _masks(ctx, desc, torch):
+    from mydemcfd_amd import cases
+
+    return {f: torch.tensor(cases.state_mask(desc["n"], desc["periodic"], f), device="cuda")
+            for f in ("u", "v", "w", "p")}
+
+
+def _step_out(ctx, st, dt, opts, nsteps=1):
+    """Run nsteps from state st; return the new state tensors."""
+    torch = _torch()
+    torch.cuda.synchronize()
+    for f, t in st.items():
+        ctx.upload_device(f, t.data_ptr(), t.numel())
+    for _ in range(nsteps):
+        s = ctx.step(dt, opts)
+        assert s.converged, s.as_dict()
+    out = {f: torch.empty_like(t) for f, t in st.items()}
+    for f, t in out.items():
+        ctx.download_device(f, t.data_ptr(), t.numel())
+    torch.cuda.synchronize()
+    return out
+
+
+def _dot(a, b, masks):
+    return sum(float((a[f] * b[f])[masks[f]].sum()) for f in a)
+
+
+def _setup(kind, hip):
+    from mydemcfd_amd import cases
+
+    rng = np.random.default_rng(7)
+    if kind == "periodic":
+        n = 16
+        desc = cases.periodic_desc(n, re=500.0)
+        u, v, w = cases.adjoint_fields(n, kmax=8)
+        fields = {"u": u, "v": v, "w": w}
+        dt = 2e-3
+    elif kind == "cavity":
+        desc = cases.cavity_desc(12, re=100.0)
+        ctx0 = hip.SMACContext(**desc)
+        fields = {f: 0.3 * rng.standard_normal(ctx0.shape(f)) for f in ("u", "v", "w")}
+        del ctx0
+        dt = 1e-3
+    else:  # two-phase column
+        n = (16, 8, 8)
+        desc = cases.dam_break_desc(n, length=(2.0, 1.0, 1.0))
+        ctx0 = hip.SMACContext(**desc)
+        fields = {f: 0.05 * rng.standard_normal(ctx0.shape(f)) for f in ("u", "v", "w")}
+        fields["alpha"] = cases.dam_break_alpha(n)
+        del ctx0
+        dt = 1e-3
+    ctx = hip.SMACContext(**desc)
+    if "alpha" in fields:
+        ctx.upload("alpha", fields.pop("alpha"))
+    return ctx, desc, fields, dt
+
+
+@pytest.mark.parametrize("kind", ["periodic", "cavity", "two_phase"])
+def test_adjoint_transpose_identity_one_step(hip, kind):
+    torch = _torch()
+    print(f"case: adjoint transpose identity, {kind}", flush=True)
+    ctx, desc, fields, dt = _setup(kind, hip)
+    masks = _masks(ctx, desc, torch)
+    opts = ctx.opts(**TIGHT)
+    st0 = _state(ctx, fields)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(11)
+    d = {f: torch.randn(t.shape, generator=gen, device="cuda", dtype=torch.float64) * masks[f]
+         for f, t in st0.items()}
+    wv = {f: torch.randn(t.shape, generator=gen, device="cuda", dtype=torch.float64) * masks[f]
+          for f, t in st0.items()}
+    eps = 1e-6
+    plus = _step_out(ctx, {f: st0[f] + eps * d[f] for f in st0}, dt, opts)
+    minus = _step_out(ctx, {f: st0[f] - eps * d[f] for f in st0}, dt, opts)
+    lhs = (_dot(wv, plus, masks) - _dot(wv, minus, masks)) / (2 * eps)
+    # adjoint
+    _step_out(ctx, st0, dt, opts, nsteps=0)
+    u_n = [st0[f] for f in ("u", "v", "w")]
+    ub = [wv[f].clone() for f in ("u", "v", "w")]
+    pb = wv["p"].clone()
+    torch.cuda.synchronize()
+    s = ctx.adjoint_step(dt, [t.data_ptr() for t in u_n], [t.data_ptr() for t in ub], pb.data_ptr(), opts)
+    assert s.converged, s.as_dict()
+    torch.cuda.synchronize()
+    grad = {"u": ub[0], "v": ub[1], "w": ub[2], "p": pb}
+    for f in grad:  # non-state entries are zero on return
+        assert float(grad[f][~masks[f]].abs().max()) == 0.0, f
+    rhs = _dot(grad, d, masks)
+    print(f"  <w, J d> = {lhs:.12e}  <J^T w, d> = {rhs:.12e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-6 * abs(rhs), (lhs, rhs)
+
+
+def _energy(out, masks, h3):
+    return 0.5 * h3 * sum(float((out[f] ** 2)[masks[f]].sum()) for f in ("u", "v", "w"))
+
+
+def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
+    """Gradient of J = 1/2 sum |u_N|^2 dV with respect to u_0 through the
+    autograd binding, against central differences along ndir directions."""
+    torch = _torch()
+    from mydemcfd_amd import autograd, cases
+
+    desc = cases.periodic_desc(n, re=500.0)
+    ctx = hip.SMACContext(**desc)
+    masks = _masks(ctx, desc, torch)
+    u, v, w = cases.adjoint_fields(n)
+    st0 = _state(ctx, {"u": u, "v": v, "w": w})
+    dt = 2e-3
+    opts = ctx.opts(precond="mg", stop_rule="rel_l2", tol=1e-13, max_iter=400, precision=precision,
+                    flexible=0 if precision == "fp64" else 1)
+    h3 = (2 * math.pi / n) ** 3
+    x = {f: t.clone().requires_grad_(f != "p") for f, t in st0.items()}
+    out = autograd.rollout(ctx, x["u"], x["v"], x["w"], x["p"], dt, nsteps, opts)
+    J = 0.5 * h3 * sum(((o ** 2) * masks[f]).sum() for o, f in zip(out[:3], ("u", "v", "w")))
+    J.backward()
+    grad = {f: x[f].grad for f in ("u", "v", "w")}
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0x5EED0014)
+    fd_opts = ctx.opts(**TIGHT)
+    errs = []
+    for _ in range(ndir):
+        d = {f: torch.randn(st0[f].shape, generator=gen, device="cuda", dtype=torch.float64) * masks[f]
+             for f in ("u", "v", "w")}
+        d["p"] = torch.zeros_like(st0["p"])
+        eps = 1e-6
+        jp = _energy(_step_out(ctx, {f: st0[f] + eps * d[f] for f in st0}, dt, fd_opts, nsteps), masks, h3)
+        jm = _energy(_step_out(ctx, {f: st0[f] - eps * d[f] for f in st0}, dt, fd_opts, nsteps), masks, h3)
+        fd = (jp - jm) / (2 * eps)
+        ad = sum(float((grad[f] * d[f]).sum()) for f in ("u", "v", "w"))
+        errs.append(abs(fd - ad) / abs(fd))
+        print(f"  direction: FD {fd:.10e} adjoint {ad:.10e} rel {errs[-1]:.2e}", flush=True)
+    assert max(errs) <= tol_rel, errs
+    return grad
+
+
+def test_rollout_gradient_vs_finite_differences(hip):
+    print("case: rollout gradient, periodic 32^3 x4 steps, 4 directions", flush=True)
+    _rollout_grad_check(hip, 32, 4, 4)
+
+
+def test_mixed_adjoint_vs_fp64(hip):
+    """The adjoint with FP32 multigrid storage gives the FP64 gradient to the
+    solver tolerance."""
+    torch = _torch()
+    from mydemcfd_amd import autograd, cases
+
+    print("case: mixed vs FP64 adjoint gradient, periodic 32^3 x3", flush=True)
+    n = 32
+    desc = cases.periodic_desc(n, re=500.0)
+    grads = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        ctx = hip.SMACContext(**desc)
+        masks = _masks(ctx, desc, torch)
+        u, v, w = cases.adjoint_fields(n)
+        st0 = _state(ctx, {"u": u, "v": v, "w": w})
+        opts = ctx.opts(precond="mg", stop_rule="rel_l2", tol=1e-12, max_iter=400, precision=prec,
+                        flexible=0 if prec == "fp64" else 1)
+        x = {f: t.clone().requires_grad_(f != "p") for f, t in st0.items()}
+        out = autograd.rollout(ctx, x["u"], x["v"], x["w"], x["p"], 2e-3, 3, opts)
+        J = 0.5 * sum(((o ** 2) * masks[f]).sum() for o, f in zip(out[:3], ("u", "v", "w")))
+        J.backward()
+        grads[prec] = torch.cat([x[f].grad[masks[f]] for f in ("u", "v", "w")])
+    ref = grads["fp64"]
+    for prec in ("fp32", "bf16"):
+        err = float((grads[prec] - ref).abs().max() / ref.abs().max())
+        print(f"  {prec}: max rel diff vs FP64 gradient {err:.2e}", flush=True)
+        assert err <= 1e-6, (prec, err)

@@ -1,6 +1,6 @@
 """GPU (HIP, gfx950) vs CPU FP64 oracle: the FP64 form of the SMAC step.
 
-Bar (BASELINE north star): max|a-b|/max|b| <= 1e-10 on u, v, w and p after N
+Bar: max|a-b|/max|b| <= 1e-10 on u, v, w and p after N
 steps on identical seeded inputs (p compared modulo its mean). Both sides
 share the arithmetic contract (no FMA contraction, canonical reduction
 order), so in practice the fields agree exactly.
