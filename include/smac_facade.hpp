@@ -36,6 +36,42 @@ inline void check(int s, const char* what) {
   if (s != SMAC_OK) throw smac_error(s, what);
 }
 
+// CFDTime (CFD/CFDTime.h, CFD/CFDTime.cpp:22-48): the CFL time-step rule of the
+// reference loop. The first five steps take min(initdt, dt*cfl_thresh/(cfl+EPS));
+// later steps take dt*cfl_thresh/(cfl+EPS), cut to reach the next output time
+// and to max_dt; a step landing within 1e-8 of an output time marks it.
+enum Time_mode { CONST_TIME_STEP, VARIBALE_TIME_STEP };
+struct CFDTime {
+  double dt_, initdt_, current_time_ = 0.0;
+  int current_steps_ = 0;
+  int outStepFreq_ = 0;
+  double max_dt_, nextOut_, cfl_thresh_, end_time_, out_freq_time_;
+  int mode_;
+  bool isOutStep_ = false;
+  CFDTime(double initdt, double max_dt, double outfreqtime, double end_time, double cfl_thresh, Time_mode mode)
+      : dt_(initdt), initdt_(initdt), max_dt_(max_dt), nextOut_(outfreqtime), cfl_thresh_(cfl_thresh),
+        end_time_(end_time), out_freq_time_(outfreqtime), mode_(mode) {}
+  void updateTime(double cfl) {
+    const double EPS = 1.E-12;  // CFD/hardCodedParameters.h:7
+    if (current_steps_ < 5) {
+      double dt_tmp = dt_ * cfl_thresh_ / (cfl + EPS);
+      dt_ = initdt_ < dt_tmp ? initdt_ : dt_tmp;
+    } else {
+      dt_ = dt_ * cfl_thresh_ / (cfl + EPS);
+      double till_out = nextOut_ - current_time_;
+      if (dt_ >= till_out) dt_ = till_out;
+      if (dt_ > max_dt_) dt_ = max_dt_;
+      double t_next = current_time_ + dt_;
+      if ((t_next - nextOut_ < 0 ? nextOut_ - t_next : t_next - nextOut_) <= 1e-8) {
+        isOutStep_ = true;
+        nextOut_ += out_freq_time_;
+      }
+    }
+    current_time_ += dt_;
+    current_steps_ += 1;
+  }
+};
+
 // The host-side parameters the reference passes by value as SMACSolver (dt, gravity).
 struct SMACSolver {
   double dt_ = 0.001;
