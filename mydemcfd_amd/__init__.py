@@ -19,11 +19,21 @@ _lib = None
 
 
 def load():
-    """Load the HIP library. Raises RuntimeError when it has not been built."""
+    """Load the HIP library. Raises RuntimeError when it has not been built.
+
+    When PyTorch is installed it is imported first: its HIP runtime is then the
+    one the library binds to (same soname), so one process holds a single HIP
+    runtime and torch tensors and library calls share the device. Loading the
+    library first would bring in a second runtime, after which torch cannot
+    initialize the GPU."""
     global _lib
     if _lib is None:
         if not _os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = _abi.bind(_C.CDLL(LIB_PATH, mode=_C.RTLD_LOCAL))
     return _lib
 

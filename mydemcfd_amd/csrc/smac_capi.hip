@@ -63,7 +63,7 @@ struct smac_ctx {
   double* u[3] = {nullptr, nullptr, nullptr};
   double* us[3] = {nullptr, nullptr, nullptr};
   double *p = nullptr, *dp = nullptr, *rhs = nullptr, *alpha = nullptr;
-  double *r = nullptr, *z = nullptr, *dv = nullptr, *dv2 = nullptr, *Ad = nullptr, *res0 = nullptr;
+  double *r = nullptr, *z = nullptr, *dv = nullptr, *Ad = nullptr, *res0 = nullptr;
   double* part[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int nrows = 0;
   double* S = nullptr;
@@ -301,7 +301,7 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
   int rc = SMAC_OK;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) rc = SMAC_ERR_HIP;
   double** arrs[] = {&c->u[0], &c->u[1], &c->u[2], &c->us[0], &c->us[1], &c->us[2], &c->p, &c->dp, &c->rhs,
-                     &c->r, &c->z, &c->dv, &c->dv2, &c->Ad, &c->res0};
+                     &c->r, &c->z, &c->dv, &c->Ad, &c->res0};
   for (auto a : arrs)
     if (rc == SMAC_OK) rc = dalloc(a, c->tot);
   for (int m = 0; m < 5 && rc == SMAC_OK; m++) rc = dalloc(&c->part[m], (size_t)c->nrows);
@@ -356,7 +356,7 @@ extern "C" void smac_destroy(smac_ctx* c) {
   mg_free(c);
   rg_free(c);
   double* arrs[] = {c->u[0],   c->u[1],    c->u[2],    c->us[0],   c->us[1],  c->us[2],  c->p,      c->dp,
-                    c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->dv2,    c->Ad,     c->res0,
+                    c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->Ad,     c->res0,
                     c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
                     c->muf[0], c->muf[1],  c->muf[2]};
   for (auto a : arrs)
@@ -1013,13 +1013,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     launch_finalize(fa, c->S, s);
     launch_add_scalar<T>(g, z, -1. * size_inv, c->S, S_SUMZ, s);
   }
-  // fused direction update + SpMV (2.5-D tiled) when the grid fits; the
-  // direction then lives in a ping-pong pair (dcur is the current direction)
-  const bool fused = !var && xpay_spmv_fits(g);
-  double* dcur = c->dv;
-  double* dnxt = c->dv2;
-  if (fused) launch_dot2<T>(g, c->r, z, c->Ad, 0, P[0], P[1], s);
-  else launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s);
+  launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s);
   {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_RZ);
@@ -1035,12 +1029,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   if (bb == 0.0) conv = 1;
   bool have_update = false;
   for (iter = 0; iter < o.max_iter && !conv; ++iter) {
-    if (fused) {
-      KT(c, "xpay_spmv", launch_xpay_spmv<T>(g, z, dcur, dnxt, c->Ad, c->S, iter == 0 ? -1 : S_BETA, P[0], s));
-      std::swap(dcur, dnxt);
-    } else {
-      KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
-    }
+    KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
     {
       FinalizeArgs fa = fin(c);
       fin_add(fa, P[0], 0, S_PAP);
@@ -1050,7 +1039,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       fa.sb = S_PAP;
       launch_finalize(fa, c->S, s);
     }
-    KT(c, "update", launch_update<T>(g, c->dp, dcur, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
+    KT(c, "update", launch_update<T>(g, c->dp, c->dv, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
     have_update = true;
     if (refg && pure) {
       // solve_pureneumann_gmgpcg removes the mean of r after every update
@@ -1108,7 +1097,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
         }
       }
     }
-    if (!fused) KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->S, S_BETA, s));
+    KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->S, S_BETA, s));
     launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }
   HIPOK(hipGetLastError());

@@ -92,12 +92,6 @@ template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st);
 template <typename F>
 void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st);
-bool xpay_spmv_fits(const Geo& g);
-// fused d_new = z + beta d_old (beta_slot < 0: d_new = z), Ad = A d_new, row
-// partials of d_new.Ad; returns false when the grid does not fit the tiling
-template <typename Z>
-bool launch_xpay_spmv(const Geo& g, const Z* z, const double* dold, double* dnew, double* Ad, const double* S,
-                      int beta_slot, double* p0, hipStream_t st);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
 void launch_correct(const Geo& g, const BetaF* bf, double* const us[3], const double* dp, double* u[3], double dt,

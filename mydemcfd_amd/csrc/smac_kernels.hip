@@ -356,109 +356,6 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
   if (lane == 0) part0[row] = acc;
 }
 
-// Fused d_new = z + beta d_old, Ad = A d_new, part0 <- sum d_new*Ad, 2.5-D
-// tiled (nx <= 64*VPL, ny % BY == 0). A block of BY waves owns rows j0+1..j0+BY
-// of a y-strip and marches k over KC planes. Each wave keeps d_new of rows
-// (j,k-1), (j,k), (j,k+1) in registers in natural-lane order (lane l holds
-// i = 1+l+64m, as the canonical row reduction) and publishes row (j,k) in LDS
-// (natural order with ghost slots 0 and nx+1) for the x- and y-neighbours;
-// waves 0 and BY-1 form the halo rows j0 and j0+BY+1 from z and d_old.
-// d_old and d_new are different buffers, so halo reads never see a new value.
-// beta_slot < 0: d_new = z (the first direction). Per cell the arithmetic is
-// that of k_xpay followed by k_spmv_dot.
-template <typename Z, int VPL, int BY>
-__global__ __launch_bounds__(64 * BY) void k_xpay_spmv(Geo g, const Z* __restrict__ z, const double* __restrict__ dold,
-                                                       double* __restrict__ dnew, double* __restrict__ Ad,
-                                                       const double* __restrict__ S, int beta_slot,
-                                                       double* __restrict__ part0, int KC) {
-  extern __shared__ double lds[];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  const int nyb = ny / BY;
-  const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int kc = b / nyb, yb = b % nyb;
-  const int j = yb * BY + w + 1;
-  const int k0 = kc * KC + 1;
-  const int k1 = min(k0 + KC - 1, nz);
-  if (k0 > nz) return;
-  const bool first = beta_slot < 0;
-  const double beta = first ? 0.0 : S[beta_slot];
-  const int ldr = nx + 2;
-  auto dn_of = [&](size_t o) -> double {
-    double zz = (double)St<Z>::ld(z, o);
-    return first ? zz : zz + beta * dold[o];
-  };
-  int pos[VPL];
-#pragma unroll
-  for (int m = 0; m < VPL; m++) {
-    const int i = 1 + lane + 64 * m;
-    pos[m] = i <= nx ? spos(i, nx) : 0;
-  }
-  double dm[VPL], dc[VPL], dq[VPL];
-  {
-    const size_t rm = idx(g, 0, j, W(g, 2, k0 - 1)), rc = idx(g, 0, j, k0);
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const bool v = 1 + lane + 64 * m <= nx;
-      dm[m] = v ? dn_of(rm + pos[m]) : 0.0;
-      dc[m] = v ? dn_of(rc + pos[m]) : 0.0;
-    }
-  }
-  double* own = lds + (size_t)(w + 1) * ldr;
-  for (int k = k0; k <= k1; k++) {
-    const size_t rp = idx(g, 0, j, W(g, 2, k + 1));
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const bool v = 1 + lane + 64 * m <= nx;
-      dq[m] = v ? dn_of(rp + pos[m]) : 0.0;
-    }
-    // publish row (j,k) in natural order, ghost slots 0 and nx+1
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int i = 1 + lane + 64 * m;
-      if (i <= nx) {
-        own[i] = dc[m];
-        if (i == nx) own[0] = g.per[0] ? dc[m] : 0.0;
-        if (i == 1) own[nx + 1] = g.per[0] ? dc[m] : 0.0;
-      }
-    }
-    if (w == 0 || w == BY - 1) {
-      const int slot = w == 0 ? 0 : BY + 1;
-      const int jh = w == 0 ? W(g, 1, yb * BY) : W(g, 1, yb * BY + BY + 1);
-      double* dst = lds + (size_t)slot * ldr;
-      const size_t rh = idx(g, 0, jh, k);
-      for (int i = 1 + lane; i <= nx; i += 64) dst[i] = dn_of(rh + spos(i, nx));
-    }
-    __syncthreads();
-    const size_t rb = idx(g, 0, j, k);
-    const double* yq = lds + (size_t)(w + 2) * ldr;
-    const double* ym = lds + (size_t)w * ldr;
-    double acc = 0.0;
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int i = 1 + lane + 64 * m;
-      if (i <= nx) {
-        Coef c = coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
-        double xc = dc[m];
-        double xp = own[i + 1], xm = own[i - 1];
-        double yp = yq[i], ymv = ym[i], zp = dq[m], zm = dm[m];
-        double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ymv + c.a[4] * zp + c.a[5] * zm;
-        double v = c.diag * xc - s;
-        dnew[rb + pos[m]] = xc;
-        Ad[rb + pos[m]] = v;
-        acc += xc * v;
-      }
-    }
-    acc = wave_tree_sum(acc);
-    if (lane == 0) part0[(size_t)(k - 1) * (size_t)ny + (size_t)(j - 1)] = acc;
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      dm[m] = dc[m];
-      dc[m] = dq[m];
-    }
-  }
-}
 
 // dp += alpha d (dp natural); r -= alpha Ad (k_update_p_r, G_PCGSolver.cu:66-85);
 // part0 <- sum r^2, part1 <- max |r|
@@ -819,27 +716,6 @@ template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st) {
   hipLaunchKernelGGL(k_xpay<Z>, rows_grid(g), ROWB, 0, st, g, z, d, S, slot_beta);
 }
-template <typename Z, int VPL>
-static void xpay_spmv_v(const Geo& g, const Z* z, const double* dold, double* dnew, double* Ad, const double* S,
-                        int beta_slot, double* p0, hipStream_t st) {
-  constexpr int BY = 8;
-  const int KC = 16;
-  const int nyb = g.n[1] / BY, nkc = (g.n[2] + KC - 1) / KC;
-  size_t shm = (size_t)(BY + 2) * (size_t)(g.n[0] + 2) * sizeof(double);
-  hipLaunchKernelGGL((k_xpay_spmv<Z, VPL, BY>), dim3((unsigned)(nyb * nkc)), dim3(64 * BY), shm, st, g, z, dold,
-                     dnew, Ad, S, beta_slot, p0, KC);
-}
-bool xpay_spmv_fits(const Geo& g) { return g.n[1] % 8 == 0 && g.n[0] <= 512 && g.n[2] >= 2; }
-template <typename Z>
-bool launch_xpay_spmv(const Geo& g, const Z* z, const double* dold, double* dnew, double* Ad, const double* S,
-                      int beta_slot, double* p0, hipStream_t st) {
-  if (!xpay_spmv_fits(g)) return false;
-  if (g.n[0] <= 64) xpay_spmv_v<Z, 1>(g, z, dold, dnew, Ad, S, beta_slot, p0, st);
-  else if (g.n[0] <= 128) xpay_spmv_v<Z, 2>(g, z, dold, dnew, Ad, S, beta_slot, p0, st);
-  else if (g.n[0] <= 256) xpay_spmv_v<Z, 4>(g, z, dold, dnew, Ad, S, beta_slot, p0, st);
-  else xpay_spmv_v<Z, 8>(g, z, dold, dnew, Ad, S, beta_slot, p0, st);
-  return true;
-}
 template <typename F>
 void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st) {
   hipLaunchKernelGGL(k_cvt<F>, rows_grid(g), ROWB, 0, st, g, r, f0);
@@ -853,9 +729,7 @@ void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st) {
   template void launch_dot2<Z>(const Geo&, const double*, const Z*, const double*, int, double*, double*,  \
                                hipStream_t);                                                              \
   template void launch_xpay<Z>(const Geo&, const Z*, double*, const double*, int, hipStream_t);           \
-  template void launch_cvt<Z>(const Geo&, const double*, Z*, hipStream_t);                                  \
-  template bool launch_xpay_spmv<Z>(const Geo&, const Z*, const double*, double*, double*, const double*, int,  \
-                                    double*, hipStream_t);
+  template void launch_cvt<Z>(const Geo&, const double*, Z*, hipStream_t);
 SMAC_INST_Z(double)
 SMAC_INST_Z(float)
 SMAC_INST_Z(bf16s)

@@ -84,103 +84,6 @@ __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const 
   }
 }
 
-// 2.5-D red-black half sweep (fine levels, nx even, ny % BY == 0).
-// A block of BY waves owns rows j0+1..j0+BY of a y-strip and marches k over a
-// chunk of KC planes. During a half sweep of colour col every row is read only
-// on its other-colour half: the x-neighbours inside the row, the y-neighbours
-// in rows j+-1 and the z-neighbours in rows (j,k+-1) are all cells of colour
-// 1-col, and they sit at the same relative half index as the active cell (y, z)
-// or one off (x). A wave keeps the other-colour halves of (j,k-1), (j,k),
-// (j,k+1) in registers and publishes (j,k) in LDS for the x- and y-neighbours;
-// waves 0 and BY-1 also fetch the halo rows j0 and j0+BY+1. Each row's
-// other-colour half is fetched once per sweep (plus the y/z halos), against ~1.7
-// times for the one-row-per-wave kernel. Arithmetic is the same per cell.
-template <typename T, int VPL, int BY>
-__global__ __launch_bounds__(64 * BY) void k_mg_rbgs25(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
-                                                       int KC) {
-  using C = typename St<T>::C;
-  extern __shared__ unsigned char sm_raw[];
-  C* sm = reinterpret_cast<C*>(sm_raw);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nx = L.n[0], ny = L.n[1], nz = L.n[2], nE = nx >> 1;
-  const int nyb = ny / BY;
-  const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int kc = b / nyb, yb = b % nyb;
-  const int j = yb * BY + w + 1;
-  const int k0 = kc * KC + 1;
-  const int k1 = min(k0 + KC - 1, nz);
-  if (k0 > nz) return;  // uniform per block
-  const int jm = LW(L, 1, j - 1), jp = LW(L, 1, j + 1);
-  (void)jm;
-  (void)jp;
-  // other-colour half offset of row (jj,kk): active parity ((col+jj+kk)&1) odd -> even half
-  auto ooff = [&](int jj, int kk) { return ((col + jj + kk) & 1) ? 0 : nE; };
-  C zm[VPL], zc[VPL], zp[VPL];
-  {
-    const int km = LW(L, 2, k0 - 1);
-    const size_t rm = lrow(L, j, km) + 1 + ooff(j, km);
-    const size_t rc = lrow(L, j, k0) + 1 + ooff(j, k0);
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int h = lane + 64 * m;
-      zm[m] = h < nE ? St<T>::ld(e, rm + h) : (C)0;
-      zc[m] = h < nE ? St<T>::ld(e, rc + h) : (C)0;
-    }
-  }
-  C* row_own = sm + (size_t)(w + 1) * (size_t)nE;
-  for (int k = k0; k <= k1; k++) {
-    const int kp = LW(L, 2, k + 1);
-    const size_t rp = lrow(L, j, kp) + 1 + ooff(j, kp);
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int h = lane + 64 * m;
-      zp[m] = h < nE ? St<T>::ld(e, rp + h) : (C)0;
-    }
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int h = lane + 64 * m;
-      if (h < nE) row_own[h] = zc[m];
-    }
-    if (w == 0 || w == BY - 1) {
-      const int slot = w == 0 ? 0 : BY + 1;
-      const int jh = w == 0 ? LW(L, 1, yb * BY) : LW(L, 1, yb * BY + BY + 1);
-      const size_t rh = lrow(L, jh, k) + 1 + ooff(jh, k);
-      C* dst = sm + (size_t)slot * (size_t)nE;
-      for (int h = lane; h < nE; h += 64) dst[h] = St<T>::ld(e, rh + h);
-    }
-    __syncthreads();
-    {
-      const int par = (col + j + k) & 1;  // parity of the active i
-      const int offA = par ? nE : 0;
-      const size_t rb = lrow(L, j, k);
-      const C* ym = sm + (size_t)w * (size_t)nE;
-      const C* yq = sm + (size_t)(w + 2) * (size_t)nE;
-#pragma unroll
-      for (int m = 0; m < VPL; m++) {
-        const int h = lane + 64 * m;
-        if (h < nE) {
-          const int i = par ? 2 * h + 1 : 2 * h + 2;
-          CoefC<C> c = level_coef<C>(L, i, j, k);
-          int hxm = par ? h - 1 : h;
-          int hxp = par ? h : h + 1;
-          hxm = hxm < 0 ? (L.per[0] ? nE - 1 : 0) : hxm;
-          hxp = hxp >= nE ? (L.per[0] ? 0 : nE - 1) : hxp;
-          C xp = row_own[hxp], xm = row_own[hxm];
-          C yp = yq[h], ymv = ym[h];
-          C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ymv + c.a[4] * zp[m] + c.a[5] * zm[m];
-          const size_t o = rb + 1 + offA + h;
-          St<T>::st(e, o, (St<T>::ld(f, o) + S) / c.diag);
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      zm[m] = zc[m];
-      zc[m] = zp[m];
-    }
-  }
-}
 
 // 1-D restriction stencil of coarse index I (transpose of cell-centred linear
 // prolongation; at a wall the mirrored weight folds onto the boundary cell)
@@ -249,101 +152,6 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
   }
 }
 
-// 2.5-D tiled form of k_mg_resid_rx (nx even, ny % BY == 0): a block of BY
-// waves owns rows j0+1..j0+BY and marches k over KC planes; e rows (j,k-1..k+1)
-// stay in registers (position order), row (j,k) of every wave is published in
-// LDS for the x- and y-neighbours, waves 0 and BY-1 load the y halo rows. Each
-// wave writes its residual row (natural order) to a private LDS row and
-// restricts it along x. Per cell the arithmetic is that of k_mg_resid_rx.
-template <typename T, int VPL, int BY>
-__global__ __launch_bounds__(64 * BY) void k_mg_resid_rx25(Lv F, Lv Cl, const T* __restrict__ e,
-                                                           const T* __restrict__ f,
-                                                           typename St<T>::C* __restrict__ tb, int KC) {
-  using C = typename St<T>::C;
-  extern __shared__ unsigned char sm_raw[];
-  C* sm = reinterpret_cast<C*>(sm_raw);
-  const Lv& L = F;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nx = L.n[0], ny = L.n[1], nz = L.n[2], nE = nx >> 1;
-  const int nyb = ny / BY;
-  const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
-  const int kc = b / nyb, yb = b % nyb;
-  const int j = yb * BY + w + 1;
-  const int k0 = kc * KC + 1;
-  const int k1 = min(k0 + KC - 1, nz);
-  if (k0 > nz) return;
-  const int ldr = nx + 2;
-  C* erow = sm + (size_t)(w + 1) * ldr;             // e rows, position order (slot 0..BY+1)
-  C* rrow = sm + (size_t)(BY + 2) * ldr + (size_t)w * ldr;  // residual row of this wave, natural order
-  C em[VPL], ec[VPL], ep[VPL];
-  {
-    const size_t rm = lrow(L, j, LW(L, 2, k0 - 1)), rc = lrow(L, j, k0);
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int p = 1 + lane + 64 * m;
-      em[m] = p <= nx ? St<T>::ld(e, rm + p) : (C)0;
-      ec[m] = p <= nx ? St<T>::ld(e, rc + p) : (C)0;
-    }
-  }
-  const size_t pxT = (size_t)(Cl.n[0] + 3);
-  for (int k = k0; k <= k1; k++) {
-    const size_t rp = lrow(L, j, LW(L, 2, k + 1));
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int p = 1 + lane + 64 * m;
-      ep[m] = p <= nx ? St<T>::ld(e, rp + p) : (C)0;
-    }
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int p = 1 + lane + 64 * m;
-      if (p <= nx) erow[p] = ec[m];
-    }
-    if (lane == 0) {
-      erow[0] = (C)0;  // ghost slots: only read with a zero coefficient
-      erow[nx + 1] = (C)0;
-    }
-    if (w == 0 || w == BY - 1) {
-      const int slot = w == 0 ? 0 : BY + 1;
-      const int jh = w == 0 ? LW(L, 1, yb * BY) : LW(L, 1, yb * BY + BY + 1);
-      C* dst = sm + (size_t)slot * ldr;
-      const size_t rh = lrow(L, jh, k);
-      for (int p = 1 + lane; p <= nx; p += 64) dst[p] = St<T>::ld(e, rh + p);
-    }
-    __syncthreads();
-    const size_t rb = lrow(L, j, k);
-    const C* yq = sm + (size_t)(w + 2) * ldr;
-    const C* ym = sm + (size_t)w * ldr;
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      const int p = 1 + lane + 64 * m;
-      if (p <= nx) {
-        const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
-        CoefC<C> c = level_coef<C>(L, i, j, k);
-        const int ixp = LW(L, 0, i + 1), ixm = LW(L, 0, i - 1);
-        C xp = erow[ixp >= 1 && ixp <= nx ? spos(ixp, nx) : 0];
-        C xm = erow[ixm >= 1 && ixm <= nx ? spos(ixm, nx) : 0];
-        C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yq[p] + c.a[3] * ym[p] + c.a[4] * ep[m] + c.a[5] * em[m];
-        rrow[i] = St<T>::ld(f, rb + p) - (c.diag * ec[m] - S);
-      }
-    }
-    __syncthreads();
-    const size_t tr = ((size_t)k * (size_t)L.py + (size_t)j) * pxT;
-    for (int I = 1 + lane; I <= Cl.n[0]; I += 64) {
-      int ix[4];
-      double wx[4];
-      int mx = rweights(Cl.per[0], Cl.n[0], I, ix, wx);
-      C tx = 0;
-      for (int cc = 0; cc < mx; cc++) tx += (C)wx[cc] * rrow[ix[cc]];
-      tb[tr + I] = tx;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < VPL; m++) {
-      em[m] = ec[m];
-      ec[m] = ep[m];
-    }
-  }
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_mg_ryz(Lv F, Lv Cl, const typename St<T>::C* __restrict__ tb,
@@ -451,53 +259,17 @@ static inline dim3 cell_grid(const Lv& L) {
 }
 static const dim3 ROWB(256), CELLB(64, 4, 1);
 
-template <typename T, int VPL>
-static void rbgs25(const Lv& L, T* e, const T* f, int col, hipStream_t st) {
-  constexpr int BY = 8;
-  const int KC = 16;
-  const int nyb = L.n[1] / BY, nkc = (L.n[2] + KC - 1) / KC;
-  size_t shm = (size_t)(BY + 2) * (size_t)(L.n[0] / 2) * sizeof(typename St<T>::C);
-  hipLaunchKernelGGL((k_mg_rbgs25<T, VPL, BY>), dim3((unsigned)(nyb * nkc)), dim3(64 * BY), shm, st, L, e, f, col,
-                     KC);
-}
 
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
-  const int nE = L.n[0] / 2;
-  const bool tiled = !zero_start && !L.b[0] && (L.n[0] % 2 == 0) && (L.n[1] % 8 == 0) && L.n[1] >= 8 && L.n[2] >= 2 &&
-                     nE >= 16 && nE <= 512;
-  if (tiled) {
-    if (nE <= 64) return rbgs25<T, 1>(L, e, f, col, st);
-    if (nE <= 128) return rbgs25<T, 2>(L, e, f, col, st);
-    if (nE <= 256) return rbgs25<T, 4>(L, e, f, col, st);
-    return rbgs25<T, 8>(L, e, f, col, st);
-  }
   hipLaunchKernelGGL(k_mg_rbgs<T>, lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start);
-}
-template <typename T, int VPL>
-static void resid_rx25(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, hipStream_t st) {
-  constexpr int BY = 8;
-  const int KC = 16;
-  const int nyb = F.n[1] / BY, nkc = (F.n[2] + KC - 1) / KC;
-  size_t shm = (size_t)(2 * BY + 2) * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
-  hipLaunchKernelGGL((k_mg_resid_rx25<T, VPL, BY>), dim3((unsigned)(nyb * nkc)), dim3(64 * BY), shm, st, F, C, e,
-                     f, tb, KC);
 }
 
 template <typename T>
 void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
                         hipStream_t st) {
-  const int nx = F.n[0];
-  const bool tiled = !F.b[0] && nx % 2 == 0 && F.n[1] % 8 == 0 && F.n[1] >= 8 && F.n[2] >= 2 && nx >= 32 && nx <= 512;
-  if (tiled) {
-    if (nx <= 64) resid_rx25<T, 1>(F, C, e, f, tb, st);
-    else if (nx <= 128) resid_rx25<T, 2>(F, C, e, f, tb, st);
-    else if (nx <= 256) resid_rx25<T, 4>(F, C, e, f, tb, st);
-    else resid_rx25<T, 8>(F, C, e, f, tb, st);
-  } else {
-    size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
-    hipLaunchKernelGGL(k_mg_resid_rx<T>, lrows_grid(F), ROWB, shm, st, F, C, e, f, tb);
-  }
+  size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
+  hipLaunchKernelGGL(k_mg_resid_rx<T>, lrows_grid(F), ROWB, shm, st, F, C, e, f, tb);
   hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
 }
 template <typename T>
