@@ -185,12 +185,146 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, Props P, Vel V, const do
   ustar[IX(g, q)] = predict_face<BC, PR, A>(g, P, V, p, q, dt);
 }
 
-// All three components in one pass: the thread of cell (i,j,k) computes u* on
-// x-face i, v* on y-face j and w* on z-face k (the cell's minus faces), each
-// only where that face is interior. The neighbourhoods of the three faces
-// overlap, so their loads are shared through registers and L1; per face the
-// arithmetic is exactly predict_face<BC, A>. Each thread marches KC planes in
-// z so that planes are reused while they are still in L2.
+// Periodic one-phase form of predict_face<PeriodicBC, UniP, A> with the whole
+// stencil loaded before any branch: the own component at q + s e_d for
+// s = -2..2 along each axis, the two cross components on the four faces around
+// the edge, p on both sides. The upwind side and the minmod slope are then
+// register selects, so no load waits on another load's value. The arithmetic
+// is that of predict_face, term for term, so the results are bit-identical.
+struct PerIdx {
+  int x[5], y[5], z[5];  // wrapped coordinates i-2..i+2, j-2..j+2, k-2..k+2
+  int px, py;
+  __device__ __forceinline__ size_t at(int di, int dj, int dk) const {
+    return ((size_t)z[dk + 2] * (size_t)py + (size_t)y[dj + 2]) * (size_t)px + (size_t)x[di + 2];
+  }
+};
+
+template <int A>
+__device__ __forceinline__ size_t per_at(const PerIdx& X, int d, int s) {
+  return d == 0 ? X.at(s, 0, 0) : (d == 1 ? X.at(0, s, 0) : X.at(0, 0, s));
+}
+template <int A, int B>
+__device__ __forceinline__ size_t per_at2(const PerIdx& X, int sb, int sa) {
+  // offset sb along B and sa along A
+  int o[3] = {0, 0, 0};
+  o[B] += sb;
+  o[A] += sa;
+  return X.at(o[0], o[1], o[2]);
+}
+
+// one side of the convective flux along E, MUSCL-minmod on the five values
+// c[-2..2] = u_A at q + s e_E (upwind side from the sign of m)
+__device__ __forceinline__ double flux_plus(double m, const double c[5]) {
+  // + face: up = q (c0) when m > 0, else q + e (c1)
+  double dpv, dmv, uw;
+  if (m > 0.) {
+    dpv = c[3] - c[2];
+    dmv = c[2] - c[1];
+    uw = c[2];
+  } else {
+    dpv = c[4] - c[3];
+    dmv = c[3] - c[2];
+    uw = c[3];
+  }
+  double sl = minmod(dpv, dmv);
+  int dir = sgn(m);
+  return m * (uw + dir * 0.5 * sl);
+}
+__device__ __forceinline__ double flux_minus(double m, const double c[5]) {
+  // - face: up = q - e (c-1) when m > 0, else q (c0)
+  double dpv, dmv, uw;
+  if (m > 0.) {
+    dpv = c[2] - c[1];
+    dmv = c[1] - c[0];
+    uw = c[1];
+  } else {
+    dpv = c[3] - c[2];
+    dmv = c[2] - c[1];
+    uw = c[2];
+  }
+  double sl = minmod(dpv, dmv);
+  int dir = sgn(m);
+  return m * (uw + dir * 0.5 * sl);
+}
+
+template <int A>
+__device__ __forceinline__ double predict_face_pu(const Geo& g, const Vel& V, const double* __restrict__ p,
+                                                  const PerIdx& X, double dt) {
+  constexpr int B1 = A == 0 ? 1 : 0;
+  constexpr int B2 = A == 2 ? 1 : 2;
+  const double* ua = V.c[A];
+  const double* ub1 = V.c[B1];
+  const double* ub2 = V.c[B2];
+  double ca[5], c1[5], c2[5];
+#pragma unroll
+  for (int s = -2; s <= 2; s++) {
+    ca[s + 2] = ua[per_at<A>(X, A, s)];
+    c1[s + 2] = s == 0 ? 0.0 : ua[per_at<A>(X, B1, s)];
+    c2[s + 2] = s == 0 ? 0.0 : ua[per_at<A>(X, B2, s)];
+  }
+  const double v0 = ca[2];
+  c1[2] = v0;
+  c2[2] = v0;
+  // cross components on the faces around the edges: (q+eB), (q+eB-eA), q, (q-eA)
+  const double b1pp = ub1[per_at2<A, B1>(X, 1, 0)], b1mp = ub1[per_at2<A, B1>(X, 1, -1)];
+  const double b1p0 = ub1[per_at2<A, B1>(X, 0, 0)], b1m0 = ub1[per_at2<A, B1>(X, 0, -1)];
+  const double b2pp = ub2[per_at2<A, B2>(X, 1, 0)], b2mp = ub2[per_at2<A, B2>(X, 1, -1)];
+  const double b2p0 = ub2[per_at2<A, B2>(X, 0, 0)], b2m0 = ub2[per_at2<A, B2>(X, 0, -1)];
+  const double pq = p[X.at(0, 0, 0)], pm = p[per_at<A>(X, A, -1)];
+
+  const double rho0 = g.rho0, mu0 = g.mu0;
+  const double vp1 = ca[3], vm1 = ca[1];
+  double tau_p = mu0 * (vp1 - v0);
+  double tau_m = mu0 * (v0 - vm1);
+  double tmp = 2. * (tau_p - tau_m) * g.inv_d2[A];
+  {  // visc_cross B1
+    const double mu_p = 0.5 * (mu0 + mu0);
+    double t_p = mu_p * ((b1pp - b1mp) * g.inv_d[A] + (c1[3] - v0) * g.inv_d[B1]);
+    const double mu_m = 0.5 * (mu0 + mu0);
+    double t_m = mu_m * ((b1p0 - b1m0) * g.inv_d[A] + (v0 - c1[1]) * g.inv_d[B1]);
+    tmp += (t_p - t_m) * g.inv_d[B1];
+  }
+  {  // visc_cross B2
+    const double mu_p = 0.5 * (mu0 + mu0);
+    double t_p = mu_p * ((b2pp - b2mp) * g.inv_d[A] + (c2[3] - v0) * g.inv_d[B2]);
+    const double mu_m = 0.5 * (mu0 + mu0);
+    double t_m = mu_m * ((b2p0 - b2m0) * g.inv_d[A] + (v0 - c2[1]) * g.inv_d[B2]);
+    tmp += (t_p - t_m) * g.inv_d[B2];
+  }
+  // convection along x, y, z
+#pragma unroll
+  for (int E = 0; E < 3; E++) {
+    double mp, mm;
+    const double* c;
+    if (E == A) {
+      mp = 0.5 * (rho0 * ca[2] + rho0 * ca[3]);
+      mm = 0.5 * (rho0 * ca[1] + rho0 * ca[2]);
+      c = ca;
+    } else if (E == B1) {
+      mp = 0.5 * (rho0 * b1mp + rho0 * b1pp);
+      mm = 0.5 * (rho0 * b1p0 + rho0 * b1m0);
+      c = c1;
+    } else {
+      mp = 0.5 * (rho0 * b2mp + rho0 * b2pp);
+      mm = 0.5 * (rho0 * b2p0 + rho0 * b2m0);
+      c = c2;
+    }
+    double Mp = flux_plus(mp, c);
+    double Mm = flux_minus(mm, c);
+    tmp -= (Mp - Mm) * g.inv_d[E];
+  }
+  const double beta = g.beta0;
+  const double rho_old_f = 0.5 * (rho0 + rho0);
+  tmp -= (pq - pm) * g.inv_d[A];
+  return beta * (v0 * rho_old_f) + dt * (beta * tmp + g.g[A]);
+}
+
+// All three components in one pass (triply periodic, one phase): the thread of
+// cell (i,j,k) computes u* on x-face i, v* on y-face j and w* on z-face k (the
+// cell's minus faces). The neighbourhoods of the three faces overlap, so their
+// loads are shared through L1; per face the arithmetic is exactly
+// predict_face<PeriodicBC, UniP, A>. Each thread marches KC planes in z so that
+// planes are reused while they are still in L2.
 template <class BC>
 __global__ __launch_bounds__(256) void k_predict_fused(Geo g, Vel V, const double* __restrict__ p,
                                                        double* __restrict__ us0, double* __restrict__ us1,
@@ -202,14 +336,22 @@ __global__ __launch_bounds__(256) void k_predict_fused(Geo g, Vel V, const doubl
   const int k1 = min(k0 + KC - 1, g.n[2]);
   const bool xi = g.per[0] || i >= 2;
   const bool yi = g.per[1] || j >= 2;
+  PerIdx X;
+  X.px = g.px;
+  X.py = g.py;
+#pragma unroll
+  for (int s = -2; s <= 2; s++) {
+    X.x[s + 2] = wrapi(i + s, g.n[0]);
+    X.y[s + 2] = wrapi(j + s, g.n[1]);
+  }
   for (int k = k0; k <= k1; k++) {
     const bool zi = g.per[2] || k >= 2;
-    I3 q = mk(i, j, k);
-    const size_t id = IX(g, q);
-    const Props P{};
-    if (xi) us0[id] = predict_face<BC, UniP, 0>(g, P, V, p, q, dt);
-    if (yi) us1[id] = predict_face<BC, UniP, 1>(g, P, V, p, q, dt);
-    if (zi) us2[id] = predict_face<BC, UniP, 2>(g, P, V, p, q, dt);
+#pragma unroll
+    for (int s = -2; s <= 2; s++) X.z[s + 2] = wrapi(k + s, g.n[2]);
+    const size_t id = X.at(0, 0, 0);
+    if (xi) us0[id] = predict_face_pu<0>(g, V, p, X, dt);
+    if (yi) us1[id] = predict_face_pu<1>(g, V, p, X, dt);
+    if (zi) us2[id] = predict_face_pu<2>(g, V, p, X, dt);
   }
 }
 
