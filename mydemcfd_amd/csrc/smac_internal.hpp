@@ -46,6 +46,34 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   (void)rb;
 
 #define ROW_SETUP ROW_SETUP_N(g.n[0], g.n[1], g.n[2], g.px, g.py)
+
+// Cells per lane of a row kernel over rows of n cells (a wave owns a row and
+// lane l the cells 1+l+64m): 1, 2, 4 or 8, so that the lane issues all its
+// loads before the first use; 0 = runtime loop for longer rows.
+inline int row_vpl(int n) { return n <= 64 ? 1 : (n <= 128 ? 2 : (n <= 256 ? 4 : (n <= 512 ? 8 : 0))); }
+#define SMAC_VPL_DISPATCH(N, CALL)   \
+  switch (row_vpl(N)) {              \
+    case 1: {                        \
+      constexpr int VPL = 1;         \
+      CALL;                          \
+    } break;                         \
+    case 2: {                        \
+      constexpr int VPL = 2;         \
+      CALL;                          \
+    } break;                         \
+    case 4: {                        \
+      constexpr int VPL = 4;         \
+      CALL;                          \
+    } break;                         \
+    case 8: {                        \
+      constexpr int VPL = 8;         \
+      CALL;                          \
+    } break;                         \
+    default: {                       \
+      constexpr int VPL = 0;         \
+      CALL;                          \
+    } break;                         \
+  }
 #define LROW_SETUP ROW_SETUP_N(L.n[0], L.n[1], L.n[2], L.px, L.py)
 
 

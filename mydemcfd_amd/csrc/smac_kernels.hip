@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z
 }
 
 // Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124)
-template <bool VAR>
+template <bool VAR, int VPL>
 __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double* __restrict__ d,
                                                   double* __restrict__ Ad, double* __restrict__ part0) {
   ROW_SETUP
@@ -483,16 +483,48 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
   const size_t ypr = idx(g, 0, W(g, 1, j + 1), k), ymr = idx(g, 0, W(g, 1, j - 1), k);
   const size_t zpr = idx(g, 0, j, W(g, 2, k + 1)), zmr = idx(g, 0, j, W(g, 2, k - 1));
   double acc = 0.0;
-  for (int i = 1 + lane; i <= nx; i += 64) {
-    Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
-    const int p = spos(i, nx);
-    double xc = d[rb + p];
-    double xp = d[rb + spos(W(g, 0, i + 1), nx)], xm = d[rb + spos(W(g, 0, i - 1), nx)];
-    double yp = d[ypr + p], ym = d[ymr + p], zp = d[zpr + p], zm = d[zmr + p];
-    double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
-    double v = c.diag * xc - s;
-    Ad[rb + p] = v;
-    acc += xc * v;
+  if (VPL == 0) {
+    for (int i = 1 + lane; i <= nx; i += 64) {
+      Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+      const int p = spos(i, nx);
+      double xc = d[rb + p];
+      double xp = d[rb + spos(W(g, 0, i + 1), nx)], xm = d[rb + spos(W(g, 0, i - 1), nx)];
+      double yp = d[ypr + p], ym = d[ymr + p], zp = d[zpr + p], zm = d[zmr + p];
+      double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
+      double v = c.diag * xc - s;
+      Ad[rb + p] = v;
+      acc += xc * v;
+    }
+  } else {
+    // all loads of the lane's VPL cells first, then the arithmetic (same order)
+    constexpr int NV = VPL > 0 ? VPL : 1;
+    double xc[NV], xp[NV], xm[NV], yp[NV], ym[NV], zp[NV], zm[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int i = 1 + lane + 64 * m;
+      if (i <= nx) {
+        const int p = spos(i, nx);
+        xc[m] = d[rb + p];
+        xp[m] = d[rb + spos(W(g, 0, i + 1), nx)];
+        xm[m] = d[rb + spos(W(g, 0, i - 1), nx)];
+        yp[m] = d[ypr + p];
+        ym[m] = d[ymr + p];
+        zp[m] = d[zpr + p];
+        zm[m] = d[zmr + p];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int i = 1 + lane + 64 * m;
+      if (i <= nx) {
+        Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+        double s = c.a[0] * xp[m] + c.a[1] * xm[m] + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] +
+                   c.a[5] * zm[m];
+        double v = c.diag * xc[m] - s;
+        Ad[rb + spos(i, nx)] = v;
+        acc += xc[m] * v;
+      }
+    }
   }
   acc = wave_tree_sum(acc);
   if (lane == 0) part0[row] = acc;
@@ -503,24 +535,46 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
 // part0 <- sum r^2, part1 <- max |r|
 // F: multigrid storage; when f0 is given it receives (F) r (the fine-level rhs
 // of the next V-cycle in the mixed forms)
-template <typename F>
+template <typename F, int VPL>
 __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, const double* __restrict__ d,
                                                 double* __restrict__ r, const double* __restrict__ Ad,
                                                 const double* __restrict__ S, int slot_alpha,
                                                 double* __restrict__ part0, double* __restrict__ part1,
                                                 F* __restrict__ f0) {
   ROW_SETUP
+  const int nx = g.n[0];
   const double alpha = S[slot_alpha];
   double a0 = 0.0, mx = 0.0;
-  for (int i = 1 + lane; i <= g.n[0]; i += 64) {
-    const size_t o = rb + spos(i, g.n[0]);
-    dp[rb + i] += alpha * d[o];
-    double rn = r[o] - alpha * Ad[o];
-    r[o] = rn;
-    if (f0) St<F>::st(f0, o, (typename St<F>::C)rn);
-    a0 += rn * rn;
-    double ar = fabs(rn);
-    if (ar > mx || ar != ar) mx = ar;
+  constexpr int NV = VPL > 0 ? VPL : 1;
+  const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
+  for (int t = 0; t < trips; t++) {
+    // VPL > 0: one trip covering the row, loads first; VPL == 0: one cell per trip
+    double vdp[NV], vd[NV], vr[NV], vA[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (i <= nx) {
+        const size_t o = rb + spos(i, nx);
+        vdp[m] = dp[rb + i];
+        vd[m] = d[o];
+        vr[m] = r[o];
+        vA[m] = Ad[o];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (i <= nx) {
+        const size_t o = rb + spos(i, nx);
+        dp[rb + i] = vdp[m] + alpha * vd[m];
+        double rn = vr[m] - alpha * vA[m];
+        r[o] = rn;
+        if (f0) St<F>::st(f0, o, (typename St<F>::C)rn);
+        a0 += rn * rn;
+        double ar = fabs(rn);
+        if (ar > mx || ar != ar) mx = ar;
+      }
+    }
   }
   a0 = wave_tree_sum(a0);
   mx = wave_max(mx);
@@ -531,17 +585,35 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, 
 }
 
 // part0 <- sum r*z; part1 <- sum z*Ad (flexible only)
-template <typename Z>
+template <typename Z, int VPL>
 __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ r, const Z* __restrict__ z,
                                               const double* __restrict__ Ad, int flexible,
                                               double* __restrict__ part0, double* __restrict__ part1) {
   ROW_SETUP
+  const int nx = g.n[0];
   double a0 = 0.0, a1 = 0.0;
-  for (int i = 1 + lane; i <= g.n[0]; i += 64) {
-    const size_t o = rb + spos(i, g.n[0]);
-    double zz = (double)St<Z>::ld(z, o);
-    a0 += r[o] * zz;
-    if (flexible) a1 += zz * Ad[o];
+  constexpr int NV = VPL > 0 ? VPL : 1;
+  const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
+  for (int t = 0; t < trips; t++) {
+    double vr[NV], vz[NV], vA[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (i <= nx) {
+        const size_t o = rb + spos(i, nx);
+        vr[m] = r[o];
+        vz[m] = (double)St<Z>::ld(z, o);
+        vA[m] = flexible ? Ad[o] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (i <= nx) {
+        a0 += vr[m] * vz[m];
+        if (flexible) a1 += vz[m] * vA[m];
+      }
+    }
   }
   a0 = wave_tree_sum(a0);
   if (flexible) a1 = wave_tree_sum(a1);
@@ -552,12 +624,30 @@ __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ 
 }
 
 // d = z + beta d (k_update_dir, G_PCGSolver.cu:276-288)
-template <typename Z>
+template <typename Z, int VPL>
 __global__ __launch_bounds__(256) void k_xpay(Geo g, const Z* __restrict__ z, double* __restrict__ d,
                                               const double* __restrict__ S, int slot_beta) {
   ROW_SETUP
+  const int nx = g.n[0];
   const double beta = S[slot_beta];
-  for (int p = 1 + lane; p <= g.n[0]; p += 64) d[rb + p] = (double)St<Z>::ld(z, rb + p) + beta * d[rb + p];
+  constexpr int NV = VPL > 0 ? VPL : 1;
+  const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
+  for (int t = 0; t < trips; t++) {
+    double vz[NV], vd[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (p <= nx) {
+        vz[m] = (double)St<Z>::ld(z, rb + p);
+        vd[m] = d[rb + p];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (p <= nx) d[rb + p] = vz[m] + beta * vd[m];
+    }
+  }
 }
 
 // f0 = (F) r on interior cells (fine-level rhs of the mixed V-cycle)
@@ -841,22 +931,28 @@ void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, doubl
   hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0);
 }
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st) {
-  if (bf) hipLaunchKernelGGL(k_spmv_dot<true>, rows_grid(g), ROWB, 0, st, g, *bf, d, Ad, p0);
-  else hipLaunchKernelGGL(k_spmv_dot<false>, rows_grid(g), ROWB, 0, st, g, BetaF{}, d, Ad, p0);
+  if (bf) {
+    SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_spmv_dot<true, VPL>), rows_grid(g), ROWB, 0, st, g, *bf, d, Ad, p0))
+  } else {
+    SMAC_VPL_DISPATCH(g.n[0],
+                      hipLaunchKernelGGL((k_spmv_dot<false, VPL>), rows_grid(g), ROWB, 0, st, g, BetaF{}, d, Ad, p0))
+  }
 }
 template <typename F>
 void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,
                    int slot_alpha, double* p0, double* p1, F* f0, hipStream_t st) {
-  hipLaunchKernelGGL(k_update<F>, rows_grid(g), ROWB, 0, st, g, dp, d, r, Ad, S, slot_alpha, p0, p1, f0);
+  SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_update<F, VPL>), rows_grid(g), ROWB, 0, st, g, dp, d, r, Ad, S,
+                                               slot_alpha, p0, p1, f0))
 }
 template <typename Z>
 void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, int flexible, double* p0,
                  double* p1, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot2<Z>, rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1);
+  SMAC_VPL_DISPATCH(g.n[0],
+                    hipLaunchKernelGGL((k_dot2<Z, VPL>), rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1))
 }
 template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st) {
-  hipLaunchKernelGGL(k_xpay<Z>, rows_grid(g), ROWB, 0, st, g, z, d, S, slot_beta);
+  SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_xpay<Z, VPL>), rows_grid(g), ROWB, 0, st, g, z, d, S, slot_beta))
 }
 template <typename F>
 void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st) {

@@ -57,7 +57,9 @@ __device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k)
 // Red-black Gauss-Seidel half sweep of colour col ((i+j+k)&1 == col) on the
 // active half row: e = (f + S)/diag with S = sum of a_nb e_nb in the order
 // xp, xm, yp, ym, zp, zm. zero_start: e is zero on entry, so e = f/diag.
-template <typename T>
+// VPL > 0: a lane owns VPL cells h = 1+lane+64m, all loaded before any is
+// computed (one memory round per wave instead of VPL); VPL = 0: any row length.
+template <typename T, int VPL>
 __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
                                                  int zero_start) {
   using C = typename St<T>::C;
@@ -68,18 +70,55 @@ __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const 
   const int off = par ? nE : 0;
   const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
   const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
-  for (int h = 1 + lane; h <= cnt; h += 64) {
-    const int i = par ? 2 * h - 1 : 2 * h;
-    const int p = off + h;
-    CoefC<C> c = level_coef<C>(L, i, j, k);
-    if (zero_start) {
-      St<T>::st(e, rb + p, St<T>::ld(f, rb + p) / c.diag);
-    } else {
-      C xp = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx)), xm = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
-      C yp = St<T>::ld(e, ypr + p), ym = St<T>::ld(e, ymr + p);
-      C zp = St<T>::ld(e, zpr + p), zm = St<T>::ld(e, zmr + p);
-      C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
-      St<T>::st(e, rb + p, (St<T>::ld(f, rb + p) + S) / c.diag);
+  if (VPL == 0) {
+    for (int h = 1 + lane; h <= cnt; h += 64) {
+      const int i = par ? 2 * h - 1 : 2 * h;
+      const int p = off + h;
+      CoefC<C> c = level_coef<C>(L, i, j, k);
+      if (zero_start) {
+        St<T>::st(e, rb + p, St<T>::ld(f, rb + p) / c.diag);
+      } else {
+        C xp = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx)), xm = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
+        C yp = St<T>::ld(e, ypr + p), ym = St<T>::ld(e, ymr + p);
+        C zp = St<T>::ld(e, zpr + p), zm = St<T>::ld(e, zmr + p);
+        C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
+        St<T>::st(e, rb + p, (St<T>::ld(f, rb + p) + S) / c.diag);
+      }
+    }
+    return;
+  }
+  constexpr int NV = VPL > 0 ? VPL : 1;
+  C fv[NV], xp[NV], xm[NV], yp[NV], ym[NV], zp[NV], zm[NV];
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    const int h = 1 + lane + 64 * m;
+    if (h <= cnt) {
+      const int i = par ? 2 * h - 1 : 2 * h;
+      const int p = off + h;
+      fv[m] = St<T>::ld(f, rb + p);
+      if (!zero_start) {
+        xp[m] = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx));
+        xm[m] = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
+        yp[m] = St<T>::ld(e, ypr + p);
+        ym[m] = St<T>::ld(e, ymr + p);
+        zp[m] = St<T>::ld(e, zpr + p);
+        zm[m] = St<T>::ld(e, zmr + p);
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NV; m++) {
+    const int h = 1 + lane + 64 * m;
+    if (h <= cnt) {
+      const int i = par ? 2 * h - 1 : 2 * h;
+      const int p = off + h;
+      CoefC<C> c = level_coef<C>(L, i, j, k);
+      if (zero_start) {
+        St<T>::st(e, rb + p, fv[m] / c.diag);
+      } else {
+        C S = c.a[0] * xp[m] + c.a[1] * xm[m] + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
+        St<T>::st(e, rb + p, (fv[m] + S) / c.diag);
+      }
     }
   }
 }
@@ -109,7 +148,7 @@ __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], doubl
 // res = f - (diag e - S) of one fine row in LDS and restricts it along x into
 // tb (coarse x, fine y and z, natural order, pitch C.n[0]+3). Pass 2 restricts
 // tb along y and z.
-template <typename T>
+template <typename T, int VPL>
 __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __restrict__ e, const T* __restrict__ f,
                                                      typename St<T>::C* __restrict__ tb) {
   using C = typename St<T>::C;
@@ -129,14 +168,36 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
     const size_t rb = lrow(L, j, k);
     const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
     const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
-    for (int p = 1 + lane; p <= nx; p += 64) {
-      const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
-      CoefC<C> c = level_coef<C>(L, i, j, k);
-      C xp = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx)), xm = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
-      C yp = St<T>::ld(e, ypr + p), ym = St<T>::ld(e, ymr + p);
-      C zp = St<T>::ld(e, zpr + p), zm = St<T>::ld(e, zmr + p);
-      C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
-      buf[i] = St<T>::ld(f, rb + p) - (c.diag * St<T>::ld(e, rb + p) - S);
+    constexpr int NV = VPL > 0 ? VPL : 1;
+    const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
+    for (int t = 0; t < trips; t++) {
+      // VPL > 0: the lane's cells p = 1+lane+64m, all loads before the arithmetic
+      C ec[NV], ef[NV], xp[NV], xm[NV], yp[NV], ym[NV], zp[NV], zm[NV];
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
+        if (p <= nx) {
+          const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
+          xp[m] = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx));
+          xm[m] = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
+          yp[m] = St<T>::ld(e, ypr + p);
+          ym[m] = St<T>::ld(e, ymr + p);
+          zp[m] = St<T>::ld(e, zpr + p);
+          zm[m] = St<T>::ld(e, zmr + p);
+          ef[m] = St<T>::ld(f, rb + p);
+          ec[m] = St<T>::ld(e, rb + p);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
+        if (p <= nx) {
+          const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
+          CoefC<C> c = level_coef<C>(L, i, j, k);
+          C S = c.a[0] * xp[m] + c.a[1] * xm[m] + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
+          buf[i] = ef[m] - (c.diag * ec[m] - S);
+        }
+      }
     }
   }
   __syncthreads();
@@ -187,7 +248,7 @@ __device__ __forceinline__ void pweights(int per, int nc, int i, int ix[2], doub
 // e_f += sum over 8 coarse cells of (wz*wy*wx) e_c (cell-centred trilinear).
 // col >= 0: only the fine cells of that colour (the other colour is
 // recomputed by the next half sweep without being read); col < 0: all cells.
-template <typename T>
+template <typename T, int VPL>
 __global__ __launch_bounds__(256) void k_mg_prolong(Lv Cl, Lv F, const T* __restrict__ ec, T* __restrict__ ef,
                                                     int col) {
   using C = typename St<T>::C;
@@ -206,24 +267,49 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Lv Cl, Lv F, const T* __rest
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) cr[a][b] = lrow(Cl, iy[b], iz[a]);
-  for (int h = 1 + lane; h <= cnt; h += 64) {
-    const int pp = off + h;
-    const int i = pp <= nE ? 2 * pp : 2 * (pp - nE) - 1;
-    int ix[2];
-    double wx[2];
-    pweights(Cl.per[0], Cl.n[0], i, ix, wx);
-    const int c0 = spos(ix[0], Cl.n[0]), c1 = spos(ix[1], Cl.n[0]);
-    C acc = 0;
+  constexpr int NV = VPL > 0 ? VPL : 1;
+  const int trips = VPL > 0 ? 1 : (cnt + 63) / 64;
+  for (int t = 0; t < trips; t++) {
+    // VPL > 0: all coarse and fine loads of the lane's cells first
+    C cv[NV][2][2][2], fv[NV];
+    double wxs[NV][2];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int m = 0; m < NV; m++) {
+      const int h = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (h <= cnt) {
+        const int pp = off + h;
+        const int i = pp <= nE ? 2 * pp : 2 * (pp - nE) - 1;
+        int ix[2];
+        pweights(Cl.per[0], Cl.n[0], i, ix, wxs[m]);
+        const int c0 = spos(ix[0], Cl.n[0]), c1 = spos(ix[1], Cl.n[0]);
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        C w0 = (C)(wz[a] * wy[b] * wx[0]);
-        acc += w0 * St<T>::ld(ec, cr[a][b] + c0);
-        C w1 = (C)(wz[a] * wy[b] * wx[1]);
-        acc += w1 * St<T>::ld(ec, cr[a][b] + c1);
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) {
+            cv[m][a][b][0] = St<T>::ld(ec, cr[a][b] + c0);
+            cv[m][a][b][1] = St<T>::ld(ec, cr[a][b] + c1);
+          }
+        fv[m] = St<T>::ld(ef, rb + pp);
       }
-    St<T>::st(ef, rb + pp, St<T>::ld(ef, rb + pp) + acc);
+    }
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int h = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (h <= cnt) {
+        const int pp = off + h;
+        C acc = 0;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) {
+            C w0 = (C)(wz[a] * wy[b] * wxs[m][0]);
+            acc += w0 * cv[m][a][b][0];
+            C w1 = (C)(wz[a] * wy[b] * wxs[m][1]);
+            acc += w1 * cv[m][a][b][1];
+          }
+        St<T>::st(ef, rb + pp, fv[m] + acc);
+      }
+    }
   }
 }
 
@@ -262,19 +348,21 @@ static const dim3 ROWB(256), CELLB(64, 4, 1);
 
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
-  hipLaunchKernelGGL(k_mg_rbgs<T>, lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start);
+  const int half = L.n[0] - (L.n[0] >> 1);  // longest active half row
+  SMAC_VPL_DISPATCH(half, hipLaunchKernelGGL((k_mg_rbgs<T, VPL>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start))
 }
 
 template <typename T>
 void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
                         hipStream_t st) {
   size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
-  hipLaunchKernelGGL(k_mg_resid_rx<T>, lrows_grid(F), ROWB, shm, st, F, C, e, f, tb);
+  SMAC_VPL_DISPATCH(F.n[0], hipLaunchKernelGGL((k_mg_resid_rx<T, VPL>), lrows_grid(F), ROWB, shm, st, F, C, e, f, tb))
   hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
 }
 template <typename T>
 void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hipStream_t st) {
-  hipLaunchKernelGGL(k_mg_prolong<T>, lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col);
+  const int cntmax = col < 0 ? F.n[0] : F.n[0] - (F.n[0] >> 1);
+  SMAC_VPL_DISPATCH(cntmax, hipLaunchKernelGGL((k_mg_prolong<T, VPL>), lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col))
 }
 template <typename T>
 void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st) {
