@@ -36,14 +36,34 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return (b & 7) * q + (b >> 3);
 }
 
+// Rows are visited band-major: the y-rows are cut into bands of JB rows and a
+// band is swept through all z-planes before the next, so the y- and z-
+// neighbour rows of a stencil are read again while they are still in L2
+// (a plane-major order needs three whole planes resident, 6 MB per FP64 plane
+// triple at 512^2, more than an XCD's 4 MB L2).
+__device__ __forceinline__ int band_rows(int ny) {
+  return (ny % 32 == 0) ? 32 : ((ny % 16 == 0) ? 16 : ((ny % 8 == 0) ? 8 : ny));
+}
+// visit index r -> row (j, k), both from 1
+__device__ __forceinline__ void band_jk(int r, int ny, int nz, int& j, int& k) {
+  const int jb = band_rows(ny);
+  const int t = r / jb;
+  k = t % nz + 1;
+  j = (t / nz) * jb + (r % jb) + 1;
+}
+
+// row = (k-1)*NY + (j-1) is the canonical row index (row partial sums are
+// stored there whatever the visiting order)
 #define ROW_SETUP_N(NX, NY, NZ, PX, PY)                                        \
   const int lane = threadIdx.x & 63;                                          \
-  const int row = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)(threadIdx.x >> 6); \
-  if (row >= (NY) * (NZ)) return;                                             \
-  const int j = row % (NY) + 1;                                               \
-  const int k = row / (NY) + 1;                                               \
+  const int vrow_ = xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + (int)(threadIdx.x >> 6); \
+  if (vrow_ >= (NY) * (NZ)) return;                                           \
+  int j, k;                                                                   \
+  band_jk(vrow_, (NY), (NZ), j, k);                                           \
+  const int row = (k - 1) * (NY) + (j - 1);                                   \
   const size_t rb = ((size_t)k * (size_t)(PY) + (size_t)j) * (size_t)(PX);    \
-  (void)rb;
+  (void)rb;                                                                   \
+  (void)row;
 
 #define ROW_SETUP ROW_SETUP_N(g.n[0], g.n[1], g.n[2], g.px, g.py)
 

@@ -886,7 +886,7 @@ static void predict_components(const Geo& g, const Props& P, const Vel& V, const
 
 void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
                     hipStream_t st) {
-  const int KC = 4;
+  const int KC = 16;  // z-planes per thread: the k+-2 halo is re-read once per 16 planes
   const bool allper = g.per[0] && g.per[1] && g.per[2];
   if (P) {
     // two-phase grids: one kernel per component, properties from the arrays

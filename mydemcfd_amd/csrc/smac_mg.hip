@@ -163,8 +163,7 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
   C* buf = sm + (size_t)w * (size_t)(nx + 2);
   int j = 1, k = 1;
   if (valid) {
-    j = row % L.n[1] + 1;
-    k = row / L.n[1] + 1;
+    band_jk(row, L.n[1], L.n[2], j, k);
     const size_t rb = lrow(L, j, k);
     const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
     const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));

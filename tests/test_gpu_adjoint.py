@@ -18,7 +18,11 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-TIGHT = dict(precond="mg", stop_rule="rel_l2", tol=1e-13, max_iter=400)
+# finite differences need the same solver map on both sides: a fixed number of
+# MG-PCG iterations (tol = 0) that takes the residual to rounding level, so the
+# forward is a smooth function of its input and the iteration count cannot
+# differ between the + and - runs
+TIGHT = dict(precond="mg", stop_rule="rel_l2", tol=0.0, max_iter=18)
 
 
 def _torch():
@@ -53,7 +57,8 @@ def _step_out(ctx, st, dt, opts, nsteps=1):
         ctx.upload_device(f, t.data_ptr(), t.numel())
     for _ in range(nsteps):
         s = ctx.step(dt, opts)
-        assert s.converged, s.as_dict()
+        assert s.iters == opts.max_iter or s.converged, s.as_dict()
+        assert s.rel_res < 1e-10, s.as_dict()
     out = {f: torch.empty_like(t) for f, t in st.items()}
     for f, t in out.items():
         ctx.download_device(f, t.data_ptr(), t.numel())
@@ -101,14 +106,14 @@ def _transpose_check(ctx, st0, d, wv, masks, dt, opts):
     eps = 1e-6
     plus = _step_out(ctx, {f: st0[f] + eps * d[f] for f in st0}, dt, opts)
     minus = _step_out(ctx, {f: st0[f] - eps * d[f] for f in st0}, dt, opts)
-    lhs = (_dot(wv, plus, masks) - _dot(wv, minus, masks)) / (2 * eps)
+    lhs = _dot(wv, {f: plus[f] - minus[f] for f in plus}, masks) / (2 * eps)
     _step_out(ctx, st0, dt, opts, nsteps=0)
     u_n = [st0[f] for f in ("u", "v", "w")]
     ub = [wv[f].clone() for f in ("u", "v", "w")]
     pb = wv["p"].clone()
     torch.cuda.synchronize()
     s = ctx.adjoint_step(dt, [t.data_ptr() for t in u_n], [t.data_ptr() for t in ub], pb.data_ptr(), opts)
-    assert s.converged, s.as_dict()
+    assert s.rel_res < 1e-10, s.as_dict()
     torch.cuda.synchronize()
     grad = {"u": ub[0], "v": ub[1], "w": ub[2], "p": pb}
     for f in grad:  # non-state entries are zero on return
@@ -163,8 +168,7 @@ def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
     u, v, w = cases.adjoint_fields(n)
     st0 = _state(ctx, {"u": u, "v": v, "w": w})
     dt = 2e-3
-    opts = ctx.opts(precond="mg", stop_rule="rel_l2", tol=1e-13, max_iter=400, precision=precision,
-                    flexible=0 if precision == "fp64" else 1)
+    opts = ctx.opts(**dict(TIGHT, precision=precision, flexible=0 if precision == "fp64" else 1))
     h3 = (2 * math.pi / n) ** 3
     x = {f: t.clone().requires_grad_(f != "p") for f, t in st0.items()}
     out = autograd.rollout(ctx, x["u"], x["v"], x["w"], x["p"], dt, nsteps, opts)
@@ -180,9 +184,11 @@ def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
              for f in ("u", "v", "w")}
         d["p"] = torch.zeros_like(st0["p"])
         eps = 1e-6
-        jp = _energy(_step_out(ctx, {f: st0[f] + eps * d[f] for f in st0}, dt, fd_opts, nsteps), masks, h3)
-        jm = _energy(_step_out(ctx, {f: st0[f] - eps * d[f] for f in st0}, dt, fd_opts, nsteps), masks, h3)
-        fd = (jp - jm) / (2 * eps)
+        op = _step_out(ctx, {f: st0[f] + eps * d[f] for f in st0}, dt, fd_opts, nsteps)
+        om = _step_out(ctx, {f: st0[f] - eps * d[f] for f in st0}, dt, fd_opts, nsteps)
+        # J(+) - J(-) = h^3/2 sum (a - b)(a + b): formed from the difference, not from two large sums
+        dj = 0.5 * h3 * sum(float(((op[f] - om[f]) * (op[f] + om[f]))[masks[f]].sum()) for f in ("u", "v", "w"))
+        fd = dj / (2 * eps)
         ad = sum(float((grad[f] * d[f]).sum()) for f in ("u", "v", "w"))
         errs.append(abs(fd - ad) / abs(fd))
         print(f"  direction: FD {fd:.10e} adjoint {ad:.10e} rel {errs[-1]:.2e}", flush=True)
@@ -193,6 +199,15 @@ def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
 def test_rollout_gradient_vs_finite_differences(hip):
     print("case: rollout gradient, periodic 32^3 x4 steps, 4 directions", flush=True)
     _rollout_grad_check(hip, 32, 4, 4)
+
+
+def test_c4_adjoint_128_16_steps(hip):
+    """The adjoint configuration: 128^3 periodic box, k^-5/3 initial velocity
+    with unit RMS (seed 0x5EED0004), Re = 500, dt = 2e-3, 16 steps with
+    per-step checkpoints; gradient of the final kinetic energy along 8 seeded
+    directions (seed 0x5EED0014) against central differences, eps = 1e-6."""
+    print("case: C4 adjoint, periodic 128^3 x16 steps, 8 directions", flush=True)
+    _rollout_grad_check(hip, 128, 16, 8)
 
 
 def test_mixed_adjoint_vs_fp64(hip):
