@@ -68,6 +68,7 @@ struct smac_ctx {
   int nrows = 0;
   double* S = nullptr;
   double* Sh = nullptr;
+  double* fstage = nullptr;  // 3*1024 lane sums of the two-stage reductions
   std::vector<Level> lev;
   double* cinv = nullptr;
   int cinv_n = 0;
@@ -85,6 +86,12 @@ struct smac_ctx {
   uint64_t props_ver = 1;     // bumped whenever the face coefficients may change
   uint64_t mg_ver = ~0ull;    // props_ver the multigrid hierarchy was built for
   std::vector<double*> mg_b;  // coarse-level face coefficients, 3 per level >= 1
+  // the V-cycle below the fine level, captured once as a HIP graph and
+  // replayed (its kernels are small and launch-bound); key = nu1, nu2
+  hipGraphExec_t cg_exec = nullptr;
+  int cg_nu[2] = {-1, -1};
+  bool capturing = false;
+  bool graphs = true;
   // adjoint work arrays (box layout, allocated on first use): cotangents of
   // u^{n+1}, u*, u^n (3 faces each), of p^{n+1} and of p^n
   double* adj[11] = {nullptr};
@@ -128,7 +135,7 @@ static void prof_flush(smac_ctx* c) {
 }
 // returns a record index, or -1 when profiling is off
 static int kbeg(smac_ctx* c, const char* name) {
-  if (!c->prof) return -1;
+  if (!c->prof || c->capturing) return -1;
   if (c->pool_used + 2 > 65536) prof_flush(c);
   auto it = c->name_id.find(name);
   int id;
@@ -306,6 +313,7 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
     if (rc == SMAC_OK) rc = dalloc(a, c->tot);
   for (int m = 0; m < 5 && rc == SMAC_OK; m++) rc = dalloc(&c->part[m], (size_t)c->nrows);
   if (rc == SMAC_OK) rc = dalloc(&c->S, S_COUNT);
+  if (rc == SMAC_OK) rc = dalloc(&c->fstage, 3 * 1024);
   if (rc == SMAC_OK && hipHostMalloc((void**)&c->Sh, S_COUNT * sizeof(double)) != hipSuccess) rc = SMAC_ERR_OOM;
   if (rc == SMAC_OK) {
     c->ev_ok = true;
@@ -320,7 +328,14 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
   return SMAC_OK;
 }
 
+static void graph_free(smac_ctx* c) {
+  if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
+  c->cg_exec = nullptr;
+  c->cg_nu[0] = c->cg_nu[1] = -1;
+}
+
 static void mg_free(smac_ctx* c) {
+  graph_free(c);
   for (auto b : c->mg_b)
     if (b) (void)hipFree(b);
   c->mg_b.clear();
@@ -358,7 +373,7 @@ extern "C" void smac_destroy(smac_ctx* c) {
   double* arrs[] = {c->u[0],   c->u[1],    c->u[2],    c->us[0],   c->us[1],  c->us[2],  c->p,      c->dp,
                     c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->Ad,     c->res0,
                     c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
-                    c->muf[0], c->muf[1],  c->muf[2]};
+                    c->muf[0], c->muf[1],  c->muf[2], c->fstage};
   for (auto a : arrs)
     if (a) (void)hipFree(a);
   for (int m = 0; m < 5; m++)
@@ -777,8 +792,45 @@ static int mg_setup(smac_ctx* c, int L, int prec) {
 }
 
 template <typename T>
+static void vcycle(smac_ctx* c, int l, int nu1, int nu2);
+
+// Levels 1.. of the V-cycle as one graph launch. The graph is captured on the
+// context stream the first time (and again when the sweep counts change); it
+// holds only kernel nodes (the fills are kernels), and its arguments (level
+// buffers, coefficients, coarse inverse) stay fixed until mg_free.
+template <typename T>
+static bool vcycle_graph(smac_ctx* c, int nu1, int nu2) {
+  if (!c->graphs) return false;
+  if (!c->cg_exec || c->cg_nu[0] != nu1 || c->cg_nu[1] != nu2) {
+    graph_free(c);
+    hipGraph_t gr = nullptr;
+    if (hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      c->graphs = false;
+      return false;
+    }
+    c->capturing = true;
+    vcycle<T>(c, 1, nu1, nu2);
+    c->capturing = false;
+    hipError_t e1 = hipStreamEndCapture(c->st, &gr);
+    hipError_t e2 = e1 == hipSuccess ? hipGraphInstantiate(&c->cg_exec, gr, nullptr, nullptr, 0) : e1;
+    if (gr) (void)hipGraphDestroy(gr);
+    if (e2 != hipSuccess) {
+      c->cg_exec = nullptr;
+      c->graphs = false;  // the launches above were captured, not run: fall back for good
+      (void)hipGetLastError();
+      return false;
+    }
+    c->cg_nu[0] = nu1;
+    c->cg_nu[1] = nu2;
+  }
+  KT(c, "mg_coarse_levels", (void)hipGraphLaunch(c->cg_exec, c->st));
+  return true;
+}
+
+template <typename T>
 static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   using Cc = typename St<T>::C;
+  if (l == 1 && !c->capturing && (int)c->lev.size() > 2 && vcycle_graph<T>(c, nu1, nu2)) return;
   Level& V = c->lev[l];
   T* e = (T*)V.e;
   T* f = (T*)V.f;
@@ -922,6 +974,7 @@ static FinalizeArgs fin(const smac_ctx* c) {
   FinalizeArgs fa;
   memset(&fa, 0, sizeof(fa));
   fa.nrows = c->nrows;
+  fa.stage = c->fstage;
   return fa;
 }
 static void fin_add(FinalizeArgs& fa, const double* part, int kind, int out) {

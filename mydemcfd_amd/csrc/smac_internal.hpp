@@ -107,6 +107,7 @@ struct FinalizeArgs {
   int nrows;
   int op, dst, sa, sb, sc;
   double c;
+  double* stage;  // 3*1024 doubles: per-lane sums of large reductions (may be null)
 };
 
 // device scalar slots

@@ -660,37 +660,50 @@ __global__ __launch_bounds__(256) void k_cvt(Geo g, const double* __restrict__ r
 // --------------------------------------------------------------- finalize
 // Reduce up to four row-partial arrays with the canonical 1024-lane tree
 // (kind 0 = sum, 1 = max) into S[out[m]], then apply one scalar op.
+// lane t of the canonical tree accumulates rows t, t+1024, ... in order;
+// loads are issued 16 at a time so their latencies overlap (the summation
+// order is unchanged)
+__device__ __forceinline__ double fin_lane(const double* __restrict__ part, int kind, int nrows, int t) {
+  double acc = 0.0;
+  int r = t;
+  for (; r + 15 * 1024 < nrows; r += 16 * 1024) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = part[r + u * 1024];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      if (kind == 0) {
+        acc += v[u];
+      } else if (v[u] > acc || v[u] != v[u]) {
+        acc = v[u];
+      }
+    }
+  }
+  for (; r < nrows; r += 1024) {
+    double v = part[r];
+    if (kind == 0) {
+      acc += v;
+    } else if (v > acc || v != v) {
+      acc = v;
+    }
+  }
+  return acc;
+}
+
+// first stage of a large reduction: the 1024 lane sums of array blockIdx.y,
+// 64 lanes per block so that many CUs stream the row partials
+__global__ __launch_bounds__(64) void k_fin_lanes(FinalizeArgs fa) {
+  const int m = blockIdx.y;
+  const int t = (int)blockIdx.x * 64 + (int)threadIdx.x;
+  fa.stage[m * 1024 + t] = fin_lane(fa.part[m], fa.kind[m], fa.nrows, t);
+}
+
 __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs fa, double* __restrict__ S) {
   __shared__ double lds[1024];
   const int t = threadIdx.x;
   for (int m = 0; m < fa.count; m++) {
-    const double* part = fa.part[m];
     const int kind = fa.kind[m];
-    double acc = 0.0;
-    // lane t visits rows t, t+1024, ... in order; loads are issued 16 at a
-    // time so their latencies overlap (the summation order is unchanged)
-    int r = t;
-    for (; r + 15 * 1024 < fa.nrows; r += 16 * 1024) {
-      double v[16];
-#pragma unroll
-      for (int u = 0; u < 16; u++) v[u] = part[r + u * 1024];
-#pragma unroll
-      for (int u = 0; u < 16; u++) {
-        if (kind == 0) {
-          acc += v[u];
-        } else if (v[u] > acc || v[u] != v[u]) {
-          acc = v[u];
-        }
-      }
-    }
-    for (; r < fa.nrows; r += 1024) {
-      double v = part[r];
-      if (kind == 0) {
-        acc += v;
-      } else if (v > acc || v != v) {
-        acc = v;
-      }
-    }
+    double acc = fa.stage ? fa.stage[m * 1024 + t] : fin_lane(fa.part[m], kind, fa.nrows, t);
     lds[t] = acc;
     __syncthreads();
     for (int s = 512; s >= 64; s >>= 1) {
@@ -972,7 +985,15 @@ SMAC_INST_Z(double)
 SMAC_INST_Z(float)
 SMAC_INST_Z(bf16s)
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fa, S);
+  if (fa.stage && fa.count > 0 && fa.nrows >= 16384) {
+    // large reductions: lane sums on 16 blocks per array, then the tree
+    hipLaunchKernelGGL(k_fin_lanes, dim3(16, (unsigned)fa.count), dim3(64), 0, st, fa);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fa, S);
+  } else {
+    FinalizeArgs f1 = fa;
+    f1.stage = nullptr;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, f1, S);
+  }
 }
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st) {
   hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(1), 0, st, S, dst, src);
