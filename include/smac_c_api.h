@@ -66,7 +66,10 @@ enum smac_field {
     SMAC_FIELD_DP = 7,       /* p_delta_    cells   */
     SMAC_FIELD_RHS = 8,      /* rhs_        cells   */
     SMAC_FIELD_ALPHA = 9,    /* alpha_      cells   (VoF fraction, drives rho/mu) */
-    SMAC_FIELD_COUNT = 10
+    SMAC_FIELD_GRADPX = 10,  /* f_gradp_x_  x-faces (written by smac_make_face_gradp) */
+    SMAC_FIELD_GRADPY = 11,  /* f_gradp_y_  y-faces */
+    SMAC_FIELD_GRADPZ = 12,  /* f_gradp_z_  z-faces */
+    SMAC_FIELD_COUNT = 13
 };
 
 /* PCG_TYPES (CFD/Enums.h:28-33) plus the red-black Gauss-Seidel multigrid. */
@@ -199,6 +202,24 @@ int smac_step(smac_ctx* ctx, double dt, const smac_solver_opts* opts, smac_solve
 /* G_SMACSolver::calc_cfl (CFD/G_SMACSolver.cu:1365-1430): convective and viscous
  * CFL numbers for time step dt. */
 int smac_calc_cfl(smac_ctx* ctx, double dt, double* cfl_conv, double* cfl_visc);
+
+/* ---- after the step (CFD/main.cu:316-317) -----------------------------------
+ * G_SMACSolver::update_boundary_ghost (CFD/G_SMACSolver.cu:1345-1362):
+ *  - ghost pressure p(g) = p(s) + rho(s)*(r.g) with s the nearest interior cell
+ *    and r = x(g) - x(s) (k_update_cell_ghost_pressure, CFD/G_SMACSolver.cu:1227-1267,
+ *    which overwrites every ghost k_update_cell_boundary_pressure wrote);
+ *  - tangential velocity ghosts by d_get_velocity_ghost_value with the bc id of the
+ *    face one row inside, and edge ghosts as the mean of the two adjacent ghosts
+ *    (k_update_v{x,y,z}_ghost[_corner], CFD/G_BoundaryFunctions.cu:257-540);
+ *  - outlet faces (k_update_v{x,y,z}_outlet, CFD/G_BoundaryFunctions.cu:143-255)
+ *    have no case: smac_create rejects outlet sides.
+ * Periodic directions (the extension) get periodic images instead. dt is the
+ * outlet update's time step. */
+int smac_update_boundary_ghost(smac_ctx* ctx, double dt);
+/* G_SMACSolver::make_face_gradp (CFD/G_SMACSolver.cu:1277-1343): face pressure
+ * gradient (p(c) - p(c-1))/h of each axis into SMAC_FIELD_GRADP{X,Y,Z}, on the
+ * faces whose three indices all lie in 1..N+1 (others stay 0). */
+int smac_make_face_gradp(smac_ctx* ctx);
 
 /* Adjoint (vector-Jacobian product) of one smac_step with time step dt taken
  * from the velocity u_n (three device arrays, x/y/z faces, reference layout):

@@ -180,7 +180,10 @@ struct G_SMACSolver {
     pressure_solver_->solve(*this);
   }
   void correct_vof_velocity(const SMACSolver& solv) { check(smac_correct(ctx_, solv.dt_), "smac_correct"); }
-  void update_boundary_ghost(const SMACSolver&) {}  // not needed by the next step for the built BCs
+  void update_boundary_ghost(const SMACSolver& solv) {
+    check(smac_update_boundary_ghost(ctx_, solv.dt_), "smac_update_boundary_ghost");
+  }
+  void make_face_gradp() { check(smac_make_face_gradp(ctx_), "smac_make_face_gradp"); }
 };
 
 // CFD/pressure_solver/G_PCGSolver.h:17-37. The stopping rule is the

@@ -25,9 +25,11 @@ SMAC_ERR_UNSUPPORTED = -7
 BC_NOSLIP, BC_SLIP, BC_INFLOW, BC_OUTFLOW, BC_OUTLET = range(5)
 XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX = range(6)
 
-FIELD_U, FIELD_V, FIELD_W, FIELD_USTAR, FIELD_VSTAR, FIELD_WSTAR, FIELD_P, FIELD_DP, FIELD_RHS, FIELD_ALPHA = range(10)
+(FIELD_U, FIELD_V, FIELD_W, FIELD_USTAR, FIELD_VSTAR, FIELD_WSTAR, FIELD_P, FIELD_DP, FIELD_RHS, FIELD_ALPHA,
+ FIELD_GRADPX, FIELD_GRADPY, FIELD_GRADPZ) = range(13)
 FIELD_NAMES = {"u": FIELD_U, "v": FIELD_V, "w": FIELD_W, "ustar": FIELD_USTAR, "vstar": FIELD_VSTAR,
-               "wstar": FIELD_WSTAR, "p": FIELD_P, "dp": FIELD_DP, "rhs": FIELD_RHS, "alpha": FIELD_ALPHA}
+               "wstar": FIELD_WSTAR, "p": FIELD_P, "dp": FIELD_DP, "rhs": FIELD_RHS, "alpha": FIELD_ALPHA,
+               "gradpx": FIELD_GRADPX, "gradpy": FIELD_GRADPY, "gradpz": FIELD_GRADPZ}
 
 PRECOND_JACOBI, PRECOND_MG, PRECOND_REF_GMG = 0, 1, 2
 PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16 = 0, 1, 2
@@ -88,6 +90,8 @@ SIGNATURES = {
     "smac_correct": (C.c_int, [_P, C.c_double]),
     "smac_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "smac_calc_cfl": (C.c_int, [_P, C.c_double, _D, _D]),
+    "smac_update_boundary_ghost": (C.c_int, [_P, C.c_double]),
+    "smac_make_face_gradp": (C.c_int, [_P]),
     "smac_adjoint_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(_P * 3), C.POINTER(_P * 3), _P,
                                     C.POINTER(SolveStats)]),
     "smac_default_opts": (None, [C.POINTER(SolverOpts)]),
@@ -135,11 +139,11 @@ def field_shape(n, field: int):
     """(z, y, x) numpy shape of a field in the reference padded layout."""
     nx, ny, nz = n
     sh = [nx + 2, ny + 2, nz + 2]
-    if field in (FIELD_U, FIELD_USTAR):
+    if field in (FIELD_U, FIELD_USTAR, FIELD_GRADPX):
         sh[0] += 1
-    elif field in (FIELD_V, FIELD_VSTAR):
+    elif field in (FIELD_V, FIELD_VSTAR, FIELD_GRADPY):
         sh[1] += 1
-    elif field in (FIELD_W, FIELD_WSTAR):
+    elif field in (FIELD_W, FIELD_WSTAR, FIELD_GRADPZ):
         sh[2] += 1
     return (sh[2], sh[1], sh[0])
 
@@ -254,6 +258,15 @@ class Context:
         a, b = C.c_double(), C.c_double()
         self._check(self.lib.smac_calc_cfl(self.h, dt, C.byref(a), C.byref(b)), "smac_calc_cfl")
         return a.value, b.value
+
+    def update_boundary_ghost(self, dt):
+        """Ghost pressure and tangential velocity ghosts after a step
+        (G_SMACSolver::update_boundary_ghost)."""
+        self._check(self.lib.smac_update_boundary_ghost(self.h, dt), "smac_update_boundary_ghost")
+
+    def make_face_gradp(self):
+        """Face pressure gradient into gradpx/gradpy/gradpz (G_SMACSolver::make_face_gradp)."""
+        self._check(self.lib.smac_make_face_gradp(self.h), "smac_make_face_gradp")
 
     def set_profiling(self, on=True):
         self._check(self.lib.smac_set_profiling(self.h, int(on)), "smac_set_profiling")

@@ -95,6 +95,7 @@ struct smac_ctx {
   // adjoint work arrays (box layout, allocated on first use): cotangents of
   // u^{n+1}, u*, u^n (3 faces each), of p^{n+1} and of p^n
   double* adj[11] = {nullptr};
+  double* gradp[3] = {nullptr, nullptr, nullptr};  // f_gradp_{x,y,z}, allocated on first use
   double* rgt = nullptr;  // fine ping-pong partner of z in the reference V-cycle
   bool prof = false;
   hipEvent_t ev[8];
@@ -373,7 +374,7 @@ extern "C" void smac_destroy(smac_ctx* c) {
   double* arrs[] = {c->u[0],   c->u[1],    c->u[2],    c->us[0],   c->us[1],  c->us[2],  c->p,      c->dp,
                     c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->Ad,     c->res0,
                     c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
-                    c->muf[0], c->muf[1],  c->muf[2], c->fstage};
+                    c->muf[0], c->muf[1],  c->muf[2], c->fstage, c->gradp[0], c->gradp[1], c->gradp[2]};
   for (auto a : arrs)
     if (a) (void)hipFree(a);
   for (int m = 0; m < 5; m++)
@@ -399,9 +400,9 @@ extern "C" int smac_set_fluid(smac_ctx* c, const smac_fluid_desc* fl) {
 // ------------------------------------------------------------------ data movement
 static int field_axis(int f) {
   switch (f) {
-    case SMAC_FIELD_U: case SMAC_FIELD_USTAR: return 0;
-    case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: return 1;
-    case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: return 2;
+    case SMAC_FIELD_U: case SMAC_FIELD_USTAR: case SMAC_FIELD_GRADPX: return 0;
+    case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: case SMAC_FIELD_GRADPY: return 1;
+    case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: case SMAC_FIELD_GRADPZ: return 2;
     default: return -1;
   }
 }
@@ -422,6 +423,9 @@ static double** field_slot(smac_ctx* c, int f) {
     case SMAC_FIELD_DP: return &c->dp;
     case SMAC_FIELD_RHS: return &c->rhs;
     case SMAC_FIELD_ALPHA: return &c->alpha;
+    case SMAC_FIELD_GRADPX: return &c->gradp[0];
+    case SMAC_FIELD_GRADPY: return &c->gradp[1];
+    case SMAC_FIELD_GRADPZ: return &c->gradp[2];
     default: return nullptr;
   }
 }
@@ -520,6 +524,10 @@ extern "C" int smac_upload_device(smac_ctx* c, int32_t f, const double* dsrc, si
   if (!c || !dsrc || f < 0 || f >= SMAC_FIELD_COUNT || f == SMAC_FIELD_ALPHA) return SMAC_ERR_INVALID_ARG;
   if (count != smac_field_count(c, f)) return SMAC_ERR_INVALID_ARG;
   double** slot = field_slot(c, f);
+  if (!*slot) {
+    int rc = dalloc(slot, c->tot);
+    if (rc) return rc;
+  }
   return copy_field(c, f, *slot, dsrc, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
 }
 
@@ -1258,6 +1266,30 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
   sp->ms_solve = t_solve;
   sp->ms_correct = t_corr;
   return rc;
+}
+
+// ------------------------------------------------------------------ after the step
+extern "C" int smac_update_boundary_ghost(smac_ctx* c, double dt) {
+  (void)dt;  // the outlet update's; outlet sides are rejected at create
+  if (!c) return SMAC_ERR_INVALID_ARG;
+  int rc = ensure_props(c);
+  if (rc) return rc;
+  KT(c, "ghost_pressure", launch_ghost_pressure(c->g, c->d, c->p, var_props(c) ? c->rho : nullptr, c->st));
+  KT(c, "ghost_velocity", launch_ghost_velocity(c->g, c->u, c->st));
+  HIPOK(hipGetLastError());
+  return SMAC_OK;
+}
+
+extern "C" int smac_make_face_gradp(smac_ctx* c) {
+  if (!c) return SMAC_ERR_INVALID_ARG;
+  for (int a = 0; a < 3; a++)
+    if (!c->gradp[a]) {
+      int rc = dalloc(&c->gradp[a], c->tot);
+      if (rc) return rc;
+    }
+  KT(c, "face_gradp", launch_face_gradp(c->g, c->p, c->gradp, c->st));
+  HIPOK(hipGetLastError());
+  return SMAC_OK;
 }
 
 // Adjoint of one smac_step (see smac_adjoint.hip for the derivation).

@@ -152,6 +152,20 @@ void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st);
 // inv_rho == nullptr: one-phase viscous CFL `visc`; else (mu/rho) dt sum 1/h^2 per cell
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, const double* inv_rho, const double* mu,
                 double* p0, double* p1, hipStream_t st);
+// after the step (smac_post.hip): a launch covers planes q[ax] = v[0..nv) of
+// one array, the other two axes u, w over [lo, lo+cnt)
+struct Plane {
+  int ax, nv, v[3];
+  int u, ulo, ucnt;
+  int w, wlo, wcnt;
+};
+struct Spacing {
+  double h[3];
+};
+// ghost cells of p; rho == nullptr: uniform density g.rho0
+void launch_ghost_pressure(const Geo& g, const double h[3], double* p, const double* rho, hipStream_t st);
+void launch_ghost_velocity(const Geo& g, double* const u[3], hipStream_t st);
+void launch_face_gradp(const Geo& g, const double* p, double* const gp[3], hipStream_t st);
 // multigrid (smac_mg.hip), storage T in {double, float, bf16s}; the restriction
 // buffer tb has the compute type of T
 template <typename T>

@@ -67,6 +67,7 @@ struct smac_ctx {
     double *vel[3], *vst[3], *p_, *dp, *rhs, *alpha;
     double *rho, *rho_old, *inv_rho, *mu;
     double *beta[3], *inv_rhof[3], *muf[3], *mf[3], *rhof[3];
+    double *gradp[3];    /* f_gradp_{x,y,z}_ (smac_make_face_gradp) */
     /* PCG */
     double *r, *z, *dvec, *Ad;
     double *Adiag, *Aoff[6];
@@ -1402,13 +1403,134 @@ int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
     return SMAC_OK;
 }
 
+/* ------------------------------------------------ after the step */
+/* d_get_velocity_ghost_value (CFD/G_BoundaryFunctions.cu:257-275) with the IBM
+ * solid fraction lambda = 0 (no immersed solid): bval is the face one row
+ * inside the wall, inval the face two rows inside. */
+static double ghost_value(int bt, double bv, double bval, double inval) {
+    if (bt == SMAC_BC_INFLOW) return 2.0 * (1.0 - 0.0) * bv - inval;
+    if (bt == SMAC_BC_NOSLIP) return -inval;
+    if (bt == SMAC_BC_OUTLET) return 2.0 * bval - inval;
+    if (bt == SMAC_BC_SLIP) return inval;
+    return bval;
+}
+
+/* ghost of component a across the wall of tangential axis t at q (q.v[t] is 0
+ * or n+1): the body of k_update_v{x,y,z}_ghost (CFD/G_BoundaryFunctions.cu:277-336,
+ * 365-424, 454-513), bc id of the a-face one row inside */
+static double wall_ghost(const struct smac_ctx* c, int a, int t, ivec q) {
+    int lower = q.v[t] == 0;
+    ivec qb = q, qi = q;
+    qb.v[t] = lower ? 1 : c->n[t];
+    qi.v[t] = lower ? 2 : c->n[t] - 1;
+    int bid = face_bcid(c, a, qb);
+    return ghost_value(c->bctype[bid], c->bcv[bid][a], c->vel[a][IX(c, qb)], c->vel[a][IX(c, qi)]);
+}
+
+/* value of velocity component a at position q of its face array after the
+ * ghost update: periodic directions resolve to their image; a ghost across one
+ * wall follows wall_ghost; an edge ghost across two walls is the mean of the
+ * two adjacent wall ghosts, the one across the lower-numbered axis first
+ * (k_update_v{x,y,z}_ghost_corner, CFD/G_BoundaryFunctions.cu:338-363,426-452,515-540) */
+static double ghost_vel(const struct smac_ctx* c, int a, ivec q) {
+    for (int d = 0; d < 3; d++) {
+        if (!c->per[d]) continue;
+        q.v[d] = wrap(q.v[d], c->n[d]);
+        if (d == a && q.v[d] > c->n[d]) q.v[d] -= c->n[d];
+    }
+    int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    int g1 = q.v[t1] == 0 || q.v[t1] == c->n[t1] + 1;
+    int g2 = q.v[t2] == 0 || q.v[t2] == c->n[t2] + 1;
+    if (!g1 && !g2) return c->vel[a][IX(c, q)];
+    if (g1 && !g2) return wall_ghost(c, a, t1, q);
+    if (!g1 && g2) return wall_ghost(c, a, t2, q);
+    ivec q1 = q, q2 = q;
+    q1.v[t2] = q.v[t2] == 0 ? 1 : c->n[t2];
+    q2.v[t1] = q.v[t1] == 0 ? 1 : c->n[t1];
+    return 0.5 * (wall_ghost(c, a, t1, q1) + wall_ghost(c, a, t2, q2));
+}
+
+/* G_SMACSolver::update_boundary_ghost (CFD/G_SMACSolver.cu:1345-1362) */
+int smac_update_boundary_ghost(smac_ctx* c, double dt) {
+    (void)dt; /* the outlet update's; outlet sides are rejected at create */
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    const double gv[3] = {c->fl.gx, c->fl.gy, c->fl.gz};
+    /* k_update_cell_ghost_pressure (CFD/G_SMACSolver.cu:1227-1267) */
+    for (int k = 0; k <= c->n[2] + 1; k++)
+        for (int j = 0; j <= c->n[1] + 1; j++)
+            for (int i = 0; i <= c->n[0] + 1; i++) {
+                ivec q = iv(i, j, k);
+                if (cell_interior(c, q)) continue;
+                ivec s = q;
+                double r[3];
+                for (int d = 0; d < 3; d++) {
+                    if (c->per[d]) s.v[d] = wrap(s.v[d], c->n[d]);
+                    else if (s.v[d] < 1) s.v[d] = 1;
+                    else if (s.v[d] > c->n[d]) s.v[d] = c->n[d];
+                    r[d] = c->per[d] ? 0.0 : (double)(q.v[d] - s.v[d]) * c->d[d];
+                }
+                size_t is = IX(c, s);
+                c->p_[IX(c, q)] = c->p_[is] + c->rho[is] * (r[0] * gv[0] + r[1] * gv[1] + r[2] * gv[2]);
+            }
+    /* velocity ghosts: every position of the face array with a ghost index
+     * along a tangential axis, or along its own axis when that is periodic;
+     * faces 0 and n+2 along a wall axis are not touched (the kernels run
+     * the own index over 1..n+1). Values are computed from entries that no
+     * write changes, then stored. */
+    for (int a = 0; a < 3; a++) {
+        int sh[3] = {c->n[0] + 2, c->n[1] + 2, c->n[2] + 2};
+        sh[a]++;
+        size_t cnt = (size_t)sh[0] * sh[1] * sh[2], m = 0;
+        size_t* pos = (size_t*)malloc(cnt * sizeof(size_t));
+        double* val = (double*)malloc(cnt * sizeof(double));
+        if (!pos || !val) { free(pos); free(val); return SMAC_ERR_OOM; }
+        for (int k = 0; k < sh[2]; k++)
+            for (int j = 0; j < sh[1]; j++)
+                for (int i = 0; i < sh[0]; i++) {
+                    ivec q = iv(i, j, k);
+                    int ghost = 0, skip = 0;
+                    for (int d = 0; d < 3; d++) {
+                        int x = q.v[d], n = c->n[d];
+                        if (d == a) {
+                            if (c->per[d]) ghost |= (x < 1 || x > n);
+                            else skip |= (x < 1 || x > n + 1);
+                        } else {
+                            ghost |= (x < 1 || x > n);
+                        }
+                    }
+                    if (!ghost || skip) continue;
+                    pos[m] = IX(c, q);
+                    val[m] = ghost_vel(c, a, q);
+                    m++;
+                }
+        for (size_t t = 0; t < m; t++) c->vel[a][pos[t]] = val[t];
+        free(pos);
+        free(val);
+    }
+    return SMAC_OK;
+}
+
+/* G_SMACSolver::make_face_gradp (CFD/G_SMACSolver.cu:1277-1343) */
+int smac_make_face_gradp(smac_ctx* c) {
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    for (int a = 0; a < 3; a++)
+        for (int k = 1; k <= c->n[2] + 1; k++)
+            for (int j = 1; j <= c->n[1] + 1; j++)
+                for (int i = 1; i <= c->n[0] + 1; i++) {
+                    ivec q = iv(i, j, k);
+                    c->gradp[a][IX(c, q)] = (c->p_[IX(c, q)] - c->p_[IX(c, add(q, a, -1))]) * c->inv_d[a];
+                }
+    return SMAC_OK;
+}
+
 /* --------------------------------------------------------- lifecycle, IO */
 static int shape_of(const struct smac_ctx* c, int field, int sh[3]) {
     sh[0] = c->n[0] + 2; sh[1] = c->n[1] + 2; sh[2] = c->n[2] + 2;
     switch (field) {
-        case SMAC_FIELD_U: case SMAC_FIELD_USTAR: sh[0]++; return 0;
-        case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: sh[1]++; return 1;
-        case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: sh[2]++; return 2;
+        case SMAC_FIELD_U: case SMAC_FIELD_USTAR: case SMAC_FIELD_GRADPX: sh[0]++; return 0;
+        case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: case SMAC_FIELD_GRADPY: sh[1]++; return 1;
+        case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: case SMAC_FIELD_GRADPZ: sh[2]++; return 2;
         default: return -1;
     }
 }
@@ -1424,6 +1546,9 @@ static double* field_ptr(struct smac_ctx* c, int field) {
         case SMAC_FIELD_DP: return c->dp;
         case SMAC_FIELD_RHS: return c->rhs;
         case SMAC_FIELD_ALPHA: return c->alpha;
+        case SMAC_FIELD_GRADPX: return c->gradp[0];
+        case SMAC_FIELD_GRADPY: return c->gradp[1];
+        case SMAC_FIELD_GRADPZ: return c->gradp[2];
         default: return NULL;
     }
 }
@@ -1527,7 +1652,7 @@ int smac_create(const smac_grid_desc* g, const smac_fluid_desc* f, smac_ctx** ou
                        &c->beta[2], &c->inv_rhof[0], &c->inv_rhof[1], &c->inv_rhof[2], &c->muf[0], &c->muf[1],
                        &c->muf[2], &c->mf[0], &c->mf[1], &c->mf[2], &c->rhof[0], &c->rhof[1], &c->rhof[2],
                        &c->r, &c->z, &c->dvec, &c->Ad, &c->Adiag, &c->Aoff[0], &c->Aoff[1], &c->Aoff[2],
-                       &c->Aoff[3], &c->Aoff[4], &c->Aoff[5]};
+                       &c->Aoff[3], &c->Aoff[4], &c->Aoff[5], &c->gradp[0], &c->gradp[1], &c->gradp[2]};
     for (size_t t = 0; t < sizeof(arrs) / sizeof(arrs[0]); t++) {
         *arrs[t] = (double*)calloc(c->tot, sizeof(double));
         if (!*arrs[t]) { smac_destroy(c); return SMAC_ERR_OOM; }
@@ -1544,7 +1669,8 @@ void smac_destroy(smac_ctx* c) {
                       c->alpha, c->rho, c->rho_old, c->inv_rho, c->mu, c->beta[0], c->beta[1], c->beta[2],
                       c->inv_rhof[0], c->inv_rhof[1], c->inv_rhof[2], c->muf[0], c->muf[1], c->muf[2], c->mf[0],
                       c->mf[1], c->mf[2], c->rhof[0], c->rhof[1], c->rhof[2], c->r, c->z, c->dvec, c->Ad,
-                      c->Adiag, c->Aoff[0], c->Aoff[1], c->Aoff[2], c->Aoff[3], c->Aoff[4], c->Aoff[5]};
+                      c->Adiag, c->Aoff[0], c->Aoff[1], c->Aoff[2], c->Aoff[3], c->Aoff[4], c->Aoff[5],
+                      c->gradp[0], c->gradp[1], c->gradp[2]};
     for (size_t t = 0; t < sizeof(arrs) / sizeof(arrs[0]); t++) free(arrs[t]);
     free(c);
 }

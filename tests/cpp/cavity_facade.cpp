@@ -57,12 +57,17 @@ int main(int argc, char** argv) {
       g.update_vstar_boundary();
       g.solve_poisson();
       g.correct_vof_velocity(solv);
+      g.update_boundary_ghost(solv);
+      g.make_face_gradp();
       std::printf("step %d PCG iter = %d rel_res = %.3e\n", step, g.last_stats_.iters, g.last_stats_.rel_res);
     }
     dump(g, SMAC_FIELD_U, out + "/u.f64");
     dump(g, SMAC_FIELD_V, out + "/v.f64");
     dump(g, SMAC_FIELD_W, out + "/w.f64");
     dump(g, SMAC_FIELD_P, out + "/p.f64");
+    dump(g, SMAC_FIELD_GRADPX, out + "/gradpx.f64");
+    dump(g, SMAC_FIELD_GRADPY, out + "/gradpy.f64");
+    dump(g, SMAC_FIELD_GRADPZ, out + "/gradpz.f64");
   } catch (const smac_error& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return 1;

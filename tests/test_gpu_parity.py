@@ -209,7 +209,9 @@ def test_cpp_facade_driver(hip, oracle_mod, tmp_path, mode):
     for _ in range(5):
         o.step(1e-3, precond=precond, stop_rule="ref_linf_dt", tol=1e-8,
                check_every=10 if mode == "jacobi" else 5, mg_levels=0 if mode == "jacobi" else 4)
-    for f in ("u", "v", "w", "p"):
+        o.update_boundary_ghost(1e-3)
+        o.make_face_gradp()
+    for f in ("u", "v", "w", "p", "gradpx", "gradpy", "gradpz"):
         a = np.fromfile(os.path.join(tmp_path, f + ".f64"), dtype=np.float64).reshape(o.shape(f))
         b = o.download(f)
         err = rel_err_mod_mean(a, b, (n, n, n)) if f == "p" else rel_err(a, b)
@@ -359,3 +361,42 @@ def test_two_phase_mixed_vs_fp64_oracle(hip, oracle_mod, precision):
         # BF16 smoothing is weaker across the 1000:1 density jump: more iterations, same tolerance
         assert sg.converged and sg.iters <= 2 * so.iters, (s, sg.as_dict(), so.as_dict())
     _compare(g, o, n, tol=1e-6)
+
+
+@pytest.mark.parametrize("case", ["walls_mixed", "periodic_xz", "two_phase"])
+def test_post_step_ghosts_and_gradp(hip, oracle_mod, case):
+    """update_boundary_ghost + make_face_gradp from identical random state
+    (ghosts included): every entry of u, v, w, p and of the three face
+    gradients equals the oracle's."""
+    from mydemcfd_amd import _abi, cases
+
+    print(f"case: post-step ghosts and face gradient, {case}", flush=True)
+    rng = np.random.default_rng(31)
+    if case == "walls_mixed":
+        n = (12, 10, 8)
+        desc = dict(n=n, size=(1.2, 1.0, 0.8), origin=(0.0, 0.0, 0.0), periodic=(0, 0, 0),
+                    side_bcid=(2, 1, 3, 1, 0, 1),
+                    bc_table=((_abi.BC_SLIP, 0, 0, 0), (_abi.BC_NOSLIP, 0.3, -0.2, 0.1),
+                              (_abi.BC_INFLOW, 0.5, 0.0, 0.0), (_abi.BC_SLIP, 0, 0, 0)),
+                    rho=(2.0, 2.0), mu=(0.01, 0.01), g=(0.5, -9.81, 1.25))
+    elif case == "periodic_xz":
+        n = (16, 8, 12)
+        desc = dict(n=n, size=(1.0, 0.5, 0.75), origin=(0.0, 0.0, 0.0), periodic=(1, 0, 1),
+                    side_bcid=(0, 0, 1, 1, 0, 0),
+                    bc_table=((_abi.BC_NOSLIP, 0, 0, 0), (_abi.BC_NOSLIP, 1.0, 0.0, 0.0)),
+                    rho=(1.0, 1.0), mu=(0.01, 0.01), g=(0.0, -1.0, 0.0))
+    else:
+        n = (16, 8, 8)
+        desc = cases.dam_break_desc(n, length=(2.0, 1.0, 1.0))
+    g, o = _pair(hip, oracle_mod, desc)
+    fields = {f: rng.standard_normal(g.shape(f)) for f in ("u", "v", "w", "p")}
+    if case == "two_phase":
+        fields["alpha"] = cases.dam_break_alpha(n)
+    _upload_both(g, o, fields)
+    for c in (g, o):
+        c.update_boundary_ghost(1e-3)
+        c.make_face_gradp()
+    for f in ("u", "v", "w", "p", "gradpx", "gradpy", "gradpz"):
+        a, b = g.download(f), o.download(f)
+        diff = float(np.max(np.abs(a - b)))
+        assert diff == 0.0, (f, diff)
