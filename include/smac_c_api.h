@@ -69,7 +69,10 @@ enum smac_field {
     SMAC_FIELD_GRADPX = 10,  /* f_gradp_x_  x-faces (written by smac_make_face_gradp) */
     SMAC_FIELD_GRADPY = 11,  /* f_gradp_y_  y-faces */
     SMAC_FIELD_GRADPZ = 12,  /* f_gradp_z_  z-faces */
-    SMAC_FIELD_COUNT = 13
+    SMAC_FIELD_FLUXX = 13,   /* f_Fx_accum_ x-faces (VoF alpha flux of the last transport) */
+    SMAC_FIELD_FLUXY = 14,   /* f_Fy_accum_ y-faces */
+    SMAC_FIELD_FLUXZ = 15,   /* f_Fz_accum_ z-faces */
+    SMAC_FIELD_COUNT = 16
 };
 
 /* PCG_TYPES (CFD/Enums.h:28-33) plus the red-black Gauss-Seidel multigrid. */
@@ -181,8 +184,19 @@ size_t smac_field_count(const smac_ctx* ctx, int32_t field);
  * host form SMACSolver::update_properties_by_alpha_initial
  * (CFD/SMACSolver.cpp:283-421) when initial != 0 (rho_old = rho). */
 int smac_update_properties(smac_ctx* ctx, int32_t initial);
-/* Mass flux mf = rho0*u_f (alpha flux = 0):
- * G_SMACSolver::compute_mass_flux_from_alpha_flux (CFD/G_SMACSolver_VoF.cu:1708-1795). */
+/* VoF transport of alpha over one time step dt by the split THINC/WLIC scheme
+ * (CFD/main.cu:283-295): G_SMACSolver::clear_alpha_flux_accum, then for each of
+ * `substeps` substeps clear_alpha_flux, alpha_flux_thincwlic_split(dt/substeps, step)
+ * and alpha_flux_accum (CFD/G_SMACSolver_VoF.cu:451-1190,1575-1706). step (the
+ * time-step counter) picks the direction order, step % 3 -> xyz, yzx, zxy. The
+ * accumulated face fluxes (SMAC_FIELD_FLUX{X,Y,Z}) enter the mass flux of the next
+ * predictor. Call smac_update_properties(ctx, 1) once before the first
+ * transport, as the reference's t = 0 setup does. */
+int smac_transport_alpha(smac_ctx* ctx, double dt, int32_t substeps, int32_t step);
+/* Mass flux G_SMACSolver::compute_mass_flux_from_alpha_flux
+ * (CFD/G_SMACSolver_VoF.cu:1708-1795): mf = rho0*u_f + (rho1-rho0)*F_accum*h/dt
+ * right after smac_transport_alpha, else mf = rho0*u_f (no alpha flux). The
+ * predictor forms it itself; the transported flux serves one predictor. */
 int smac_compute_mass_flux(smac_ctx* ctx, double dt);
 /* G_SMACSolver::get_vof_vstar_rhouu_consistent + update_vstar_boundary
  * (CFD/G_SMACSolver.h:71,74; CFD/G_SMACSolver.cu:1044-1069). */

@@ -26,10 +26,11 @@ BC_NOSLIP, BC_SLIP, BC_INFLOW, BC_OUTFLOW, BC_OUTLET = range(5)
 XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX = range(6)
 
 (FIELD_U, FIELD_V, FIELD_W, FIELD_USTAR, FIELD_VSTAR, FIELD_WSTAR, FIELD_P, FIELD_DP, FIELD_RHS, FIELD_ALPHA,
- FIELD_GRADPX, FIELD_GRADPY, FIELD_GRADPZ) = range(13)
+ FIELD_GRADPX, FIELD_GRADPY, FIELD_GRADPZ, FIELD_FLUXX, FIELD_FLUXY, FIELD_FLUXZ) = range(16)
 FIELD_NAMES = {"u": FIELD_U, "v": FIELD_V, "w": FIELD_W, "ustar": FIELD_USTAR, "vstar": FIELD_VSTAR,
                "wstar": FIELD_WSTAR, "p": FIELD_P, "dp": FIELD_DP, "rhs": FIELD_RHS, "alpha": FIELD_ALPHA,
-               "gradpx": FIELD_GRADPX, "gradpy": FIELD_GRADPY, "gradpz": FIELD_GRADPZ}
+               "gradpx": FIELD_GRADPX, "gradpy": FIELD_GRADPY, "gradpz": FIELD_GRADPZ,
+               "fluxx": FIELD_FLUXX, "fluxy": FIELD_FLUXY, "fluxz": FIELD_FLUXZ}
 
 PRECOND_JACOBI, PRECOND_MG, PRECOND_REF_GMG = 0, 1, 2
 PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16 = 0, 1, 2
@@ -85,6 +86,7 @@ SIGNATURES = {
     "smac_field_count": (C.c_size_t, [_P, C.c_int32]),
     "smac_update_properties": (C.c_int, [_P, C.c_int32]),
     "smac_compute_mass_flux": (C.c_int, [_P, C.c_double]),
+    "smac_transport_alpha": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32]),
     "smac_predictor": (C.c_int, [_P, C.c_double]),
     "smac_pressure_solve": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "smac_correct": (C.c_int, [_P, C.c_double]),
@@ -139,11 +141,11 @@ def field_shape(n, field: int):
     """(z, y, x) numpy shape of a field in the reference padded layout."""
     nx, ny, nz = n
     sh = [nx + 2, ny + 2, nz + 2]
-    if field in (FIELD_U, FIELD_USTAR, FIELD_GRADPX):
+    if field in (FIELD_U, FIELD_USTAR, FIELD_GRADPX, FIELD_FLUXX):
         sh[0] += 1
-    elif field in (FIELD_V, FIELD_VSTAR, FIELD_GRADPY):
+    elif field in (FIELD_V, FIELD_VSTAR, FIELD_GRADPY, FIELD_FLUXY):
         sh[1] += 1
-    elif field in (FIELD_W, FIELD_WSTAR, FIELD_GRADPZ):
+    elif field in (FIELD_W, FIELD_WSTAR, FIELD_GRADPZ, FIELD_FLUXZ):
         sh[2] += 1
     return (sh[2], sh[1], sh[0])
 
@@ -258,6 +260,11 @@ class Context:
         a, b = C.c_double(), C.c_double()
         self._check(self.lib.smac_calc_cfl(self.h, dt, C.byref(a), C.byref(b)), "smac_calc_cfl")
         return a.value, b.value
+
+    def transport_alpha(self, dt, substeps=1, step=0):
+        """VoF transport of alpha over one time step (split THINC/WLIC); the
+        accumulated flux enters the next predictor's mass flux."""
+        self._check(self.lib.smac_transport_alpha(self.h, dt, int(substeps), int(step)), "smac_transport_alpha")
 
     def update_boundary_ghost(self, dt):
         """Ghost pressure and tangential velocity ghosts after a step

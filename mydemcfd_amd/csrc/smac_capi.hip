@@ -96,6 +96,14 @@ struct smac_ctx {
   // u^{n+1}, u*, u^n (3 faces each), of p^{n+1} and of p^n
   double* adj[11] = {nullptr};
   double* gradp[3] = {nullptr, nullptr, nullptr};  // f_gradp_{x,y,z}, allocated on first use
+  // VoF transport (allocated on first use): alpha_new_, f_F{x,y,z}_,
+  // f_F{x,y,z}_accum_, correct_coeff_; vof_armed = the accumulated flux
+  // belongs to the coming predictor
+  double* alpha_new = nullptr;
+  double* vf[3] = {nullptr, nullptr, nullptr};
+  double* vfacc[3] = {nullptr, nullptr, nullptr};
+  unsigned char* vcoeff = nullptr;
+  bool vof_armed = false;
   double* rgt = nullptr;  // fine ping-pong partner of z in the reference V-cycle
   bool prof = false;
   hipEvent_t ev[8];
@@ -374,13 +382,15 @@ extern "C" void smac_destroy(smac_ctx* c) {
   double* arrs[] = {c->u[0],   c->u[1],    c->u[2],    c->us[0],   c->us[1],  c->us[2],  c->p,      c->dp,
                     c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->Ad,     c->res0,
                     c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
-                    c->muf[0], c->muf[1],  c->muf[2], c->fstage, c->gradp[0], c->gradp[1], c->gradp[2]};
+                    c->muf[0], c->muf[1],  c->muf[2], c->fstage, c->gradp[0], c->gradp[1], c->gradp[2],
+                    c->alpha_new, c->vf[0], c->vf[1], c->vf[2], c->vfacc[0], c->vfacc[1], c->vfacc[2]};
   for (auto a : arrs)
     if (a) (void)hipFree(a);
   for (int m = 0; m < 5; m++)
     if (c->part[m]) (void)hipFree(c->part[m]);
   for (int m = 0; m < 11; m++)
     if (c->adj[m]) (void)hipFree(c->adj[m]);
+  if (c->vcoeff) (void)hipFree(c->vcoeff);
   if (c->Sh) (void)hipHostFree(c->Sh);
   if (c->ev_ok)
     for (int e = 0; e < 8; e++) (void)hipEventDestroy(c->ev[e]);
@@ -403,6 +413,9 @@ static int field_axis(int f) {
     case SMAC_FIELD_U: case SMAC_FIELD_USTAR: case SMAC_FIELD_GRADPX: return 0;
     case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: case SMAC_FIELD_GRADPY: return 1;
     case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: case SMAC_FIELD_GRADPZ: return 2;
+    case SMAC_FIELD_FLUXX: return 0;
+    case SMAC_FIELD_FLUXY: return 1;
+    case SMAC_FIELD_FLUXZ: return 2;
     default: return -1;
   }
 }
@@ -426,6 +439,9 @@ static double** field_slot(smac_ctx* c, int f) {
     case SMAC_FIELD_GRADPX: return &c->gradp[0];
     case SMAC_FIELD_GRADPY: return &c->gradp[1];
     case SMAC_FIELD_GRADPZ: return &c->gradp[2];
+    case SMAC_FIELD_FLUXX: return &c->vfacc[0];
+    case SMAC_FIELD_FLUXY: return &c->vfacc[1];
+    case SMAC_FIELD_FLUXZ: return &c->vfacc[2];
     default: return nullptr;
   }
 }
@@ -582,7 +598,21 @@ extern "C" int smac_predictor(smac_ctx* c, double dt) {
   Vel V;
   for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
   const Props P = props_of(c);
-  KT(c, "predict", launch_predict(c->g, var_props(c) ? &P : nullptr, V, c->p, c->us, dt, c->st));
+  // right after a VoF transport of a two-phase grid the mass flux carries the
+  // accumulated alpha flux, mf = rho0 u + (rho1 - rho0) F h/dt
+  const double drho = c->fl.rho1 - c->fl.rho0;
+  VofF M{};
+  const bool vof = c->vof_armed && var_props(c) && drho != 0.0;
+  if (vof) {
+    const double inv_dt = 1. / dt;
+    for (int a = 0; a < 3; a++) {
+      M.F[a] = c->vfacc[a];
+      M.hbydt[a] = c->d[a] * inv_dt;
+    }
+    M.drho = drho;
+  }
+  c->vof_armed = false;
+  KT(c, "predict", launch_predict(c->g, var_props(c) ? &P : nullptr, V, c->p, c->us, dt, c->st, vof ? &M : nullptr));
   HIPOK(hipGetLastError());
   return SMAC_OK;
 }
@@ -1268,6 +1298,34 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
   return rc;
 }
 
+// ------------------------------------------------------------------ VoF transport
+static int vof_alloc(smac_ctx* c) {
+  if (c->alpha_new) return SMAC_OK;
+  int rc = c->alpha ? SMAC_OK : dalloc(&c->alpha, c->tot);
+  if (!rc) rc = dalloc(&c->alpha_new, c->tot);
+  for (int a = 0; a < 3 && !rc; a++) rc = dalloc(&c->vf[a], c->tot);
+  for (int a = 0; a < 3 && !rc; a++) rc = dalloc(&c->vfacc[a], c->tot);
+  if (!rc) rc = balloc((void**)&c->vcoeff, c->tot);
+  return rc;
+}
+
+extern "C" int smac_transport_alpha(smac_ctx* c, double dt, int32_t substeps, int32_t step) {
+  if (!c || !(dt > 0) || substeps < 1 || step < 0) return SMAC_ERR_INVALID_ARG;
+  int rc = vof_alloc(c);
+  if (rc) return rc;
+  for (int a = 0; a < 3; a++) HIPOK(hipMemsetAsync(c->vfacc[a], 0, c->tot * sizeof(double), c->st));
+  Vel V;
+  for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
+  const double sub_dt = dt / (double)substeps;
+  for (int s = 0; s < substeps; s++)
+    KT(c, "vof_substep", launch_vof_substep(c->g, c->d, V, &c->alpha, &c->alpha_new, c->vf, c->vfacc, c->vcoeff,
+                                            sub_dt, step, c->st));
+  HIPOK(hipGetLastError());
+  c->vof_armed = true;
+  c->props_ver++;  // the face coefficients follow the new alpha at the next property update
+  return SMAC_OK;
+}
+
 // ------------------------------------------------------------------ after the step
 extern "C" int smac_update_boundary_ghost(smac_ctx* c, double dt) {
   (void)dt;  // the outlet update's; outlet sides are rejected at create
@@ -1298,6 +1356,9 @@ extern "C" int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts*
   if (!c || !(dt > 0) || !u_n || !ubar || !pbar) return SMAC_ERR_INVALID_ARG;
   for (int a = 0; a < 3; a++)
     if (!u_n[a] || !ubar[a]) return SMAC_ERR_INVALID_ARG;
+  // the adjoint holds alpha fixed (mass flux rho0 u): a pending VoF flux is
+  // outside it
+  if (c->vof_armed) return SMAC_ERR_UNSUPPORTED;
   int rc = ensure_props(c);
   if (rc) return rc;
   for (int m = 0; m < 11 && !rc; m++)

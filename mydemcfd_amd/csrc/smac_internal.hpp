@@ -116,9 +116,17 @@ enum {
   S_RR, S_LINF, S_ZAD, S_CFLC, S_CFLV, S_COUNT = 32
 };
 
-// P == nullptr: one-phase (uniform) properties from Geo
+// alpha flux of the VoF transport for the predictor's mass flux:
+// mf = rho0 u_f + drho * F h/dt (hbydt = h/dt per axis)
+struct VofF {
+  const double* F[3];
+  double drho;
+  double hbydt[3];
+};
+// P == nullptr: one-phase (uniform) properties from Geo. M == nullptr: mass
+// flux rho0 u_f; M requires the two-phase properties P
 void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
-                    hipStream_t st);
+                    hipStream_t st, const VofF* M = nullptr);
 void launch_rhs(const Geo& g, double* const us[3], double* rhs, double inv_dt, double* part0, hipStream_t st);
 void launch_init_r(const Geo& g, const double* rhs, double* r, double* dp, const double* S, int slot_mean,
                    double* p0, double* p1, hipStream_t st);
@@ -166,6 +174,10 @@ struct Spacing {
 void launch_ghost_pressure(const Geo& g, const double h[3], double* p, const double* rho, hipStream_t st);
 void launch_ghost_velocity(const Geo& g, double* const u[3], hipStream_t st);
 void launch_face_gradp(const Geo& g, const double* p, double* const gp[3], hipStream_t st);
+// one substep of the split VoF transport (smac_vof.hip); swaps *al and *an
+// once per direction, accumulates the face fluxes into Facc
+void launch_vof_substep(const Geo& g, const double h[3], const Vel& V, double** al, double** an, double* const F[3],
+                        double* const Facc[3], unsigned char* coeff, double dt, int step, hipStream_t st);
 // multigrid (smac_mg.hip), storage T in {double, float, bf16s}; the restriction
 // buffer tb has the compute type of T
 template <typename T>

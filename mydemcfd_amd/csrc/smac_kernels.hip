@@ -69,17 +69,33 @@ __device__ __forceinline__ double visc_cross(const Geo& g, const Props& P, const
   return (t_p - t_m) * g.inv_d[B];
 }
 
+// Mass flux policies of the predictor, the face value mf of
+// k_compute_mass_flux_from_alpha_flux (CFD/G_SMACSolver_VoF.cu:1708-1790):
+// MfRho0 has no alpha flux (mf = rho0 u_f); MfVof adds (rho1-rho0) F h/dt with
+// F the alpha flux accumulated by the VoF transport.
+struct MfRho0 {
+  template <int E>
+  static __device__ __forceinline__ double mf(const Geo& g, const VofF&, I3, double fv) {
+    return g.rho0 * fv;
+  }
+};
+struct MfVof {
+  template <int E>
+  static __device__ __forceinline__ double mf(const Geo& g, const VofF& M, I3 r, double fv) {
+    return g.rho0 * fv + M.drho * (M.F[E][IX(g, wr(g, r))] * M.hbydt[E]);
+  }
+};
+
 // Convective flux difference (M+ - M-)/h_E of component A along axis E,
 // MUSCL-minmod with first-order upwind where the stencil leaves the interior.
-// Convective flux difference (M+ - M-)/h_E of component A along axis E,
-// MUSCL-minmod with first-order upwind where the stencil leaves the interior.
-template <class BC, int A, int E>
-__device__ __forceinline__ double conv_term(const Geo& g, const Vel& V, I3 q, double v0, double vp1, double vm1,
-                                            double rho0) {
+template <class BC, class MF, int A, int E>
+__device__ __forceinline__ double conv_term(const Geo& g, const Vel& V, const VofF& M, I3 q, double v0, double vp1,
+                                            double vm1) {
   double mp, mm, up_p, up_m;
   if (E == A) {
-    mp = 0.5 * (rho0 * BC::face(g, V, A, q) + rho0 * BC::face(g, V, A, sh(q, A, +1)));
-    mm = 0.5 * (rho0 * BC::face(g, V, A, sh(q, A, -1)) + rho0 * BC::face(g, V, A, q));
+    const I3 qp = sh(q, A, +1), qm = sh(q, A, -1);
+    mp = 0.5 * (MF::template mf<A>(g, M, q, BC::face(g, V, A, q)) + MF::template mf<A>(g, M, qp, BC::face(g, V, A, qp)));
+    mm = 0.5 * (MF::template mf<A>(g, M, qm, BC::face(g, V, A, qm)) + MF::template mf<A>(g, M, q, BC::face(g, V, A, q)));
     I3 up = mp > 0. ? q : sh(q, A, +1);
     if (!BC::interior(g, A, up)) {
       up_p = mp > 0. ? v0 : vp1;
@@ -105,8 +121,9 @@ __device__ __forceinline__ double conv_term(const Geo& g, const Vel& V, I3 q, do
   } else {
     double vcm = BC::dir(g, V, A, E, q, -1);
     double vcp = BC::dir(g, V, A, E, q, +1);
-    mp = 0.5 * (rho0 * BC::face(g, V, E, sh(sh(q, E, +1), A, -1)) + rho0 * BC::face(g, V, E, sh(q, E, +1)));
-    mm = 0.5 * (rho0 * BC::face(g, V, E, q) + rho0 * BC::face(g, V, E, sh(q, A, -1)));
+    const I3 r1 = sh(sh(q, E, +1), A, -1), r2 = sh(q, E, +1), r3 = sh(q, A, -1);
+    mp = 0.5 * (MF::template mf<E>(g, M, r1, BC::face(g, V, E, r1)) + MF::template mf<E>(g, M, r2, BC::face(g, V, E, r2)));
+    mm = 0.5 * (MF::template mf<E>(g, M, q, BC::face(g, V, E, q)) + MF::template mf<E>(g, M, r3, BC::face(g, V, E, r3)));
     I3 up = mp > 0. ? q : sh(q, E, +1);
     I3 upw = wr(g, up);
     if (!BC::interior(g, A, upw)) {
@@ -140,17 +157,16 @@ __device__ __forceinline__ double conv_term(const Geo& g, const Vel& V, I3 q, do
 }
 
 // One component of k_get_vof_vstar_rhouu_consistent_{x,y,z}
-// (CFD/G_SMACSolver.cu:139-392, 441-662, 763-1003). Mass flux mf = rho0*u_f
-// (CFD/G_SMACSolver_VoF.cu:1708-1790 with no VoF flux); mu, face mu, beta and
-// rho_old from the property policy PR. Terms are accumulated in the
-// reference's order: own-direction stress, cross stresses in increasing
-// axis order, convection along x, y, z, old pressure gradient.
-template <class BC, class PR, int A>
-__device__ double predict_face(const Geo& g, const Props& P, const Vel& V, const double* __restrict__ p, I3 q,
-                               double dt) {
+// (CFD/G_SMACSolver.cu:139-392, 441-662, 763-1003). Mass flux from the
+// policy MF; mu, face mu, beta and rho_old from the property policy PR. Terms
+// are accumulated in the reference's order: own-direction stress, cross
+// stresses in increasing axis order, convection along x, y, z, old pressure
+// gradient.
+template <class BC, class PR, class MF, int A>
+__device__ double predict_face(const Geo& g, const Props& P, const Vel& V, const VofF& M, const double* __restrict__ p,
+                               I3 q, double dt) {
   constexpr int B1 = A == 0 ? 1 : 0;
   constexpr int B2 = A == 2 ? 1 : 2;
-  const double rho0 = g.rho0;
   double v0 = V.c[A][IX(g, q)];
   double vp1 = BC::face(g, V, A, sh(q, A, +1));
   double vm1 = BC::face(g, V, A, sh(q, A, -1));
@@ -161,9 +177,9 @@ __device__ double predict_face(const Geo& g, const Props& P, const Vel& V, const
   double tmp = 2. * (tau_p - tau_m) * g.inv_d2[A];
   tmp += visc_cross<BC, PR, A, B1>(g, P, V, q, v0);
   tmp += visc_cross<BC, PR, A, B2>(g, P, V, q, v0);
-  tmp -= conv_term<BC, A, 0>(g, V, q, v0, vp1, vm1, rho0);
-  tmp -= conv_term<BC, A, 1>(g, V, q, v0, vp1, vm1, rho0);
-  tmp -= conv_term<BC, A, 2>(g, V, q, v0, vp1, vm1, rho0);
+  tmp -= conv_term<BC, MF, A, 0>(g, V, M, q, v0, vp1, vm1);
+  tmp -= conv_term<BC, MF, A, 1>(g, V, M, q, v0, vp1, vm1);
+  tmp -= conv_term<BC, MF, A, 2>(g, V, M, q, v0, vp1, vm1);
 
   double beta = PR::beta(g, P, A, q);
   double rho_old_f = PR::rho_old_f(g, P, q, cm);
@@ -172,8 +188,8 @@ __device__ double predict_face(const Geo& g, const Props& P, const Vel& V, const
 }
 
 // grid: x = face index along i (block 64), y = j (block 4), z = k
-template <class BC, class PR, int A>
-__global__ __launch_bounds__(256) void k_predict(Geo g, Props P, Vel V, const double* __restrict__ p,
+template <class BC, class PR, class MF, int A>
+__global__ __launch_bounds__(256) void k_predict(Geo g, Props P, Vel V, VofF M, const double* __restrict__ p,
                                                  double* __restrict__ ustar, double dt) {
   int lo[3] = {1, 1, 1}, hi[3] = {g.n[0], g.n[1], g.n[2]};
   lo[A] = g.per[A] ? 1 : 2;
@@ -182,7 +198,7 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, Props P, Vel V, const do
   int k = lo[2] + (int)blockIdx.z;
   if (i > hi[0] || j > hi[1] || k > hi[2]) return;
   I3 q = mk(i, j, k);
-  ustar[IX(g, q)] = predict_face<BC, PR, A>(g, P, V, p, q, dt);
+  ustar[IX(g, q)] = predict_face<BC, PR, MF, A>(g, P, V, M, p, q, dt);
 }
 
 // Periodic one-phase form of predict_face<PeriodicBC, UniP, A> with the whole
@@ -882,36 +898,41 @@ void launch_props(const Geo& g, const double* alpha, double* rho, double* rho_ol
   hipLaunchKernelGGL(k_props_faces<2>, fg, CELLB, 0, st, g, rho, mu, beta[2], muf[2]);
 }
 
-template <class BC, class PR>
-static void predict_components(const Geo& g, const Props& P, const Vel& V, const double* p, double* us[3],
-                               double dt, hipStream_t st) {
+template <class BC, class PR, class MF>
+static void predict_components(const Geo& g, const Props& P, const Vel& V, const VofF& M, const double* p,
+                               double* us[3], double dt, hipStream_t st) {
   for (int a = 0; a < 3; a++) {
     int lo = g.per[a] ? 1 : 2;
     int nf[3] = {g.n[0], g.n[1], g.n[2]};
     nf[a] = g.n[a] - lo + 1;
     if (nf[0] <= 0 || nf[1] <= 0 || nf[2] <= 0) continue;
     dim3 cg((unsigned)((nf[0] + 63) / 64), (unsigned)((nf[1] + 3) / 4), (unsigned)nf[2]);
-    if (a == 0) hipLaunchKernelGGL((k_predict<BC, PR, 0>), cg, CELLB, 0, st, g, P, V, p, us[0], dt);
-    if (a == 1) hipLaunchKernelGGL((k_predict<BC, PR, 1>), cg, CELLB, 0, st, g, P, V, p, us[1], dt);
-    if (a == 2) hipLaunchKernelGGL((k_predict<BC, PR, 2>), cg, CELLB, 0, st, g, P, V, p, us[2], dt);
+    if (a == 0) hipLaunchKernelGGL((k_predict<BC, PR, MF, 0>), cg, CELLB, 0, st, g, P, V, M, p, us[0], dt);
+    if (a == 1) hipLaunchKernelGGL((k_predict<BC, PR, MF, 1>), cg, CELLB, 0, st, g, P, V, M, p, us[1], dt);
+    if (a == 2) hipLaunchKernelGGL((k_predict<BC, PR, MF, 2>), cg, CELLB, 0, st, g, P, V, M, p, us[2], dt);
   }
 }
 
 void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
-                    hipStream_t st) {
+                    hipStream_t st, const VofF* M) {
   const int KC = 16;  // z-planes per thread: the k+-2 halo is re-read once per 16 planes
   const bool allper = g.per[0] && g.per[1] && g.per[2];
-  if (P) {
+  const VofF M0{};
+  if (P && M) {
+    // two-phase grids right after a VoF transport: mass flux with the alpha flux
+    if (allper) predict_components<PeriodicBC, VarP, MfVof>(g, *P, V, *M, p, us, dt, st);
+    else predict_components<WallBC, VarP, MfVof>(g, *P, V, *M, p, us, dt, st);
+  } else if (P) {
     // two-phase grids: one kernel per component, properties from the arrays
-    if (allper) predict_components<PeriodicBC, VarP>(g, *P, V, p, us, dt, st);
-    else predict_components<WallBC, VarP>(g, *P, V, p, us, dt, st);
+    if (allper) predict_components<PeriodicBC, VarP, MfRho0>(g, *P, V, M0, p, us, dt, st);
+    else predict_components<WallBC, VarP, MfRho0>(g, *P, V, M0, p, us, dt, st);
   } else if (allper) {
     dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4), (unsigned)((g.n[2] + KC - 1) / KC));
     hipLaunchKernelGGL(k_predict_fused<PeriodicBC>, grid, CELLB, 0, st, g, V, p, us[0], us[1], us[2], dt, KC);
     return;
   } else {
     // walls: one kernel per component (the fused wall kernel exceeds the register file)
-    predict_components<WallBC, UniP>(g, Props{}, V, p, us, dt, st);
+    predict_components<WallBC, UniP, MfRho0>(g, Props{}, V, M0, p, us, dt, st);
   }
   if (allper) return;
   int m = g.n[0] > g.n[1] ? g.n[0] : g.n[1];

@@ -68,6 +68,12 @@ struct smac_ctx {
     double *rho, *rho_old, *inv_rho, *mu;
     double *beta[3], *inv_rhof[3], *muf[3], *mf[3], *rhof[3];
     double *gradp[3];    /* f_gradp_{x,y,z}_ (smac_make_face_gradp) */
+    /* VoF transport (allocated on first use): alpha_new_, f_F{x,y,z}_,
+     * f_F{x,y,z}_accum_, correct_coeff_; vof_armed = the accumulated fluxes
+     * belong to the coming predictor */
+    double *alpha_new, *vf[3], *vfacc[3];
+    unsigned char* vcoeff;
+    int vof_armed;
     /* PCG */
     double *r, *z, *dvec, *Ad;
     double *Adiag, *Aoff[6];
@@ -317,6 +323,7 @@ int smac_predictor(smac_ctx* c, double dt) {
     if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
     if (!c->props_ready) smac_update_properties(c, 1);
     smac_compute_mass_flux(c, dt);
+    c->vof_armed = 0; /* the accumulated alpha flux serves one predictor */
     for (int a = 0; a < 3; a++) {
         int lo[3], hi[3];
         for (int d = 0; d < 3; d++) { lo[d] = 1; hi[d] = c->n[d]; }
@@ -478,11 +485,30 @@ int smac_update_properties(smac_ctx* c, int32_t initial) {
     return SMAC_OK;
 }
 
-/* k_compute_mass_flux_from_alpha_flux with zero alpha flux:
- * mf = rho0*u_f + drho*0 (CFD/G_SMACSolver_VoF.cu:1708-1790) */
+/* k_compute_mass_flux_from_alpha_flux (CFD/G_SMACSolver_VoF.cu:1708-1790):
+ * mf = rho0*u_f + (rho1 - rho0)*(F_accum*h/dt) after a VoF transport, else
+ * mf = rho0*u_f (no alpha flux) */
 int smac_compute_mass_flux(smac_ctx* c, double dt) {
-    (void)dt;
     if (!c) return SMAC_ERR_INVALID_ARG;
+    if (c->vof_armed) {
+        if (!(dt > 0)) return SMAC_ERR_INVALID_ARG;
+        double inv_dt = 1. / dt, drho = c->fl.rho1 - c->fl.rho0;
+        for (int a = 0; a < 3; a++) {
+            double hbydt = c->d[a] * inv_dt;
+            for (int k = 0; k < c->p[2]; k++)
+                for (int j = 0; j < c->p[1]; j++)
+                    for (int i = 0; i < c->p[0]; i++) {
+                        ivec q = iv(i, j, k);
+                        int ok = 1;
+                        for (int d = 0; d < 3; d++)
+                            if (q.v[d] > c->n[d] + (d == a ? 2 : 1)) ok = 0;
+                        if (!ok) continue;
+                        size_t id = IX(c, q);
+                        c->mf[a][id] = c->fl.rho0 * get_face(c, a, q) + drho * (c->vfacc[a][id] * hbydt);
+                    }
+        }
+        return SMAC_OK;
+    }
     for (int a = 0; a < 3; a++)
         for (int k = 0; k < c->p[2]; k++)
             for (int j = 0; j < c->p[1]; j++)
@@ -1403,6 +1429,153 @@ int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
     return SMAC_OK;
 }
 
+/* ------------------------------------------------ VoF transport */
+#define THINC_BETA 2.0              /* BETA, CFD/hardCodedParameters.h:6 */
+#define VOF_EPS 1.E-12              /* EPS, CFD/hardCodedParameters.h:7 */
+#define VOF_PI 3.14159265358979323846
+
+/* find_xi0_analytic (CFD/G_SMACSolver_VoF.cu:459-465) */
+static double thinc_xi0(double alpha, double gamma) {
+    double A = exp(THINC_BETA * gamma * (2. * alpha - 1.));
+    return 1. / (2. * THINC_BETA) * log((exp(THINC_BETA) - A) / (A - exp(-THINC_BETA)));
+}
+/* integrate_thinc (CFD/G_SMACSolver_VoF.cu:451-455) */
+static double thinc_integral(double a, double b, double gamma, double xi) {
+    return 0.5 * (b - a) + gamma / (2. * THINC_BETA) *
+                               (log(cosh(THINC_BETA * (b - xi))) - log(cosh(THINC_BETA * (a - xi))));
+}
+static inline double sgnd(double a) { return a < 0 ? -1. : 1.; }
+
+/* one tangential component of the donor cell's normal (-grad alpha):
+ * central, one-sided where a neighbour is not interior, else 0 */
+static double vof_normal_t(const struct smac_ctx* c, ivec dn, int t) {
+    ivec qp = add(dn, t, +1), qm = add(dn, t, -1);
+    int ip = c->per[t] || (qp.v[t] >= 1 && qp.v[t] <= c->n[t]);
+    int im = c->per[t] || (qm.v[t] >= 1 && qm.v[t] <= c->n[t]);
+    const double* al = c->alpha;
+    if (ip && im) return -(al[IX(c, wr(c, qp))] - al[IX(c, wr(c, qm))]) * (1. / (2. * c->d[t]));
+    if (!ip && im) return -(al[IX(c, dn)] - al[IX(c, wr(c, qm))]) * c->inv_d[t];
+    if (ip && !im) return -(al[IX(c, wr(c, qp))] - al[IX(c, dn)]) * c->inv_d[t];
+    return 0.;
+}
+
+/* THINC/WLIC flux of the a-face q: k_alpha_flux_thincwlic_{x,y,z}
+ * (CFD/G_SMACSolver_VoF.cu:507-666, 668-827, 829-984). q is a computed face
+ * (2..N along a wall axis, 1..N along a periodic one), so its donor cell is
+ * interior and the reference's F_BOUNDARY branch has no case. */
+static double vof_face_flux(const struct smac_ctx* c, int a, ivec q, double dt) {
+    const double* al = c->alpha;
+    double v = c->vel[a][IX(c, q)];
+    ivec dn = wr(c, v > 0.0 ? add(q, a, -1) : q);
+    double ad = al[IX(c, dn)];
+    double dtbyd = dt * c->inv_d[a];
+    ivec dp = add(dn, a, +1), dm = add(dn, a, -1);
+    int ip = c->per[a] || dp.v[a] <= c->n[a];
+    int im = c->per[a] || dm.v[a] >= 1;
+    if (!ip || !im) return v * ad * dtbyd;
+    double gam = al[IX(c, wr(c, dp))] - al[IX(c, wr(c, dm))];
+    if (ad < VOF_EPS || ad > 1.0 - VOF_EPS || fabs(gam) < 1e-6) return v * ad * dtbyd;
+    double nrm[3];
+    nrm[a] = -gam * (1. / (2. * c->d[a]));
+    for (int t = 0; t < 3; t++)
+        if (t != a) nrm[t] = vof_normal_t(c, dn, t);
+    double s_sq = nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2];
+    double sn = sqrt(s_sq);
+    double inv_s = 1.0 / (sn + VOF_EPS);
+    double theta = acos(fabs(nrm[a]) * inv_s);
+    double wc = 2. * theta / VOF_PI;
+    double w = 1.0 - wc * wc;
+    double gamma = sgnd(gam);
+    double xi0 = thinc_xi0(ad, gamma);
+    double lambda = v * dtbyd;
+    double ft = v > 0.0 ? thinc_integral(1.0 - lambda, 1.0, gamma, xi0) : -thinc_integral(0.0, -lambda, gamma, xi0);
+    double fu = lambda * ad;
+    return w * ft + (1. - w) * fu;
+}
+
+/* k_alpha_flux_thincwlic_{x,y,z} over the computed faces, then
+ * k_transport_alpha_{x,y,z} (CFD/G_SMACSolver_VoF.cu:1011-1152) into
+ * alpha_new and the swap; the accumulation of k_alpha_flux_accum
+ * (468-505) is taken here, per computed face (other faces hold 0) */
+static void vof_direction(struct smac_ctx* c, int a, double dt) {
+    int lo[3] = {1, 1, 1}, hi[3] = {c->n[0], c->n[1], c->n[2]};
+    lo[a] = c->per[a] ? 1 : 2;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = lo[2]; k <= hi[2]; k++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++) {
+                ivec q = iv(i, j, k);
+                size_t id = IX(c, q);
+                double F = vof_face_flux(c, a, q, dt);
+                c->vf[a][id] = F;
+                c->vfacc[a][id] += F;
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                ivec q = iv(i, j, k);
+                size_t id = IX(c, q);
+                double flux = 0.;
+                flux += c->vf[a][IX(c, wr(c, add(q, a, +1)))] - c->vf[a][id];
+                double up = get_face(c, a, add(q, a, +1));
+                double um = get_face(c, a, q);
+                double coeff = (double)c->vcoeff[id];
+                flux -= coeff * (up - um) * dt * c->inv_d[a];
+                c->alpha_new[id] = c->alpha[id] - flux;
+            }
+    double* t = c->alpha;
+    c->alpha = c->alpha_new;
+    c->alpha_new = t;
+}
+
+/* G_SMACSolver::alpha_flux_thincwlic_split (CFD/G_SMACSolver_VoF.cu:1575-1674):
+ * correct_coeff from alpha (k_set_correct_coeff, 986-1009), the three
+ * directions in the order of step % 3 (xyz, yzx, zxy), then k_clip_alpha
+ * (1173-1190) */
+static void vof_split(struct smac_ctx* c, double dt, int step) {
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                size_t id = IDX(c, i, j, k);
+                c->vcoeff[id] = c->alpha[id] > 0.5 ? 1 : 0;
+            }
+    int first = step % 3;
+    for (int s = 0; s < 3; s++) vof_direction(c, (first + s) % 3, dt);
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                size_t id = IDX(c, i, j, k);
+                if (c->alpha[id] < VOF_EPS) c->alpha[id] = 0.;
+                else if (c->alpha[id] > 1.) c->alpha[id] = 1.;
+            }
+}
+
+static int vof_alloc(struct smac_ctx* c) {
+    if (c->alpha_new) return SMAC_OK;
+    c->alpha_new = (double*)calloc(c->tot, sizeof(double));
+    c->vcoeff = (unsigned char*)calloc(c->tot, 1);
+    for (int a = 0; a < 3; a++) {
+        c->vf[a] = (double*)calloc(c->tot, sizeof(double));
+        c->vfacc[a] = (double*)calloc(c->tot, sizeof(double));
+    }
+    if (!c->alpha_new || !c->vcoeff || !c->vf[0] || !c->vf[1] || !c->vf[2] || !c->vfacc[0] || !c->vfacc[1] ||
+        !c->vfacc[2])
+        return SMAC_ERR_OOM;
+    return SMAC_OK;
+}
+
+/* One time step of VoF transport (CFD/main.cu:283-295) */
+int smac_transport_alpha(smac_ctx* c, double dt, int32_t substeps, int32_t step) {
+    if (!c || !(dt > 0) || substeps < 1 || step < 0) return SMAC_ERR_INVALID_ARG;
+    if (vof_alloc(c)) return SMAC_ERR_OOM;
+    for (int a = 0; a < 3; a++) memset(c->vfacc[a], 0, c->tot * sizeof(double));
+    double sub_dt = dt / (double)substeps;
+    for (int s = 0; s < substeps; s++) vof_split(c, sub_dt, step);
+    c->vof_armed = 1;
+    return SMAC_OK;
+}
+
 /* ------------------------------------------------ after the step */
 /* d_get_velocity_ghost_value (CFD/G_BoundaryFunctions.cu:257-275) with the IBM
  * solid fraction lambda = 0 (no immersed solid): bval is the face one row
@@ -1531,6 +1704,9 @@ static int shape_of(const struct smac_ctx* c, int field, int sh[3]) {
         case SMAC_FIELD_U: case SMAC_FIELD_USTAR: case SMAC_FIELD_GRADPX: sh[0]++; return 0;
         case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: case SMAC_FIELD_GRADPY: sh[1]++; return 1;
         case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: case SMAC_FIELD_GRADPZ: sh[2]++; return 2;
+        case SMAC_FIELD_FLUXX: sh[0]++; return 0;
+        case SMAC_FIELD_FLUXY: sh[1]++; return 1;
+        case SMAC_FIELD_FLUXZ: sh[2]++; return 2;
         default: return -1;
     }
 }
@@ -1549,6 +1725,9 @@ static double* field_ptr(struct smac_ctx* c, int field) {
         case SMAC_FIELD_GRADPX: return c->gradp[0];
         case SMAC_FIELD_GRADPY: return c->gradp[1];
         case SMAC_FIELD_GRADPZ: return c->gradp[2];
+        case SMAC_FIELD_FLUXX: return c->vfacc[0];
+        case SMAC_FIELD_FLUXY: return c->vfacc[1];
+        case SMAC_FIELD_FLUXZ: return c->vfacc[2];
         default: return NULL;
     }
 }
@@ -1566,6 +1745,7 @@ int smac_upload(smac_ctx* c, int32_t field, const double* h, size_t count) {
     if (count != smac_field_count(c, field)) return SMAC_ERR_INVALID_ARG;
     int sh[3];
     shape_of(c, field, sh);
+    if (field >= SMAC_FIELD_FLUXX && vof_alloc(c)) return SMAC_ERR_OOM;
     double* dst = field_ptr(c, field);
     for (int k = 0; k < sh[2]; k++)
         for (int j = 0; j < sh[1]; j++)
@@ -1580,6 +1760,10 @@ int smac_download(smac_ctx* c, int32_t field, double* h, size_t count) {
     int sh[3];
     int a = shape_of(c, field, sh);
     const double* src = field_ptr(c, field);
+    if (!src) {
+        memset(h, 0, count * sizeof(double));
+        return SMAC_OK;
+    }
     for (int k = 0; k < sh[2]; k++)
         for (int j = 0; j < sh[1]; j++)
             for (int i = 0; i < sh[0]; i++) {
@@ -1670,8 +1854,10 @@ void smac_destroy(smac_ctx* c) {
                       c->inv_rhof[0], c->inv_rhof[1], c->inv_rhof[2], c->muf[0], c->muf[1], c->muf[2], c->mf[0],
                       c->mf[1], c->mf[2], c->rhof[0], c->rhof[1], c->rhof[2], c->r, c->z, c->dvec, c->Ad,
                       c->Adiag, c->Aoff[0], c->Aoff[1], c->Aoff[2], c->Aoff[3], c->Aoff[4], c->Aoff[5],
-                      c->gradp[0], c->gradp[1], c->gradp[2]};
+                      c->gradp[0], c->gradp[1], c->gradp[2], c->alpha_new, c->vf[0], c->vf[1], c->vf[2],
+                      c->vfacc[0], c->vfacc[1], c->vfacc[2]};
     for (size_t t = 0; t < sizeof(arrs) / sizeof(arrs[0]); t++) free(arrs[t]);
+    free(c->vcoeff);
     free(c);
 }
 

@@ -1,0 +1,191 @@
+"""Pins of the oracle's VoF transport (no GPU): split THINC/WLIC
+(G_SMACSolver::alpha_flux_thincwlic_split, CFD/G_SMACSolver_VoF.cu:451-1190,1575-1706)
+and the mass flux it feeds (k_compute_mass_flux_from_alpha_flux, 1708-1790).
+
+What fixes the oracle here is independent of its formulas:
+  * the THINC flux of an interface donor equals the integral of the
+    hyperbolic-tangent profile H(x) = (1 + g tanh(b (x - x0)))/2, b = 2, whose
+    cell mean is the donor's alpha, over the upwind fraction of the cell --
+    x0 by root finding and the integral by quadrature (scipy), for both flow
+    signs;
+  * the WLIC weight: with the interface normal at 45 degrees in the x-y plane
+    the x-flux is 3/4 THINC + 1/4 upwind (w = 1 - (2 theta/pi)^2);
+  * the split with the divergence correction conserves the total volume
+    fraction for a discretely solenoidal velocity (Weymouth & Yue 2010);
+  * full and empty cells stay full and empty;
+  * the direction order of step % 3 (xyz, yzx, zxy): the step-1 transport of
+    a cyclically relabelled problem (x->y->z->x) is the relabelled step-0
+    transport;
+  * the mass flux: a box full of phase 1 transported and predicted with
+    rho0 != rho1 gives the single-phase predictor with rho = rho1
+    (F = u dt/h on every face, so mf = rho0 u + (rho1 - rho0) u = rho1 u).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, optimize
+
+from mydemcfd_amd import _abi, cases
+
+BETA = 2.0
+
+
+def _H(x, gam, x0):
+    return 0.5 * (1.0 + gam * np.tanh(BETA * (x - x0)))
+
+
+def _thinc_flux_quad(alpha, gam, lam):
+    """Flux (fraction of a cell) through the downwind face for a donor of
+    volume fraction alpha, profile orientation gam and Courant number lam."""
+    x0 = optimize.brentq(lambda x: integrate.quad(_H, 0.0, 1.0, args=(gam, x))[0] - alpha, -20.0, 20.0,
+                         xtol=1e-15)
+    if lam > 0:
+        return integrate.quad(_H, 1.0 - lam, 1.0, args=(gam, x0), epsabs=1e-15)[0]
+    return -integrate.quad(_H, 0.0, -lam, args=(gam, x0), epsabs=1e-15)[0]
+
+
+def _periodic(n, length=1.0, rho=(1.0, 1.0), mu=(0.01, 0.01)):
+    nn = n if isinstance(n, tuple) else (n, n, n)
+    L = length if isinstance(length, tuple) else (length,) * 3
+    return dict(n=nn, size=L, origin=(0.0, 0.0, 0.0), periodic=(1, 1, 1), side_bcid=(0,) * 6,
+                bc_table=((_abi.BC_NOSLIP, 0.0, 0.0, 0.0),), rho=rho, mu=mu, g=(0.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("U", [0.7, -0.45])
+def test_thinc_flux_matches_quadrature(oracle_mod, U):
+    n = (16, 4, 4)
+    c = oracle_mod.OracleContext(**_periodic(n))
+    prof = np.zeros(n[0])
+    prof[:4] = 1.0
+    prof[4] = 0.3  # cell i = 5 (index 1-based) is the interface donor, alpha decreasing in x
+    a = np.zeros(c.shape("alpha"))
+    a[1:-1, 1:-1, 1:-1] = prof[None, None, :]
+    c.upload("alpha", a)
+    c.upload("u", np.full(c.shape("u"), U))
+    h, dt = 1.0 / n[0], 0.05
+    c.transport_alpha(dt, 1, 0)  # x first: the x-fluxes see the initial alpha
+    F = c.download("fluxx")
+    lam = U * dt / h
+    face = 6 if U > 0 else 5  # the face whose donor is cell 5
+    expect = _thinc_flux_quad(0.3, -1.0, lam)
+    got = F[2, 2, face]
+    assert abs(got - expect) < 1e-10, (got, expect)
+    # a full donor away from the interface is plain upwind
+    assert F[2, 2, 3] == pytest.approx(lam * 1.0, abs=1e-15)
+
+
+def test_wlic_weight_diagonal_interface(oracle_mod):
+    """alpha = f(i + j): equal x and y normal components, theta = pi/4 and the
+    x-flux is 3/4 THINC + 1/4 upwind."""
+    n = (12, 12, 4)
+    c = oracle_mod.OracleContext(**_periodic(n))
+    i = np.arange(1, n[0] + 1)[None, :]
+    j = np.arange(1, n[1] + 1)[:, None]
+    s = (i + j) % n[0]
+    f = np.select([s < 5, s == 5, s == 6], [1.0, 0.6, 0.2], 0.0)
+    a = np.zeros(c.shape("alpha"))
+    a[1:-1, 1:-1, 1:-1] = f[None, :, :]
+    c.upload("alpha", a)
+    U = 0.5
+    c.upload("u", np.full(c.shape("u"), U))
+    h, dt = 1.0 / n[0], 0.06
+    lam = U * dt / h
+    c.transport_alpha(dt, 1, 0)
+    F = c.download("fluxx")
+    # donor (i, j) = (2, 3): s = 5, alpha 0.6; its x neighbours s = 4 (1.0) and 6 (0.2)
+    ii, jj = 2, 3
+    assert f[jj - 1, ii - 1] == 0.6
+    gam = -1.0
+    expect = 0.75 * _thinc_flux_quad(0.6, gam, lam) + 0.25 * lam * 0.6
+    got = F[2, jj, ii + 1]
+    assert abs(got - expect) < 1e-9, (got, expect)
+
+
+def _sphere_alpha(c, n, r=0.3):
+    h = 1.0 / n
+    x = (np.arange(n) + 0.5) * h
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    d = np.sqrt((X - 0.5) ** 2 + (Y - 0.45) ** 2 + (Z - 0.55) ** 2)
+    inside = np.clip((r - d) / h + 0.5, 0.0, 1.0)  # a one-cell smeared sphere
+    a = np.zeros(c.shape("alpha"))
+    a[1:-1, 1:-1, 1:-1] = np.transpose(inside, (2, 1, 0))
+    return a
+
+
+def test_split_conserves_volume_for_solenoidal_velocity(oracle_mod):
+    n = 16
+    c = oracle_mod.OracleContext(**_periodic(n))
+    c.upload("alpha", _sphere_alpha(c, n))
+    u, v, w = cases.white_solenoidal(n, seed=5, rms=0.5, length=1.0)
+    for f, arr in zip(("u", "v", "w"), (u, v, w)):
+        c.upload(f, cases._embed_periodic(arr, _abi.FIELD_NAMES[f]))
+    tot0 = c.download("alpha")[1:-1, 1:-1, 1:-1].sum()
+    for s in range(6):
+        c.transport_alpha(0.01, 2, s)
+        a = c.download("alpha")[1:-1, 1:-1, 1:-1]
+        assert a.min() >= 0.0 and a.max() <= 1.0
+        assert abs(a.sum() - tot0) < 1e-11 * tot0, (s, a.sum() - tot0)
+
+
+@pytest.mark.parametrize("fill", [0.0, 1.0])
+def test_full_and_empty_stay(oracle_mod, fill):
+    n = 12
+    c = oracle_mod.OracleContext(**_periodic(n))
+    c.upload("alpha", np.full(c.shape("alpha"), fill))
+    u, v, w = cases.white_solenoidal(n, seed=8, rms=0.5, length=1.0)
+    for f, arr in zip(("u", "v", "w"), (u, v, w)):
+        c.upload(f, cases._embed_periodic(arr, _abi.FIELD_NAMES[f]))
+    c.transport_alpha(0.01, 1, 0)
+    a = c.download("alpha")[1:-1, 1:-1, 1:-1]
+    assert np.max(np.abs(a - fill)) < 1e-14
+
+
+def test_direction_order_rotates_with_step(oracle_mod):
+    n = 12
+    rng = np.random.default_rng(4)
+    u, v, w = cases.white_solenoidal(n, seed=9, rms=0.5, length=1.0)
+    res = []
+    for perm in (False, True):
+        c = oracle_mod.OracleContext(**_periodic(n))
+        a0 = _sphere_alpha(c, n, r=0.27)
+        uu = cases._embed_periodic(u, _abi.FIELD_U)
+        vv = cases._embed_periodic(v, _abi.FIELD_V)
+        ww = cases._embed_periodic(w, _abi.FIELD_W)
+        if perm:  # relabel x -> y -> z -> x
+            t = lambda x: np.transpose(x, (1, 2, 0))
+            a0, uu, vv, ww = t(a0), t(ww), t(uu), t(vv)
+        c.upload("alpha", a0)
+        for f, arr in zip(("u", "v", "w"), (uu, vv, ww)):
+            c.upload(f, arr)
+        c.transport_alpha(0.02, 1, 1 if perm else 0)
+        res.append((c.download("alpha"), c.download("fluxx"), c.download("fluxy"), c.download("fluxz")))
+    (a, fx, fy, fz), (ap, fxp, fyp, fzp) = res
+    t = lambda x: np.transpose(x, (1, 2, 0))
+    assert np.max(np.abs(ap - t(a))) < 1e-13
+    assert np.max(np.abs(fyp - t(fx))) < 1e-13
+    assert np.max(np.abs(fzp - t(fy))) < 1e-13
+    assert np.max(np.abs(fxp - t(fz))) < 1e-13
+    del rng
+
+
+def test_mass_flux_full_box_is_single_phase_rho1(oracle_mod):
+    n = 12
+    rho, mu = (1.0, 1000.0), (1e-3, 0.05)
+    u, v, w = cases.white_solenoidal(n, seed=3, rms=0.5, length=1.0)
+    fields = {f: cases._embed_periodic(x, _abi.FIELD_NAMES[f]) for f, x in zip("uvw", (u, v, w))}
+    two = oracle_mod.OracleContext(**_periodic(n, rho=rho, mu=mu))
+    one = oracle_mod.OracleContext(**_periodic(n, rho=(rho[1], rho[1]), mu=(mu[1], mu[1])))
+    for c in (two, one):
+        for f, x in fields.items():
+            c.upload(f, x)
+    two.upload("alpha", np.ones(two.shape("alpha")))
+    two.update_properties(1)
+    dt = 0.01
+    two.transport_alpha(dt, 3, 0)
+    two.update_properties(0)
+    for c in (two, one):
+        c.predictor(dt)
+    for f in ("ustar", "vstar", "wstar"):
+        a, b = two.download(f), one.download(f)
+        assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(b)), f
