@@ -193,6 +193,13 @@ int smac_update_properties(smac_ctx* ctx, int32_t initial);
  * predictor. Call smac_update_properties(ctx, 1) once before the first
  * transport, as the reference's t = 0 setup does. */
 int smac_transport_alpha(smac_ctx* ctx, double dt, int32_t substeps, int32_t step);
+/* The two halves of smac_transport_alpha, for callers that keep the reference's
+ * substep loop: G_SMACSolver::clear_alpha_flux_accum
+ * (CFD/G_SMACSolver_VoF.cu:1688-1696), and one substep = clear_alpha_flux +
+ * alpha_flux_thincwlic_split(sub_dt, step) + alpha_flux_accum
+ * (CFD/G_SMACSolver_VoF.cu:468-505,1575-1706). */
+int smac_clear_alpha_flux_accum(smac_ctx* ctx);
+int smac_alpha_flux_split(smac_ctx* ctx, double sub_dt, int32_t step);
 /* Mass flux G_SMACSolver::compute_mass_flux_from_alpha_flux
  * (CFD/G_SMACSolver_VoF.cu:1708-1795): mf = rho0*u_f + (rho1-rho0)*F_accum*h/dt
  * right after smac_transport_alpha, else mf = rho0*u_f (no alpha flux). The
@@ -234,6 +241,13 @@ int smac_update_boundary_ghost(smac_ctx* ctx, double dt);
  * gradient (p(c) - p(c-1))/h of each axis into SMAC_FIELD_GRADP{X,Y,Z}, on the
  * faces whose three indices all lie in 1..N+1 (others stay 0). */
 int smac_make_face_gradp(smac_ctx* ctx);
+/* The cell data of FileInOut::output_vti_binary_cellData (CFD/FileInOut.cpp:145-255):
+ * host_out receives count = 5*Nx*Ny*Nz floats, x fastest over the interior
+ * cells: pressure[n], alpha[n], then velocity[3n] with (u, v, w) interleaved,
+ * each component the mean of the cell's two faces. Packed on the device and
+ * copied once; the VTI file itself is written by smac_config::write_vti
+ * (include/smac_config.hpp). */
+int smac_cell_output(smac_ctx* ctx, float* host_out, size_t count);
 
 /* Adjoint (vector-Jacobian product) of one smac_step with time step dt taken
  * from the velocity u_n (three device arrays, x/y/z faces, reference layout):

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "smac_c_api.h"
+#include "smac_config.hpp"
 
 namespace smac_facade {
 
@@ -180,6 +181,17 @@ struct G_SMACSolver {
     pressure_solver_->solve(*this);
   }
   void correct_vof_velocity(const SMACSolver& solv) { check(smac_correct(ctx_, solv.dt_), "smac_correct"); }
+  // VoF loop of CFD/main.cu:283-303: transport, properties, mass flux, wall faces
+  void clear_alpha_flux_accum() { check(smac_clear_alpha_flux_accum(ctx_), "smac_clear_alpha_flux_accum"); }
+  void clear_alpha_flux() {}  // each substep rewrites the face fluxes it computes
+  void alpha_flux_thincwlic_split(double sub_dt, int step) {
+    check(smac_alpha_flux_split(ctx_, sub_dt, step), "smac_alpha_flux_split");
+  }
+  void alpha_flux_accum() {}  // fused into the flux pass of smac_alpha_flux_split
+  void update_properties_by_alpha() { check(smac_update_properties(ctx_, 0), "smac_update_properties"); }
+  void compute_mass_flux_from_alpha_flux(const SMACSolver&) {}  // formed inside the predictor
+  void update_boundary_faces() {}  // part of smac_update_properties
+  void calc_surface_tension() {}   // surface tension is outside this path (sigma = 0)
   void update_boundary_ghost(const SMACSolver& solv) {
     check(smac_update_boundary_ghost(ctx_, solv.dt_), "smac_update_boundary_ghost");
   }
@@ -219,6 +231,53 @@ struct G_PCGSolver : G_PressureSolverBase {
       o.ref_coarse_sweeps = gmg_->num_iter_coarse_;
     }
     check(smac_pressure_solve(s.ctx_, s.dt_, &o, &s.last_stats_), "smac_pressure_solve");
+  }
+};
+
+// The reference's set-up from its JSON configuration (CFD/main.cu:80-250 with
+// SimulationSetup::apply_boundary_conditions / apply_initial_conditions):
+// grid, boundaries, fluid, device arrays, the VoF initial fields, t = 0
+// properties. Periodic directions are not expressible in the configuration.
+inline void setup_from_config(G_SMACSolver& g, const smac_config::SimulationConfig& c) {
+  const smac_config::GridConfig& gr = c.grid;
+  g.set_calc_properties(gr.origin_x_, gr.origin_y_, gr.origin_z_, gr.size_x, gr.size_y, gr.size_z, gr.Nx, gr.Ny,
+                        gr.Nz);
+  const smac_config::BoundarySetup b = smac_config::boundary_setup(c);
+  std::vector<smac_bc_entry> table;
+  for (const auto& e : b.table) table.push_back(smac_bc_entry{(int32_t)e.type, e.velocity.x, e.velocity.y, e.velocity.z});
+  const int periodic[3] = {0, 0, 0};
+  g.set_boundaries(periodic, b.side_bcid, table);
+  g.set_fluid(c.fluid.phase0.density, c.fluid.phase1.density, c.fluid.phase0.viscosity, c.fluid.phase1.viscosity,
+              c.fluid.gravity.x, c.fluid.gravity.y, c.fluid.gravity.z);
+  g.solver_malloc();
+  const smac_config::InitialFields f = smac_config::initial_fields(c);
+  g.upload(SMAC_FIELD_ALPHA, f.alpha);
+  g.upload(SMAC_FIELD_V, f.v);
+  check(smac_update_properties(g.ctx_, 1), "smac_update_properties");
+}
+
+// FileInOut (CFD/FileInOut.cpp:145-255): result_%06d.vti in outdir_, the cell
+// data packed on the device (smac_cell_output)
+struct FileInOut {
+  std::string outdir_ = "results";
+  explicit FileInOut(std::string outdir = "results") : outdir_(std::move(outdir)) {}
+  std::string output_vti_binary_cellData(G_SMACSolver& g, int step) const {
+    smac_config::GridConfig gr;
+    gr.Nx = g.grid_.nx;
+    gr.Ny = g.grid_.ny;
+    gr.Nz = g.grid_.nz;
+    gr.size_x = g.grid_.sizex;
+    gr.size_y = g.grid_.sizey;
+    gr.size_z = g.grid_.sizez;
+    gr.origin_x_ = g.grid_.originx;
+    gr.origin_y_ = g.grid_.originy;
+    gr.origin_z_ = g.grid_.originz;
+    const size_t n = (size_t)gr.Nx * (size_t)gr.Ny * (size_t)gr.Nz;
+    std::vector<float> buf(5 * n);
+    check(smac_cell_output(g.ctx_, buf.data(), buf.size()), "smac_cell_output");
+    const std::string name = smac_config::vti_filename(outdir_, step);
+    smac_config::write_vti(name, gr, buf.data());
+    return name;
   }
 };
 

@@ -87,6 +87,8 @@ SIGNATURES = {
     "smac_update_properties": (C.c_int, [_P, C.c_int32]),
     "smac_compute_mass_flux": (C.c_int, [_P, C.c_double]),
     "smac_transport_alpha": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32]),
+    "smac_clear_alpha_flux_accum": (C.c_int, [_P]),
+    "smac_alpha_flux_split": (C.c_int, [_P, C.c_double, C.c_int32]),
     "smac_predictor": (C.c_int, [_P, C.c_double]),
     "smac_pressure_solve": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "smac_correct": (C.c_int, [_P, C.c_double]),
@@ -94,6 +96,7 @@ SIGNATURES = {
     "smac_calc_cfl": (C.c_int, [_P, C.c_double, _D, _D]),
     "smac_update_boundary_ghost": (C.c_int, [_P, C.c_double]),
     "smac_make_face_gradp": (C.c_int, [_P]),
+    "smac_cell_output": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "smac_adjoint_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(_P * 3), C.POINTER(_P * 3), _P,
                                     C.POINTER(SolveStats)]),
     "smac_default_opts": (None, [C.POINTER(SolverOpts)]),
@@ -266,6 +269,13 @@ class Context:
         accumulated flux enters the next predictor's mass flux."""
         self._check(self.lib.smac_transport_alpha(self.h, dt, int(substeps), int(step)), "smac_transport_alpha")
 
+    def clear_alpha_flux_accum(self):
+        self._check(self.lib.smac_clear_alpha_flux_accum(self.h), "smac_clear_alpha_flux_accum")
+
+    def alpha_flux_split(self, sub_dt, step):
+        """One VoF substep (split THINC/WLIC), its flux added to the accumulator."""
+        self._check(self.lib.smac_alpha_flux_split(self.h, sub_dt, int(step)), "smac_alpha_flux_split")
+
     def update_boundary_ghost(self, dt):
         """Ghost pressure and tangential velocity ghosts after a step
         (G_SMACSolver::update_boundary_ghost)."""
@@ -274,6 +284,16 @@ class Context:
     def make_face_gradp(self):
         """Face pressure gradient into gradpx/gradpy/gradpz (G_SMACSolver::make_face_gradp)."""
         self._check(self.lib.smac_make_face_gradp(self.h), "smac_make_face_gradp")
+
+    def cell_output(self):
+        """(p, alpha, velocity) cell arrays of the VTI output as float32:
+        shapes (nz, ny, nx) and (nz, ny, nx, 3)."""
+        nx, ny, nz = self.n
+        n = nx * ny * nz
+        buf = np.empty(5 * n, dtype=np.float32)
+        self._check(self.lib.smac_cell_output(self.h, buf.ctypes.data_as(C.POINTER(C.c_float)), buf.size),
+                    "smac_cell_output")
+        return (buf[:n].reshape(nz, ny, nx), buf[n:2 * n].reshape(nz, ny, nx), buf[2 * n:].reshape(nz, ny, nx, 3))
 
     def set_profiling(self, on=True):
         self._check(self.lib.smac_set_profiling(self.h, int(on)), "smac_set_profiling")

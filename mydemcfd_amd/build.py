@@ -57,19 +57,36 @@ def build(verbose: bool = False) -> str:
 ROOT = os.path.dirname(HERE)
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "cavity_facade.cpp")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "build", "cavity_facade")
+CONFIG_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "config_driver.cpp")
+CONFIG_DRIVER = os.path.join(ROOT, "tests", "cpp", "build", "config_driver")
+CONFIG_TOOL_SRC = os.path.join(ROOT, "tests", "cpp", "config_tool.cpp")
+CONFIG_TOOL = os.path.join(ROOT, "tests", "cpp", "build", "config_tool")
 
 
 def build_facade_driver(verbose: bool = False) -> str:
-    """The C++ driver of tests/cpp (reference-shaped loop over the C++ facade)."""
+    """The C++ drivers of tests/cpp: reference-shaped loops over the C++ facade
+    (linked to the library) and the host-only configuration tool."""
     if not os.path.exists(DRIVER_SRC):
         return ""
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o", DRIVER,
-           "-L", LIBDIR, "-lsmac_hip", "-Wl,-rpath," + LIBDIR]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    link = ["-L", LIBDIR, "-lsmac_hip", "-Wl,-rpath," + LIBDIR]
+    cmds = [["g++", "-O2", "-std=c++17"] + inc + [DRIVER_SRC, "-o", DRIVER] + link,
+            ["g++", "-O2", "-std=c++17"] + inc + [CONFIG_DRIVER_SRC, "-o", CONFIG_DRIVER] + link,
+            ["g++", "-O2", "-std=c++17"] + inc + [CONFIG_TOOL_SRC, "-o", CONFIG_TOOL]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     return DRIVER
+
+
+def build_config_tool() -> str:
+    """The host-only configuration tool alone (no HIP library needed)."""
+    os.makedirs(os.path.dirname(CONFIG_TOOL), exist_ok=True)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), CONFIG_TOOL_SRC, "-o",
+                    CONFIG_TOOL], check=True)
+    return CONFIG_TOOL
 
 
 if __name__ == "__main__":

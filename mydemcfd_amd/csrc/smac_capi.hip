@@ -104,6 +104,7 @@ struct smac_ctx {
   double* vfacc[3] = {nullptr, nullptr, nullptr};
   unsigned char* vcoeff = nullptr;
   bool vof_armed = false;
+  float* cellout = nullptr;  // device buffer of smac_cell_output (5 floats per cell)
   double* rgt = nullptr;  // fine ping-pong partner of z in the reference V-cycle
   bool prof = false;
   hipEvent_t ev[8];
@@ -391,6 +392,7 @@ extern "C" void smac_destroy(smac_ctx* c) {
   for (int m = 0; m < 11; m++)
     if (c->adj[m]) (void)hipFree(c->adj[m]);
   if (c->vcoeff) (void)hipFree(c->vcoeff);
+  if (c->cellout) (void)hipFree(c->cellout);
   if (c->Sh) (void)hipHostFree(c->Sh);
   if (c->ev_ok)
     for (int e = 0; e < 8; e++) (void)hipEventDestroy(c->ev[e]);
@@ -1309,21 +1311,35 @@ static int vof_alloc(smac_ctx* c) {
   return rc;
 }
 
-extern "C" int smac_transport_alpha(smac_ctx* c, double dt, int32_t substeps, int32_t step) {
-  if (!c || !(dt > 0) || substeps < 1 || step < 0) return SMAC_ERR_INVALID_ARG;
+extern "C" int smac_clear_alpha_flux_accum(smac_ctx* c) {
+  if (!c) return SMAC_ERR_INVALID_ARG;
   int rc = vof_alloc(c);
   if (rc) return rc;
   for (int a = 0; a < 3; a++) HIPOK(hipMemsetAsync(c->vfacc[a], 0, c->tot * sizeof(double), c->st));
+  c->vof_armed = false;
+  return SMAC_OK;
+}
+
+extern "C" int smac_alpha_flux_split(smac_ctx* c, double sub_dt, int32_t step) {
+  if (!c || !(sub_dt > 0) || step < 0) return SMAC_ERR_INVALID_ARG;
+  int rc = vof_alloc(c);
+  if (rc) return rc;
   Vel V;
   for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
-  const double sub_dt = dt / (double)substeps;
-  for (int s = 0; s < substeps; s++)
-    KT(c, "vof_substep", launch_vof_substep(c->g, c->d, V, &c->alpha, &c->alpha_new, c->vf, c->vfacc, c->vcoeff,
-                                            sub_dt, step, c->st));
+  KT(c, "vof_substep", launch_vof_substep(c->g, c->d, V, &c->alpha, &c->alpha_new, c->vf, c->vfacc, c->vcoeff,
+                                          sub_dt, step, c->st));
   HIPOK(hipGetLastError());
   c->vof_armed = true;
   c->props_ver++;  // the face coefficients follow the new alpha at the next property update
   return SMAC_OK;
+}
+
+extern "C" int smac_transport_alpha(smac_ctx* c, double dt, int32_t substeps, int32_t step) {
+  if (!c || !(dt > 0) || substeps < 1 || step < 0) return SMAC_ERR_INVALID_ARG;
+  int rc = smac_clear_alpha_flux_accum(c);
+  const double sub_dt = dt / (double)substeps;
+  for (int s = 0; s < substeps && !rc; s++) rc = smac_alpha_flux_split(c, sub_dt, step);
+  return rc;
 }
 
 // ------------------------------------------------------------------ after the step
@@ -1347,6 +1363,22 @@ extern "C" int smac_make_face_gradp(smac_ctx* c) {
     }
   KT(c, "face_gradp", launch_face_gradp(c->g, c->p, c->gradp, c->st));
   HIPOK(hipGetLastError());
+  return SMAC_OK;
+}
+
+extern "C" int smac_cell_output(smac_ctx* c, float* h, size_t count) {
+  if (!c || !h) return SMAC_ERR_INVALID_ARG;
+  const size_t n = (size_t)c->n[0] * (size_t)c->n[1] * (size_t)c->n[2];
+  if (count != 5 * n) return SMAC_ERR_INVALID_ARG;
+  if (!c->cellout) {
+    int rc = balloc((void**)&c->cellout, 5 * n * sizeof(float));
+    if (rc) return rc;
+  }
+  const double* u[3] = {c->u[0], c->u[1], c->u[2]};
+  KT(c, "pack_cells", launch_pack_cells(c->g, c->p, c->alpha, u, c->cellout, c->st));
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(h, c->cellout, 5 * n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+  HIPOK(hipStreamSynchronize(c->st));
   return SMAC_OK;
 }
 

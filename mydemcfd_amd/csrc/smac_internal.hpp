@@ -174,6 +174,9 @@ struct Spacing {
 void launch_ghost_pressure(const Geo& g, const double h[3], double* p, const double* rho, hipStream_t st);
 void launch_ghost_velocity(const Geo& g, double* const u[3], hipStream_t st);
 void launch_face_gradp(const Geo& g, const double* p, double* const gp[3], hipStream_t st);
+// VTI cell data: p[n], alpha[n] (al == nullptr: 0), velocity[3n] as float
+void launch_pack_cells(const Geo& g, const double* p, const double* al, const double* const u[3], float* out,
+                       hipStream_t st);
 // one substep of the split VoF transport (smac_vof.hip); swaps *al and *an
 // once per direction, accumulates the face fluxes into Facc
 void launch_vof_substep(const Geo& g, const double h[3], const Vel& V, double** al, double** an, double* const F[3],

@@ -117,6 +117,33 @@ __global__ __launch_bounds__(256) void k_face_gradp(Geo g, const double* __restr
   gz[id] = (pc - pzm) * g.inv_d[2];
 }
 
+// cell output of the VTI writer: p, alpha, face-mean velocity as float, x
+// fastest over the interior (FileInOut::output_vti_binary_cellData,
+// CFD/FileInOut.cpp:204-236)
+__global__ __launch_bounds__(256) void k_pack_cells(Geo g, const double* __restrict__ p,
+                                                    const double* __restrict__ al, const double* __restrict__ u,
+                                                    const double* __restrict__ v, const double* __restrict__ w,
+                                                    float* __restrict__ out) {
+  const int i = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int k = 1 + (int)blockIdx.z;
+  if (i > g.n[0] || j > g.n[1]) return;
+  const size_t n = (size_t)g.n[0] * (size_t)g.n[1] * (size_t)g.n[2];
+  const size_t m = ((size_t)(k - 1) * (size_t)g.n[1] + (size_t)(j - 1)) * (size_t)g.n[0] + (size_t)(i - 1);
+  const size_t c = idx(g, i, j, k);
+  out[m] = (float)p[c];
+  out[n + m] = al ? (float)al[c] : 0.0f;
+  out[2 * n + 3 * m + 0] = (float)(0.5 * (u[c] + u[c + 1]));
+  out[2 * n + 3 * m + 1] = (float)(0.5 * (v[c] + v[c + (size_t)g.px]));
+  out[2 * n + 3 * m + 2] = (float)(0.5 * (w[c] + w[c + (size_t)g.px * (size_t)g.py]));
+}
+
+void launch_pack_cells(const Geo& g, const double* p, const double* al, const double* const u[3], float* out,
+                       hipStream_t st) {
+  dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4), (unsigned)g.n[2]);
+  hipLaunchKernelGGL(k_pack_cells, grid, dim3(64, 4), 0, st, g, p, al, u[0], u[1], u[2], out);
+}
+
 template <int KIND, int A>
 static void launch_plane(const Geo& g, const Plane& Q, const Spacing& H, double* f, const double* rho,
                          hipStream_t st) {

@@ -1565,15 +1565,32 @@ static int vof_alloc(struct smac_ctx* c) {
     return SMAC_OK;
 }
 
+/* G_SMACSolver::clear_alpha_flux_accum (CFD/G_SMACSolver_VoF.cu:1688-1696) */
+int smac_clear_alpha_flux_accum(smac_ctx* c) {
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    if (vof_alloc(c)) return SMAC_ERR_OOM;
+    for (int a = 0; a < 3; a++) memset(c->vfacc[a], 0, c->tot * sizeof(double));
+    c->vof_armed = 0;
+    return SMAC_OK;
+}
+
+/* one substep: clear_alpha_flux, alpha_flux_thincwlic_split(sub_dt, step),
+ * alpha_flux_accum (CFD/main.cu:289-292) */
+int smac_alpha_flux_split(smac_ctx* c, double sub_dt, int32_t step) {
+    if (!c || !(sub_dt > 0) || step < 0) return SMAC_ERR_INVALID_ARG;
+    if (vof_alloc(c)) return SMAC_ERR_OOM;
+    vof_split(c, sub_dt, step);
+    c->vof_armed = 1;
+    return SMAC_OK;
+}
+
 /* One time step of VoF transport (CFD/main.cu:283-295) */
 int smac_transport_alpha(smac_ctx* c, double dt, int32_t substeps, int32_t step) {
     if (!c || !(dt > 0) || substeps < 1 || step < 0) return SMAC_ERR_INVALID_ARG;
-    if (vof_alloc(c)) return SMAC_ERR_OOM;
-    for (int a = 0; a < 3; a++) memset(c->vfacc[a], 0, c->tot * sizeof(double));
+    int rc = smac_clear_alpha_flux_accum(c);
     double sub_dt = dt / (double)substeps;
-    for (int s = 0; s < substeps; s++) vof_split(c, sub_dt, step);
-    c->vof_armed = 1;
-    return SMAC_OK;
+    for (int s = 0; s < substeps && !rc; s++) rc = smac_alpha_flux_split(c, sub_dt, step);
+    return rc;
 }
 
 /* ------------------------------------------------ after the step */
@@ -1694,6 +1711,26 @@ int smac_make_face_gradp(smac_ctx* c) {
                     ivec q = iv(i, j, k);
                     c->gradp[a][IX(c, q)] = (c->p_[IX(c, q)] - c->p_[IX(c, add(q, a, -1))]) * c->inv_d[a];
                 }
+    return SMAC_OK;
+}
+
+/* FileInOut::output_vti_binary_cellData (CFD/FileInOut.cpp:204-236): cell
+ * data as float, x fastest: p, alpha, face-mean velocity interleaved */
+int smac_cell_output(smac_ctx* c, float* h, size_t count) {
+    if (!c || !h) return SMAC_ERR_INVALID_ARG;
+    size_t n = (size_t)c->n[0] * c->n[1] * c->n[2];
+    if (count != 5 * n) return SMAC_ERR_INVALID_ARG;
+    size_t m = 0;
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++, m++) {
+                size_t id = IDX(c, i, j, k);
+                h[m] = (float)c->p_[id];
+                h[n + m] = (float)c->alpha[id];
+                h[2 * n + 3 * m + 0] = (float)(0.5 * (c->vel[0][id] + c->vel[0][IDX(c, i + 1, j, k)]));
+                h[2 * n + 3 * m + 1] = (float)(0.5 * (c->vel[1][id] + c->vel[1][IDX(c, i, j + 1, k)]));
+                h[2 * n + 3 * m + 2] = (float)(0.5 * (c->vel[2][id] + c->vel[2][IDX(c, i, j, k + 1)]));
+            }
     return SMAC_OK;
 }
 

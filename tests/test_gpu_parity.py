@@ -448,3 +448,74 @@ def test_vof_transport_and_step(hip, oracle_mod, case):
             assert err <= 1e-10, (s, f, err)
         err = rel_err_mod_mean(g.download("p"), o.download("p"), n)
         assert err <= 1e-10, (s, "p", err)
+
+
+def test_config_driver_dam_break(hip, oracle_mod, tmp_path):
+    """The reference's time loop (CFD/main.cu:258-346) over the C++ facade,
+    driven by a JSON configuration (tests/golden/dam_break_config.json):
+    set-up, CFL time steps, VoF substeps, properties, predictor, reference-GMG
+    PCG, correction, ghosts, face gradient, VTI output. The same loop on the
+    oracle, from initial fields computed independently in Python, gives the
+    same time steps and fields (1e-8 relative: the VoF transport calls
+    exp/log/cosh/acos), and the final VTI holds the oracle's cell data."""
+    import json
+    import os
+    import subprocess
+
+    import config_ref
+    from mydemcfd_amd import build, cfdtime
+
+    if not os.path.exists(build.CONFIG_DRIVER):
+        pytest.skip("C++ config driver not built")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dam_break_config.json")
+    steps = 8
+    print(f"case: config driver dam break 32x16x16 x{steps}", flush=True)
+    out = subprocess.run([build.CONFIG_DRIVER, path, str(tmp_path), str(steps)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("step ")]
+    assert len(rows) == steps
+    with open(path) as f:
+        cfg = json.load(f)
+    desc = config_ref.desc(cfg)
+    o = oracle_mod.OracleContext(**desc)
+    alpha0, v0 = config_ref.initial_fields(cfg)
+    o.upload("alpha", alpha0)
+    o.upload("v", v0)
+    o.update_properties(1)
+    t = cfg["time"]
+    ct = cfdtime.CFDTime(t["initial_dt"], t["maximum_dt"], t["interval"], t["end_time"], t["cfl"])
+    substeps = max(1, int(np.ceil(t["cfl"] / t["vof_cfl"])))
+    levels = cfg["pressure_solver"]["gmg_levels"]
+    dt_prev = 0.001  # the facade's (and the reference's) G_SMACSolver::dt_ before the first step
+    for s in range(steps):
+        tol = 1e-11 if s == 0 else cfg["pressure_solver"]["tol"]
+        cc, cv = o.calc_cfl(dt_prev)
+        dt = ct.update_time(cc if cc > cv else cv)
+        dt_prev = dt
+        assert abs(dt - float(rows[s][3])) <= 1e-9 * dt, (s, dt, rows[s])
+        o.clear_alpha_flux_accum()
+        for _ in range(substeps):
+            o.alpha_flux_split(dt / substeps, s)
+        o.update_properties(0)
+        o.predictor(dt)
+        o.pressure_solve(dt, precond="ref_gmg", stop_rule="ref_linf_dt", tol=tol, check_every=5, max_iter=2000,
+                         mg_levels=levels, ref_sweeps=4, ref_coarse_sweeps=30)
+        o.correct(dt)
+        o.update_boundary_ghost(dt)
+        o.make_face_gradp()
+    n = desc["n"]
+    for f in ("u", "v", "w", "alpha"):
+        a = np.fromfile(os.path.join(tmp_path, f + ".f64")).reshape(o.shape(f))
+        err = rel_err(a, o.download(f))
+        assert err <= 1e-8, (f, err)
+    a = np.fromfile(os.path.join(tmp_path, "p.f64")).reshape(o.shape("p"))
+    assert rel_err_mod_mean(a, o.download("p"), n) <= 1e-8
+    attrs, arrs = config_ref.read_vti(os.path.join(tmp_path, "result_999999.vti"))
+    p_o, al_o, vel_o = o.cell_output()
+    assert attrs["extent"] == [0, n[0], 0, n[1], 0, n[2]]
+    for name, ref in (("alpha", al_o), ("velocity", vel_o)):
+        got = arrs[name][0]
+        err = float(np.max(np.abs(got - ref.ravel()))) / max(float(np.max(np.abs(ref))), 1e-30)
+        assert err <= 1e-6, (name, err)
+    assert os.path.exists(os.path.join(tmp_path, "result_000000.vti"))
