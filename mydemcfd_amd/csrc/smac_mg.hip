@@ -24,10 +24,13 @@ struct CoefC {
   C a[6];
   C diag;
 };
-template <typename C>
+// VB: the level carries face coefficients L.b (variable beta); else the
+// uniform beta0 (a separate instantiation, so the uniform kernels do not hold
+// the registers of the variable path)
+template <typename C, bool VB>
 __device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k) {
   Coef c;
-  if (L.b[0]) {
+  if (VB) {
     // variable coefficients: a = beta/h^2 per face, diag = sum of all six,
     // off-diagonals zero across walls
     const int q[3] = {i, j, k};
@@ -59,7 +62,7 @@ __device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k)
 // xp, xm, yp, ym, zp, zm. zero_start: e is zero on entry, so e = f/diag.
 // VPL > 0: a lane owns VPL cells h = 1+lane+64m, all loaded before any is
 // computed (one memory round per wave instead of VPL); VPL = 0: any row length.
-template <typename T, int VPL>
+template <typename T, int VPL, bool VB>
 __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
                                                  int zero_start) {
   using C = typename St<T>::C;
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const 
     for (int h = 1 + lane; h <= cnt; h += 64) {
       const int i = par ? 2 * h - 1 : 2 * h;
       const int p = off + h;
-      CoefC<C> c = level_coef<C>(L, i, j, k);
+      CoefC<C> c = level_coef<C, VB>(L, i, j, k);
       if (zero_start) {
         St<T>::st(e, rb + p, St<T>::ld(f, rb + p) / c.diag);
       } else {
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const 
     if (h <= cnt) {
       const int i = par ? 2 * h - 1 : 2 * h;
       const int p = off + h;
-      CoefC<C> c = level_coef<C>(L, i, j, k);
+      CoefC<C> c = level_coef<C, VB>(L, i, j, k);
       if (zero_start) {
         St<T>::st(e, rb + p, fv[m] / c.diag);
       } else {
@@ -148,7 +151,7 @@ __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], doubl
 // res = f - (diag e - S) of one fine row in LDS and restricts it along x into
 // tb (coarse x, fine y and z, natural order, pitch C.n[0]+3). Pass 2 restricts
 // tb along y and z.
-template <typename T, int VPL>
+template <typename T, int VPL, bool VB>
 __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __restrict__ e, const T* __restrict__ f,
                                                      typename St<T>::C* __restrict__ tb) {
   using C = typename St<T>::C;
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
         const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
         if (p <= nx) {
           const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
-          CoefC<C> c = level_coef<C>(L, i, j, k);
+          CoefC<C> c = level_coef<C, VB>(L, i, j, k);
           C S = c.a[0] * xp[m] + c.a[1] * xm[m] + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
           buf[i] = ef[m] - (c.diag * ec[m] - S);
         }
@@ -348,14 +351,26 @@ static const dim3 ROWB(256), CELLB(64, 4, 1);
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
   const int half = L.n[0] - (L.n[0] >> 1);  // longest active half row
-  SMAC_VPL_DISPATCH(half, hipLaunchKernelGGL((k_mg_rbgs<T, VPL>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start))
+  if (L.b[0]) {
+    SMAC_VPL_DISPATCH(half,
+                      hipLaunchKernelGGL((k_mg_rbgs<T, VPL, true>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start))
+  } else {
+    SMAC_VPL_DISPATCH(half,
+                      hipLaunchKernelGGL((k_mg_rbgs<T, VPL, false>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start))
+  }
 }
 
 template <typename T>
 void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
                         hipStream_t st) {
   size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
-  SMAC_VPL_DISPATCH(F.n[0], hipLaunchKernelGGL((k_mg_resid_rx<T, VPL>), lrows_grid(F), ROWB, shm, st, F, C, e, f, tb))
+  if (F.b[0]) {
+    SMAC_VPL_DISPATCH(F.n[0], hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, true>), lrows_grid(F), ROWB, shm, st, F, C, e,
+                                                 f, tb))
+  } else {
+    SMAC_VPL_DISPATCH(F.n[0], hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, false>), lrows_grid(F), ROWB, shm, st, F, C,
+                                                 e, f, tb))
+  }
   hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
 }
 template <typename T>
