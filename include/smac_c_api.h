@@ -138,6 +138,11 @@ typedef struct smac_solver_opts {
                                     after the coarse correction, num_iter_fine_ (4) */
     int32_t ref_coarse_sweeps;   /* REF_GMG sweeps on the coarsest level,
                                     num_iter_coarse_ (30, G_GMGSolver.h:21-22) */
+    int32_t stationary;          /* 0 = preconditioned CG (default); 1 = the
+                                    stationary iteration x += M r with the same
+                                    preconditioner M: with tol = 0 a fixed linear
+                                    map of the right-hand side (finite-difference
+                                    checks of the adjoint). Extension. */
 } smac_solver_opts;
 
 typedef struct smac_solve_stats {

@@ -58,7 +58,8 @@ class SolverOpts(C.Structure):
     _fields_ = [("precond", C.c_int32), ("precision", C.c_int32), ("stop_rule", C.c_int32),
                 ("tol", C.c_double), ("check_every", C.c_int32), ("max_iter", C.c_int32),
                 ("mg_levels", C.c_int32), ("nu1", C.c_int32), ("nu2", C.c_int32), ("flexible", C.c_int32),
-                ("ref_sweeps", C.c_int32), ("ref_coarse_sweeps", C.c_int32)]
+                ("ref_sweeps", C.c_int32), ("ref_coarse_sweeps", C.c_int32),
+                ("stationary", C.c_int32)]
 
 
 class SolveStats(C.Structure):

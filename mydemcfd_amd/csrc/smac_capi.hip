@@ -1043,6 +1043,7 @@ extern "C" void smac_default_opts(smac_solver_opts* o) {
   o->flexible = 0;
   o->ref_sweeps = 4;
   o->ref_coarse_sweeps = 30;
+  o->stationary = 0;
 }
 
 // The PCG of G_PCGSolver::solve_* (CFD/pressure_solver/G_PCGSolver.cu:307-922)
@@ -1121,9 +1122,15 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   double rr = bb, linf = 0.0;
   if (bb == 0.0) conv = 1;
   bool have_update = false;
+  // stationary iteration: alpha = 1, beta = 0 (d = z) throughout
+  const bool stat = o.stationary != 0;
+  if (stat) {
+    launch_scalar_set(c->S, S_ALPHA, 1.0, s);
+    launch_scalar_set(c->S, S_BETA, 0.0, s);
+  }
   for (iter = 0; iter < o.max_iter && !conv; ++iter) {
     KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
-    {
+    if (!stat) {
       FinalizeArgs fa = fin(c);
       fin_add(fa, P[0], 0, S_PAP);
       fa.op = OP_DIV;
@@ -1144,19 +1151,23 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       KT(c, "add_scalar_norms", launch_add_scalar_norms(g, c->r, -1. * size_inv, c->S, S_SUMR, P[0], P[1], s));
     }
     precond();
-    KT(c, "dot2", launch_dot2<T>(g, c->r, z, c->Ad, o.flexible, P[2], P[3], s));
+    if (!stat) KT(c, "dot2", launch_dot2<T>(g, c->r, z, c->Ad, o.flexible, P[2], P[3], s));
     bool check = (iter % o.check_every) == 0;
     {
-      FinalizeArgs fa = fin(c);
-      fin_add(fa, P[2], 0, S_RZN);
-      if (o.flexible) fin_add(fa, P[3], 0, S_ZAD);
-      launch_finalize(fa, c->S, s);
+      if (!stat) {
+        FinalizeArgs fa = fin(c);
+        fin_add(fa, P[2], 0, S_RZN);
+        if (o.flexible) fin_add(fa, P[3], 0, S_ZAD);
+        launch_finalize(fa, c->S, s);
+      }
       FinalizeArgs fb = fin(c);
       if (check) {
         fin_add(fb, P[0], 0, S_RR);
         fin_add(fb, P[1], 1, S_LINF);
       }
-      if (o.flexible) {
+      if (stat) {
+        // no scalar operation: alpha and beta stay 1 and 0
+      } else if (o.flexible) {
         fb.op = OP_BETA_PR;
         fb.dst = S_BETA;
         fb.sa = S_ALPHA;
@@ -1168,7 +1179,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
         fb.sa = S_RZN;
         fb.sb = S_RZ;
       }
-      launch_finalize(fb, c->S, s);
+      if (fb.count > 0 || fb.op != OP_NONE) launch_finalize(fb, c->S, s);
     }
     if (check) {
       HIPOK(hipGetLastError());
@@ -1191,7 +1202,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       }
     }
     KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->S, S_BETA, s));
-    launch_scalar_copy(c->S, S_RZ, S_RZN, s);
+    if (!stat) launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }
   HIPOK(hipGetLastError());
   if (st) {

@@ -151,6 +151,7 @@ template <typename F>
 void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
+void launch_scalar_set(double* S, int dst, double v, hipStream_t st);
 void launch_correct(const Geo& g, const BetaF* bf, double* const us[3], const double* dp, double* u[3], double dt,
                     hipStream_t st);
 void launch_props(const Geo& g, const double* alpha, double* rho, double* rho_old, double* inv_rho, double* mu,

@@ -759,6 +759,7 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs fa, double* __re
 
 // S[dst] = S[src] (e.g. rz <- rz_new)
 __global__ void k_scalar_copy(double* S, int dst, int src) { S[dst] = S[src]; }
+__global__ void k_scalar_set(double* S, int dst, double v) { S[dst] = v; }
 
 // ------------------------------------------------------------- correction
 // k_correct_vof_velocity (CFD/G_SMACSolver.cu:1072-1142), component A (dp natural)
@@ -1018,6 +1019,9 @@ void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st) {
 }
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st) {
   hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(1), 0, st, S, dst, src);
+}
+void launch_scalar_set(double* S, int dst, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_scalar_set, dim3(1), dim3(1), 0, st, S, dst, v);
 }
 void launch_correct(const Geo& g, const BetaF* bf, double* const us[3], const double* dp, double* u[3], double dt,
                     hipStream_t st) {

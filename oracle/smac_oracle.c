@@ -1248,6 +1248,7 @@ void smac_default_opts(smac_solver_opts* o) {
     o->flexible = 0;
     o->ref_sweeps = 4;
     o->ref_coarse_sweeps = 30;
+    o->stationary = 0;
 }
 
 int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st) {
@@ -1306,7 +1307,8 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
             for (int j = 1; j <= ny; j++)
                 for (int i = 1; i <= nx; i++) c->Ad[IDX(c, i, j, k)] = apply_A(c, c->dvec, i, j, k);
         double pAp = csum(c, RED_DOT, c->dvec, c->Ad);
-        double alpha = rz / pAp;
+        /* stationary: x += z, r -= A z (a fixed linear map of b for tol = 0) */
+        double alpha = o.stationary ? 1.0 : rz / pAp;
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= ny; j++)
@@ -1332,7 +1334,7 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
                 if (sqrt(rr) < o.tol * bnorm) { iter++; conv = 1; break; }
             }
         }
-        double beta = o.flexible ? (-alpha * zAd) / rz : rz_new / rz;
+        double beta = o.stationary ? 0.0 : (o.flexible ? (-alpha * zAd) / rz : rz_new / rz);
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= ny; j++)

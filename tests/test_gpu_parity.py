@@ -408,8 +408,12 @@ def test_vof_transport_and_step(hip, oracle_mod, case):
     with the step), property update and a full step with the alpha flux in the
     mass flux, four times. The transport evaluates exp/log/cosh/acos, whose
     device and host libraries may differ in the last bit, so the fields are
-    compared to 1e-10 relative instead of bit for bit; the solve runs a fixed
-    30 MG-PCG iterations so both sides take the same path."""
+    compared to 1e-10 of their scale instead of bit for bit (the velocity
+    components and the flux components share one scale: the dam break has no
+    y motion, so v and the y flux are rounding noise about zero); the solve is
+    40 stationary multigrid cycles, a fixed linear map, so last-bit
+    differences of the right-hand side are not amplified by Krylov
+    coefficients."""
     from mydemcfd_amd import _abi, cases
 
     print(f"case: VoF transport + step, {case}", flush=True)
@@ -438,14 +442,20 @@ def test_vof_transport_and_step(hip, oracle_mod, case):
     _upload_both(g, o, fields)
     for c in (g, o):
         c.update_properties(1)
-    opts = dict(precond="mg", stop_rule="rel_l2", tol=0.0, max_iter=30)
+    opts = dict(precond="mg", stop_rule="rel_l2", tol=0.0, max_iter=40, stationary=1)
     for s in range(4):
         for c in (g, o):
             c.transport_alpha(dt, 2, s)
             c.step(dt, **opts)
-        for f in ("alpha", "fluxx", "fluxy", "fluxz", "u", "v", "w"):
-            err = rel_err(g.download(f), o.download(f))
-            assert err <= 1e-10, (s, f, err)
+        err = rel_err(g.download("alpha"), o.download("alpha"))
+        assert err <= 1e-10, (s, "alpha", err)
+        for group in (("fluxx", "fluxy", "fluxz"), ("u", "v", "w")):
+            a = {f: g.download(f) for f in group}
+            b = {f: o.download(f) for f in group}
+            scale = max(float(np.max(np.abs(b[f]))) for f in group)
+            for f in group:
+                err = float(np.max(np.abs(a[f] - b[f]))) / scale
+                assert err <= 1e-10, (s, f, err)
         err = rel_err_mod_mean(g.download("p"), o.download("p"), n)
         assert err <= 1e-10, (s, "p", err)
 
