@@ -763,7 +763,12 @@ static int mg_coeffs(smac_ctx* c) {
     int rc = dalloc(&c->cinv, (size_t)n * n);
     if (rc) return rc;
   }
-  HIPOK(hipMemcpy(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+  // column-major on the device: the thread of row r reads entry (r, col) at
+  // col*n + r, so a wave's loads of one column are contiguous
+  std::vector<double> ht((size_t)n * n);
+  for (int r = 0; r < n; r++)
+    for (int col = 0; col < n; col++) ht[(size_t)col * n + r] = h[(size_t)r * n + col];
+  HIPOK(hipMemcpy(c->cinv, ht.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
   c->cinv_n = n;
   c->mg_ver = c->props_ver;
   return SMAC_OK;

@@ -220,9 +220,16 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_mg_ryz(Lv F, Lv Cl, const typename St<T>::C* __restrict__ tb,
                                                 T* __restrict__ fc) {
   using C = typename St<T>::C;
-  const int I = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int J = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int K = 1 + (int)blockIdx.z;
+  // blocks in x-fastest order, remapped so that each XCD takes a contiguous
+  // range of coarse planes (the fine rows of tb are shared by neighbouring
+  // coarse rows and planes, and each XCD has its own L2)
+  const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
+  const int lin = xcd_block((int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)),
+                            (int)(gridDim.x * gridDim.y * gridDim.z));
+  const int bx = lin % nbx, by = (lin / nbx) % nby, bz = lin / (nbx * nby);
+  const int I = 1 + bx * 64 + (int)threadIdx.x;
+  const int J = 1 + by * 4 + (int)threadIdx.y;
+  const int K = 1 + bz;
   if (I > Cl.n[0] || J > Cl.n[1] || K > Cl.n[2]) return;
   int iy[4], iz[4];
   double wy[4], wz[4];
@@ -316,18 +323,26 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Lv Cl, Lv F, const T* __rest
 }
 
 // coarsest level: e = Ainv f with the dense pseudo-inverse (rows in natural
-// x-fastest order), FP64 sequential accumulation per row
+// x-fastest order), FP64 sequential accumulation per row. ainv is stored
+// column-major (entry (row, col) at col*n + row): the lanes of a wave read
+// one column at consecutive addresses, and the f value is the same for all.
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_coarse(Lv Cl, const double* __restrict__ ainv, const T* __restrict__ f,
-                                                   T* __restrict__ e) {
+__global__ __launch_bounds__(64) void k_mg_coarse(Lv Cl, const double* __restrict__ ainv, const T* __restrict__ f,
+                                                  T* __restrict__ e) {
   const int n = Cl.n[0] * Cl.n[1] * Cl.n[2];
   const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (row >= n) return;
-  const double* a = ainv + (size_t)row * (size_t)n;
   double acc = 0.0;
+  int ci = 1, cj = 1, ck = 1;
   for (int col = 0; col < n; col++) {
-    int ci = col % Cl.n[0] + 1, cj = (col / Cl.n[0]) % Cl.n[1] + 1, ck = col / (Cl.n[0] * Cl.n[1]) + 1;
-    acc += a[col] * (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
+    acc += ainv[(size_t)col * (size_t)n + (size_t)row] * (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
+    if (++ci > Cl.n[0]) {
+      ci = 1;
+      if (++cj > Cl.n[1]) {
+        cj = 1;
+        ++ck;
+      }
+    }
   }
   int i = row % Cl.n[0] + 1, j = (row / Cl.n[0]) % Cl.n[1] + 1, k = row / (Cl.n[0] * Cl.n[1]) + 1;
   St<T>::st(e, lrow(Cl, j, k) + spos(i, Cl.n[0]), (typename St<T>::C)acc);
@@ -381,7 +396,7 @@ void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hi
 template <typename T>
 void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st) {
   int n = C.n[0] * C.n[1] * C.n[2];
-  hipLaunchKernelGGL(k_mg_coarse<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C, ainv, f, e);
+  hipLaunchKernelGGL(k_mg_coarse<T>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, C, ainv, f, e);
 }
 // coarse face coefficient of axis d (mg_coarsen_beta of the oracle): faces
 // off the level's face range are 0
