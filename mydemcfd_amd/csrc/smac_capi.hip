@@ -893,7 +893,7 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   }
   Level& C = c->lev[l + 1];
   KT(c, fine ? "mg_restrict_l0" : "mg_coarse_levels",
-     launch_mg_restrict<T>(V.lv, C.lv, e, f, (Cc*)V.tb, (T*)C.f, c->st));
+     launch_mg_restrict<T>(V.lv, C.lv, e, f, (Cc*)V.tb, (T*)C.f, nu1 > 0, c->st));
   vcycle<T>(c, l + 1, nu1, nu2);
   // trilinear prolongation: with post-smoothing the black cells are recomputed
   // before they are read, so only the red cells take the correction

@@ -186,8 +186,10 @@ void launch_vof_substep(const Geo& g, const double h[3], const Vel& V, double** 
 // buffer tb has the compute type of T
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st);
+// red != 0: the residual is 0 on black cells (the pre-smoothing ended with a
+// black half sweep) and formed on the red cells only
 template <typename T>
-void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
+void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc, int red,
                         hipStream_t st);
 template <typename T>
 void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hipStream_t st);

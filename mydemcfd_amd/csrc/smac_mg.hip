@@ -151,7 +151,10 @@ __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], doubl
 // res = f - (diag e - S) of one fine row in LDS and restricts it along x into
 // tb (coarse x, fine y and z, natural order, pitch C.n[0]+3). Pass 2 restricts
 // tb along y and z.
-template <typename T, int VPL, bool VB>
+// RED: the pre-smoothing ended with a black half sweep, which leaves the black
+// equations satisfied, so the residual is taken as 0 on black cells and formed
+// on the red half row only (half the loads; n even on every restricted level).
+template <typename T, int VPL, bool VB, bool RED>
 __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __restrict__ e, const T* __restrict__ f,
                                                      typename St<T>::C* __restrict__ tb) {
   using C = typename St<T>::C;
@@ -171,15 +174,20 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
     const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
     const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
     constexpr int NV = VPL > 0 ? VPL : 1;
-    const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
+    // RED: active positions p = off + h, h = 1..nE, of colour-0 cells
+    const int par = (j + k) & 1;
+    const int off = par ? nE : 0;
+    const int cnt = RED ? nE : nx;
+    const int trips = VPL > 0 ? 1 : (cnt + 63) / 64;
     for (int t = 0; t < trips; t++) {
-      // VPL > 0: the lane's cells p = 1+lane+64m, all loads before the arithmetic
+      // VPL > 0: the lane's cells h = 1+lane+64m, all loads before the arithmetic
       C ec[NV], ef[NV], xp[NV], xm[NV], yp[NV], ym[NV], zp[NV], zm[NV];
 #pragma unroll
       for (int m = 0; m < NV; m++) {
-        const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
-        if (p <= nx) {
-          const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
+        const int h = 1 + lane + 64 * (VPL > 0 ? m : t);
+        const int p = RED ? off + h : h;
+        if (h <= cnt) {
+          const int i = RED ? (par ? 2 * h - 1 : 2 * h) : (p <= nE ? 2 * p : 2 * (p - nE) - 1);
           xp[m] = St<T>::ld(e, rb + spos(LW(L, 0, i + 1), nx));
           xm[m] = St<T>::ld(e, rb + spos(LW(L, 0, i - 1), nx));
           yp[m] = St<T>::ld(e, ypr + p);
@@ -192,12 +200,14 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
       }
 #pragma unroll
       for (int m = 0; m < NV; m++) {
-        const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
-        if (p <= nx) {
-          const int i = p <= nE ? 2 * p : 2 * (p - nE) - 1;
+        const int h = 1 + lane + 64 * (VPL > 0 ? m : t);
+        const int p = RED ? off + h : h;
+        if (h <= cnt) {
+          const int i = RED ? (par ? 2 * h - 1 : 2 * h) : (p <= nE ? 2 * p : 2 * (p - nE) - 1);
           CoefC<C> c = level_coef<C, VB>(L, i, j, k);
           C S = c.a[0] * xp[m] + c.a[1] * xm[m] + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
           buf[i] = ef[m] - (c.diag * ec[m] - S);
+          if (RED) buf[par ? i + 1 : i - 1] = (C)0;  // the black neighbour of the same pair
         }
       }
     }
@@ -376,15 +386,24 @@ void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipS
 }
 
 template <typename T>
-void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc,
+void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc, int red,
                         hipStream_t st) {
   size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
-  if (F.b[0]) {
-    SMAC_VPL_DISPATCH(F.n[0], hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, true>), lrows_grid(F), ROWB, shm, st, F, C, e,
-                                                 f, tb))
+  const int nx = F.n[0];
+  if (red) {
+    if (F.b[0]) {
+      SMAC_VPL_DISPATCH(nx / 2, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, true, true>), lrows_grid(F), ROWB, shm, st,
+                                                   F, C, e, f, tb))
+    } else {
+      SMAC_VPL_DISPATCH(nx / 2, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, false, true>), lrows_grid(F), ROWB, shm,
+                                                   st, F, C, e, f, tb))
+    }
+  } else if (F.b[0]) {
+    SMAC_VPL_DISPATCH(nx, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, true, false>), lrows_grid(F), ROWB, shm, st, F,
+                                             C, e, f, tb))
   } else {
-    SMAC_VPL_DISPATCH(F.n[0], hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, false>), lrows_grid(F), ROWB, shm, st, F, C,
-                                                 e, f, tb))
+    SMAC_VPL_DISPATCH(nx, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, false, false>), lrows_grid(F), ROWB, shm, st, F,
+                                             C, e, f, tb))
   }
   hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
 }
@@ -444,7 +463,7 @@ void launch_fill(const Lv& L, T* x, hipStream_t st) {
 
 #define SMAC_INST_MG(T)                                                                                 \
   template void launch_mg_rbgs<T>(const Lv&, T*, const T*, int, int, hipStream_t);                     \
-  template void launch_mg_restrict<T>(const Lv&, const Lv&, const T*, const T*, typename St<T>::C*, T*, \
+  template void launch_mg_restrict<T>(const Lv&, const Lv&, const T*, const T*, typename St<T>::C*, T*, int, \
                                       hipStream_t);                                                    \
   template void launch_mg_prolong<T>(const Lv&, const Lv&, const T*, T*, int, hipStream_t);            \
   template void launch_mg_coarse<T>(const Lv&, const double*, const T*, T*, hipStream_t);              \

@@ -793,11 +793,18 @@ static void rbgs(const struct smac_ctx* c, level* V, int col) {
                 V->e[LIDX(V, i, j, k)] = (V->f[LIDX(V, i, j, k)] + S) / diag;
             }
 }
-static void residual(const struct smac_ctx* c, level* V) {
+/* red_only: the pre-smoothing ended with a black half sweep, which leaves the
+ * black equations satisfied: the residual is 0 on black cells and formed on
+ * red cells only */
+static void residual(const struct smac_ctx* c, level* V, int red_only) {
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= V->n[2]; k++)
         for (int j = 1; j <= V->n[1]; j++)
             for (int i = 1; i <= V->n[0]; i++) {
+                if (red_only && ((i + j + k) & 1)) {
+                    V->res[LIDX(V, i, j, k)] = 0.0;
+                    continue;
+                }
                 double a6[6], diag;
                 lcoef(c, V, i, j, k, a6, &diag);
                 double S = lnbsum(c, V, V->e, i, j, k, a6);
@@ -907,7 +914,7 @@ static void vcycle(const struct smac_ctx* c, int l, int nu1, int nu2) {
     }
     zero_level(V);
     for (int s = 0; s < nu1; s++) { rbgs(c, V, 0); rbgs(c, V, 1); }
-    residual(c, V);
+    residual(c, V, nu1 > 0);
     level* C = (level*)&c->lev[l + 1];
     restrict_res(c, V, C);
     vcycle(c, l + 1, nu1, nu2);
