@@ -21,7 +21,9 @@ Two properties of the forward shape these checks:
     check. The forward solves of these checks use the stationary iteration
     x += M r with the multigrid V-cycle (opts.stationary = 1, a fixed linear
     map, 40 cycles to a relative residual ~1e-17), whose Jacobian is the
-    converged inverse the adjoint applies.
+    converged inverse the adjoint applies. At the two-phase density ratio
+    (1000:1) that iteration diverges, and the two-phase check runs CG to its
+    floor instead.
 The exact step does not depend on p^n at all (the solve absorbs the old
 pressure gradient: p^{n+1} = A^+ (-D F(u^n)/dt) up to a constant), so the
 pressure direction checks that the adjoint returns a constant pressure
@@ -37,6 +39,10 @@ pytestmark = pytest.mark.gpu
 # the forward map of the difference quotients: 40 stationary multigrid cycles
 # (tol = 0): linear in the right-hand side and converged to rounding
 TIGHT = dict(precond="mg", stop_rule="rel_l2", tol=0.0, max_iter=40, stationary=1)
+# the stationary iteration diverges at a 1000:1 density ratio (the V-cycle's
+# spectrum exceeds 2 there although it is a good CG preconditioner): the
+# two-phase check keeps CG, 30 iterations to its floor
+TIGHT_PCG = dict(precond="mg", stop_rule="rel_l2", tol=0.0, max_iter=30)
 
 
 def _torch():
@@ -72,7 +78,7 @@ def _step_out(ctx, st, dt, opts, nsteps=1):
     for _ in range(nsteps):
         s = ctx.step(dt, opts)
         assert s.iters == opts.max_iter or s.converged, s.as_dict()
-        assert s.rel_res < 1e-12, s.as_dict()
+        assert s.rel_res < (1e-12 if opts.stationary else 1e-10), s.as_dict()
     out = {f: torch.empty_like(t) for f, t in st.items()}
     for f, t in out.items():
         ctx.download_device(f, t.data_ptr(), t.numel())
@@ -141,7 +147,7 @@ def _transpose_check(ctx, st0, d, wv, masks, dt, opts):
     pb = wv["p"].clone()
     torch.cuda.synchronize()
     s = ctx.adjoint_step(dt, [t.data_ptr() for t in u_n], [t.data_ptr() for t in ub], pb.data_ptr(), opts)
-    assert s.rel_res < 1e-12, s.as_dict()
+    assert s.rel_res < (1e-12 if opts.stationary else 1e-10), s.as_dict()
     torch.cuda.synchronize()
     grad = {"u": ub[0], "v": ub[1], "w": ub[2], "p": pb}
     for f in grad:  # non-state entries are zero on return
@@ -155,7 +161,7 @@ def test_adjoint_transpose_identity_one_step(hip, kind):
     print(f"case: adjoint transpose identity, {kind}", flush=True)
     ctx, desc, fields, dt = _setup(kind, hip)
     masks = _masks(ctx, desc, torch)
-    opts = ctx.opts(**TIGHT)
+    opts = ctx.opts(**(TIGHT_PCG if kind == "two_phase" else TIGHT))
     st0 = _state(ctx, fields)
     rng = np.random.default_rng(11)
     d = {f: torch.tensor(smooth_field(t.shape, rng), device="cuda") * masks[f] for f, t in st0.items()}
@@ -211,7 +217,7 @@ def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
     J.backward()
     grad = {f: x[f].grad for f in ("u", "v", "w")}
     fd_opts = ctx.opts(**TIGHT)
-    errs = []
+    fds, ads = [], []
     for k in range(ndir):
         # smooth random directions (seeds 0x5EED0014 + k): k^-5/3 fields with |k| <= 4
         du, dv, dw = cases.adjoint_fields(n, seed=0x5EED0014 + k, kmax=4)
@@ -224,8 +230,15 @@ def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
         dj = 0.5 * h3 * sum(float(((op[f] - om[f]) * (op[f] + om[f]))[masks[f]].sum()) for f in ("u", "v", "w"))
         fd = dj / (2 * eps)
         ad = sum(float((grad[f] * d[f]).sum()) for f in ("u", "v", "w"))
-        errs.append(abs(fd - ad) / abs(fd))
-        print(f"  direction: FD {fd:.10e} adjoint {ad:.10e} rel {errs[-1]:.2e}", flush=True)
+        fds.append(fd)
+        ads.append(ad)
+        print(f"  direction: FD {fd:.10e} adjoint {ad:.10e} rel {abs(fd - ad) / abs(fd):.2e}", flush=True)
+    # relative to each directional derivative, floored at a tenth of the
+    # largest: the difference quotient carries an absolute noise (~5e-7 here,
+    # the same for every direction), so a direction nearly orthogonal to the
+    # gradient would otherwise measure that noise against a tiny derivative
+    floor = 0.1 * max(abs(f) for f in fds)
+    errs = [abs(f - a) / max(abs(f), floor) for f, a in zip(fds, ads)]
     assert max(errs) <= tol_rel, errs
     return grad
 
