@@ -243,13 +243,34 @@ __global__ __launch_bounds__(256) void k_mg_ryz(Lv F, Lv Cl, const typename St<T
   if (I > Cl.n[0] || J > Cl.n[1] || K > Cl.n[2]) return;
   int iy[4], iz[4];
   double wy[4], wz[4];
-  int my = rweights(Cl.per[1], Cl.n[1], J, iy, wy);
-  int mz = rweights(Cl.per[2], Cl.n[2], K, iz, wz);
+  const int my = rweights(Cl.per[1], Cl.n[1], J, iy, wy);
+  const int mz = rweights(Cl.per[2], Cl.n[2], K, iz, wz);
+  // pad to 4 x 4 with weight 0 on a valid index: the padded terms come last
+  // and add exact zeros, so the sums are those of the 3- or 4-term stencil;
+  // all 16 loads are issued before the arithmetic
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    if (m >= my) {
+      iy[m] = iy[my - 1];
+      wy[m] = 0.0;
+    }
+    if (m >= mz) {
+      iz[m] = iz[mz - 1];
+      wz[m] = 0.0;
+    }
+  }
   const size_t pxT = (size_t)(Cl.n[0] + 3);
+  C v[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) v[a][b] = tb[((size_t)iz[a] * (size_t)F.py + (size_t)iy[b]) * pxT + I];
   C tz = 0;
-  for (int a = 0; a < mz; a++) {
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
     C ty = 0;
-    for (int b = 0; b < my; b++) ty += (C)wy[b] * tb[((size_t)iz[a] * (size_t)F.py + (size_t)iy[b]) * pxT + I];
+#pragma unroll
+    for (int b = 0; b < 4; b++) ty += (C)wy[b] * v[a][b];
     tz += (C)wz[a] * ty;
   }
   St<T>::st(fc, lrow(Cl, J, K) + spos(I, Cl.n[0]), (C)0.125 * tz);
@@ -344,7 +365,26 @@ __global__ __launch_bounds__(64) void k_mg_coarse(Lv Cl, const double* __restric
   if (row >= n) return;
   double acc = 0.0;
   int ci = 1, cj = 1, ck = 1;
-  for (int col = 0; col < n; col++) {
+  constexpr int U = 16;  // columns whose loads are issued before their (sequential) adds
+  int col = 0;
+  for (; col + U <= n; col += U) {
+    double a[U], fv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      a[u] = ainv[(size_t)(col + u) * (size_t)n + (size_t)row];
+      fv[u] = (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
+      if (++ci > Cl.n[0]) {
+        ci = 1;
+        if (++cj > Cl.n[1]) {
+          cj = 1;
+          ++ck;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) acc += a[u] * fv[u];
+  }
+  for (; col < n; col++) {
     acc += ainv[(size_t)col * (size_t)n + (size_t)row] * (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
     if (++ci > Cl.n[0]) {
       ci = 1;

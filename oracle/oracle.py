@@ -44,6 +44,8 @@ def lib():
         l.smac_oracle_rg_beta.restype = C.c_int
         l.smac_oracle_rg_beta.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), C.c_int, C.c_int,
                                           C.POINTER(C.c_int * 3), _abi._D, C.c_size_t]
+        l.smac_oracle_vofmath.restype = C.c_int
+        l.smac_oracle_vofmath.argtypes = [C.c_int, _abi._D, _abi._D, C.c_size_t]
         l.smac_oracle_threads.restype = C.c_int
         l.smac_oracle_threads.argtypes = []
         _lib = l
@@ -94,3 +96,11 @@ class OracleContext(_abi.Context):
 
 def threads() -> int:
     return lib().smac_oracle_threads()
+
+
+def vofmath(fn, x):
+    """The VoF transport's exp (fn 0), log (1), cosh (2) and acos on [0, 1] (3)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    lib().smac_oracle_vofmath(int(fn), x.ctypes.data_as(_abi._D), y.ctypes.data_as(_abi._D), x.size)
+    return y

@@ -189,3 +189,25 @@ def test_mass_flux_full_box_is_single_phase_rho1(oracle_mod):
     for f in ("ustar", "vstar", "wstar"):
         a, b = two.download(f), one.download(f)
         assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(b)), f
+
+
+def _ulps(a, b):
+    """|a - b| in units of the last place of b."""
+    return np.abs(a - b) / np.spacing(np.abs(b))
+
+
+@pytest.mark.parametrize("fn,lo,hi", [(0, -40.0, 40.0), (1, 1e-12, 1e6), (2, -20.0, 20.0), (3, 0.0, 1.0)])
+def test_transcendentals_against_libm(oracle_mod, fn, lo, hi):
+    """The explicit exp/log/cosh/acos sequences of the transport (shared bit
+    for bit with the device kernels) stay within a few ulp of libm."""
+    rng = np.random.default_rng(fn)
+    if fn == 1:
+        x = np.exp(rng.uniform(np.log(lo), np.log(hi), 20000))
+    else:
+        x = rng.uniform(lo, hi, 20000)
+    x = np.concatenate([x, [lo, hi, 0.5, 1.0] if fn == 3 else [lo, hi]])
+    ref = {0: np.exp, 1: np.log, 2: np.cosh, 3: np.arccos}[fn](x)
+    got = oracle_mod.vofmath(fn, x)
+    assert np.max(_ulps(got, ref)) <= 4.0, np.max(_ulps(got, ref))
+    if fn == 3:
+        assert np.isnan(oracle_mod.vofmath(3, np.array([1.0 + 2.0 ** -52]))[0])
