@@ -21,9 +21,12 @@ Two properties of the forward shape these checks:
     check. The forward solves of these checks use the stationary iteration
     x += M r with the multigrid V-cycle (opts.stationary = 1, a fixed linear
     map, 40 cycles to a relative residual ~1e-17), whose Jacobian is the
-    converged inverse the adjoint applies. At the two-phase density ratio
-    (1000:1) that iteration diverges, and the two-phase check runs CG to its
-    floor instead.
+    converged inverse the adjoint applies. At a 1000:1 density ratio that
+    iteration diverges, and CG stalls at a relative residual ~1.4e-10 (the
+    FP64 floor eps * cond of that operator), whose noise over 2 eps is ~1e-5
+    of the quotient; the two-phase check therefore takes a 20:1 density ratio
+    (the stationary iteration converges to ~1e-17 up to 30:1), which runs the
+    same variable-coefficient code as any ratio.
 The exact step does not depend on p^n at all (the solve absorbs the old
 pressure gradient: p^{n+1} = A^+ (-D F(u^n)/dt) up to a constant), so the
 pressure direction checks that the adjoint returns a constant pressure
@@ -39,10 +42,10 @@ pytestmark = pytest.mark.gpu
 # the forward map of the difference quotients: 40 stationary multigrid cycles
 # (tol = 0): linear in the right-hand side and converged to rounding
 TIGHT = dict(precond="mg", stop_rule="rel_l2", tol=0.0, max_iter=40, stationary=1)
-# the stationary iteration diverges at a 1000:1 density ratio (the V-cycle's
-# spectrum exceeds 2 there although it is a good CG preconditioner): the
-# two-phase check keeps CG, 60 iterations to its floor
-TIGHT_PCG = dict(precond="mg", stop_rule="rel_l2", tol=0.0, max_iter=60)
+# density ratio of the two-phase check (the stationary iteration diverges at
+# 1000:1: the V-cycle's spectrum exceeds 2 there although it is a good CG
+# preconditioner)
+TWO_PHASE_RHO = (1.0, 20.0)
 
 
 def _torch():
@@ -122,7 +125,7 @@ def _setup(kind, hip):
         dt = 1e-3
     else:  # two-phase column
         n = (16, 8, 8)
-        desc = cases.dam_break_desc(n, length=(2.0, 1.0, 1.0))
+        desc = cases.dam_break_desc(n, length=(2.0, 1.0, 1.0), rho=TWO_PHASE_RHO)
         ctx0 = hip.SMACContext(**desc)
         fields = {f: 0.05 * smooth_field(ctx0.shape(f), rng) for f in ("u", "v", "w")}
         fields["alpha"] = cases.dam_break_alpha(n)
@@ -161,7 +164,7 @@ def test_adjoint_transpose_identity_one_step(hip, kind):
     print(f"case: adjoint transpose identity, {kind}", flush=True)
     ctx, desc, fields, dt = _setup(kind, hip)
     masks = _masks(ctx, desc, torch)
-    opts = ctx.opts(**(TIGHT_PCG if kind == "two_phase" else TIGHT))
+    opts = ctx.opts(**TIGHT)
     st0 = _state(ctx, fields)
     rng = np.random.default_rng(11)
     d = {f: torch.tensor(smooth_field(t.shape, rng), device="cuda") * masks[f] for f, t in st0.items()}
