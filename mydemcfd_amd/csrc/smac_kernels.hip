@@ -490,56 +490,105 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z
   if (lane == 0) part0[row] = acc;
 }
 
-// Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124)
-template <bool VAR, int VPL>
+// Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124).
+// Lanes own split positions: lane l holds q = 1+l, 1+l+64, ... of the row
+// (contiguous loads and stores), and the row sum of d*Ad is accumulated per
+// lane in that order (natural cell i = 2q for q <= nx/2, 2(q - nx/2) - 1
+// above).
+// FAST (nx = 64 VPL, VPL even): the row is loaded once; the x neighbours of a
+// cell are the other half's value in the same lane and, one lane over, a
+// rotate shuffle (wall neighbours are the zero ghost value). LDSY (FAST and
+// ny % 4 == 0): the four waves of a block own rows j0..j0+3 of one plane and
+// share them, with rows j0-1 and j0+4, through LDS for the y neighbours.
+template <bool VAR, int VPL, bool FAST, bool LDSY>
 __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double* __restrict__ d,
                                                   double* __restrict__ Ad, double* __restrict__ part0) {
+  constexpr int NV = VPL > 0 ? VPL : 1;
+  __shared__ double ysm[LDSY ? 6 * 64 * NV : 1];
   ROW_SETUP
-  const int nx = g.n[0];
+  const int nx = g.n[0], nh = nx >> 1;
   const size_t ypr = idx(g, 0, W(g, 1, j + 1), k), ymr = idx(g, 0, W(g, 1, j - 1), k);
   const size_t zpr = idx(g, 0, j, W(g, 2, k + 1)), zmr = idx(g, 0, j, W(g, 2, k - 1));
   double acc = 0.0;
-  if (VPL == 0) {
-    for (int i = 1 + lane; i <= nx; i += 64) {
+  if constexpr (!FAST) {
+    for (int q = 1 + lane; q <= nx; q += 64) {
+      const int i = q <= nh ? 2 * q : 2 * (q - nh) - 1;
       Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
-      const int p = spos(i, nx);
-      double xc = d[rb + p];
+      double xc = d[rb + q];
       double xp = d[rb + spos(W(g, 0, i + 1), nx)], xm = d[rb + spos(W(g, 0, i - 1), nx)];
-      double yp = d[ypr + p], ym = d[ymr + p], zp = d[zpr + p], zm = d[zmr + p];
+      double yp = d[ypr + q], ym = d[ymr + q], zp = d[zpr + q], zm = d[zmr + q];
       double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
       double v = c.diag * xc - s;
-      Ad[rb + p] = v;
+      Ad[rb + q] = v;
       acc += xc * v;
     }
   } else {
-    // all loads of the lane's VPL cells first, then the arithmetic (same order)
-    constexpr int NV = VPL > 0 ? VPL : 1;
-    double xc[NV], xp[NV], xm[NV], yp[NV], ym[NV], zp[NV], zm[NV];
+    constexpr int HV = NV / 2;  // lane slots per half row
+    double xc[NV], yp[NV], ym[NV], zp[NV], zm[NV];
+    if constexpr (LDSY) {
+      const int w = (int)(threadIdx.x >> 6);
+      double hv[NV];
+      const size_t hr = w == 0 ? ymr : ypr;  // wave 0: row j0-1, wave 3: row j0+4
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-      const int i = 1 + lane + 64 * m;
-      if (i <= nx) {
-        const int p = spos(i, nx);
-        xc[m] = d[rb + p];
-        xp[m] = d[rb + spos(W(g, 0, i + 1), nx)];
-        xm[m] = d[rb + spos(W(g, 0, i - 1), nx)];
-        yp[m] = d[ypr + p];
-        ym[m] = d[ymr + p];
-        zp[m] = d[zpr + p];
-        zm[m] = d[zmr + p];
+      for (int m = 0; m < NV; m++) {
+        const int q = 1 + lane + 64 * m;
+        xc[m] = d[rb + q];
+        zp[m] = d[zpr + q];
+        zm[m] = d[zmr + q];
+        if (w == 0 || w == 3) hv[m] = d[hr + q];
+      }
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int o = lane + 64 * m;
+        ysm[(w + 1) * 64 * NV + o] = xc[m];
+        if (w == 0) ysm[o] = hv[m];
+        if (w == 3) ysm[5 * 64 * NV + o] = hv[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int o = lane + 64 * m;
+        ym[m] = ysm[w * 64 * NV + o];
+        yp[m] = ysm[(w + 2) * 64 * NV + o];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int q = 1 + lane + 64 * m;
+        xc[m] = d[rb + q];
+        yp[m] = d[ypr + q];
+        ym[m] = d[ymr + q];
+        zp[m] = d[zpr + q];
+        zm[m] = d[zmr + q];
       }
     }
+    const bool perx = g.per[0] != 0;
+    const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
 #pragma unroll
     for (int m = 0; m < NV; m++) {
-      const int i = 1 + lane + 64 * m;
-      if (i <= nx) {
-        Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
-        double s = c.a[0] * xp[m] + c.a[1] * xm[m] + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] +
-                   c.a[5] * zm[m];
-        double v = c.diag * xc[m] - s;
-        Ad[rb + spos(i, nx)] = v;
-        acc += xc[m] * v;
+      const int q = 1 + lane + 64 * m;
+      const int i = m < HV ? 2 * q : 2 * (q - nh) - 1;
+      double xp, xm;
+      if (m < HV) {
+        // even i = 2q: i-1 is the other half's q (this lane), i+1 its q+1
+        // (next lane; lane 63 takes lane 0 of the next slot, i = nx wraps to i = 1)
+        const double src = lane == 0 ? xc[m + HV + 1 < NV ? m + HV + 1 : HV] : xc[m + HV];
+        xm = xc[m + HV];
+        xp = __shfl(src, up);
+        if (!perx && i == nx) xp = 0.0;
+      } else {
+        // odd i: i+1 is the other half's slot (this lane), i-1 one lane down
+        // (lane 0 takes lane 63 of the previous slot, i = 1 wraps to i = nx)
+        const double src = lane == 63 ? xc[m - HV - 1 >= 0 ? m - HV - 1 : HV - 1] : xc[m - HV];
+        xp = xc[m - HV];
+        xm = __shfl(src, dn);
+        if (!perx && i == 1) xm = 0.0;
       }
+      Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+      double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
+      double v = c.diag * xc[m] - s;
+      Ad[rb + q] = v;
+      acc += xc[m] * v;
     }
   }
   acc = wave_tree_sum(acc);
@@ -966,12 +1015,30 @@ void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, doubl
   hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0);
 }
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st) {
-  if (bf) {
-    SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_spmv_dot<true, VPL>), rows_grid(g), ROWB, 0, st, g, *bf, d, Ad, p0))
-  } else {
-    SMAC_VPL_DISPATCH(g.n[0],
-                      hipLaunchKernelGGL((k_spmv_dot<false, VPL>), rows_grid(g), ROWB, 0, st, g, BetaF{}, d, Ad, p0))
+  const int vpl = row_vpl(g.n[0]);
+  const bool fast = vpl >= 2 && g.n[0] == 64 * vpl;
+  const bool ldsy = fast && g.n[1] % 4 == 0;
+#define SMAC_SPMV(V, P, F, L) \
+  hipLaunchKernelGGL((k_spmv_dot<V, P, F, L>), rows_grid(g), ROWB, 0, st, g, bf ? *bf : BetaF{}, d, Ad, p0)
+#define SMAC_SPMV_V(V)                                   \
+  if (!fast) SMAC_SPMV(V, 0, false, false);              \
+  else if (vpl == 2) {                                   \
+    if (ldsy) SMAC_SPMV(V, 2, true, true);               \
+    else SMAC_SPMV(V, 2, true, false);                   \
+  } else if (vpl == 4) {                                 \
+    if (ldsy) SMAC_SPMV(V, 4, true, true);               \
+    else SMAC_SPMV(V, 4, true, false);                   \
+  } else {                                               \
+    if (ldsy) SMAC_SPMV(V, 8, true, true);               \
+    else SMAC_SPMV(V, 8, true, false);                   \
   }
+  if (bf) {
+    SMAC_SPMV_V(true)
+  } else {
+    SMAC_SPMV_V(false)
+  }
+#undef SMAC_SPMV_V
+#undef SMAC_SPMV
 }
 template <typename F>
 void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,

@@ -62,10 +62,19 @@ __device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k)
 // xp, xm, yp, ym, zp, zm. zero_start: e is zero on entry, so e = f/diag.
 // VPL > 0: a lane owns VPL cells h = 1+lane+64m, all loaded before any is
 // computed (one memory round per wave instead of VPL); VPL = 0: any row length.
-template <typename T, int VPL, bool VB>
+// FAST (half row = 64 VPL): the lane loads the other half of its row once;
+// an active cell's two x neighbours are that half's value in the same lane
+// and, one lane over, a rotate shuffle (wall neighbours are the zero ghost
+// value). LDSY (FAST and ny % 4 == 0): the four waves of a block own rows
+// j0..j0+3 of one plane; the other half a wave loads is exactly the y
+// neighbour half of rows j0+w+-1, so it goes through LDS together with the
+// matching halves of rows j0-1 and j0+4.
+template <typename T, int VPL, bool VB, bool FAST, bool LDSY>
 __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
                                                  int zero_start) {
   using C = typename St<T>::C;
+  constexpr int NV = VPL > 0 ? VPL : 1;
+  __shared__ C ysm[LDSY ? 6 * 64 * NV : 1];
   LROW_SETUP
   const int nx = L.n[0], nE = nx >> 1;
   const int par = (col + j + k) & 1;  // parity of the active i
@@ -73,6 +82,89 @@ __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const 
   const int off = par ? nE : 0;
   const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
   const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+  if constexpr (FAST) {
+    // nx = 128 VPL: both halves hold 64 VPL cells
+    const int oo = par ? 0 : nE;  // the other half
+    C fv[NV], ov[NV], yp[NV], ym[NV], zp[NV], zm[NV];
+    if (zero_start) {
+#pragma unroll
+      for (int m = 0; m < NV; m++) fv[m] = St<T>::ld(f, rb + off + 1 + lane + 64 * m);
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int h = 1 + lane + 64 * m;
+        const int i = par ? 2 * h - 1 : 2 * h;
+        CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+        St<T>::st(e, rb + off + h, fv[m] / c.diag);
+      }
+      return;
+    }
+    if constexpr (LDSY) {
+      const int w = (int)(threadIdx.x >> 6);
+      C hv[NV];
+      const size_t hr = w == 0 ? ymr : ypr;  // wave 0: row j0-1, wave 3: row j0+4
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int h = 1 + lane + 64 * m;
+        fv[m] = St<T>::ld(f, rb + off + h);
+        ov[m] = St<T>::ld(e, rb + oo + h);
+        zp[m] = St<T>::ld(e, zpr + off + h);
+        zm[m] = St<T>::ld(e, zmr + off + h);
+        if (w == 0 || w == 3) hv[m] = St<T>::ld(e, hr + off + h);
+      }
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int o = lane + 64 * m;
+        ysm[(w + 1) * 64 * NV + o] = ov[m];
+        if (w == 0) ysm[o] = hv[m];
+        if (w == 3) ysm[5 * 64 * NV + o] = hv[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int o = lane + 64 * m;
+        ym[m] = ysm[w * 64 * NV + o];
+        yp[m] = ysm[(w + 2) * 64 * NV + o];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int h = 1 + lane + 64 * m;
+        fv[m] = St<T>::ld(f, rb + off + h);
+        ov[m] = St<T>::ld(e, rb + oo + h);
+        yp[m] = St<T>::ld(e, ypr + off + h);
+        ym[m] = St<T>::ld(e, ymr + off + h);
+        zp[m] = St<T>::ld(e, zpr + off + h);
+        zm[m] = St<T>::ld(e, zmr + off + h);
+      }
+    }
+    const bool perx = L.per[0] != 0;
+    const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+      const int h = 1 + lane + 64 * m;
+      const int i = par ? 2 * h - 1 : 2 * h;
+      C xp, xm;
+      if (!par) {
+        // even i = 2h: i-1 is the other half's slot h (this lane), i+1 slot
+        // h+1 (next lane; lane 63 takes lane 0 of the next slot, i = nx wraps to 1)
+        const C src = lane == 0 ? ov[m + 1 < NV ? m + 1 : 0] : ov[m];
+        xm = ov[m];
+        xp = __shfl(src, up);
+        if (!perx && i == nx) xp = (C)0;
+      } else {
+        // odd i = 2h-1: i+1 is slot h (this lane), i-1 slot h-1 (previous
+        // lane; lane 0 takes lane 63 of the previous slot, i = 1 wraps to nx)
+        const C src = lane == 63 ? ov[m >= 1 ? m - 1 : NV - 1] : ov[m];
+        xp = ov[m];
+        xm = __shfl(src, dn);
+        if (!perx && i == 1) xm = (C)0;
+      }
+      CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+      C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
+      St<T>::st(e, rb + off + h, (fv[m] + S) / c.diag);
+    }
+    return;
+  }
   if (VPL == 0) {
     for (int h = 1 + lane; h <= cnt; h += 64) {
       const int i = par ? 2 * h - 1 : 2 * h;
@@ -90,7 +182,6 @@ __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const 
     }
     return;
   }
-  constexpr int NV = VPL > 0 ? VPL : 1;
   C fv[NV], xp[NV], xm[NV], yp[NV], ym[NV], zp[NV], zm[NV];
 #pragma unroll
   for (int m = 0; m < NV; m++) {
@@ -416,13 +507,29 @@ static const dim3 ROWB(256), CELLB(64, 4, 1);
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
   const int half = L.n[0] - (L.n[0] >> 1);  // longest active half row
-  if (L.b[0]) {
-    SMAC_VPL_DISPATCH(half,
-                      hipLaunchKernelGGL((k_mg_rbgs<T, VPL, true>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start))
-  } else {
-    SMAC_VPL_DISPATCH(half,
-                      hipLaunchKernelGGL((k_mg_rbgs<T, VPL, false>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start))
+  const int vpl = row_vpl(half);
+  const bool fast = vpl >= 1 && L.n[0] == 128 * vpl;
+  const bool ldsy = fast && L.n[1] % 4 == 0;
+#define SMAC_RBGS(V, P, F, Y) \
+  hipLaunchKernelGGL((k_mg_rbgs<T, P, V, F, Y>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start)
+#define SMAC_RBGS_V(V)                                                \
+  if (fast) {                                                         \
+    switch (vpl) {                                                    \
+      case 1: if (ldsy) SMAC_RBGS(V, 1, true, true); else SMAC_RBGS(V, 1, true, false); break; \
+      case 2: if (ldsy) SMAC_RBGS(V, 2, true, true); else SMAC_RBGS(V, 2, true, false); break; \
+      case 4: if (ldsy) SMAC_RBGS(V, 4, true, true); else SMAC_RBGS(V, 4, true, false); break; \
+      default: if (ldsy) SMAC_RBGS(V, 8, true, true); else SMAC_RBGS(V, 8, true, false); break; \
+    }                                                                 \
+  } else {                                                            \
+    SMAC_VPL_DISPATCH(half, SMAC_RBGS(V, VPL, false, false))          \
   }
+  if (L.b[0]) {
+    SMAC_RBGS_V(true)
+  } else {
+    SMAC_RBGS_V(false)
+  }
+#undef SMAC_RBGS_V
+#undef SMAC_RBGS
 }
 
 template <typename T>

@@ -377,8 +377,11 @@ static double final_reduce(const double* part, size_t nrows) {
     return tree(lanes, 1024);
 }
 
-/* generic dot-like sum: sum_c x[c]*y[c] (y may be NULL: sum x; sq: x*x) */
-enum { RED_SUM = 0, RED_DOT = 1, RED_SQ = 2 };
+/* generic dot-like sum: sum_c x[c]*y[c] (y may be NULL: sum x; sq: x*x).
+ * RED_DOT_SPLIT: the dot in the order of a row held in the row-split layout
+ * (even i first, then odd i): lane l accumulates split positions
+ * q = 1+l, 1+l+64, ... (cell i = 2q for q <= nx/2, else 2(q - nx/2) - 1). */
+enum { RED_SUM = 0, RED_DOT = 1, RED_SQ = 2, RED_DOT_SPLIT = 3 };
 static double csum(const struct smac_ctx* c, int mode, const double* x, const double* y) {
     int nx = c->n[0], ny = c->n[1], nz = c->n[2];
     size_t nrows = (size_t)ny * (size_t)nz;
@@ -389,9 +392,11 @@ static double csum(const struct smac_ctx* c, int mode, const double* x, const do
         double lanes[64];
         for (int l = 0; l < 64; l++) {
             double acc = 0.0;
-            for (int i = 1 + l; i <= nx; i += 64) {
+            for (int q = 1 + l; q <= nx; q += 64) {
+                int i = q;
+                if (mode == RED_DOT_SPLIT) i = q <= nx / 2 ? 2 * q : 2 * (q - nx / 2) - 1;
                 size_t id = IDX(c, i, j, k);
-                double v = mode == RED_SUM ? x[id] : (mode == RED_DOT ? x[id] * y[id] : x[id] * x[id]);
+                double v = mode == RED_SUM ? x[id] : (mode == RED_SQ ? x[id] * x[id] : x[id] * y[id]);
                 acc += v;
             }
             lanes[l] = acc;
@@ -1313,7 +1318,7 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= ny; j++)
                 for (int i = 1; i <= nx; i++) c->Ad[IDX(c, i, j, k)] = apply_A(c, c->dvec, i, j, k);
-        double pAp = csum(c, RED_DOT, c->dvec, c->Ad);
+        double pAp = csum(c, RED_DOT_SPLIT, c->dvec, c->Ad);
         /* stationary: x += z, r -= A z (a fixed linear map of b for tol = 0) */
         double alpha = o.stationary ? 1.0 : rz / pAp;
 #pragma omp parallel for collapse(2) schedule(static)
