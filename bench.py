@@ -19,9 +19,10 @@ width of the V-cycle):
          B_it   = 104 + 17.43*W_s (+16 when the PCG is flexible)
   kernels (per launch, per cell of the level it runs on; W_z = width of z,
   8 in FP64, W_s in the mixed forms):
-         xpay_spmv 24 + W_z (d_new = z + beta d_old, Ad = A d_new, d_new.Ad),
-         spmv_dot 16, update 48 (+W_s: the mixed forms also write (T) r),
-         dot2 8 + W_z (+8 flexible), xpay 16 + W_z,
+         spmv_dot 16, update 24 (r, Ad; +W_s: the mixed forms also write
+         (T) r), dot2 8 + W_z (+8 flexible), xpay 32 + W_z (z, d, dp: the
+         dp += alpha d of the previous update rides on the direction update),
+         dp_flush 24 (that dp update after the last iteration),
          mg_rbgs_l0 1.5*W_s (a half sweep), mg_restrict_l0 2.125*W_s
          (residual + restriction), mg_prolong_l0 1.125*W_s (red cells),
          mg_fill_l0 W_s, predict 56, rhs 32, correct 56, add_dp 32
@@ -46,10 +47,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (flex: the mixed forms)
     "spmv_dot": lambda W, flex: 16.0,
-    "xpay_spmv": lambda W, flex: 24.0 + W,
-    "update": lambda W, flex: 48.0 + (W if flex else 0.0),
+    "update": lambda W, flex: 24.0 + (W if flex else 0.0),
     "dot2": lambda W, flex: 8.0 + W + (8.0 if flex else 0.0),
-    "xpay": lambda W, flex: 16.0 + W,
+    "xpay": lambda W, flex: 32.0 + W,
+    "dp_flush": lambda W, flex: 24.0,
     "mg_rbgs_l0": lambda W, flex: 1.5 * W,
     "mg_restrict_l0": lambda W, flex: 2.125 * W,
     "mg_prolong_l0": lambda W, flex: 1.125 * W,

@@ -1126,7 +1126,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   int iter = 0, conv = 0;
   double rr = bb, linf = 0.0;
   if (bb == 0.0) conv = 1;
-  bool have_update = false;
+  bool have_update = false, dp_pending = false;
   // stationary iteration: alpha = 1, beta = 0 (d = z) throughout
   const bool stat = o.stationary != 0;
   if (stat) {
@@ -1144,8 +1144,9 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       fa.sb = S_PAP;
       launch_finalize(fa, c->S, s);
     }
-    KT(c, "update", launch_update<T>(g, c->dp, c->dv, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
+    KT(c, "update", launch_update<T>(g, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
     have_update = true;
+    dp_pending = true;  // dp += alpha d waits for the next xpay (or the flush)
     if (refg && pure) {
       // solve_pureneumann_gmgpcg removes the mean of r after every update
       // (G_PCGSolver.cu:866); the check norms are taken after it
@@ -1206,9 +1207,12 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
         }
       }
     }
-    KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->S, S_BETA, s));
+    KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->dp, c->S, S_ALPHA, S_BETA, s));
+    dp_pending = false;
     if (!stat) launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }
+  // the last update's dp += alpha d (S_ALPHA still holds its alpha)
+  if (dp_pending) KT(c, "dp_flush", launch_dp_flush(g, c->dp, c->dv, c->S, S_ALPHA, s));
   HIPOK(hipGetLastError());
   if (st) {
     memset(st, 0, sizeof(*st));

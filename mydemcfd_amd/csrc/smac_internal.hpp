@@ -140,13 +140,15 @@ template <typename Z>
 void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st);
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st);
 template <typename F>
-void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,
-                   int slot_alpha, double* p0, double* p1, F* f0, hipStream_t st);
+void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
+                   double* p1, F* f0, hipStream_t st);
+void launch_dp_flush(const Geo& g, double* dp, const double* d, const double* S, int slot_alpha, hipStream_t st);
 template <typename Z>
 void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, int flexible, double* p0,
                  double* p1, hipStream_t st);
 template <typename Z>
-void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st);
+void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* S, int slot_alpha, int slot_beta,
+                 hipStream_t st);
 template <typename F>
 void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);

@@ -596,13 +596,14 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
 }
 
 
-// dp += alpha d (dp natural); r -= alpha Ad (k_update_p_r, G_PCGSolver.cu:66-85);
-// part0 <- sum r^2, part1 <- max |r|
+// r -= alpha Ad (the residual half of k_update_p_r, G_PCGSolver.cu:66-85);
+// part0 <- sum r^2, part1 <- max |r|. The dp += alpha d half is deferred to
+// the next k_xpay, which reads d anyway, or to k_dp_flush after the last
+// iteration (the same dp + alpha d per cell).
 // F: multigrid storage; when f0 is given it receives (F) r (the fine-level rhs
 // of the next V-cycle in the mixed forms)
 template <typename F, int VPL>
-__global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, const double* __restrict__ d,
-                                                double* __restrict__ r, const double* __restrict__ Ad,
+__global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ r, const double* __restrict__ Ad,
                                                 const double* __restrict__ S, int slot_alpha,
                                                 double* __restrict__ part0, double* __restrict__ part1,
                                                 F* __restrict__ f0) {
@@ -614,14 +615,12 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, 
   const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
   for (int t = 0; t < trips; t++) {
     // VPL > 0: one trip covering the row, loads first; VPL == 0: one cell per trip
-    double vdp[NV], vd[NV], vr[NV], vA[NV];
+    double vr[NV], vA[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++) {
       const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
       if (i <= nx) {
         const size_t o = rb + spos(i, nx);
-        vdp[m] = dp[rb + i];
-        vd[m] = d[o];
         vr[m] = r[o];
         vA[m] = Ad[o];
       }
@@ -631,7 +630,6 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, 
       const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
       if (i <= nx) {
         const size_t o = rb + spos(i, nx);
-        dp[rb + i] = vdp[m] + alpha * vd[m];
         double rn = vr[m] - alpha * vA[m];
         r[o] = rn;
         if (f0) St<F>::st(f0, o, (typename St<F>::C)rn);
@@ -647,6 +645,15 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ dp, 
     part0[row] = a0;
     part1[row] = mx;
   }
+}
+
+// dp += alpha d (dp natural, d split): the deferred half of k_update_p_r
+// after the last iteration
+__global__ __launch_bounds__(256) void k_dp_flush(Geo g, double* __restrict__ dp, const double* __restrict__ d,
+                                                  const double* __restrict__ S, int slot_alpha) {
+  ROW_SETUP
+  const double alpha = S[slot_alpha];
+  for (int i = 1 + lane; i <= g.n[0]; i += 64) dp[rb + i] = dp[rb + i] + alpha * d[rb + spos(i, g.n[0])];
 }
 
 // part0 <- sum r*z; part1 <- sum z*Ad (flexible only)
@@ -688,29 +695,38 @@ __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ 
   }
 }
 
-// d = z + beta d (k_update_dir, G_PCGSolver.cu:276-288)
+// dp += alpha d_old (natural dp; the deferred half of k_update_p_r), then
+// d = z + beta d_old (k_update_dir, G_PCGSolver.cu:276-288)
 template <typename Z, int VPL>
 __global__ __launch_bounds__(256) void k_xpay(Geo g, const Z* __restrict__ z, double* __restrict__ d,
-                                              const double* __restrict__ S, int slot_beta) {
+                                              double* __restrict__ dp, const double* __restrict__ S, int slot_alpha,
+                                              int slot_beta) {
   ROW_SETUP
-  const int nx = g.n[0];
+  const int nx = g.n[0], nh = nx >> 1;
+  const double alpha = S[slot_alpha];
   const double beta = S[slot_beta];
   constexpr int NV = VPL > 0 ? VPL : 1;
   const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
   for (int t = 0; t < trips; t++) {
-    double vz[NV], vd[NV];
+    double vz[NV], vd[NV], vdp[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++) {
       const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
       if (p <= nx) {
+        const int i = p <= nh ? 2 * p : 2 * (p - nh) - 1;
         vz[m] = (double)St<Z>::ld(z, rb + p);
         vd[m] = d[rb + p];
+        vdp[m] = dp[rb + i];
       }
     }
 #pragma unroll
     for (int m = 0; m < NV; m++) {
       const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
-      if (p <= nx) d[rb + p] = vz[m] + beta * vd[m];
+      if (p <= nx) {
+        const int i = p <= nh ? 2 * p : 2 * (p - nh) - 1;
+        dp[rb + i] = vdp[m] + alpha * vd[m];
+        d[rb + p] = vz[m] + beta * vd[m];
+      }
     }
   }
 }
@@ -1041,10 +1057,13 @@ void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad,
 #undef SMAC_SPMV
 }
 template <typename F>
-void launch_update(const Geo& g, double* dp, const double* d, double* r, const double* Ad, const double* S,
-                   int slot_alpha, double* p0, double* p1, F* f0, hipStream_t st) {
-  SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_update<F, VPL>), rows_grid(g), ROWB, 0, st, g, dp, d, r, Ad, S,
-                                               slot_alpha, p0, p1, f0))
+void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
+                   double* p1, F* f0, hipStream_t st) {
+  SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_update<F, VPL>), rows_grid(g), ROWB, 0, st, g, r, Ad, S, slot_alpha,
+                                               p0, p1, f0))
+}
+void launch_dp_flush(const Geo& g, double* dp, const double* d, const double* S, int slot_alpha, hipStream_t st) {
+  hipLaunchKernelGGL(k_dp_flush, rows_grid(g), ROWB, 0, st, g, dp, d, S, slot_alpha);
 }
 template <typename Z>
 void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, int flexible, double* p0,
@@ -1053,8 +1072,10 @@ void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, in
                     hipLaunchKernelGGL((k_dot2<Z, VPL>), rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1))
 }
 template <typename Z>
-void launch_xpay(const Geo& g, const Z* z, double* d, const double* S, int slot_beta, hipStream_t st) {
-  SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_xpay<Z, VPL>), rows_grid(g), ROWB, 0, st, g, z, d, S, slot_beta))
+void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* S, int slot_alpha, int slot_beta,
+                 hipStream_t st) {
+  SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_xpay<Z, VPL>), rows_grid(g), ROWB, 0, st, g, z, d, dp, S,
+                                               slot_alpha, slot_beta))
 }
 template <typename F>
 void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st) {
@@ -1064,11 +1085,11 @@ void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st) {
   template void launch_add_scalar<Z>(const Geo&, Z*, double, const double*, int, hipStream_t);            \
   template void launch_sum<Z>(const Geo&, const Z*, double*, hipStream_t);                                \
   template void launch_copy_dot<Z>(const Geo&, const Z*, double*, const double*, double*, hipStream_t);   \
-  template void launch_update<Z>(const Geo&, double*, const double*, double*, const double*, const double*, \
-                                 int, double*, double*, Z*, hipStream_t);                                 \
+  template void launch_update<Z>(const Geo&, double*, const double*, const double*, int, double*, double*, \
+                                 Z*, hipStream_t);                                                        \
   template void launch_dot2<Z>(const Geo&, const double*, const Z*, const double*, int, double*, double*,  \
                                hipStream_t);                                                              \
-  template void launch_xpay<Z>(const Geo&, const Z*, double*, const double*, int, hipStream_t);           \
+  template void launch_xpay<Z>(const Geo&, const Z*, double*, double*, const double*, int, int, hipStream_t); \
   template void launch_cvt<Z>(const Geo&, const double*, Z*, hipStream_t);
 SMAC_INST_Z(double)
 SMAC_INST_Z(float)

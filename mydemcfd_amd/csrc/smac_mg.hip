@@ -245,10 +245,16 @@ __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], doubl
 // RED: the pre-smoothing ended with a black half sweep, which leaves the black
 // equations satisfied, so the residual is taken as 0 on black cells and formed
 // on the red half row only (half the loads; n even on every restricted level).
-template <typename T, int VPL, bool VB, bool RED>
+// FAST (RED, half row = 64 VPL, ny % 4 == 0): the smoother's access pattern --
+// the lane loads the other half of its row once (x neighbours by a rotate
+// shuffle), and the four rows of a block share those halves through LDS as
+// each other's y neighbours (with rows j0-1 and j0+4); same arithmetic.
+template <typename T, int VPL, bool VB, bool RED, bool FAST>
 __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __restrict__ e, const T* __restrict__ f,
                                                      typename St<T>::C* __restrict__ tb) {
   using C = typename St<T>::C;
+  constexpr int NVF = VPL > 0 ? VPL : 1;
+  __shared__ C ysm[FAST ? 6 * 64 * NVF : 1];
   extern __shared__ unsigned char sm_raw[];
   C* sm = reinterpret_cast<C*>(sm_raw);
   const Lv& L = F;
@@ -269,6 +275,58 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
     const int par = (j + k) & 1;
     const int off = par ? nE : 0;
     const int cnt = RED ? nE : nx;
+    if constexpr (FAST) {
+      const int oo = par ? 0 : nE;
+      C ec[NV], ef[NV], ov[NV], yp[NV], ym[NV], zp[NV], zm[NV], hv[NV];
+      const size_t hr = w == 0 ? ymr : ypr;  // wave 0: row j0-1, wave 3: row j0+4
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int p = off + 1 + lane + 64 * m;
+        ef[m] = St<T>::ld(f, rb + p);
+        ec[m] = St<T>::ld(e, rb + p);
+        ov[m] = St<T>::ld(e, rb + oo + 1 + lane + 64 * m);
+        zp[m] = St<T>::ld(e, zpr + p);
+        zm[m] = St<T>::ld(e, zmr + p);
+        if (w == 0 || w == 3) hv[m] = St<T>::ld(e, hr + p);
+      }
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int o = lane + 64 * m;
+        ysm[(w + 1) * 64 * NV + o] = ov[m];
+        if (w == 0) ysm[o] = hv[m];
+        if (w == 3) ysm[5 * 64 * NV + o] = hv[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int o = lane + 64 * m;
+        ym[m] = ysm[w * 64 * NV + o];
+        yp[m] = ysm[(w + 2) * 64 * NV + o];
+      }
+      const bool perx = L.per[0] != 0;
+      const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        const int h = 1 + lane + 64 * m;
+        const int i = par ? 2 * h - 1 : 2 * h;
+        C xp, xm;
+        if (!par) {  // even i = 2h: i-1 in this lane's slot, i+1 one lane up
+          const C src = lane == 0 ? ov[m + 1 < NV ? m + 1 : 0] : ov[m];
+          xm = ov[m];
+          xp = __shfl(src, up);
+          if (!perx && i == nx) xp = (C)0;
+        } else {     // odd i = 2h-1: i+1 in this lane's slot, i-1 one lane down
+          const C src = lane == 63 ? ov[m >= 1 ? m - 1 : NV - 1] : ov[m];
+          xp = ov[m];
+          xm = __shfl(src, dn);
+          if (!perx && i == 1) xm = (C)0;
+        }
+        CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+        C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
+        buf[i] = ef[m] - (c.diag * ec[m] - S);
+        buf[par ? i + 1 : i - 1] = (C)0;  // the black neighbour of the same pair
+      }
+    } else {
     const int trips = VPL > 0 ? 1 : (cnt + 63) / 64;
     for (int t = 0; t < trips; t++) {
       // VPL > 0: the lane's cells h = 1+lane+64m, all loads before the arithmetic
@@ -301,6 +359,7 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
           if (RED) buf[par ? i + 1 : i - 1] = (C)0;  // the black neighbour of the same pair
         }
       }
+    }
     }
   }
   __syncthreads();
@@ -444,6 +503,88 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Lv Cl, Lv F, const T* __rest
   }
 }
 
+// The same one-colour prolongation for a block of four fine rows j0..j0+3
+// (j0 = 4b+1) of one plane, whose coarse cells lie in the coarse rows
+// 2b..2b+3 (periodic images or wall ghosts at the ends) of the two coarse
+// planes of k: those eight coarse rows are staged once in LDS in natural x
+// order (coalesced loads of the split rows), and each fine cell reads its
+// eight coarse values there. Needs col >= 0, ny % 4 == 0 and a half row of
+// 64 VPL cells; same arithmetic, in the same order, as k_mg_prolong.
+template <typename T, int VPL>
+__global__ __launch_bounds__(256) void k_mg_prolong_lds(Lv Cl, Lv F, const T* __restrict__ ec,
+                                                        T* __restrict__ ef, int col) {
+  using C = typename St<T>::C;
+  constexpr int NC = 64 * VPL;  // coarse row length
+  __shared__ C cs[8 * NC];
+  const Lv& L = F;
+  LROW_SETUP
+  const int nE = L.n[0] >> 1;
+  const int w = (int)(threadIdx.x >> 6);
+  const int par = (col + j + k) & 1;
+  const int off = par ? nE : 0;
+  int iy[2], iz[2];
+  double wy[2], wz[2];
+  pweights(Cl.per[1], Cl.n[1], j, iy, wy);
+  pweights(Cl.per[2], Cl.n[2], k, iz, wz);
+  const int j0 = j - w;
+  const int jb = (j0 - 1) >> 1;  // coarse row of LDS slot 0: 2b
+  // LDS slot of each coarse y index of this row: J - 2b, or, for the
+  // neighbour row, the unwrapped neighbour - 2b (its wrapped image is what
+  // slot 0 or 3 holds); a wall-clamped neighbour is J itself
+  int sy[2];
+  {
+    const int J = (j + 1) >> 1;
+    const int nb = (j & 1) ? J - 1 : J + 1;
+    sy[0] = J - jb;
+    sy[1] = (Cl.per[1] || (nb >= 1 && nb <= Cl.n[1])) ? nb - jb : sy[0];
+  }
+  // stage coarse rows (2b + s, iz[a]) -> slot a*4 + s, natural order; all
+  // loads (and the fine cells') are issued before the first LDS store
+  const int t = (int)threadIdx.x;
+  constexpr int PER = (NC + 255) / 256;
+  C stg[8][PER], fv[VPL];
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const size_t crow = lrow(Cl, LW(Cl, 1, jb + (r & 3)), iz[r >> 2]);
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int q = 1 + t + 256 * u;
+      if (q <= NC) stg[r][u] = St<T>::ld(ec, crow + q);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < VPL; m++) fv[m] = St<T>::ld(ef, rb + off + 1 + lane + 64 * m);
+#pragma unroll
+  for (int r = 0; r < 8; r++)
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int q = 1 + t + 256 * u;
+      if (q <= NC) cs[r * NC + (q <= NC / 2 ? 2 * q : 2 * (q - NC / 2) - 1) - 1] = stg[r][u];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < VPL; m++) {
+    const int h = 1 + lane + 64 * m;
+    const int pp = off + h;
+    const int i = pp <= nE ? 2 * pp : 2 * (pp - nE) - 1;
+    int ix[2];
+    double wx[2];
+    pweights(Cl.per[0], Cl.n[0], i, ix, wx);
+    C acc = 0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const C* rowp = cs + (a * 4 + sy[b]) * NC;
+        C w0 = (C)(wz[a] * wy[b] * wx[0]);
+        acc += w0 * rowp[ix[0] - 1];
+        C w1 = (C)(wz[a] * wy[b] * wx[1]);
+        acc += w1 * rowp[ix[1] - 1];
+      }
+    St<T>::st(ef, rb + pp, fv[m] + acc);
+  }
+}
+
 // coarsest level: e = Ainv f with the dense pseudo-inverse (rows in natural
 // x-fastest order), FP64 sequential accumulation per row. ainv is stored
 // column-major (entry (row, col) at col*n + row): the lanes of a wave read
@@ -537,26 +678,48 @@ void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typena
                         hipStream_t st) {
   size_t shm = 4 * (size_t)(F.n[0] + 2) * sizeof(typename St<T>::C);
   const int nx = F.n[0];
-  if (red) {
-    if (F.b[0]) {
-      SMAC_VPL_DISPATCH(nx / 2, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, true, true>), lrows_grid(F), ROWB, shm, st,
-                                                   F, C, e, f, tb))
-    } else {
-      SMAC_VPL_DISPATCH(nx / 2, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, false, true>), lrows_grid(F), ROWB, shm,
-                                                   st, F, C, e, f, tb))
-    }
-  } else if (F.b[0]) {
-    SMAC_VPL_DISPATCH(nx, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, true, false>), lrows_grid(F), ROWB, shm, st, F,
-                                             C, e, f, tb))
-  } else {
-    SMAC_VPL_DISPATCH(nx, hipLaunchKernelGGL((k_mg_resid_rx<T, VPL, false, false>), lrows_grid(F), ROWB, shm, st, F,
-                                             C, e, f, tb))
+  const int vpl = row_vpl(nx / 2);
+  const bool fast = red && vpl >= 1 && nx == 128 * vpl && F.n[1] % 4 == 0;
+#define SMAC_RX(V, P, R, FA) \
+  hipLaunchKernelGGL((k_mg_resid_rx<T, P, V, R, FA>), lrows_grid(F), ROWB, shm, st, F, C, e, f, tb)
+#define SMAC_RX_V(V)                                                         \
+  if (fast) {                                                                \
+    switch (vpl) {                                                           \
+      case 1: SMAC_RX(V, 1, true, true); break;                              \
+      case 2: SMAC_RX(V, 2, true, true); break;                              \
+      case 4: SMAC_RX(V, 4, true, true); break;                              \
+      default: SMAC_RX(V, 8, true, true); break;                             \
+    }                                                                        \
+  } else if (red) {                                                          \
+    SMAC_VPL_DISPATCH(nx / 2, SMAC_RX(V, VPL, true, false))                  \
+  } else {                                                                   \
+    SMAC_VPL_DISPATCH(nx, SMAC_RX(V, VPL, false, false))                     \
   }
+  if (F.b[0]) {
+    SMAC_RX_V(true)
+  } else {
+    SMAC_RX_V(false)
+  }
+#undef SMAC_RX_V
+#undef SMAC_RX
   hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
 }
 template <typename T>
 void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hipStream_t st) {
   const int cntmax = col < 0 ? F.n[0] : F.n[0] - (F.n[0] >> 1);
+  const int vpl = row_vpl(cntmax);
+  if (col >= 0 && vpl >= 1 && F.n[0] == 128 * vpl && F.n[1] % 4 == 0) {
+#define SMAC_PROLONG_LDS(P) \
+  hipLaunchKernelGGL((k_mg_prolong_lds<T, P>), lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col)
+    switch (vpl) {
+      case 1: SMAC_PROLONG_LDS(1); break;
+      case 2: SMAC_PROLONG_LDS(2); break;
+      case 4: SMAC_PROLONG_LDS(4); break;
+      default: SMAC_PROLONG_LDS(8); break;
+    }
+#undef SMAC_PROLONG_LDS
+    return;
+  }
   SMAC_VPL_DISPATCH(cntmax, hipLaunchKernelGGL((k_mg_prolong<T, VPL>), lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col))
 }
 template <typename T>
