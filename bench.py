@@ -45,9 +45,9 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (flex: the mixed forms)
+KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (< 8: a mixed form)
     "spmv_dot": lambda W, flex: 16.0,
-    "update": lambda W, flex: 24.0 + (W if flex else 0.0),
+    "update": lambda W, flex: 24.0 + (W if W < 8.0 else 0.0),
     "dot2": lambda W, flex: 8.0 + W + (8.0 if flex else 0.0),
     "xpay": lambda W, flex: 32.0 + W,
     "dp_flush": lambda W, flex: 24.0,
@@ -63,8 +63,8 @@ KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (
 
 FORMS = {
     "fp64": dict(precision="fp64", W=8.0, flexible=0),
-    "mixed_fp32": dict(precision="fp32", W=4.0, flexible=1),
-    "mixed_bf16": dict(precision="bf16", W=2.0, flexible=1),
+    "mixed_fp32": dict(precision="fp32", W=4.0, flexible=0),
+    "mixed_bf16": dict(precision="bf16", W=2.0, flexible=0),
 }
 
 

@@ -263,30 +263,39 @@ __device__ __forceinline__ double flux_minus(double m, const double c[5]) {
   return m * (uw + dir * 0.5 * sl);
 }
 
-template <int A>
-__device__ __forceinline__ double predict_face_pu(const Geo& g, const Vel& V, const double* __restrict__ p,
-                                                  const PerIdx& X, double dt) {
+// ld(c, di, dj, dk) = component c (0..2 = u, v, w; 3 = p) at (i+di, j+dj, k+dk)
+// of the face's cell, periodic images included. The offsets are compile-time
+// constants after unrolling, so every loader resolves to a register, an LDS
+// or a global read.
+template <int A, class LD>
+__device__ __forceinline__ double predict_face_gen(const Geo& g, const LD& ld, double dt) {
   constexpr int B1 = A == 0 ? 1 : 0;
   constexpr int B2 = A == 2 ? 1 : 2;
-  const double* ua = V.c[A];
-  const double* ub1 = V.c[B1];
-  const double* ub2 = V.c[B2];
+  // offset s along axis d
+  auto along = [&](int c, int d, int s) { return ld(c, d == 0 ? s : 0, d == 1 ? s : 0, d == 2 ? s : 0); };
+  // offset sb along B and sa along A
+  auto along2 = [&](int c, int B, int sb, int sa) {
+    int o[3] = {0, 0, 0};
+    o[B] += sb;
+    o[A] += sa;
+    return ld(c, o[0], o[1], o[2]);
+  };
   double ca[5], c1[5], c2[5];
 #pragma unroll
   for (int s = -2; s <= 2; s++) {
-    ca[s + 2] = ua[per_at<A>(X, A, s)];
-    c1[s + 2] = s == 0 ? 0.0 : ua[per_at<A>(X, B1, s)];
-    c2[s + 2] = s == 0 ? 0.0 : ua[per_at<A>(X, B2, s)];
+    ca[s + 2] = along(A, A, s);
+    c1[s + 2] = s == 0 ? 0.0 : along(A, B1, s);
+    c2[s + 2] = s == 0 ? 0.0 : along(A, B2, s);
   }
   const double v0 = ca[2];
   c1[2] = v0;
   c2[2] = v0;
   // cross components on the faces around the edges: (q+eB), (q+eB-eA), q, (q-eA)
-  const double b1pp = ub1[per_at2<A, B1>(X, 1, 0)], b1mp = ub1[per_at2<A, B1>(X, 1, -1)];
-  const double b1p0 = ub1[per_at2<A, B1>(X, 0, 0)], b1m0 = ub1[per_at2<A, B1>(X, 0, -1)];
-  const double b2pp = ub2[per_at2<A, B2>(X, 1, 0)], b2mp = ub2[per_at2<A, B2>(X, 1, -1)];
-  const double b2p0 = ub2[per_at2<A, B2>(X, 0, 0)], b2m0 = ub2[per_at2<A, B2>(X, 0, -1)];
-  const double pq = p[X.at(0, 0, 0)], pm = p[per_at<A>(X, A, -1)];
+  const double b1pp = along2(B1, B1, 1, 0), b1mp = along2(B1, B1, 1, -1);
+  const double b1p0 = along2(B1, B1, 0, 0), b1m0 = along2(B1, B1, 0, -1);
+  const double b2pp = along2(B2, B2, 1, 0), b2mp = along2(B2, B2, 1, -1);
+  const double b2p0 = along2(B2, B2, 0, 0), b2m0 = along2(B2, B2, 0, -1);
+  const double pq = ld(3, 0, 0, 0), pm = along(3, A, -1);
 
   const double rho0 = g.rho0, mu0 = g.mu0;
   const double vp1 = ca[3], vm1 = ca[1];
@@ -333,6 +342,153 @@ __device__ __forceinline__ double predict_face_pu(const Geo& g, const Vel& V, co
   const double rho_old_f = 0.5 * (rho0 + rho0);
   tmp -= (pq - pm) * g.inv_d[A];
   return beta * (v0 * rho_old_f) + dt * (beta * tmp + g.g[A]);
+}
+
+// global loads over the wrapped neighbourhood X
+template <int A>
+__device__ __forceinline__ double predict_face_pu(const Geo& g, const Vel& V, const double* __restrict__ p,
+                                                  const PerIdx& X, double dt) {
+  auto ld = [&](int c, int di, int dj, int dk) {
+    const size_t o = X.at(di, dj, dk);
+    return c == 3 ? p[o] : V.c[c][o];
+  };
+  return predict_face_gen<A>(g, ld, dt);
+}
+
+// The fused periodic predictor as a z-march over a 64 x 4 column tile: the
+// block keeps planes k-1 and k of u and v, k and k+1 of w and k of p (each
+// with a two-cell x/y halo) in LDS, and every thread its own column's u, v, w
+// at k-2..k+2 and p at k-1..k in registers. Every value a face needs is in
+// one of those (the cross components reach one plane up for w and one plane
+// down for u and v; all x/y offsets stay in plane k but those of w at k+1
+// and u, v at k-1, which are within one cell). A plane step writes the next
+// planes (loaded during the previous step) into the ring slots the step no
+// longer reads, then issues the loads for the following step before the
+// arithmetic. Needs nx % 64 == 0, ny % 4 == 0 and nz >= 3 (one periodic
+// wrap reaches k-2 and k+3); the per-face arithmetic is
+// predict_face_gen, so the results are those of k_predict_fused bit for bit.
+constexpr int PT_X = 64, PT_Y = 4, PT_W = PT_X + 4, PT_PL = (PT_X + 4) * (PT_Y + 4);
+__global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const double* __restrict__ p,
+                                                      double* __restrict__ us0, double* __restrict__ us1,
+                                                      double* __restrict__ us2, double dt, int KC) {
+  __shared__ double Ush[2 * PT_PL], Vsh[2 * PT_PL], Wsh[2 * PT_PL], Psh[PT_PL];
+  const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+  const int t = ty * PT_X + tx;
+  const int i0 = 1 + (int)blockIdx.x * PT_X, j0 = 1 + (int)blockIdx.y * PT_Y;
+  const int i = i0 + tx, j = j0 + ty;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const size_t px = (size_t)g.px, plane = (size_t)g.px * (size_t)g.py;
+  const int k0 = 1 + (int)blockIdx.z * KC;
+  const int k1 = min(k0 + KC - 1, nz);
+  // tile entries of this thread: e = t, t+256, t+512 (< PT_PL)
+  size_t toff[3];
+  bool tval[3];
+#pragma unroll
+  for (int u = 0; u < 3; u++) {
+    const int e = t + 256 * u;
+    tval[u] = e < PT_PL;
+    const int ey = e / PT_W - 2, ex = e % PT_W - 2;
+    toff[u] = tval[u] ? (size_t)wrapi(j0 + ey, ny) * px + (size_t)wrapi(i0 + ex, nx) : 0;
+  }
+  const size_t own_off = (size_t)j * px + (size_t)i;
+  const int own = (ty + 2) * PT_W + tx + 2;
+  auto kpl = [&](int kk) { return (size_t)wrapi(kk, nz) * plane; };
+  auto tile_ld = [&](const double* a, int kk, double out[3]) {
+    const size_t kb = kpl(kk);
+#pragma unroll
+    for (int u = 0; u < 3; u++)
+      if (tval[u]) out[u] = a[kb + toff[u]];
+  };
+  auto tile_st = [&](double* sl, const double in[3]) {
+#pragma unroll
+    for (int u = 0; u < 3; u++)
+      if (tval[u]) sl[t + 256 * u] = in[u];
+  };
+  const double* ua = V.c[0];
+  const double* va = V.c[1];
+  const double* wa = V.c[2];
+  // prologue: LDS u, v at k0-1, w at k0; own windows u, v, w at k0-2..k0+1, p at k0-1
+  double uz[5], vz[5], wz[5], pz[2];
+  {
+    double tu[3], tv[3], tw[3];
+    tile_ld(ua, k0 - 1, tu);
+    tile_ld(va, k0 - 1, tv);
+    tile_ld(wa, k0, tw);
+#pragma unroll
+    for (int m = 1; m < 5; m++) {
+      const size_t o = kpl(k0 - 3 + m) + own_off;
+      uz[m] = ua[o];
+      vz[m] = va[o];
+      wz[m] = wa[o];
+    }
+    pz[1] = p[kpl(k0 - 1) + own_off];
+    tile_st(Ush + ((k0 - 1) & 1) * PT_PL, tu);
+    tile_st(Vsh + ((k0 - 1) & 1) * PT_PL, tv);
+    tile_st(Wsh + (k0 & 1) * PT_PL, tw);
+  }
+  // prefetch of the first step: u, v, p at k0, w at k0+1; own u, v, w at k0+2
+  double nu[3], nv[3], nw[3], np[3], ou, ov, ow;
+  tile_ld(ua, k0, nu);
+  tile_ld(va, k0, nv);
+  tile_ld(wa, k0 + 1, nw);
+  tile_ld(p, k0, np);
+  {
+    const size_t o = kpl(k0 + 2) + own_off;
+    ou = ua[o];
+    ov = va[o];
+    ow = wa[o];
+  }
+  for (int k = k0; k <= k1; k++) {
+    __syncthreads();  // the previous step no longer reads the slots written here
+    double* U0 = Ush + (k & 1) * PT_PL;
+    double* V0 = Vsh + (k & 1) * PT_PL;
+    double* W1 = Wsh + ((k + 1) & 1) * PT_PL;
+    tile_st(U0, nu);
+    tile_st(V0, nv);
+    tile_st(W1, nw);
+    tile_st(Psh, np);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      uz[m] = uz[m + 1];
+      vz[m] = vz[m + 1];
+      wz[m] = wz[m + 1];
+    }
+    uz[4] = ou;
+    vz[4] = ov;
+    wz[4] = ow;
+    pz[0] = pz[1];
+    __syncthreads();
+    pz[1] = Psh[own];
+    if (k < k1) {
+      tile_ld(ua, k + 1, nu);
+      tile_ld(va, k + 1, nv);
+      tile_ld(wa, k + 2, nw);
+      tile_ld(p, k + 1, np);
+      const size_t o = kpl(k + 3) + own_off;
+      ou = ua[o];
+      ov = va[o];
+      ow = wa[o];
+    }
+    const double* U1 = Ush + ((k - 1) & 1) * PT_PL;
+    const double* V1 = Vsh + ((k - 1) & 1) * PT_PL;
+    const double* W0 = Wsh + (k & 1) * PT_PL;
+    auto ld = [&](int c, int di, int dj, int dk) -> double {
+      if (di == 0 && dj == 0) {
+        if (c == 0) return uz[dk + 2];
+        if (c == 1) return vz[dk + 2];
+        if (c == 2) return wz[dk + 2];
+        return pz[dk + 1];
+      }
+      const int o = own + dj * PT_W + di;
+      if (dk == 0) return c == 0 ? U0[o] : (c == 1 ? V0[o] : (c == 2 ? W0[o] : Psh[o]));
+      if (dk == 1) return W1[o];      // w one plane up
+      return c == 0 ? U1[o] : V1[o];  // u, v one plane down
+    };
+    const size_t id = kpl(k) + own_off;
+    us0[id] = predict_face_gen<0>(g, ld, dt);
+    us1[id] = predict_face_gen<1>(g, ld, dt);
+    us2[id] = predict_face_gen<2>(g, ld, dt);
+  }
 }
 
 // All three components in one pass (triply periodic, one phase): the thread of
@@ -992,6 +1148,11 @@ void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p,
     // two-phase grids: one kernel per component, properties from the arrays
     if (allper) predict_components<PeriodicBC, VarP, MfRho0>(g, *P, V, M0, p, us, dt, st);
     else predict_components<WallBC, VarP, MfRho0>(g, *P, V, M0, p, us, dt, st);
+  } else if (allper && g.n[0] % PT_X == 0 && g.n[1] % PT_Y == 0 && g.n[2] >= 3) {
+    const int KT = 32;  // z-planes per block column
+    dim3 grid((unsigned)(g.n[0] / PT_X), (unsigned)(g.n[1] / PT_Y), (unsigned)((g.n[2] + KT - 1) / KT));
+    hipLaunchKernelGGL(k_predict_tile, grid, dim3(PT_X, PT_Y, 1), 0, st, g, V, p, us[0], us[1], us[2], dt, KT);
+    return;
   } else if (allper) {
     dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4), (unsigned)((g.n[2] + KC - 1) / KC));
     hipLaunchKernelGGL(k_predict_fused<PeriodicBC>, grid, CELLB, 0, st, g, V, p, us[0], us[1], us[2], dt, KC);

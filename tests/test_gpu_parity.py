@@ -51,11 +51,16 @@ def test_predictor_cavity(hip, oracle_mod):
         assert rel_err(g.download(f), o.download(f)) == 0.0, f
 
 
-def test_predictor_periodic(hip, oracle_mod):
+@pytest.mark.parametrize("n", [(16, 8, 12), (128, 8, 40), (64, 12, 33), (64, 4, 4)])
+def test_predictor_periodic(hip, oracle_mod, n):
+    """Triply periodic predictor: the cell-parallel kernel (16x8x12) and the
+    z-marching tile kernel (nx % 64 == 0, ny % 4 == 0): two and three z
+    chunks of 32 planes (the last one partial, down to one plane), several
+    y tiles, and a box of four planes (the z wrap reaches k-2 and k+3)."""
     from mydemcfd_amd import cases
 
-    print("case: predictor periodic 16x8x12", flush=True)
-    desc = cases.periodic_desc((16, 8, 12))
+    print(f"case: predictor periodic {n}", flush=True)
+    desc = cases.periodic_desc(n)
     g, o = _pair(hip, oracle_mod, desc)
     rng = np.random.default_rng(2)
     fields = {f: rng.standard_normal(g.shape(f)) for f in ("u", "v", "w", "p")}
@@ -219,8 +224,9 @@ def test_cpp_facade_driver(hip, oracle_mod, tmp_path, mode):
 
 
 def test_periodic_mg_wide_rows(hip, oracle_mod):
-    """Anisotropic periodic box whose rows take the 2-values-per-lane path of
-    the tiled smoother (nx = 256 -> 128 cells per half row)."""
+    """Anisotropic periodic box with 256-cell rows: the shuffle/LDS paths of
+    the smoother, residual, SpMV and prolongation at four cells per lane, and
+    the tiled predictor."""
     from mydemcfd_amd import cases
 
     print("case: TG-like periodic 256x16x8 x3 mg", flush=True)
@@ -238,9 +244,8 @@ def test_periodic_mg_wide_rows(hip, oracle_mod):
 
 
 def test_cavity_mg_tiled(hip, oracle_mod):
-    """Walled box whose fine levels take the 2.5-D tiled smoother and the
-    tiled residual/restriction (nx >= 32, ny % 8 == 0) and whose coarse levels
-    take the row kernels."""
+    """Walled box of 64-cell rows: every level takes the load-per-neighbour
+    row kernels (the shuffle/LDS paths need rows of 128 cells or more)."""
     from mydemcfd_amd import cases
 
     print("case: cavity 64x32x24 x3 mg", flush=True)
@@ -251,6 +256,41 @@ def test_cavity_mg_tiled(hip, oracle_mod):
         sg = g.step(2e-3, precond="mg", tol=1e-8)
         so = o.step(2e-3, precond="mg", tol=1e-8)
         assert sg.iters == so.iters and sg.mg_levels == so.mg_levels, (s, sg.as_dict(), so.as_dict())
+    _compare(g, o, n)
+
+
+@pytest.mark.parametrize("n", [(128, 16, 12), (256, 8, 8)])
+def test_cavity_mg_wide_rows(hip, oracle_mod, n):
+    """Walled box whose fine level takes the shuffle/LDS paths of the
+    smoother, red residual, SpMV and prolongation (rows of 128 and 256 cells,
+    ny % 4 == 0): the wall ghosts of the x shuffles and the LDS halo rows."""
+    from mydemcfd_amd import cases
+
+    print(f"case: cavity {n} x3 mg", flush=True)
+    desc = cases.cavity_desc(n, re=100.0)
+    g, o = _pair(hip, oracle_mod, desc)
+    for s in range(3):
+        sg = g.step(2e-3, precond="mg", tol=1e-8)
+        so = o.step(2e-3, precond="mg", tol=1e-8)
+        assert sg.iters == so.iters and sg.mg_levels == so.mg_levels, (s, sg.as_dict(), so.as_dict())
+    _compare(g, o, n)
+
+
+def test_two_phase_dam_break_wide_rows(hip, oracle_mod):
+    """The two-phase dam break on 128-cell rows: the variable-coefficient
+    shuffle/LDS paths of the smoother, residual and SpMV."""
+    from mydemcfd_amd import cases
+
+    n = (128, 8, 12)
+    print(f"case: dam break {n} x2 mg", flush=True)
+    desc = cases.dam_break_desc(n, length=(4.0, 0.25, 0.375))
+    g, o = _pair(hip, oracle_mod, desc)
+    _upload_both(g, o, {"alpha": cases.dam_break_alpha(n, length=(4.0, 0.25, 0.375), x_front=1.0, height=0.2)})
+    kw = dict(precond="mg", tol=1e-8, max_iter=3000)
+    for s in range(2):
+        sg = g.step(1e-3, **kw)
+        so = o.step(1e-3, **kw)
+        assert sg.iters == so.iters and sg.converged == so.converged, (s, sg.as_dict(), so.as_dict())
     _compare(g, o, n)
 
 
