@@ -63,8 +63,8 @@ KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (
 
 FORMS = {
     "fp64": dict(precision="fp64", W=8.0, flexible=0),
-    "mixed_fp32": dict(precision="fp32", W=4.0, flexible=0),
-    "mixed_bf16": dict(precision="bf16", W=2.0, flexible=0),
+    "mixed_fp32": dict(precision="fp32", W=4.0, flexible=1),
+    "mixed_bf16": dict(precision="bf16", W=2.0, flexible=1),
 }
 
 
