@@ -14,6 +14,8 @@
 #include <map>
 #include <string>
 #include <utility>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/smac_c_api.h"
@@ -183,22 +185,43 @@ static void kend(smac_ctx* c, int rec) {
     }                                    \
   } while (0)
 
-static int balloc(void** p, size_t bytes) {
-  if (hipMalloc(p, bytes) != hipSuccess) {
-    *p = nullptr;
-    return SMAC_ERR_OOM;
+// Device arrays are allocated with their element 1 on a 128-byte boundary
+// (the pointer handed out is the allocation + 128 - esize bytes): with the
+// row pitch a multiple of 64 elements, the first interior cell of every row
+// of a box array then starts a cache line. The allocation is zeroed. dfree
+// releases by the handed-out pointer.
+static std::mutex g_alloc_mu;
+static std::unordered_map<void*, void*> g_alloc_raw;
+static int balloc(void** p, size_t bytes, size_t esize = 8) {
+  void* raw = nullptr;
+  const size_t off = 128 - esize;
+  *p = nullptr;
+  if (hipMalloc(&raw, bytes + 256) != hipSuccess) return SMAC_ERR_OOM;
+  if (hipMemset(raw, 0, bytes + 256) != hipSuccess) {
+    (void)hipFree(raw);
+    return SMAC_ERR_HIP;
   }
-  if (hipMemset(*p, 0, bytes) != hipSuccess) return SMAC_ERR_HIP;
+  *p = (char*)raw + off;
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  g_alloc_raw[*p] = raw;
   return SMAC_OK;
 }
-static int dalloc(double** p, size_t n) {
-  if (hipMalloc((void**)p, n * sizeof(double)) != hipSuccess) {
-    *p = nullptr;
-    return SMAC_ERR_OOM;
+static int dalloc(double** p, size_t n) { return balloc((void**)p, n * sizeof(double), sizeof(double)); }
+static void dfree(void* p) {
+  if (!p) return;
+  void* raw = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_alloc_raw.find(p);
+    if (it == g_alloc_raw.end()) return;
+    raw = it->second;
+    g_alloc_raw.erase(it);
   }
-  if (hipMemset(*p, 0, n * sizeof(double)) != hipSuccess) return SMAC_ERR_HIP;
-  return SMAC_OK;
+  (void)hipFree(raw);
 }
+// row pitch of box arrays: a multiple of 64 elements (128 bytes in BF16,
+// 256 in FP32, 512 in FP64)
+static int box_pitch(int n) { return ((n + 3 + 63) / 64) * 64; }
 
 static int pure_neumann(const smac_ctx* c) {
   for (int d = 0; d < 3; d++) {
@@ -296,7 +319,7 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
     c->g.inv_d[d] = 1. / c->d[d];
     c->g.inv_d2[d] = 1. / (c->d[d] * c->d[d]);
   }
-  c->g.px = n[0] + 3;
+  c->g.px = box_pitch(n[0]);
   c->g.py = n[1] + 3;
   for (int s = 0; s < 6; s++) {
     const smac_bc_entry& e = c->bc[c->side_bcid[s]];
@@ -313,7 +336,7 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
   else c->fl = smac_fluid_desc{1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   set_props(c);
 
-  c->tot = (size_t)(n[0] + 3) * (size_t)(n[1] + 3) * (size_t)(n[2] + 3);
+  c->tot = (size_t)c->g.px * (size_t)c->g.py * (size_t)(n[2] + 3);
   c->nrows = n[1] * n[2];
   int rc = SMAC_OK;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) rc = SMAC_ERR_HIP;
@@ -347,17 +370,17 @@ static void graph_free(smac_ctx* c) {
 static void mg_free(smac_ctx* c) {
   graph_free(c);
   for (auto b : c->mg_b)
-    if (b) (void)hipFree(b);
+    dfree(b);
   c->mg_b.clear();
   c->mg_ver = ~0ull;
   for (auto& L : c->lev) {
-    if (L.own_e) (void)hipFree(L.e);
-    if (L.own_f) (void)hipFree(L.f);
-    if (L.own_tb) (void)hipFree(L.tb);
+    if (L.own_e) dfree(L.e);
+    if (L.own_f) dfree(L.f);
+    if (L.own_tb) dfree(L.tb);
   }
   c->lev.clear();
   c->mg_prec = -1;
-  if (c->cinv) (void)hipFree(c->cinv);
+  dfree(c->cinv);
   c->cinv = nullptr;
   c->cinv_n = 0;
   c->mg_L = 0;
@@ -366,12 +389,11 @@ static void mg_free(smac_ctx* c) {
 static void rg_free(smac_ctx* c) {
   for (auto& L : c->rg) {
     double* arrs[] = {L.q, L.qt, L.rhs, L.b[0], L.b[1], L.b[2]};
-    for (auto a : arrs)
-      if (a) (void)hipFree(a);
+    for (auto a : arrs) dfree(a);
   }
   c->rg.clear();
   c->rg_req = 0;
-  if (c->rgt) (void)hipFree(c->rgt);
+  dfree(c->rgt);
   c->rgt = nullptr;
 }
 
@@ -385,14 +407,11 @@ extern "C" void smac_destroy(smac_ctx* c) {
                     c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
                     c->muf[0], c->muf[1],  c->muf[2], c->fstage, c->gradp[0], c->gradp[1], c->gradp[2],
                     c->alpha_new, c->vf[0], c->vf[1], c->vf[2], c->vfacc[0], c->vfacc[1], c->vfacc[2]};
-  for (auto a : arrs)
-    if (a) (void)hipFree(a);
-  for (int m = 0; m < 5; m++)
-    if (c->part[m]) (void)hipFree(c->part[m]);
-  for (int m = 0; m < 11; m++)
-    if (c->adj[m]) (void)hipFree(c->adj[m]);
-  if (c->vcoeff) (void)hipFree(c->vcoeff);
-  if (c->cellout) (void)hipFree(c->cellout);
+  for (auto a : arrs) dfree(a);
+  for (int m = 0; m < 5; m++) dfree(c->part[m]);
+  for (int m = 0; m < 11; m++) dfree(c->adj[m]);
+  dfree(c->vcoeff);
+  dfree(c->cellout);
   if (c->Sh) (void)hipHostFree(c->Sh);
   if (c->ev_ok)
     for (int e = 0; e < 8; e++) (void)hipEventDestroy(c->ev[e]);
@@ -763,12 +782,8 @@ static int mg_coeffs(smac_ctx* c) {
     int rc = dalloc(&c->cinv, (size_t)n * n);
     if (rc) return rc;
   }
-  // column-major on the device: the thread of row r reads entry (r, col) at
-  // col*n + r, so a wave's loads of one column are contiguous
-  std::vector<double> ht((size_t)n * n);
-  for (int r = 0; r < n; r++)
-    for (int col = 0; col < n; col++) ht[(size_t)col * n + r] = h[(size_t)r * n + col];
-  HIPOK(hipMemcpy(c->cinv, ht.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+  // row-major: the wave of row r reads the row's entries at consecutive addresses
+  HIPOK(hipMemcpy(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
   c->cinv_n = n;
   c->mg_ver = c->props_ver;
   return SMAC_OK;
@@ -793,7 +808,9 @@ static int mg_setup(smac_ctx* c, int L, int prec) {
       double h = c->d[d] * (double)(1 << l);
       V.lv.inv_d2[d] = 1.0 / (h * h);
     }
-    V.lv.px = V.lv.n[0] + 3;
+    // level 0 shares arrays with the fine grid (its pitch); coarser levels
+    // take the same 64-element pitch rule
+    V.lv.px = l == 0 ? c->g.px : box_pitch(V.lv.n[0]);
     V.lv.py = V.lv.n[1] + 3;
     V.lv.beta0 = c->g.beta0;
     size_t t = (size_t)V.lv.px * V.lv.py * (size_t)(V.lv.n[2] + 3);
@@ -804,14 +821,14 @@ static int mg_setup(smac_ctx* c, int L, int prec) {
       V.f = c->r;
       V.tb = c->res0;
     } else {
-      rc = balloc(&V.e, t * sb);
+      rc = balloc(&V.e, t * sb, sb);
       V.own_e = rc == SMAC_OK;
       if (!rc) {
-        rc = balloc(&V.f, t * sb);
+        rc = balloc(&V.f, t * sb, sb);
         V.own_f = rc == SMAC_OK;
       }
       if (!rc) {
-        rc = balloc(&V.tb, t * cb);
+        rc = balloc(&V.tb, t * cb, cb);
         V.own_tb = rc == SMAC_OK;
       }
     }
@@ -1327,7 +1344,7 @@ static int vof_alloc(smac_ctx* c) {
   if (!rc) rc = dalloc(&c->alpha_new, c->tot);
   for (int a = 0; a < 3 && !rc; a++) rc = dalloc(&c->vf[a], c->tot);
   for (int a = 0; a < 3 && !rc; a++) rc = dalloc(&c->vfacc[a], c->tot);
-  if (!rc) rc = balloc((void**)&c->vcoeff, c->tot);
+  if (!rc) rc = balloc((void**)&c->vcoeff, c->tot, 1);
   return rc;
 }
 
@@ -1391,7 +1408,7 @@ extern "C" int smac_cell_output(smac_ctx* c, float* h, size_t count) {
   const size_t n = (size_t)c->n[0] * (size_t)c->n[1] * (size_t)c->n[2];
   if (count != 5 * n) return SMAC_ERR_INVALID_ARG;
   if (!c->cellout) {
-    int rc = balloc((void**)&c->cellout, 5 * n * sizeof(float));
+    int rc = balloc((void**)&c->cellout, 5 * n * sizeof(float), sizeof(float));
     if (rc) return rc;
   }
   const double* u[3] = {c->u[0], c->u[1], c->u[2]};

@@ -852,13 +852,17 @@ __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ 
 }
 
 // dp += alpha d_old (natural dp; the deferred half of k_update_p_r), then
-// d = z + beta d_old (k_update_dir, G_PCGSolver.cu:276-288)
+// d = z + beta d_old (k_update_dir, G_PCGSolver.cu:276-288). Lanes run over
+// natural i, so the dp stores are contiguous and those of d and the loads of
+// z and d are two contiguous half-row runs per instruction (lanes over split
+// positions would store dp at stride two, which costs a second write-back of
+// every dp line).
 template <typename Z, int VPL>
 __global__ __launch_bounds__(256) void k_xpay(Geo g, const Z* __restrict__ z, double* __restrict__ d,
                                               double* __restrict__ dp, const double* __restrict__ S, int slot_alpha,
                                               int slot_beta) {
   ROW_SETUP
-  const int nx = g.n[0], nh = nx >> 1;
+  const int nx = g.n[0];
   const double alpha = S[slot_alpha];
   const double beta = S[slot_beta];
   constexpr int NV = VPL > 0 ? VPL : 1;
@@ -867,21 +871,21 @@ __global__ __launch_bounds__(256) void k_xpay(Geo g, const Z* __restrict__ z, do
     double vz[NV], vd[NV], vdp[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++) {
-      const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
-      if (p <= nx) {
-        const int i = p <= nh ? 2 * p : 2 * (p - nh) - 1;
-        vz[m] = (double)St<Z>::ld(z, rb + p);
-        vd[m] = d[rb + p];
+      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (i <= nx) {
+        const size_t o = rb + spos(i, nx);
+        vz[m] = (double)St<Z>::ld(z, o);
+        vd[m] = d[o];
         vdp[m] = dp[rb + i];
       }
     }
 #pragma unroll
     for (int m = 0; m < NV; m++) {
-      const int p = 1 + lane + 64 * (VPL > 0 ? m : t);
-      if (p <= nx) {
-        const int i = p <= nh ? 2 * p : 2 * (p - nh) - 1;
+      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
+      if (i <= nx) {
+        const size_t o = rb + spos(i, nx);
         dp[rb + i] = vdp[m] + alpha * vd[m];
-        d[rb + p] = vz[m] + beta * vd[m];
+        d[o] = vz[m] + beta * vd[m];
       }
     }
   }

@@ -585,49 +585,43 @@ __global__ __launch_bounds__(256) void k_mg_prolong_lds(Lv Cl, Lv F, const T* __
   }
 }
 
-// coarsest level: e = Ainv f with the dense pseudo-inverse (rows in natural
-// x-fastest order), FP64 sequential accumulation per row. ainv is stored
-// column-major (entry (row, col) at col*n + row): the lanes of a wave read
-// one column at consecutive addresses, and the f value is the same for all.
+// coarsest level: e = Ainv f with the dense pseudo-inverse (rows and columns
+// in natural x-fastest order, row-major). One wave per row: lane l
+// accumulates the columns l, l+64, ... in order (FP64), and the 64 lane sums
+// are combined by the canonical halving tree -- the oracle's order.
 template <typename T>
-__global__ __launch_bounds__(64) void k_mg_coarse(Lv Cl, const double* __restrict__ ainv, const T* __restrict__ f,
-                                                  T* __restrict__ e) {
+__global__ __launch_bounds__(256) void k_mg_coarse(Lv Cl, const double* __restrict__ ainv, const T* __restrict__ f,
+                                                   T* __restrict__ e) {
   const int n = Cl.n[0] * Cl.n[1] * Cl.n[2];
-  const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int lane = threadIdx.x & 63;
+  const int row = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (row >= n) return;
+  const double* a = ainv + (size_t)row * (size_t)n;
+  const int nxy = Cl.n[0] * Cl.n[1];
   double acc = 0.0;
-  int ci = 1, cj = 1, ck = 1;
-  constexpr int U = 16;  // columns whose loads are issued before their (sequential) adds
-  int col = 0;
-  for (; col + U <= n; col += U) {
-    double a[U], fv[U];
+  constexpr int U = 8;  // columns per lane whose loads are issued before their (sequential) adds
+  int col = lane;
+  for (; col + 64 * (U - 1) < n; col += 64 * U) {
+    double av[U], fv[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      a[u] = ainv[(size_t)(col + u) * (size_t)n + (size_t)row];
+      const int cc = col + 64 * u;
+      const int ci = cc % Cl.n[0] + 1, cj = (cc / Cl.n[0]) % Cl.n[1] + 1, ck = cc / nxy + 1;
+      av[u] = a[cc];
       fv[u] = (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
-      if (++ci > Cl.n[0]) {
-        ci = 1;
-        if (++cj > Cl.n[1]) {
-          cj = 1;
-          ++ck;
-        }
-      }
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) acc += a[u] * fv[u];
+    for (int u = 0; u < U; u++) acc += av[u] * fv[u];
   }
-  for (; col < n; col++) {
-    acc += ainv[(size_t)col * (size_t)n + (size_t)row] * (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
-    if (++ci > Cl.n[0]) {
-      ci = 1;
-      if (++cj > Cl.n[1]) {
-        cj = 1;
-        ++ck;
-      }
-    }
+  for (; col < n; col += 64) {
+    const int ci = col % Cl.n[0] + 1, cj = (col / Cl.n[0]) % Cl.n[1] + 1, ck = col / nxy + 1;
+    acc += a[col] * (double)St<T>::ld(f, lrow(Cl, cj, ck) + spos(ci, Cl.n[0]));
   }
-  int i = row % Cl.n[0] + 1, j = (row / Cl.n[0]) % Cl.n[1] + 1, k = row / (Cl.n[0] * Cl.n[1]) + 1;
-  St<T>::st(e, lrow(Cl, j, k) + spos(i, Cl.n[0]), (typename St<T>::C)acc);
+  acc = wave_tree_sum(acc);
+  if (lane == 0) {
+    const int i = row % Cl.n[0] + 1, j = (row / Cl.n[0]) % Cl.n[1] + 1, k = row / nxy + 1;
+    St<T>::st(e, lrow(Cl, j, k) + spos(i, Cl.n[0]), (typename St<T>::C)acc);
+  }
 }
 
 // x = 0 on interior cells of a level (a fill kernel, safe under graph capture)
@@ -725,7 +719,7 @@ void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hi
 template <typename T>
 void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st) {
   int n = C.n[0] * C.n[1] * C.n[2];
-  hipLaunchKernelGGL(k_mg_coarse<T>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, C, ainv, f, e);
+  hipLaunchKernelGGL(k_mg_coarse<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, C, ainv, f, e);
 }
 // coarse face coefficient of axis d (mg_coarsen_beta of the oracle): faces
 // off the level's face range are 0

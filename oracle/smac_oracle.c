@@ -899,13 +899,19 @@ static void coarse_solve(const struct smac_ctx* c, level* V) {
         for (int j = 1; j <= V->n[1]; j++)
             for (int i = 1; i <= V->n[0]; i++)
                 rv[((k - 1) * V->n[1] + (j - 1)) * V->n[0] + (i - 1)] = V->f[LIDX(V, i, j, k)];
+    /* row sum in the canonical order: lane l of 64 accumulates columns
+     * l, l+64, ... in order, then the halving tree */
 #pragma omp parallel for schedule(static)
     for (int row = 0; row < n; row++) {
-        double acc = 0.0;
+        double lanes[64];
         const double* a = c->cinv + (size_t)row * n;
-        for (int col = 0; col < n; col++) acc += a[col] * rv[col];
+        for (int l = 0; l < 64; l++) {
+            double acc = 0.0;
+            for (int col = l; col < n; col += 64) acc += a[col] * rv[col];
+            lanes[l] = acc;
+        }
         int i = row % V->n[0] + 1, j = (row / V->n[0]) % V->n[1] + 1, k = row / (V->n[0] * V->n[1]) + 1;
-        V->e[LIDX(V, i, j, k)] = acc;
+        V->e[LIDX(V, i, j, k)] = tree(lanes, 64);
     }
     free(rv);
 }
