@@ -72,7 +72,14 @@ enum smac_field {
     SMAC_FIELD_FLUXX = 13,   /* f_Fx_accum_ x-faces (VoF alpha flux of the last transport) */
     SMAC_FIELD_FLUXY = 14,   /* f_Fy_accum_ y-faces */
     SMAC_FIELD_FLUXZ = 15,   /* f_Fz_accum_ z-faces */
-    SMAC_FIELD_COUNT = 16
+    /* CFD-DEM (two-way) inputs, written by the coupling (CFD/memberList/gridMembers.def:144-152) */
+    SMAC_FIELD_EPS = 16,      /* void_fraction_        cells   (fluid volume fraction, t^{n+1}) */
+    SMAC_FIELD_EPS_OLD = 17,  /* void_fraction_old_    cells   (t^n) */
+    SMAC_FIELD_EPS_HALF = 18, /* void_fraction_half_   cells   (t^{n+1/2}, in the two-way mass flux) */
+    SMAC_FIELD_IMPX = 19,     /* f_coupling_impulse_x_ x-faces (particle momentum impulse over the step) */
+    SMAC_FIELD_IMPY = 20,     /* f_coupling_impulse_y_ y-faces */
+    SMAC_FIELD_IMPZ = 21,     /* f_coupling_impulse_z_ z-faces */
+    SMAC_FIELD_COUNT = 22
 };
 
 /* PCG_TYPES (CFD/Enums.h:28-33) plus the red-black Gauss-Seidel multigrid. */
@@ -213,10 +220,29 @@ int smac_compute_mass_flux(smac_ctx* ctx, double dt);
 /* G_SMACSolver::get_vof_vstar_rhouu_consistent + update_vstar_boundary
  * (CFD/G_SMACSolver.h:71,74; CFD/G_SMACSolver.cu:1044-1069). */
 int smac_predictor(smac_ctx* ctx, double dt);
+/* Two-way CFD-DEM variants (the coupled loop Coupling/main.cu:531-540,661-670).
+ * G_SMACSolver::compute_mass_flux_from_alpha_flux_two_way +
+ * get_vof_vstar_rhouu_consistent_two_way + update_vstar_boundary
+ * (CFD/G_SMACSolver_VoF.cu:2472-2557; CFD/G_SMACSolver.cu:1538-2203,1044-1052):
+ * the predictor with the stresses and the old pressure gradient weighted by the
+ * t^n volume fraction, the mass flux rho0*eps_half_f*u_f (+ the alpha flux term
+ * right after a VoF transport), and u* = (beta/eps_f)(eps^n_f rho^n_f u + dt*rhs
+ * + J/V) + dt*g with J the impulse fields (zero when not uploaded). Needs
+ * SMAC_FIELD_EPS, EPS_OLD and EPS_HALF uploaded (ghost cells included, as the
+ * coupling fills them); else SMAC_ERR_INVALID_ARG. */
+int smac_predictor_two_way(smac_ctx* ctx, double dt);
+/* G_CFDDEMCoupling::update_poisson_beta_two_way (Coupling/G_CFDDEMCoupling.cu:419-459):
+ * the Poisson face coefficients become eps_f/rho_f (eps_f the mean of the two cells'
+ * SMAC_FIELD_EPS) until the next smac_update_properties, which restores 1/rho_f as
+ * the reference's face-property update does. The correction keeps 1/rho_f. */
+int smac_update_poisson_beta_two_way(smac_ctx* ctx);
 /* G_SMACSolver::solve_poisson -> G_PCGSolver::solve (CFD/G_SMACSolver.cu:1499-1501,
  * CFD/pressure_solver/G_PCGSolver.cu:592-617): RHS assembly
  * (G_PressureSolverBase.cu:49-118), operator build (G_PCGSolver.cu:126-212) and the
- * preconditioned CG on A dp = b, dp starting from 0. */
+ * preconditioned CG on A dp = b, dp starting from 0. Once SMAC_FIELD_EPS and
+ * EPS_OLD are uploaded the RHS carries the volume fraction,
+ * b = -(1/dt) D(eps_f u*) - (eps - eps_old)/dt^2 (the reference's general form;
+ * without them eps = 1). */
 int smac_pressure_solve(smac_ctx* ctx, double dt, const smac_solver_opts* opts,
                         smac_solve_stats* stats);
 /* G_SMACSolver::correct_vof_velocity (CFD/G_SMACSolver.cu:1072-1142,1269-1275):

@@ -26,11 +26,14 @@ BC_NOSLIP, BC_SLIP, BC_INFLOW, BC_OUTFLOW, BC_OUTLET = range(5)
 XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX = range(6)
 
 (FIELD_U, FIELD_V, FIELD_W, FIELD_USTAR, FIELD_VSTAR, FIELD_WSTAR, FIELD_P, FIELD_DP, FIELD_RHS, FIELD_ALPHA,
- FIELD_GRADPX, FIELD_GRADPY, FIELD_GRADPZ, FIELD_FLUXX, FIELD_FLUXY, FIELD_FLUXZ) = range(16)
+ FIELD_GRADPX, FIELD_GRADPY, FIELD_GRADPZ, FIELD_FLUXX, FIELD_FLUXY, FIELD_FLUXZ,
+ FIELD_EPS, FIELD_EPS_OLD, FIELD_EPS_HALF, FIELD_IMPX, FIELD_IMPY, FIELD_IMPZ) = range(22)
 FIELD_NAMES = {"u": FIELD_U, "v": FIELD_V, "w": FIELD_W, "ustar": FIELD_USTAR, "vstar": FIELD_VSTAR,
                "wstar": FIELD_WSTAR, "p": FIELD_P, "dp": FIELD_DP, "rhs": FIELD_RHS, "alpha": FIELD_ALPHA,
                "gradpx": FIELD_GRADPX, "gradpy": FIELD_GRADPY, "gradpz": FIELD_GRADPZ,
-               "fluxx": FIELD_FLUXX, "fluxy": FIELD_FLUXY, "fluxz": FIELD_FLUXZ}
+               "fluxx": FIELD_FLUXX, "fluxy": FIELD_FLUXY, "fluxz": FIELD_FLUXZ,
+               "eps": FIELD_EPS, "eps_old": FIELD_EPS_OLD, "eps_half": FIELD_EPS_HALF,
+               "impx": FIELD_IMPX, "impy": FIELD_IMPY, "impz": FIELD_IMPZ}
 
 PRECOND_JACOBI, PRECOND_MG, PRECOND_REF_GMG = 0, 1, 2
 PREC_FP64, PREC_MIXED_FP32, PREC_MIXED_BF16 = 0, 1, 2
@@ -91,6 +94,8 @@ SIGNATURES = {
     "smac_clear_alpha_flux_accum": (C.c_int, [_P]),
     "smac_alpha_flux_split": (C.c_int, [_P, C.c_double, C.c_int32]),
     "smac_predictor": (C.c_int, [_P, C.c_double]),
+    "smac_predictor_two_way": (C.c_int, [_P, C.c_double]),
+    "smac_update_poisson_beta_two_way": (C.c_int, [_P]),
     "smac_pressure_solve": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "smac_correct": (C.c_int, [_P, C.c_double]),
     "smac_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
@@ -145,11 +150,11 @@ def field_shape(n, field: int):
     """(z, y, x) numpy shape of a field in the reference padded layout."""
     nx, ny, nz = n
     sh = [nx + 2, ny + 2, nz + 2]
-    if field in (FIELD_U, FIELD_USTAR, FIELD_GRADPX, FIELD_FLUXX):
+    if field in (FIELD_U, FIELD_USTAR, FIELD_GRADPX, FIELD_FLUXX, FIELD_IMPX):
         sh[0] += 1
-    elif field in (FIELD_V, FIELD_VSTAR, FIELD_GRADPY, FIELD_FLUXY):
+    elif field in (FIELD_V, FIELD_VSTAR, FIELD_GRADPY, FIELD_FLUXY, FIELD_IMPY):
         sh[1] += 1
-    elif field in (FIELD_W, FIELD_WSTAR, FIELD_GRADPZ, FIELD_FLUXZ):
+    elif field in (FIELD_W, FIELD_WSTAR, FIELD_GRADPZ, FIELD_FLUXZ, FIELD_IMPZ):
         sh[2] += 1
     return (sh[2], sh[1], sh[0])
 
@@ -232,6 +237,16 @@ class Context:
 
     def predictor(self, dt):
         self._check(self.lib.smac_predictor(self.h, dt), "smac_predictor")
+
+    def predictor_two_way(self, dt):
+        """G_SMACSolver::get_vof_vstar_rhouu_consistent_two_way (with its mass flux
+        and the u* boundary update); needs eps, eps_old and eps_half uploaded."""
+        self._check(self.lib.smac_predictor_two_way(self.h, dt), "smac_predictor_two_way")
+
+    def update_poisson_beta_two_way(self):
+        """G_CFDDEMCoupling::update_poisson_beta_two_way: Poisson coefficients
+        eps_f/rho_f until the next update_properties."""
+        self._check(self.lib.smac_update_poisson_beta_two_way(self.h), "smac_update_poisson_beta_two_way")
 
     def pressure_solve(self, dt, opts=None, **kw):
         o = opts if opts is not None else default_opts(self.lib, **kw)

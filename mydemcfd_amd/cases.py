@@ -240,3 +240,36 @@ def state_mask(n, periodic, field):
     m = np.zeros(shape, dtype=bool)
     m[tuple(sl)] = True
     return m
+
+
+def void_fraction(n, periodic, seed, lo=0.45, hi=0.95, kmax=2, modes=5):
+    """A smooth fluid volume fraction on cells in the reference layout ((z, y, x)
+    with ghosts), values in [lo, hi]: a sum of random cosine modes over the
+    interior, ghost cells the periodic image (periodic axes) or the nearest
+    interior cell (walls), as a coupling fills them."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    idx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    dims = (nz, ny, nx)
+    f = np.zeros(dims)
+    for _ in range(modes):
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        ph = rng.uniform(0.0, 2.0 * math.pi)
+        f += rng.standard_normal() * np.cos(sum(2.0 * math.pi * k[d] * idx[d] / dims[d] for d in range(3)) + ph)
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-30)
+    f = lo + (hi - lo) * f
+    out = np.zeros((nz + 2, ny + 2, nx + 2))
+    out[1:-1, 1:-1, 1:-1] = f
+    per = (periodic[2], periodic[1], periodic[0])  # (z, y, x) order
+    for ax in range(3):
+        src_lo = -2 if per[ax] else 1
+        src_hi = 1 if per[ax] else -2
+        sl_lo = [slice(None)] * 3
+        sl_hi = [slice(None)] * 3
+        s_lo = [slice(None)] * 3
+        s_hi = [slice(None)] * 3
+        sl_lo[ax], sl_hi[ax] = 0, -1
+        s_lo[ax], s_hi[ax] = src_lo, src_hi
+        out[tuple(sl_lo)] = out[tuple(s_lo)]
+        out[tuple(sl_hi)] = out[tuple(s_hi)]
+    return out

@@ -107,6 +107,18 @@ struct smac_ctx {
   unsigned char* vcoeff = nullptr;
   bool vof_armed = false;
   float* cellout = nullptr;  // device buffer of smac_cell_output (5 floats per cell)
+  // CFD-DEM (two-way) fields, allocated on upload: void_fraction_,
+  // void_fraction_old_, void_fraction_half_ (cells) and f_coupling_impulse_*
+  // (faces); bpois = the two-way Poisson face coefficients eps_f/rho_f, in use
+  // (bpois_on) from smac_update_poisson_beta_two_way until the next
+  // smac_update_properties (which restores 1/rho_f, as the reference's face
+  // property update rewrites f_b)
+  double* eps = nullptr;
+  double* eps_old = nullptr;
+  double* eps_half = nullptr;
+  double* imp[3] = {nullptr, nullptr, nullptr};
+  double* bpois[3] = {nullptr, nullptr, nullptr};
+  bool bpois_on = false;
   double* rgt = nullptr;  // fine ping-pong partner of z in the reference V-cycle
   bool prof = false;
   hipEvent_t ev[8];
@@ -269,6 +281,15 @@ static BetaF beta_of(const smac_ctx* c) {
   for (int d = 0; d < 3; d++) b.b[d] = c->beta[d];
   return b;
 }
+// Poisson operator coefficients: the two-way eps_f/rho_f when in use, else
+// the face 1/rho_f (variable) or beta0 (uniform)
+static bool pois_var(const smac_ctx* c) { return c->bpois_on || var_props(c); }
+static BetaF pois_of(const smac_ctx* c) {
+  if (!c->bpois_on) return beta_of(c);
+  BetaF b;
+  for (int d = 0; d < 3; d++) b.b[d] = c->bpois[d];
+  return b;
+}
 
 // ------------------------------------------------------------------ profiling
 static void prof_begin(smac_ctx* c, int slot) {
@@ -406,7 +427,9 @@ extern "C" void smac_destroy(smac_ctx* c) {
                     c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->Ad,     c->res0,
                     c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
                     c->muf[0], c->muf[1],  c->muf[2], c->fstage, c->gradp[0], c->gradp[1], c->gradp[2],
-                    c->alpha_new, c->vf[0], c->vf[1], c->vf[2], c->vfacc[0], c->vfacc[1], c->vfacc[2]};
+                    c->alpha_new, c->vf[0], c->vf[1], c->vf[2], c->vfacc[0], c->vfacc[1], c->vfacc[2],
+                    c->eps,    c->eps_old, c->eps_half, c->imp[0], c->imp[1], c->imp[2], c->bpois[0],
+                    c->bpois[1], c->bpois[2]};
   for (auto a : arrs) dfree(a);
   for (int m = 0; m < 5; m++) dfree(c->part[m]);
   for (int m = 0; m < 11; m++) dfree(c->adj[m]);
@@ -434,9 +457,9 @@ static int field_axis(int f) {
     case SMAC_FIELD_U: case SMAC_FIELD_USTAR: case SMAC_FIELD_GRADPX: return 0;
     case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: case SMAC_FIELD_GRADPY: return 1;
     case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: case SMAC_FIELD_GRADPZ: return 2;
-    case SMAC_FIELD_FLUXX: return 0;
-    case SMAC_FIELD_FLUXY: return 1;
-    case SMAC_FIELD_FLUXZ: return 2;
+    case SMAC_FIELD_FLUXX: case SMAC_FIELD_IMPX: return 0;
+    case SMAC_FIELD_FLUXY: case SMAC_FIELD_IMPY: return 1;
+    case SMAC_FIELD_FLUXZ: case SMAC_FIELD_IMPZ: return 2;
     default: return -1;
   }
 }
@@ -463,6 +486,12 @@ static double** field_slot(smac_ctx* c, int f) {
     case SMAC_FIELD_FLUXX: return &c->vfacc[0];
     case SMAC_FIELD_FLUXY: return &c->vfacc[1];
     case SMAC_FIELD_FLUXZ: return &c->vfacc[2];
+    case SMAC_FIELD_EPS: return &c->eps;
+    case SMAC_FIELD_EPS_OLD: return &c->eps_old;
+    case SMAC_FIELD_EPS_HALF: return &c->eps_half;
+    case SMAC_FIELD_IMPX: return &c->imp[0];
+    case SMAC_FIELD_IMPY: return &c->imp[1];
+    case SMAC_FIELD_IMPZ: return &c->imp[2];
     default: return nullptr;
   }
 }
@@ -583,6 +612,10 @@ extern "C" int smac_download_device(smac_ctx* c, int32_t f, double* ddst, size_t
 extern "C" int smac_update_properties(smac_ctx* c, int32_t initial) {
   if (!c) return SMAC_ERR_INVALID_ARG;
   set_props(c);
+  if (c->bpois_on) {
+    c->bpois_on = false;
+    c->props_ver++;
+  }
   if (!var_props(c)) {
     c->props_ready = true;
     return SMAC_OK;
@@ -635,6 +668,56 @@ extern "C" int smac_predictor(smac_ctx* c, double dt) {
   c->vof_armed = false;
   KT(c, "predict", launch_predict(c->g, var_props(c) ? &P : nullptr, V, c->p, c->us, dt, c->st, vof ? &M : nullptr));
   HIPOK(hipGetLastError());
+  return SMAC_OK;
+}
+
+// --- two-way (CFD-DEM) variants, Coupling/main.cu:531-540,661-670
+// G_SMACSolver::compute_mass_flux_from_alpha_flux_two_way +
+// get_vof_vstar_rhouu_consistent_two_way + update_vstar_boundary
+// (CFD/G_SMACSolver_VoF.cu:2472-2557; CFD/G_SMACSolver.cu:1538-2203,1044-1052)
+extern "C" int smac_predictor_two_way(smac_ctx* c, double dt) {
+  if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
+  if (!c->eps || !c->eps_old || !c->eps_half) return SMAC_ERR_INVALID_ARG;
+  int rc = ensure_props(c);
+  if (rc) return rc;
+  Vel V;
+  for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
+  const Props P = props_of(c);
+  const double drho = c->fl.rho1 - c->fl.rho0;
+  const double inv_dt = 1. / dt;
+  VofF M{};
+  const bool vof = c->vof_armed && var_props(c) && drho != 0.0;
+  for (int a = 0; a < 3; a++) {
+    M.F[a] = vof ? c->vfacc[a] : nullptr;
+    M.hbydt[a] = c->d[a] * inv_dt;
+  }
+  M.drho = drho;
+  M.eps_half = c->eps_half;
+  c->vof_armed = false;
+  EpsF E{};
+  E.eps = c->eps;
+  E.eps_old = c->eps_old;
+  for (int a = 0; a < 3; a++) E.imp[a] = c->imp[a];
+  E.inv_vol = c->g.inv_d[0] * c->g.inv_d[1] * c->g.inv_d[2];
+  KT(c, "predict_two_way",
+     launch_predict_two_way(c->g, var_props(c) ? &P : nullptr, V, c->p, c->us, dt, c->st, &M, E));
+  HIPOK(hipGetLastError());
+  return SMAC_OK;
+}
+
+// G_CFDDEMCoupling::update_poisson_beta_two_way (Coupling/G_CFDDEMCoupling.cu:419-459)
+extern "C" int smac_update_poisson_beta_two_way(smac_ctx* c) {
+  if (!c) return SMAC_ERR_INVALID_ARG;
+  if (!c->eps) return SMAC_ERR_INVALID_ARG;
+  int rc = ensure_props(c);
+  if (rc) return rc;
+  for (int a = 0; a < 3 && !rc; a++)
+    if (!c->bpois[a]) rc = dalloc(&c->bpois[a], c->tot);
+  if (rc) return rc;
+  KT(c, "pois_two_way", launch_pois_two_way(c->g, var_props(c) ? c->beta : nullptr, c->eps, c->bpois, c->st));
+  HIPOK(hipGetLastError());
+  c->bpois_on = true;
+  c->props_ver++;  // the multigrid rebuilds its coarse coefficients
   return SMAC_OK;
 }
 
@@ -790,9 +873,10 @@ static int mg_coeffs(smac_ctx* c) {
 }
 
 static int mg_setup(smac_ctx* c, int L, int prec) {
-  const bool var = var_props(c);
+  const bool var = pois_var(c);
+  const BetaF pb = pois_of(c);
   if (c->mg_L == L && c->mg_prec == prec && !c->lev.empty() && c->lev[0].lv.beta0 == c->g.beta0 &&
-      (c->lev[0].lv.b[0] != nullptr) == var) {
+      (c->lev[0].lv.b[0] != nullptr) == var && (!var || c->lev[0].lv.b[0] == pb.b[0])) {
     if (!var || c->mg_ver == c->props_ver) return SMAC_OK;
     return mg_coeffs(c);  // same hierarchy, new coefficients
   }
@@ -835,7 +919,7 @@ static int mg_setup(smac_ctx* c, int L, int prec) {
     if (var) {
       for (int d = 0; d < 3 && !rc; d++) {
         if (l == 0) {
-          V.lv.b[d] = c->beta[d];
+          V.lv.b[d] = pb.b[d];
         } else {
           double* b = nullptr;
           rc = dalloc(&b, t);
@@ -928,7 +1012,7 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
 // prolongs through levels_[-1], G_GMGSolver.cu:971).
 static int rg_setup(smac_ctx* c, int req) {
   if (req < 1 || req > 16) return SMAC_ERR_INVALID_ARG;
-  const uint64_t ver = var_props(c) ? c->props_ver : 0;
+  const uint64_t ver = pois_var(c) ? c->props_ver : 0;
   if (!c->rg.empty() && c->rg_req == req && c->rg_beta0 == c->g.beta0 && c->rg_ver == ver) return (int)c->rg.size();
   rg_free(c);
   std::vector<RgLv> lv;
@@ -972,7 +1056,7 @@ static int rg_setup(smac_ctx* c, int req) {
     return rc;
   }
   // face coefficients (create_coeffs); rebuilt when the grid's coefficients change
-  const BetaF gb = beta_of(c);
+  const BetaF gb = pois_of(c);
   for (size_t l = 0; l < c->rg.size(); l++)
     for (int d = 0; d < 3; d++)
       launch_rg_beta(c->g, ver ? &gb : nullptr, c->rg[l].lv, l ? c->rg[l - 1].lv : c->rg[l].lv,
@@ -989,8 +1073,8 @@ static void rg_vcycle(smac_ctx* c, const smac_solver_opts& o) {
   const Geo& g = c->g;
   hipStream_t s = c->st;
   const int nf = o.ref_sweeps, nc = o.ref_coarse_sweeps, L = (int)c->rg.size();
-  const BetaF gb = beta_of(c);
-  const BetaF* gbp = var_props(c) ? &gb : nullptr;
+  const BetaF gb = pois_of(c);
+  const BetaF* gbp = pois_var(c) ? &gb : nullptr;
   double* x = c->z;
   double* t = c->rgt;
   for (auto& V : c->rg) KT(c, "rg_levels", launch_rg_zero(V.q, V.tot, s));
@@ -1083,8 +1167,8 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   const bool refg = o.precond == SMAC_PRECOND_REF_GMG;  // FP64 only (checked by the caller)
   T* z = mg ? (T*)c->lev[0].e : (T*)c->z;
   T* f0 = (mg && (void*)c->lev[0].f != (void*)c->r) ? (T*)c->lev[0].f : nullptr;
-  const bool var = var_props(c);
-  const BetaF bf = beta_of(c);
+  const bool var = pois_var(c);
+  const BetaF bf = pois_of(c);
   const BetaF* bfp = var ? &bf : nullptr;
   auto precond = [&]() {
     if (refg) {
@@ -1101,6 +1185,8 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   double** P = c->part;
 
   if (rhs_given) launch_rowsum_nat(g, c->rhs, P[0], s);  // b supplied in c->rhs (adjoint solve)
+  else if (c->eps && c->eps_old)  // with the fluid volume fraction (CFD-DEM grids)
+    KT(c, "rhs", launch_rhs_eps(g, c->us, c->eps, c->eps_old, c->rhs, inv_dt, P[0], s));
   else KT(c, "rhs", launch_rhs(g, c->us, c->rhs, inv_dt, P[0], s));
   if (pure) {
     FinalizeArgs fa = fin(c);
@@ -1428,6 +1514,8 @@ extern "C" int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts*
   // the adjoint holds alpha fixed (mass flux rho0 u): a pending VoF flux is
   // outside it
   if (c->vof_armed) return SMAC_ERR_UNSUPPORTED;
+  // the adjoint differentiates the one-fluid step: no volume fraction terms
+  if (c->eps || c->bpois_on) return SMAC_ERR_UNSUPPORTED;
   int rc = ensure_props(c);
   if (rc) return rc;
   for (int m = 0; m < 11 && !rc; m++)

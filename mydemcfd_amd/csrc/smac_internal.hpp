@@ -118,11 +118,38 @@ enum {
 
 // alpha flux of the VoF transport for the predictor's mass flux:
 // mf = rho0 u_f + drho * F h/dt (hbydt = h/dt per axis)
+// Mass-flux context of the predictor: the accumulated alpha flux F (with
+// drho = rho1 - rho0 and h/dt per axis) and, for the two-way form, the fluid
+// volume fraction at t^{n+1/2} on cells (eps_half).
 struct VofF {
   const double* F[3];
   double drho;
   double hbydt[3];
+  const double* eps_half;
 };
+// CFD-DEM (two-way) fields of the predictor: fluid volume fraction at t^{n+1}
+// (eps) and t^n (eps_old) on cells, the particles' momentum impulse over the
+// step on faces (imp, nullptr = none) and 1/(dx dy dz). Volume fractions are
+// read at the raw (ghost-inclusive) cell indices the reference reads; the
+// caller fills the ghost cells.
+struct EpsF {
+  const double* eps;
+  const double* eps_old;
+  const double* imp[3];
+  double inv_vol;
+};
+// Two-way predictor: k_get_vof_vstar_rhouu_consistent_{x,y,z}_two_way with the
+// mass flux of k_compute_mass_flux_from_alpha_flux_two_way (M: the alpha flux
+// when a VoF transport preceded, else nullptr)
+void launch_predict_two_way(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
+                            hipStream_t st, const VofF* M, const EpsF& E);
+// Poisson rhs with the volume fraction: b = -(1/dt) D(eps_f u*) - (eps - eps_old)/dt^2
+void launch_rhs_eps(const Geo& g, double* const us[3], const double* eps, const double* eps_old, double* rhs,
+                    double inv_dt, double* part0, hipStream_t st);
+// Poisson face coefficients of a two-way grid: bp = eps_f * beta (beta: the
+// face 1/rho_f arrays, or nullptr for one phase: beta0 on interior faces, 0
+// elsewhere)
+void launch_pois_two_way(const Geo& g, double* const beta[3], const double* eps, double* bp[3], hipStream_t st);
 // P == nullptr: one-phase (uniform) properties from Geo. M == nullptr: mass
 // flux rho0 u_f; M requires the two-phase properties P
 void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,

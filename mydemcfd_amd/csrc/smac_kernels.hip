@@ -201,6 +201,142 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, Props P, Vel V, VofF M, 
   ustar[IX(g, q)] = predict_face<BC, PR, MF, A>(g, P, V, M, p, q, dt);
 }
 
+// ------------------------------------------------------- two-way (CFD-DEM)
+// Mass flux of k_compute_mass_flux_from_alpha_flux_two_way
+// (CFD/G_SMACSolver_VoF.cu:2472-2557): mf = rho0 (eps_f u_f) + (rho1-rho0) F h/dt
+// on the faces 1..N+1 of axis E and 0 outside them, eps_f the t^{n+1/2}
+// volume fraction of the face -- the mean of its two cells, the inner cell on
+// faces 1 and N+1 (d_get_void_fraction_half_*face, 1954-1967). F enters when
+// a VoF transport preceded (M.F[E] != nullptr).
+struct MfEps {
+  template <int E>
+  static __device__ __forceinline__ double mf(const Geo& g, const VofF& M, I3 r, double fv) {
+    const int m = r.v[E];
+    if (m < 1 || m > g.n[E] + 1) return 0.0;
+    double ef;
+    if (m == 1) {
+      ef = M.eps_half[IX(g, r)];
+    } else if (m == g.n[E] + 1) {
+      ef = M.eps_half[IX(g, sh(r, E, -1))];
+    } else {
+      ef = 0.5 * (M.eps_half[IX(g, sh(r, E, -1))] + M.eps_half[IX(g, r)]);
+    }
+    const double q = ef * fv;
+    if (!M.F[E]) return g.rho0 * q;
+    return g.rho0 * q + M.drho * (M.F[E][IX(g, wr(g, r))] * M.hbydt[E]);
+  }
+};
+
+// Cross viscous term of the two-way predictor: the stresses of visc_cross,
+// each weighted by the t^n volume fraction of its edge, the mean of the four
+// cells around the edge summed in memory (x-fastest) order
+// (CFD/G_SMACSolver.cu:1608-1623, 1826-1844, 2044-2062).
+template <class BC, class PR, int A, int B>
+__device__ __forceinline__ double visc_cross_tw(const Geo& g, const Props& P, const Vel& V, const EpsF& E, I3 q,
+                                                double v0) {
+  double vbm = BC::dir(g, V, A, B, q, -1);
+  double vbp = BC::dir(g, V, A, B, q, +1);
+  double wpp = BC::face(g, V, B, sh(q, B, +1));
+  double wmp = BC::face(g, V, B, sh(sh(q, B, +1), A, -1));
+  double wp0 = BC::face(g, V, B, q);
+  double wm0 = BC::face(g, V, B, sh(q, A, -1));
+  double mu_p = PR::mu_edge(g, P, B, wr(g, sh(q, B, +1)), wr(g, sh(sh(q, B, +1), A, -1)));
+  double t_p = mu_p * ((wpp - wmp) * g.inv_d[A] + (vbp - v0) * g.inv_d[B]);
+  double mu_m = PR::mu_edge(g, P, B, wr(g, q), wr(g, sh(q, A, -1)));
+  double t_m = mu_m * ((wp0 - wm0) * g.inv_d[A] + (v0 - vbm) * g.inv_d[B]);
+  const double* eo = E.eps_old;
+  const I3 qa = sh(q, A, -1);
+  double ep, em;
+  if (B > A) {
+    ep = 0.25 * (eo[IX(g, qa)] + eo[IX(g, q)] + eo[IX(g, sh(qa, B, +1))] + eo[IX(g, sh(q, B, +1))]);
+    em = 0.25 * (eo[IX(g, sh(qa, B, -1))] + eo[IX(g, sh(q, B, -1))] + eo[IX(g, qa)] + eo[IX(g, q)]);
+  } else {
+    ep = 0.25 * (eo[IX(g, qa)] + eo[IX(g, sh(qa, B, +1))] + eo[IX(g, q)] + eo[IX(g, sh(q, B, +1))]);
+    em = 0.25 * (eo[IX(g, sh(qa, B, -1))] + eo[IX(g, qa)] + eo[IX(g, sh(q, B, -1))] + eo[IX(g, q)]);
+  }
+  return (ep * t_p - em * t_m) * g.inv_d[B];
+}
+
+// One component of k_get_vof_vstar_rhouu_consistent_{x,y,z}_two_way
+// (CFD/G_SMACSolver.cu:1538-2194): the one-fluid predictor of predict_face
+// with the stresses weighted by the t^n volume fraction (cell values for the
+// own-direction stress, edge means for the cross stresses), the mass flux of
+// MfEps, the old pressure gradient weighted by the face mean of eps^n, and
+//   u* = (beta/eps^{n+1}_f) (eps^n_f rho^n_f u + dt tmp + J/V) + dt g
+// with J the particles' momentum impulse on the face. The IBM solid fraction
+// factor (1 - 0) of the reference is 1.
+template <class BC, class PR, class MF, int A>
+__device__ double predict_face_tw(const Geo& g, const Props& P, const Vel& V, const VofF& M, const EpsF& E,
+                                  const double* __restrict__ p, I3 q, double dt) {
+  constexpr int B1 = A == 0 ? 1 : 0;
+  constexpr int B2 = A == 2 ? 1 : 2;
+  double v0 = V.c[A][IX(g, q)];
+  double vp1 = BC::face(g, V, A, sh(q, A, +1));
+  double vm1 = BC::face(g, V, A, sh(q, A, -1));
+  const I3 cr = sh(q, A, -1);  // the minus cell at its raw index (volume fractions)
+  const I3 cm = wr(g, cr);
+  const double* eo = E.eps_old;
+
+  double tau_p = PR::mu(g, P, q) * (vp1 - v0);
+  double tau_m = PR::mu(g, P, cm) * (v0 - vm1);
+  double tmp = 2. * (eo[IX(g, q)] * tau_p - eo[IX(g, cr)] * tau_m) * g.inv_d2[A];
+  tmp += visc_cross_tw<BC, PR, A, B1>(g, P, V, E, q, v0);
+  tmp += visc_cross_tw<BC, PR, A, B2>(g, P, V, E, q, v0);
+  tmp -= conv_term<BC, MF, A, 0>(g, V, M, q, v0, vp1, vm1);
+  tmp -= conv_term<BC, MF, A, 1>(g, V, M, q, v0, vp1, vm1);
+  tmp -= conv_term<BC, MF, A, 2>(g, V, M, q, v0, vp1, vm1);
+
+  const double eo_f = 0.5 * (eo[IX(g, cr)] + eo[IX(g, q)]);
+  const double en_f = 0.5 * (E.eps[IX(g, cr)] + E.eps[IX(g, q)]);
+  tmp -= eo_f * (p[IX(g, q)] - p[IX(g, cm)]) * g.inv_d[A];
+  const double beta = PR::beta(g, P, A, q);
+  const double rho_old_f = PR::rho_old_f(g, P, q, cm);
+  const double jd = E.imp[A] ? E.imp[A][IX(g, q)] * E.inv_vol : 0.0;
+  const double inv_eps_rho = beta / en_f;
+  return inv_eps_rho * (eo_f * rho_old_f * v0 + dt * tmp + jd) + dt * g.g[A];
+}
+
+template <class BC, class PR, int A>
+__global__ __launch_bounds__(256) void k_predict_tw(Geo g, Props P, Vel V, VofF M, EpsF E,
+                                                    const double* __restrict__ p, double* __restrict__ ustar,
+                                                    double dt) {
+  int lo[3] = {1, 1, 1}, hi[3] = {g.n[0], g.n[1], g.n[2]};
+  lo[A] = g.per[A] ? 1 : 2;
+  int i = lo[0] + (int)(blockIdx.x * 64 + threadIdx.x);
+  int j = lo[1] + (int)(blockIdx.y * 4 + threadIdx.y);
+  int k = lo[2] + (int)blockIdx.z;
+  if (i > hi[0] || j > hi[1] || k > hi[2]) return;
+  I3 q = mk(i, j, k);
+  ustar[IX(g, q)] = predict_face_tw<BC, PR, MfEps, A>(g, P, V, M, E, p, q, dt);
+}
+
+// Poisson face coefficients of a two-way grid (k_update_poisson_beta_two_way,
+// Coupling/G_CFDDEMCoupling.cu:419-455): on the faces 1..N+1 of axis A with
+// the other indices in 1..N, bp = eps_f * beta_f, eps_f the mean of the two
+// cells' eps^{n+1}; other faces keep beta_f. beta_f = the face 1/rho_f array,
+// or (one phase) beta0 on interior faces and 0 on boundary and ghost faces.
+template <int A>
+__global__ __launch_bounds__(256) void k_pois_two_way(Geo g, const double* __restrict__ beta,
+                                                      const double* __restrict__ eps, double* __restrict__ bp) {
+  const int i = (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int k = (int)blockIdx.z;
+  if (i > g.n[0] + 2 || j > g.n[1] + 2) return;
+  const I3 q = mk(i, j, k);
+  const size_t id = IX(g, q);
+  const double b = beta ? beta[id] : (face_type(g, A, q) == F_INTERIOR ? g.beta0 : 0.0);
+  bool in = q.v[A] >= 1 && q.v[A] <= g.n[A] + 1;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    if (d != A) in = in && q.v[d] >= 1 && q.v[d] <= g.n[d];
+  if (in) {
+    const double epsf = 0.5 * (eps[IX(g, sh(q, A, -1))] + eps[id]);
+    bp[id] = epsf * b;
+  } else {
+    bp[id] = b;
+  }
+}
+
 // Periodic one-phase form of predict_face<PeriodicBC, UniP, A> with the whole
 // stencil loaded before any branch: the own component at q + s e_d for
 // s = -2..2 along each axis, the two cross components on the four faces around
@@ -569,6 +705,37 @@ __global__ __launch_bounds__(256) void k_rhs(Geo g, const double* __restrict__ u
                  (us2[zp + i] - us2[rb + i]) * g.inv_d[2];
     double b = -inv_dt * div;
     rhs[rb + i] = b;
+    acc += b;
+  }
+  acc = wave_tree_sum(acc);
+  if (lane == 0) part0[row] = acc;
+}
+
+// base_k_make_poisson_rhs with the volume fraction, no surface tension
+// (CFD/pressure_solver/G_PressureSolverBase.cu:49-118):
+// b = -(1/dt) sum_d (eps_{d+} u*_{d+} - eps_{d-} u*_{d-})/h_d - (eps - eps_old)/dt^2
+// with eps_{d+-} the face means of eps^{n+1} at the raw neighbour cells.
+// part0 <- row sums of b.
+__global__ __launch_bounds__(256) void k_rhs_eps(Geo g, const double* __restrict__ us0,
+                                                 const double* __restrict__ us1, const double* __restrict__ us2,
+                                                 const double* __restrict__ eps, const double* __restrict__ eps_old,
+                                                 double* __restrict__ rhs, double inv_dt, double* __restrict__ part0) {
+  ROW_SETUP
+  const size_t yp = idx(g, 0, W(g, 1, j + 1), k), zp = idx(g, 0, j, W(g, 2, k + 1));
+  const size_t ey = (size_t)g.px, ez = (size_t)g.px * (size_t)g.py;
+  double acc = 0.0;
+  for (int i = 1 + lane; i <= g.n[0]; i += 64) {
+    const int ip = W(g, 0, i + 1);
+    const size_t c = rb + i;
+    const double exp_ = 0.5 * (eps[c] + eps[c + 1]), exm = 0.5 * (eps[c - 1] + eps[c]);
+    const double eyp = 0.5 * (eps[c] + eps[c + ey]), eym = 0.5 * (eps[c - ey] + eps[c]);
+    const double ezp = 0.5 * (eps[c] + eps[c + ez]), ezm = 0.5 * (eps[c - ez] + eps[c]);
+    const double div_eps = (exp_ * us0[rb + ip] - exm * us0[c]) * g.inv_d[0] +
+                           (eyp * us1[yp + i] - eym * us1[c]) * g.inv_d[1] +
+                           (ezp * us2[zp + i] - ezm * us2[c]) * g.inv_d[2];
+    const double deps = eps[c] - eps_old[c];
+    const double b = -inv_dt * div_eps - deps * inv_dt * inv_dt;
+    rhs[c] = b;
     acc += b;
   }
   acc = wave_tree_sum(acc);
@@ -1172,6 +1339,48 @@ void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p,
   hipLaunchKernelGGL(k_star_bc, sgrid, CELLB, 0, st, g, us[0], us[1], us[2]);
 }
 
+void launch_predict_two_way(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
+                            hipStream_t st, const VofF* M, const EpsF& E) {
+  const bool allper = g.per[0] && g.per[1] && g.per[2];
+  VofF MM{};
+  if (M) MM = *M;
+  const Props PP = P ? *P : Props{};
+  for (int a = 0; a < 3; a++) {
+    int lo = g.per[a] ? 1 : 2;
+    int nf[3] = {g.n[0], g.n[1], g.n[2]};
+    nf[a] = g.n[a] - lo + 1;
+    if (nf[0] <= 0 || nf[1] <= 0 || nf[2] <= 0) continue;
+    dim3 cg((unsigned)((nf[0] + 63) / 64), (unsigned)((nf[1] + 3) / 4), (unsigned)nf[2]);
+#define SMAC_TW(BCT, PRT, AX) \
+  hipLaunchKernelGGL((k_predict_tw<BCT, PRT, AX>), cg, CELLB, 0, st, g, PP, V, MM, E, p, us[AX], dt)
+#define SMAC_TW_A(BCT, PRT)              \
+  if (a == 0) SMAC_TW(BCT, PRT, 0);      \
+  else if (a == 1) SMAC_TW(BCT, PRT, 1); \
+  else SMAC_TW(BCT, PRT, 2);
+    if (allper) {
+      if (P) { SMAC_TW_A(PeriodicBC, VarP) } else { SMAC_TW_A(PeriodicBC, UniP) }
+    } else {
+      if (P) { SMAC_TW_A(WallBC, VarP) } else { SMAC_TW_A(WallBC, UniP) }
+    }
+#undef SMAC_TW_A
+#undef SMAC_TW
+  }
+  if (allper) return;
+  int m = g.n[0] > g.n[1] ? g.n[0] : g.n[1];
+  m = m > g.n[2] ? m : g.n[2];
+  dim3 sgrid((unsigned)((m + 63) / 64), (unsigned)((m + 3) / 4), 6);
+  hipLaunchKernelGGL(k_star_bc, sgrid, CELLB, 0, st, g, us[0], us[1], us[2]);
+}
+void launch_rhs_eps(const Geo& g, double* const us[3], const double* eps, const double* eps_old, double* rhs,
+                    double inv_dt, double* part0, hipStream_t st) {
+  hipLaunchKernelGGL(k_rhs_eps, rows_grid(g), ROWB, 0, st, g, us[0], us[1], us[2], eps, eps_old, rhs, inv_dt, part0);
+}
+void launch_pois_two_way(const Geo& g, double* const beta[3], const double* eps, double* bp[3], hipStream_t st) {
+  dim3 fg((unsigned)((g.n[0] + 3 + 63) / 64), (unsigned)((g.n[1] + 3 + 3) / 4), (unsigned)(g.n[2] + 3));
+  hipLaunchKernelGGL(k_pois_two_way<0>, fg, CELLB, 0, st, g, beta ? beta[0] : nullptr, eps, bp[0]);
+  hipLaunchKernelGGL(k_pois_two_way<1>, fg, CELLB, 0, st, g, beta ? beta[1] : nullptr, eps, bp[1]);
+  hipLaunchKernelGGL(k_pois_two_way<2>, fg, CELLB, 0, st, g, beta ? beta[2] : nullptr, eps, bp[2]);
+}
 void launch_rhs(const Geo& g, double* const us[3], double* rhs, double inv_dt, double* part0, hipStream_t st) {
   hipLaunchKernelGGL(k_rhs, rows_grid(g), ROWB, 0, st, g, us[0], us[1], us[2], rhs, inv_dt, part0);
 }

@@ -74,6 +74,12 @@ struct smac_ctx {
     double *alpha_new, *vf[3], *vfacc[3];
     unsigned char* vcoeff;
     int vof_armed;
+    /* CFD-DEM (two-way) inputs (allocated on upload): void_fraction_,
+     * void_fraction_old_, void_fraction_half_ (cells), f_coupling_impulse_*
+     * (faces); bpois = the Poisson face coefficients eps_f/rho_f in use from
+     * smac_update_poisson_beta_two_way to the next smac_update_properties */
+    double *eps, *eps_old, *eps_half, *imp[3], *bpois[3];
+    int bpois_on;
     /* PCG */
     double *r, *z, *dvec, *Ad;
     double *Adiag, *Aoff[6];
@@ -195,17 +201,53 @@ static inline int sgn(double a) { return a < 0 ? -1 : 1; }
 /* ------------------------------------------------------------- predictor */
 /* One component of k_get_vof_vstar_rhouu_consistent_{x,y,z}
  * (CFD/G_SMACSolver.cu:139-392 / 441-662 / 763-1003) at interior a-face q. */
-static double predict_face(const struct smac_ctx* c, int a, ivec q, double dt) {
+/* Mass flux of k_compute_mass_flux_from_alpha_flux_two_way
+ * (CFD/G_SMACSolver_VoF.cu:2472-2557) at the raw a-face index r: 0 off the
+ * faces 1..N+1 of axis a; eps_f = the t^{n+1/2} volume fraction of the face,
+ * the mean of its two cells, the inner cell on faces 1 and N+1
+ * (d_get_void_fraction_half_*face, 1954-1967); the alpha-flux term after a
+ * VoF transport. */
+static double mf_two_way(const struct smac_ctx* c, int a, ivec r, int with_flux, double inv_dt) {
+    int m = r.v[a];
+    if (m < 1 || m > c->n[a] + 1) return 0.0;
+    double ef;
+    if (m == 1) ef = c->eps_half[IX(c, r)];
+    else if (m == c->n[a] + 1) ef = c->eps_half[IX(c, add(r, a, -1))];
+    else ef = 0.5 * (c->eps_half[IX(c, add(r, a, -1))] + c->eps_half[IX(c, r)]);
+    double q = ef * get_face(c, a, r);
+    if (!with_flux) return c->fl.rho0 * q;
+    double hbydt = c->d[a] * inv_dt;
+    return c->fl.rho0 * q + (c->fl.rho1 - c->fl.rho0) * (c->vfacc[a][IX(c, wr(c, r))] * hbydt);
+}
+/* mass flux seen by the predictor at a-face r: the precomputed mf (one-fluid
+ * form) or the two-way form (tw: 1 = without, 2 = with the alpha flux) */
+static inline double mf_at(const struct smac_ctx* c, int a, ivec r, int tw, double inv_dt) {
+    if (tw) return mf_two_way(c, a, r, tw == 2, inv_dt);
+    return c->mf[a][IX(c, wr(c, r))];
+}
+
+/* tw != 0: the two-way form k_get_vof_vstar_rhouu_consistent_{x,y,z}_two_way
+ * (CFD/G_SMACSolver.cu:1538-2194): stresses weighted by the t^n volume
+ * fraction (cells for the own direction, the mean of the four cells around an
+ * edge, in memory order, for a cross direction), mass flux mf_two_way, the old
+ * pressure gradient weighted by the face mean of eps^n, and
+ * u* = (beta/eps^{n+1}_f)(eps^n_f rho^n_f u + dt tmp + J/V) + dt g; volume
+ * fractions at raw cell indices (the caller fills the ghosts). */
+static double predict_face(const struct smac_ctx* c, int a, ivec q, double dt, int tw) {
     const double* inv_d = c->inv_d;
+    const double inv_dt = 1. / dt;
     double v0 = c->vel[a][IX(c, q)];
     double vp1 = get_face(c, a, add(q, a, +1));
     double vm1 = get_face(c, a, add(q, a, -1));
-    ivec cm = wr(c, add(q, a, -1)); /* cell on the minus side */
+    ivec cr = add(q, a, -1);  /* minus cell, raw index */
+    ivec cm = wr(c, cr);      /* cell on the minus side */
+    const double* eo = c->eps_old;
 
     /* viscous, own direction */
     double tau_p = c->mu[IX(c, q)] * (vp1 - v0);
     double tau_m = c->mu[IX(c, cm)] * (v0 - vm1);
-    double tmp = 2. * (tau_p - tau_m) * c->inv_d2[a];
+    double tmp = tw ? 2. * (eo[IX(c, q)] * tau_p - eo[IX(c, cr)] * tau_m) * c->inv_d2[a]
+                    : 2. * (tau_p - tau_m) * c->inv_d2[a];
 
     /* viscous, cross directions in increasing axis order */
     for (int b = 0; b < 3; b++) {
@@ -222,15 +264,28 @@ static double predict_face(const struct smac_ctx* c, int a, ivec q, double dt) {
         double t_p = mu_p * ((wpp - wmp) * inv_d[a] + (vbp - v0) * inv_d[b]);
         double mu_m = 0.5 * (c->muf[b][IX(c, f3)] + c->muf[b][IX(c, f4)]);
         double t_m = mu_m * ((wp0 - wm0) * inv_d[a] + (v0 - vbm) * inv_d[b]);
-        tmp += (t_p - t_m) * inv_d[b];
+        if (tw) {
+            ivec qa = add(q, a, -1);
+            double ep, em;
+            if (b > a) {
+                ep = 0.25 * (eo[IX(c, qa)] + eo[IX(c, q)] + eo[IX(c, add(qa, b, +1))] + eo[IX(c, add(q, b, +1))]);
+                em = 0.25 * (eo[IX(c, add(qa, b, -1))] + eo[IX(c, add(q, b, -1))] + eo[IX(c, qa)] + eo[IX(c, q)]);
+            } else {
+                ep = 0.25 * (eo[IX(c, qa)] + eo[IX(c, add(qa, b, +1))] + eo[IX(c, q)] + eo[IX(c, add(q, b, +1))]);
+                em = 0.25 * (eo[IX(c, add(qa, b, -1))] + eo[IX(c, qa)] + eo[IX(c, add(q, b, -1))] + eo[IX(c, q)]);
+            }
+            tmp += (ep * t_p - em * t_m) * inv_d[b];
+        } else {
+            tmp += (t_p - t_m) * inv_d[b];
+        }
     }
 
     /* convection, MUSCL-minmod, directions x, y, z */
     for (int e = 0; e < 3; e++) {
         double mp, mm, up_p, up_m, Mp, Mm;
         if (e == a) {
-            mp = 0.5 * (c->mf[a][IX(c, wr(c, q))] + c->mf[a][IX(c, wr(c, add(q, a, +1)))]);
-            mm = 0.5 * (c->mf[a][IX(c, wr(c, add(q, a, -1)))] + c->mf[a][IX(c, wr(c, q))]);
+            mp = 0.5 * (mf_at(c, a, q, tw, inv_dt) + mf_at(c, a, add(q, a, +1), tw, inv_dt));
+            mm = 0.5 * (mf_at(c, a, add(q, a, -1), tw, inv_dt) + mf_at(c, a, q, tw, inv_dt));
             /* + face */
             ivec up = mp > 0. ? q : add(q, a, +1);
             if (face_type(c, a, wr(c, up)) != F_INTERIOR) {
@@ -260,9 +315,8 @@ static double predict_face(const struct smac_ctx* c, int a, ivec q, double dt) {
         } else {
             double vcm = get_dir(c, a, e, q, -1);
             double vcp = get_dir(c, a, e, q, +1);
-            mp = 0.5 * (c->mf[e][IX(c, wr(c, add(add(q, e, +1), a, -1)))] +
-                        c->mf[e][IX(c, wr(c, add(q, e, +1)))]);
-            mm = 0.5 * (c->mf[e][IX(c, wr(c, q))] + c->mf[e][IX(c, wr(c, add(q, a, -1)))]);
+            mp = 0.5 * (mf_at(c, e, add(add(q, e, +1), a, -1), tw, inv_dt) + mf_at(c, e, add(q, e, +1), tw, inv_dt));
+            mm = 0.5 * (mf_at(c, e, q, tw, inv_dt) + mf_at(c, e, add(q, a, -1), tw, inv_dt));
             ivec up = mp > 0. ? q : add(q, e, +1);
             ivec upw = wr(c, up);
             if (face_type(c, a, upw) != F_INTERIOR) {
@@ -297,8 +351,17 @@ static double predict_face(const struct smac_ctx* c, int a, ivec q, double dt) {
 
     double beta = c->beta[a][IX(c, q)];
     double rho_old_f = 0.5 * (c->rho_old[IX(c, q)] + c->rho_old[IX(c, cm)]);
-    tmp -= (c->p_[IX(c, q)] - c->p_[IX(c, cm)]) * inv_d[a];
     double g = a == 0 ? c->fl.gx : (a == 1 ? c->fl.gy : c->fl.gz);
+    if (tw) {
+        double eo_f = 0.5 * (eo[IX(c, cr)] + eo[IX(c, q)]);
+        double en_f = 0.5 * (c->eps[IX(c, cr)] + c->eps[IX(c, q)]);
+        tmp -= eo_f * (c->p_[IX(c, q)] - c->p_[IX(c, cm)]) * inv_d[a];
+        double inv_vol = c->inv_d[0] * c->inv_d[1] * c->inv_d[2];
+        double jd = c->imp[a] ? c->imp[a][IX(c, q)] * inv_vol : 0.0;
+        double inv_eps_rho = beta / en_f;
+        return inv_eps_rho * (eo_f * rho_old_f * v0 + dt * tmp + jd) + dt * g;
+    }
+    tmp -= (c->p_[IX(c, q)] - c->p_[IX(c, cm)]) * inv_d[a];
     return beta * (v0 * rho_old_f) + dt * (beta * tmp + g);
 }
 
@@ -319,11 +382,28 @@ static double get_star_boundary(const struct smac_ctx* c, int a, ivec q) {
     return 0.;
 }
 
+static void predictor_run(smac_ctx* c, double dt, int tw);
 int smac_predictor(smac_ctx* c, double dt) {
     if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
     if (!c->props_ready) smac_update_properties(c, 1);
     smac_compute_mass_flux(c, dt);
     c->vof_armed = 0; /* the accumulated alpha flux serves one predictor */
+    predictor_run(c, dt, 0);
+    return SMAC_OK;
+}
+/* two-way form (G_SMACSolver::compute_mass_flux_from_alpha_flux_two_way +
+ * get_vof_vstar_rhouu_consistent_two_way + update_vstar_boundary,
+ * CFD/G_SMACSolver_VoF.cu:2472-2557, CFD/G_SMACSolver.cu:1538-2203) */
+int smac_predictor_two_way(smac_ctx* c, double dt) {
+    if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
+    if (!c->eps || !c->eps_old || !c->eps_half) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    int tw = c->vof_armed ? 2 : 1;
+    c->vof_armed = 0;
+    predictor_run(c, dt, tw);
+    return SMAC_OK;
+}
+static void predictor_run(smac_ctx* c, double dt, int tw) {
     for (int a = 0; a < 3; a++) {
         int lo[3], hi[3];
         for (int d = 0; d < 3; d++) { lo[d] = 1; hi[d] = c->n[d]; }
@@ -334,7 +414,7 @@ int smac_predictor(smac_ctx* c, double dt) {
             for (int j = lo[1]; j <= hi[1]; j++)
                 for (int i = lo[0]; i <= hi[0]; i++) {
                     ivec q = iv(i, j, k);
-                    c->vst[a][IX(c, q)] = predict_face(c, a, q, dt);
+                    c->vst[a][IX(c, q)] = predict_face(c, a, q, dt, tw);
                 }
     }
     /* update_vstar_boundary: wall faces of each component */
@@ -351,7 +431,6 @@ int smac_predictor(smac_ctx* c, double dt) {
                 }
         }
     }
-    return SMAC_OK;
 }
 
 /* -------------------------------------------------- canonical reductions */
@@ -421,6 +500,7 @@ static double cmaxabs(const struct smac_ctx* c, const double* x) {
 /* ----------------------------------------------------------- properties */
 int smac_update_properties(smac_ctx* c, int32_t initial) {
     if (!c) return SMAC_ERR_INVALID_ARG;
+    if (c->bpois_on) { c->bpois_on = 0; c->mg_ready = 0; }
     const smac_fluid_desc* f = &c->fl;
     size_t tot = c->tot;
     if (!initial) { /* rho_old <- rho (the swap of update_properties_by_alpha) */
@@ -542,6 +622,61 @@ static void make_rhs(struct smac_ctx* c, double inv_dt) {
                 c->rhs[IX(c, q)] = -inv_dt * div;
             }
 }
+/* base_k_make_poisson_rhs with the volume fraction and no surface tension
+ * (CFD/pressure_solver/G_PressureSolverBase.cu:49-118):
+ * b = -(1/dt) sum_d (eps_{d+} u*_{d+} - eps_{d-} u*_{d-})/h_d - (eps - eps_old)/dt^2,
+ * eps_{d+-} the face means of eps^{n+1} with the raw neighbour cells. */
+static void make_rhs_eps(struct smac_ctx* c, double inv_dt) {
+    const double* e = c->eps;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= c->n[2]; k++)
+        for (int j = 1; j <= c->n[1]; j++)
+            for (int i = 1; i <= c->n[0]; i++) {
+                ivec q = iv(i, j, k);
+                size_t id = IX(c, q);
+                double exp_ = 0.5 * (e[id] + e[IX(c, add(q, 0, 1))]), exm = 0.5 * (e[IX(c, add(q, 0, -1))] + e[id]);
+                double eyp = 0.5 * (e[id] + e[IX(c, add(q, 1, 1))]), eym = 0.5 * (e[IX(c, add(q, 1, -1))] + e[id]);
+                double ezp = 0.5 * (e[id] + e[IX(c, add(q, 2, 1))]), ezm = 0.5 * (e[IX(c, add(q, 2, -1))] + e[id]);
+                double div = (exp_ * c->vst[0][IX(c, wr(c, add(q, 0, 1)))] - exm * c->vst[0][id]) * c->inv_d[0] +
+                             (eyp * c->vst[1][IX(c, wr(c, add(q, 1, 1)))] - eym * c->vst[1][id]) * c->inv_d[1] +
+                             (ezp * c->vst[2][IX(c, wr(c, add(q, 2, 1)))] - ezm * c->vst[2][id]) * c->inv_d[2];
+                double deps = e[id] - c->eps_old[id];
+                c->rhs[id] = -inv_dt * div - deps * inv_dt * inv_dt;
+            }
+}
+/* Poisson face coefficient array of axis d: eps_f/rho_f after
+ * smac_update_poisson_beta_two_way, else 1/rho_f */
+static inline const double* pois_b(const struct smac_ctx* c, int d) { return c->bpois_on ? c->bpois[d] : c->beta[d]; }
+/* G_CFDDEMCoupling::update_poisson_beta_two_way
+ * (Coupling/G_CFDDEMCoupling.cu:419-459): on the a-faces 1..N+1 with the
+ * other indices in 1..N, bpois = eps_f * (1/rho_f) with eps_f the mean of the
+ * two cells' eps^{n+1}; other faces keep 1/rho_f. */
+int smac_update_poisson_beta_two_way(smac_ctx* c) {
+    if (!c || !c->eps) return SMAC_ERR_INVALID_ARG;
+    if (!c->props_ready) smac_update_properties(c, 1);
+    for (int a = 0; a < 3; a++) {
+        if (!c->bpois[a]) c->bpois[a] = (double*)calloc(c->tot, sizeof(double));
+        if (!c->bpois[a]) return SMAC_ERR_OOM;
+        for (int k = 0; k < c->p[2]; k++)
+            for (int j = 0; j < c->p[1]; j++)
+                for (int i = 0; i < c->p[0]; i++) {
+                    ivec q = iv(i, j, k);
+                    size_t id = IX(c, q);
+                    int in = q.v[a] >= 1 && q.v[a] <= c->n[a] + 1;
+                    for (int d = 0; d < 3; d++)
+                        if (d != a) in = in && q.v[d] >= 1 && q.v[d] <= c->n[d];
+                    if (in) {
+                        double epsf = 0.5 * (c->eps[IX(c, add(q, a, -1))] + c->eps[id]);
+                        c->bpois[a][id] = epsf * c->beta[a][id];
+                    } else {
+                        c->bpois[a][id] = c->beta[a][id];
+                    }
+                }
+    }
+    c->bpois_on = 1;
+    c->mg_ready = 0;
+    return SMAC_OK;
+}
 
 /* k_build_vof_poisson_A (CFD/pressure_solver/G_PCGSolver.cu:126-212).
  * Off-diagonal order: xp, xm, yp, ym, zp, zm. */
@@ -555,8 +690,8 @@ static void build_A(struct smac_ctx* c) {
                 double a6[6];
                 for (int d = 0; d < 3; d++) {
                     ivec fp = wr(c, add(q, d, 1));
-                    a6[2 * d] = c->beta[d][IX(c, fp)] * c->inv_d2[d];
-                    a6[2 * d + 1] = c->beta[d][id] * c->inv_d2[d];
+                    a6[2 * d] = pois_b(c, d)[IX(c, fp)] * c->inv_d2[d];
+                    a6[2 * d + 1] = pois_b(c, d)[id] * c->inv_d2[d];
                 }
                 c->Adiag[id] = a6[0] + a6[1] + a6[2] + a6[3] + a6[4] + a6[5];
                 for (int d = 0; d < 3; d++) {
@@ -769,7 +904,7 @@ static int mg_setup(struct smac_ctx* c, int L) {
         V->f = (double*)calloc(V->tot, sizeof(double));
         V->res = (double*)calloc(V->tot, sizeof(double));
         if (l == 0) {
-            for (int d = 0; d < 3; d++) V->b[d] = c->beta[d];
+            for (int d = 0; d < 3; d++) V->b[d] = (double*)pois_b(c, d);
         } else {
             for (int d = 0; d < 3; d++) V->b[d] = (double*)calloc(V->tot, sizeof(double));
             mg_coarsen_beta(c, &c->lev[l - 1], V);
@@ -1029,7 +1164,7 @@ static double rg_fine_beta(const struct smac_ctx* c, int lv, int d, int q[3]) {
     int w[3] = {q[0], q[1], q[2]};
     if (c->per[d]) w[d] = wrap(w[d], n[d]);
     else if (w[d] > n[d] + 1) return 0.0;
-    if (lv < 0) return c->beta[d][IDX(c, w[0], w[1], w[2])];
+    if (lv < 0) return pois_b(c, d)[IDX(c, w[0], w[1], w[2])];
     return c->rg[lv].b[d][RIDX(&c->rg[lv], w[0], w[1], w[2])];
 }
 
@@ -1281,7 +1416,8 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
     int pure = pure_neumann(c);
     double inv_dt = 1. / dt;
 
-    make_rhs(c, inv_dt);
+    if (c->eps && c->eps_old) make_rhs_eps(c, inv_dt);
+    else make_rhs(c, inv_dt);
     build_A(c);
     int L = 1;
     if (o.precond == SMAC_PRECOND_MG) {
@@ -1866,9 +2002,9 @@ static int shape_of(const struct smac_ctx* c, int field, int sh[3]) {
         case SMAC_FIELD_U: case SMAC_FIELD_USTAR: case SMAC_FIELD_GRADPX: sh[0]++; return 0;
         case SMAC_FIELD_V: case SMAC_FIELD_VSTAR: case SMAC_FIELD_GRADPY: sh[1]++; return 1;
         case SMAC_FIELD_W: case SMAC_FIELD_WSTAR: case SMAC_FIELD_GRADPZ: sh[2]++; return 2;
-        case SMAC_FIELD_FLUXX: sh[0]++; return 0;
-        case SMAC_FIELD_FLUXY: sh[1]++; return 1;
-        case SMAC_FIELD_FLUXZ: sh[2]++; return 2;
+        case SMAC_FIELD_FLUXX: case SMAC_FIELD_IMPX: sh[0]++; return 0;
+        case SMAC_FIELD_FLUXY: case SMAC_FIELD_IMPY: sh[1]++; return 1;
+        case SMAC_FIELD_FLUXZ: case SMAC_FIELD_IMPZ: sh[2]++; return 2;
         default: return -1;
     }
 }
@@ -1890,6 +2026,12 @@ static double* field_ptr(struct smac_ctx* c, int field) {
         case SMAC_FIELD_FLUXX: return c->vfacc[0];
         case SMAC_FIELD_FLUXY: return c->vfacc[1];
         case SMAC_FIELD_FLUXZ: return c->vfacc[2];
+        case SMAC_FIELD_EPS: return c->eps;
+        case SMAC_FIELD_EPS_OLD: return c->eps_old;
+        case SMAC_FIELD_EPS_HALF: return c->eps_half;
+        case SMAC_FIELD_IMPX: return c->imp[0];
+        case SMAC_FIELD_IMPY: return c->imp[1];
+        case SMAC_FIELD_IMPZ: return c->imp[2];
         default: return NULL;
     }
 }
@@ -1907,7 +2049,14 @@ int smac_upload(smac_ctx* c, int32_t field, const double* h, size_t count) {
     if (count != smac_field_count(c, field)) return SMAC_ERR_INVALID_ARG;
     int sh[3];
     shape_of(c, field, sh);
-    if (field >= SMAC_FIELD_FLUXX && vof_alloc(c)) return SMAC_ERR_OOM;
+    if (field >= SMAC_FIELD_FLUXX && field <= SMAC_FIELD_FLUXZ && vof_alloc(c)) return SMAC_ERR_OOM;
+    if (field >= SMAC_FIELD_EPS) { /* CFD-DEM inputs, allocated on first upload */
+        double** slot = field == SMAC_FIELD_EPS ? &c->eps
+                      : field == SMAC_FIELD_EPS_OLD ? &c->eps_old
+                      : field == SMAC_FIELD_EPS_HALF ? &c->eps_half : &c->imp[field - SMAC_FIELD_IMPX];
+        if (!*slot) *slot = (double*)calloc(c->tot, sizeof(double));
+        if (!*slot) return SMAC_ERR_OOM;
+    }
     double* dst = field_ptr(c, field);
     for (int k = 0; k < sh[2]; k++)
         for (int j = 0; j < sh[1]; j++)
@@ -2017,7 +2166,8 @@ void smac_destroy(smac_ctx* c) {
                       c->mf[1], c->mf[2], c->rhof[0], c->rhof[1], c->rhof[2], c->r, c->z, c->dvec, c->Ad,
                       c->Adiag, c->Aoff[0], c->Aoff[1], c->Aoff[2], c->Aoff[3], c->Aoff[4], c->Aoff[5],
                       c->gradp[0], c->gradp[1], c->gradp[2], c->alpha_new, c->vf[0], c->vf[1], c->vf[2],
-                      c->vfacc[0], c->vfacc[1], c->vfacc[2]};
+                      c->vfacc[0], c->vfacc[1], c->vfacc[2], c->eps, c->eps_old, c->eps_half, c->imp[0],
+                      c->imp[1], c->imp[2], c->bpois[0], c->bpois[1], c->bpois[2]};
     for (size_t t = 0; t < sizeof(arrs) / sizeof(arrs[0]); t++) free(arrs[t]);
     free(c->vcoeff);
     free(c);
