@@ -290,6 +290,8 @@ int smac_cell_output(smac_ctx* ctx, float* host_out, size_t count);
  * statistics go to stats. Entries that are not state (ghosts, periodic images)
  * are ignored on entry and zero on return. The context's velocity is
  * overwritten with u_n; two-phase properties are those of the context.
+ * SMAC_ERR_UNSUPPORTED while a VoF flux is pending or once volume fractions
+ * (SMAC_FIELD_EPS) are in the context: the adjoint is that of the one-fluid step.
  * No counterpart in the reference (it has no adjoint); it serves gradient-based
  * callers through a torch.autograd.Function (mydemcfd_amd/autograd.py). */
 int smac_adjoint_step(smac_ctx* ctx, double dt, const smac_solver_opts* opts, const double* const u_n[3],

@@ -196,6 +196,25 @@ struct G_SMACSolver {
     check(smac_update_boundary_ghost(ctx_, solv.dt_), "smac_update_boundary_ghost");
   }
   void make_face_gradp() { check(smac_make_face_gradp(ctx_), "smac_make_face_gradp"); }
+  // Two-way CFD-DEM loop (Coupling/main.cu:517-549): the coupling writes the
+  // volume fractions and impulses (SMAC_FIELD_EPS, EPS_OLD, EPS_HALF, IMP{X,Y,Z})
+  // with upload(); the mass flux is formed inside the two-way predictor.
+  void compute_mass_flux_from_alpha_flux_two_way(const SMACSolver&) {}
+  void get_vof_vstar_rhouu_consistent_two_way(const SMACSolver& solv) {
+    dt_ = solv.dt_;
+    inv_dt_ = solv.inv_dt_;
+    check(smac_predictor_two_way(ctx_, solv.dt_), "smac_predictor_two_way");
+  }
+};
+
+// Coupling/G_CFDDEMCoupling.h:149 -- the CFD side of the coupling object: the
+// Poisson coefficients eps_f/rho_f of update_poisson_beta_two_way
+// (Coupling/G_CFDDEMCoupling.cu:419-459). The DEM side (particles, drag,
+// void-fraction mapping) is outside this path.
+struct G_CFDDEMCoupling {
+  void update_poisson_beta_two_way(G_SMACSolver& g) {
+    check(smac_update_poisson_beta_two_way(g.ctx_), "smac_update_poisson_beta_two_way");
+  }
 };
 
 // CFD/pressure_solver/G_PCGSolver.h:17-37. The stopping rule is the
