@@ -281,22 +281,115 @@ struct St<float> {
   static __device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
   static __device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
 };
+// float -> bf16 bits, round to nearest even (inf stays inf, NaN stays NaN)
+__device__ __forceinline__ unsigned bf16_bits(float v) {
+  unsigned u = __float_as_uint(v);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
 template <>
 struct St<bf16s> {
   using C = float;
   static __device__ __forceinline__ float ld(const bf16s* p, size_t i) {
     return __uint_as_float(((unsigned)p[i].u) << 16);
   }
-  static __device__ __forceinline__ void st(bf16s* p, size_t i, float v) {
-    unsigned u = __float_as_uint(v);
-    unsigned r;
-    if ((u & 0x7f800000u) == 0x7f800000u) {
-      r = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);  // inf stays inf, NaN stays NaN
+  static __device__ __forceinline__ void st(bf16s* p, size_t i, float v) { p[i].u = (uint16_t)bf16_bits(v); }
+};
+
+// N consecutive storage elements in one aligned access (N * sizeof(T) <= 16,
+// the address a multiple of N * sizeof(T)), converted to / from the compute type
+template <typename T, int N>
+struct Vec;
+template <int N>
+struct Vec<double, N> {
+  static __device__ __forceinline__ void ld(const double* p, double* o) {
+    if constexpr (N == 2) {
+      const double2 v = *reinterpret_cast<const double2*>(p);
+      o[0] = v.x;
+      o[1] = v.y;
     } else {
-      u += 0x7fffu + ((u >> 16) & 1u);
-      r = u >> 16;
+      o[0] = p[0];
     }
-    p[i].u = (uint16_t)r;
+  }
+  static __device__ __forceinline__ void st(double* p, const double* v) {
+    if constexpr (N == 2) {
+      *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    } else {
+      p[0] = v[0];
+    }
+  }
+};
+template <int N>
+struct Vec<float, N> {
+  static __device__ __forceinline__ void ld(const float* p, float* o) {
+    if constexpr (N == 4) {
+      const float4 v = *reinterpret_cast<const float4*>(p);
+      o[0] = v.x;
+      o[1] = v.y;
+      o[2] = v.z;
+      o[3] = v.w;
+    } else if constexpr (N == 2) {
+      const float2 v = *reinterpret_cast<const float2*>(p);
+      o[0] = v.x;
+      o[1] = v.y;
+    } else {
+      o[0] = p[0];
+    }
+  }
+  static __device__ __forceinline__ void st(float* p, const float* v) {
+    if constexpr (N == 4) {
+      *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (N == 2) {
+      *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else {
+      p[0] = v[0];
+    }
+  }
+};
+template <int N>
+struct Vec<bf16s, N> {
+  static __device__ __forceinline__ void ld(const bf16s* p, float* o) {
+    if constexpr (N == 8) {
+      const uint4 v = *reinterpret_cast<const uint4*>(p);
+      const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        o[2 * q] = __uint_as_float(w[q] << 16);
+        o[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
+      }
+    } else if constexpr (N == 4) {
+      const uint2 v = *reinterpret_cast<const uint2*>(p);
+      o[0] = __uint_as_float(v.x << 16);
+      o[1] = __uint_as_float(v.x & 0xffff0000u);
+      o[2] = __uint_as_float(v.y << 16);
+      o[3] = __uint_as_float(v.y & 0xffff0000u);
+    } else if constexpr (N == 2) {
+      const unsigned v = *reinterpret_cast<const unsigned*>(p);
+      o[0] = __uint_as_float(v << 16);
+      o[1] = __uint_as_float(v & 0xffff0000u);
+    } else {
+      o[0] = St<bf16s>::ld(p, 0);
+    }
+  }
+  static __device__ __forceinline__ void st(bf16s* p, const float* v) {
+    if constexpr (N == 8) {
+      uint4 w;
+      w.x = bf16_bits(v[0]) | (bf16_bits(v[1]) << 16);
+      w.y = bf16_bits(v[2]) | (bf16_bits(v[3]) << 16);
+      w.z = bf16_bits(v[4]) | (bf16_bits(v[5]) << 16);
+      w.w = bf16_bits(v[6]) | (bf16_bits(v[7]) << 16);
+      *reinterpret_cast<uint4*>(p) = w;
+    } else if constexpr (N == 4) {
+      uint2 w;
+      w.x = bf16_bits(v[0]) | (bf16_bits(v[1]) << 16);
+      w.y = bf16_bits(v[2]) | (bf16_bits(v[3]) << 16);
+      *reinterpret_cast<uint2*>(p) = w;
+    } else if constexpr (N == 2) {
+      *reinterpret_cast<unsigned*>(p) = bf16_bits(v[0]) | (bf16_bits(v[1]) << 16);
+    } else {
+      St<bf16s>::st(p, 0, v[0]);
+    }
   }
 };
 

@@ -218,6 +218,125 @@ __global__ __launch_bounds__(256) void k_mg_rbgs(Lv L, T* __restrict__ e, const 
 }
 
 
+// The smoother's FAST form with each lane owning CPL consecutive cells of a
+// half row (h = 1 + CPL*lane + 64*CPL*m + t): every array access is one
+// aligned vector of CPL elements (rows start on 128-byte lines and the row
+// pitch is a multiple of 64 elements, checked by the launcher), and only the
+// x neighbour across a chunk boundary comes from another lane (one rotate
+// shuffle per chunk). The y neighbours go through LDS as in k_mg_rbgs, and the
+// arithmetic per cell is the same.
+template <typename T, int VPL, bool VB, int CPL, bool LDSY>
+__global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
+                                                     int zero_start) {
+  using C = typename St<T>::C;
+  constexpr int NM = VPL / CPL;  // chunks per lane
+  constexpr int NH = 64 * VPL;   // cells per half row
+  __shared__ C ysm[LDSY ? 6 * NH : 1];
+  LROW_SETUP
+  const int nx = L.n[0], nE = nx >> 1;
+  const int par = (col + j + k) & 1;  // parity of the active i
+  const int off = par ? nE : 0;
+  const int oo = par ? 0 : nE;  // the other half
+  const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
+  const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+  C fv[NM][CPL];
+#pragma unroll
+  for (int m = 0; m < NM; m++) Vec<T, CPL>::ld(f + rb + off + 1 + CPL * lane + 64 * CPL * m, fv[m]);
+  if (zero_start) {
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      C ev[CPL];
+#pragma unroll
+      for (int t = 0; t < CPL; t++) {
+        const int h = 1 + CPL * lane + 64 * CPL * m + t;
+        const int i = par ? 2 * h - 1 : 2 * h;
+        CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+        ev[t] = fv[m][t] / c.diag;
+      }
+      Vec<T, CPL>::st(e + rb + off + 1 + CPL * lane + 64 * CPL * m, ev);
+    }
+    return;
+  }
+  C ov[NM][CPL], yp[NM][CPL], ym[NM][CPL], zp[NM][CPL], zm[NM][CPL];
+  if constexpr (LDSY) {
+    const int w = (int)(threadIdx.x >> 6);
+    C hv[NM][CPL];
+    const size_t hr = w == 0 ? ymr : ypr;  // wave 0: row j0-1, wave 3: row j0+4
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const int p = 1 + CPL * lane + 64 * CPL * m;
+      Vec<T, CPL>::ld(e + rb + oo + p, ov[m]);
+      Vec<T, CPL>::ld(e + zpr + off + p, zp[m]);
+      Vec<T, CPL>::ld(e + zmr + off + p, zm[m]);
+      if (w == 0 || w == 3) Vec<T, CPL>::ld(e + hr + off + p, hv[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int t = 0; t < CPL; t++) {
+        const int o = CPL * lane + 64 * CPL * m + t;
+        ysm[(w + 1) * NH + o] = ov[m][t];
+        if (w == 0) ysm[o] = hv[m][t];
+        if (w == 3) ysm[5 * NH + o] = hv[m][t];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int t = 0; t < CPL; t++) {
+        const int o = CPL * lane + 64 * CPL * m + t;
+        ym[m][t] = ysm[w * NH + o];
+        yp[m][t] = ysm[(w + 2) * NH + o];
+      }
+  } else {
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      const int p = 1 + CPL * lane + 64 * CPL * m;
+      Vec<T, CPL>::ld(e + rb + oo + p, ov[m]);
+      Vec<T, CPL>::ld(e + ypr + off + p, yp[m]);
+      Vec<T, CPL>::ld(e + ymr + off + p, ym[m]);
+      Vec<T, CPL>::ld(e + zpr + off + p, zp[m]);
+      Vec<T, CPL>::ld(e + zmr + off + p, zm[m]);
+    }
+  }
+  const bool perx = L.per[0] != 0;
+  const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
+#pragma unroll
+  for (int m = 0; m < NM; m++) {
+    // the x neighbour across the chunk boundary: for even i the other half's
+    // first cell of the next lane (lane 63: lane 0 of the next chunk; the last
+    // chunk wraps to the first), for odd i its last cell of the previous lane
+    C nb;
+    if (!par) {
+      const C src = lane == 0 ? ov[(m + 1) % NM][0] : ov[m][0];
+      nb = __shfl(src, up);
+    } else {
+      const C src = lane == 63 ? ov[(m + NM - 1) % NM][CPL - 1] : ov[m][CPL - 1];
+      nb = __shfl(src, dn);
+    }
+    C ev[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; t++) {
+      const int h = 1 + CPL * lane + 64 * CPL * m + t;
+      const int i = par ? 2 * h - 1 : 2 * h;
+      C xp, xm;
+      if (!par) {
+        xm = ov[m][t];
+        xp = t + 1 < CPL ? ov[m][t + 1 < CPL ? t + 1 : 0] : nb;
+        if (!perx && i == nx) xp = (C)0;
+      } else {
+        xp = ov[m][t];
+        xm = t > 0 ? ov[m][t > 0 ? t - 1 : 0] : nb;
+        if (!perx && i == 1) xm = (C)0;
+      }
+      CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+      C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] + c.a[5] * zm[m][t];
+      ev[t] = (fv[m][t] + S) / c.diag;
+    }
+    Vec<T, CPL>::st(e + rb + off + 1 + CPL * lane + 64 * CPL * m, ev);
+  }
+}
+
 // 1-D restriction stencil of coarse index I (transpose of cell-centred linear
 // prolongation; at a wall the mirrored weight folds onto the boundary cell)
 __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], double w[4]) {
@@ -639,12 +758,41 @@ static inline dim3 cell_grid(const Lv& L) {
 static const dim3 ROWB(256), CELLB(64, 4, 1);
 
 
+// vector accesses need every row's element 1 on a 128-byte line
+template <typename T>
+static bool rows_aligned(const Lv& L, const T* a) {
+  return L.px % 64 == 0 && ((uintptr_t)(a + 1) % 128) == 0;
+}
+// cells per lane of the vector forms: 16 bytes, at most the lane's share
+template <typename T>
+static constexpr int vec_cpl(int vpl) {
+  return (int)(16 / sizeof(T)) < vpl ? (int)(16 / sizeof(T)) : vpl;
+}
+
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
   const int half = L.n[0] - (L.n[0] >> 1);  // longest active half row
   const int vpl = row_vpl(half);
   const bool fast = vpl >= 1 && L.n[0] == 128 * vpl;
   const bool ldsy = fast && L.n[1] % 4 == 0;
+  if (fast && vpl >= 2 && rows_aligned(L, e) && rows_aligned(L, f)) {
+#define SMAC_RV(V, P, Y) \
+  hipLaunchKernelGGL((k_mg_rbgs_vec<T, P, V, vec_cpl<T>(P), Y>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start)
+#define SMAC_RV_V(V)                                                              \
+  switch (vpl) {                                                                  \
+    case 2: if (ldsy) SMAC_RV(V, 2, true); else SMAC_RV(V, 2, false); break;      \
+    case 4: if (ldsy) SMAC_RV(V, 4, true); else SMAC_RV(V, 4, false); break;      \
+    default: if (ldsy) SMAC_RV(V, 8, true); else SMAC_RV(V, 8, false); break;     \
+  }
+    if (L.b[0]) {
+      SMAC_RV_V(true)
+    } else {
+      SMAC_RV_V(false)
+    }
+#undef SMAC_RV_V
+#undef SMAC_RV
+    return;
+  }
 #define SMAC_RBGS(V, P, F, Y) \
   hipLaunchKernelGGL((k_mg_rbgs<T, P, V, F, Y>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start)
 #define SMAC_RBGS_V(V)                                                \
