@@ -25,7 +25,8 @@ width of the V-cycle):
          dp_flush 24 (that dp update after the last iteration),
          mg_rbgs_l0 1.5*W_s (a half sweep), mg_restrict_l0 2.125*W_s
          (residual + restriction), mg_prolong_l0 1.125*W_s (red cells),
-         mg_fill_l0 W_s, predict 56, rhs 32, correct 56, add_dp 32
+         mg_fill_l0 W_s, predict 56, rhs 32, correct 80 (u* and dp read, u
+         written, p += dp, then dp cleared)
 
 `--copy-calib BYTES` instead times a plain device copy of BYTES bytes (the
 calibration of the FETCH_SIZE / WRITE_SIZE counters) and prints its rate.
@@ -57,8 +58,7 @@ KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (
     "mg_fill_l0": lambda W, flex: 1.0 * W,
     "predict": lambda W, flex: 56.0,
     "rhs": lambda W, flex: 32.0,
-    "correct": lambda W, flex: 56.0,
-    "add_dp": lambda W, flex: 32.0,
+    "correct": lambda W, flex: 80.0,
 }
 
 FORMS = {

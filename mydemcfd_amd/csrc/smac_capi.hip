@@ -1390,8 +1390,7 @@ extern "C" int smac_correct(smac_ctx* c, double dt) {
   int rc = ensure_props(c);
   if (rc) return rc;
   const BetaF bf = beta_of(c);
-  KT(c, "correct", launch_correct(c->g, var_props(c) ? &bf : nullptr, c->us, c->dp, c->u, dt, c->st));
-  KT(c, "add_dp", launch_add_dp(c->g, c->p, c->dp, c->st));
+  KT(c, "correct", launch_correct_all(c->g, var_props(c) ? &bf : nullptr, c->us, c->dp, c->u, c->p, dt, c->st));
   HIPOK(hipGetLastError());
   return SMAC_OK;
 }

@@ -181,12 +181,12 @@ void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
 void launch_scalar_set(double* S, int dst, double v, hipStream_t st);
-void launch_correct(const Geo& g, const BetaF* bf, double* const us[3], const double* dp, double* u[3], double dt,
-                    hipStream_t st);
 void launch_props(const Geo& g, const double* alpha, double* rho, double* rho_old, double* inv_rho, double* mu,
                   double* beta[3], double* muf[3], double rho0, double rho1, double mu0, double mu1, int initial,
                   hipStream_t st);
-void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st);
+// the correction of all components, p += dp, then dp = 0 (one pass + a clear)
+void launch_correct_all(const Geo& g, const BetaF* bf, double* const us[3], double* dp, double* u[3], double* p,
+                        double dt, hipStream_t st);
 // inv_rho == nullptr: one-phase viscous CFL `visc`; else (mu/rho) dt sum 1/h^2 per cell
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, const double* inv_rho, const double* mu,
                 double* p0, double* p1, hipStream_t st);

@@ -1,14 +1,16 @@
 // smac_kernels.hip -- gfx950 kernels of the SMAC projection step (FP64 form).
 //
 //   predictor        k_predict<A>      per face of component A (MUSCL-minmod
-//                                      advection + variable-mu stress + old dp/dx)
+//                                      advection + variable-mu stress + old dp/dx);
+//                    k_predict_tile    triply periodic one-phase grids (z-march);
+//                    k_predict_tw<A>   the two-way (CFD-DEM) form
 //   star BC          k_star_bc         wall faces of u*, v*, w*
-//   Poisson RHS      k_rhs             b = -(1/dt) div u*   (+ row partial sums)
+//   Poisson RHS      k_rhs, k_rhs_eps  b = -(1/dt) div u* (+ the volume-fraction form)
 //   PCG vectors      k_init_r, k_add_scalar, k_jacobi, k_copy_dot, k_sum,
-//                    k_spmv_dot, k_update, k_dot2, k_xpay
+//                    k_spmv_dot, k_update, k_dot2, k_xpay, k_dp_flush
 //   reductions       k_finalize        canonical 1024-lane tree + scalar ops
 //   (multigrid kernels: smac_mg.hip)
-//   correction       k_correct, k_add_dp
+//   correction       k_correct_all, k_zero_dp
 //
 // Row kernels: one wave64 per grid row (j,k); lane l owns i = 1+l, 1+l+64, ...
 // Row partial sums follow the canonical order shared with the CPU oracle.
@@ -1154,32 +1156,41 @@ __global__ void k_scalar_copy(double* S, int dst, int src) { S[dst] = S[src]; }
 __global__ void k_scalar_set(double* S, int dst, double v) { S[dst] = v; }
 
 // ------------------------------------------------------------- correction
-// k_correct_vof_velocity (CFD/G_SMACSolver.cu:1072-1142), component A (dp natural)
-template <int A>
-__global__ __launch_bounds__(256) void k_correct(Geo g, const double* __restrict__ beta,
-                                                 const double* __restrict__ us, const double* __restrict__ dp,
-                                                 double* __restrict__ u, double dt) {
-  int lo[3] = {1, 1, 1}, hi[3] = {g.n[0], g.n[1], g.n[2]};
-  lo[A] = g.per[A] ? 1 : 2;
-  int i = lo[0] + (int)(blockIdx.x * 64 + threadIdx.x);
-  int j = lo[1] + (int)(blockIdx.y * 4 + threadIdx.y);
-  int k = lo[2] + (int)blockIdx.z;
-  if (i > hi[0] || j > hi[1] || k > hi[2]) return;
-  I3 q = mk(i, j, k);
-  size_t id = IX(g, q);
-  I3 cm = wr(g, sh(q, A, -1));
-  double gr = (dp[id] - dp[IX(g, cm)]) * g.inv_d[A];
-  const double ir = beta ? beta[id] : g.beta0;  // 1/rho_f on interior faces
-  u[id] = us[id] + ir * dt * (0.0 - gr);
+// The correction of all three components and p += dp in one pass: the
+// thread of interior cell (i,j,k) corrects its minus x-, y- and z-faces
+// (those that are interior faces) with u = u* + (1/rho_f) dt (0 - (dp_c -
+// dp_{c-1})/h) (k_correct_vof_velocity, CFD/G_SMACSolver.cu:1072-1142), then
+// adds the cell's dp to p; dp is cleared afterwards by k_zero_dp (the memset
+// of CFD/G_SMACSolver.cu:1273), since the neighbouring threads read it.
+__global__ __launch_bounds__(256) void k_correct_all(Geo g, BetaF bf, int var, const double* __restrict__ us0,
+                                                     const double* __restrict__ us1,
+                                                     const double* __restrict__ us2, const double* __restrict__ dp,
+                                                     double* __restrict__ u0, double* __restrict__ u1,
+                                                     double* __restrict__ u2, double* __restrict__ p, double dt) {
+  const int i = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int k = 1 + (int)blockIdx.z;
+  if (i > g.n[0] || j > g.n[1]) return;
+  const I3 q = mk(i, j, k);
+  const size_t id = IX(g, q);
+  const double d0 = dp[id];
+  const double* const us[3] = {us0, us1, us2};
+  double* const u[3] = {u0, u1, u2};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (!g.per[a] && q.v[a] < 2) continue;  // the wall face 1 is not interior
+    const I3 cm = wr(g, sh(q, a, -1));
+    const double gr = (d0 - dp[IX(g, cm)]) * g.inv_d[a];
+    const double ir = var ? bf.b[a][id] : g.beta0;  // 1/rho_f on interior faces
+    u[a][id] = us[a][id] + ir * dt * (0.0 - gr);
+  }
+  p[id] += d0;
 }
 
-// p += dp; dp = 0 on interior cells
-__global__ __launch_bounds__(256) void k_add_dp(Geo g, double* __restrict__ p, double* __restrict__ dp) {
+// dp = 0 on interior cells (the reference's memset of p_delta after the correction)
+__global__ __launch_bounds__(256) void k_zero_dp(Geo g, double* __restrict__ dp) {
   ROW_SETUP
-  for (int i = 1 + lane; i <= g.n[0]; i += 64) {
-    p[rb + i] += dp[rb + i];
-    dp[rb + i] = 0.0;
-  }
+  for (int i = 1 + lane; i <= g.n[0]; i += 64) dp[rb + i] = 0.0;
 }
 
 // ---------------------------------------------------------------- CFL
@@ -1485,21 +1496,12 @@ void launch_scalar_copy(double* S, int dst, int src, hipStream_t st) {
 void launch_scalar_set(double* S, int dst, double v, hipStream_t st) {
   hipLaunchKernelGGL(k_scalar_set, dim3(1), dim3(1), 0, st, S, dst, v);
 }
-void launch_correct(const Geo& g, const BetaF* bf, double* const us[3], const double* dp, double* u[3], double dt,
-                    hipStream_t st) {
-  for (int a = 0; a < 3; a++) {
-    int lo = g.per[a] ? 1 : 2;
-    int nf[3] = {g.n[0], g.n[1], g.n[2]};
-    nf[a] = g.n[a] - lo + 1;
-    if (nf[0] <= 0 || nf[1] <= 0 || nf[2] <= 0) continue;
-    dim3 grid((unsigned)((nf[0] + 63) / 64), (unsigned)((nf[1] + 3) / 4), (unsigned)nf[2]);
-    if (a == 0) hipLaunchKernelGGL(k_correct<0>, grid, CELLB, 0, st, g, bf ? bf->b[0] : nullptr, us[0], dp, u[0], dt);
-    if (a == 1) hipLaunchKernelGGL(k_correct<1>, grid, CELLB, 0, st, g, bf ? bf->b[1] : nullptr, us[1], dp, u[1], dt);
-    if (a == 2) hipLaunchKernelGGL(k_correct<2>, grid, CELLB, 0, st, g, bf ? bf->b[2] : nullptr, us[2], dp, u[2], dt);
-  }
-}
-void launch_add_dp(const Geo& g, double* p, double* dp, hipStream_t st) {
-  hipLaunchKernelGGL(k_add_dp, rows_grid(g), ROWB, 0, st, g, p, dp);
+void launch_correct_all(const Geo& g, const BetaF* bf, double* const us[3], double* dp, double* u[3], double* p,
+                        double dt, hipStream_t st) {
+  dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4), (unsigned)g.n[2]);
+  hipLaunchKernelGGL(k_correct_all, grid, CELLB, 0, st, g, bf ? *bf : BetaF{}, bf ? 1 : 0, us[0], us[1], us[2], dp,
+                     u[0], u[1], u[2], p, dt);
+  hipLaunchKernelGGL(k_zero_dp, rows_grid(g), ROWB, 0, st, g, dp);
 }
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, const double* inv_rho, const double* mu,
                 double* p0, double* p1, hipStream_t st) {

@@ -368,7 +368,7 @@ __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], doubl
 // the lane loads the other half of its row once (x neighbours by a rotate
 // shuffle), and the four rows of a block share those halves through LDS as
 // each other's y neighbours (with rows j0-1 and j0+4); same arithmetic.
-template <typename T, int VPL, bool VB, bool RED, bool FAST>
+template <typename T, int VPL, bool VB, bool RED, bool FAST, int CPL = 1>
 __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __restrict__ e, const T* __restrict__ f,
                                                      typename St<T>::C* __restrict__ tb) {
   using C = typename St<T>::C;
@@ -395,55 +395,74 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
     const int off = par ? nE : 0;
     const int cnt = RED ? nE : nx;
     if constexpr (FAST) {
+      // CPL > 1: the lane owns CPL consecutive cells per chunk, h = 1 +
+      // CPL*lane + 64*CPL*m + t, read and written as aligned vectors (the
+      // launcher checks the row alignment); CPL = 1 is one cell per chunk
+      constexpr int NM = NV / CPL;
       const int oo = par ? 0 : nE;
-      C ec[NV], ef[NV], ov[NV], yp[NV], ym[NV], zp[NV], zm[NV], hv[NV];
+      C ec[NM][CPL], ef[NM][CPL], ov[NM][CPL], yp[NM][CPL], ym[NM][CPL], zp[NM][CPL], zm[NM][CPL], hv[NM][CPL];
       const size_t hr = w == 0 ? ymr : ypr;  // wave 0: row j0-1, wave 3: row j0+4
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int p = off + 1 + lane + 64 * m;
-        ef[m] = St<T>::ld(f, rb + p);
-        ec[m] = St<T>::ld(e, rb + p);
-        ov[m] = St<T>::ld(e, rb + oo + 1 + lane + 64 * m);
-        zp[m] = St<T>::ld(e, zpr + p);
-        zm[m] = St<T>::ld(e, zmr + p);
-        if (w == 0 || w == 3) hv[m] = St<T>::ld(e, hr + p);
+      for (int m = 0; m < NM; m++) {
+        const int hb = 1 + CPL * lane + 64 * CPL * m;
+        Vec<T, CPL>::ld(f + rb + off + hb, ef[m]);
+        Vec<T, CPL>::ld(e + rb + off + hb, ec[m]);
+        Vec<T, CPL>::ld(e + rb + oo + hb, ov[m]);
+        Vec<T, CPL>::ld(e + zpr + off + hb, zp[m]);
+        Vec<T, CPL>::ld(e + zmr + off + hb, zm[m]);
+        if (w == 0 || w == 3) Vec<T, CPL>::ld(e + hr + off + hb, hv[m]);
       }
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int o = lane + 64 * m;
-        ysm[(w + 1) * 64 * NV + o] = ov[m];
-        if (w == 0) ysm[o] = hv[m];
-        if (w == 3) ysm[5 * 64 * NV + o] = hv[m];
-      }
+      for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int t = 0; t < CPL; t++) {
+          const int o = CPL * lane + 64 * CPL * m + t;
+          ysm[(w + 1) * 64 * NV + o] = ov[m][t];
+          if (w == 0) ysm[o] = hv[m][t];
+          if (w == 3) ysm[5 * 64 * NV + o] = hv[m][t];
+        }
       __syncthreads();
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int o = lane + 64 * m;
-        ym[m] = ysm[w * 64 * NV + o];
-        yp[m] = ysm[(w + 2) * 64 * NV + o];
-      }
+      for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int t = 0; t < CPL; t++) {
+          const int o = CPL * lane + 64 * CPL * m + t;
+          ym[m][t] = ysm[w * 64 * NV + o];
+          yp[m][t] = ysm[(w + 2) * 64 * NV + o];
+        }
       const bool perx = L.per[0] != 0;
       const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int h = 1 + lane + 64 * m;
-        const int i = par ? 2 * h - 1 : 2 * h;
-        C xp, xm;
-        if (!par) {  // even i = 2h: i-1 in this lane's slot, i+1 one lane up
-          const C src = lane == 0 ? ov[m + 1 < NV ? m + 1 : 0] : ov[m];
-          xm = ov[m];
-          xp = __shfl(src, up);
-          if (!perx && i == nx) xp = (C)0;
-        } else {     // odd i = 2h-1: i+1 in this lane's slot, i-1 one lane down
-          const C src = lane == 63 ? ov[m >= 1 ? m - 1 : NV - 1] : ov[m];
-          xp = ov[m];
-          xm = __shfl(src, dn);
-          if (!perx && i == 1) xm = (C)0;
+      for (int m = 0; m < NM; m++) {
+        // the x neighbour across the chunk boundary (as in k_mg_rbgs_vec)
+        C nb;
+        if (!par) {
+          const C src = lane == 0 ? ov[(m + 1) % NM][0] : ov[m][0];
+          nb = __shfl(src, up);
+        } else {
+          const C src = lane == 63 ? ov[(m + NM - 1) % NM][CPL - 1] : ov[m][CPL - 1];
+          nb = __shfl(src, dn);
         }
-        CoefC<C> c = level_coef<C, VB>(L, i, j, k);
-        C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
-        buf[i] = ef[m] - (c.diag * ec[m] - S);
-        buf[par ? i + 1 : i - 1] = (C)0;  // the black neighbour of the same pair
+#pragma unroll
+        for (int t = 0; t < CPL; t++) {
+          const int h = 1 + CPL * lane + 64 * CPL * m + t;
+          const int i = par ? 2 * h - 1 : 2 * h;
+          C xp, xm;
+          if (!par) {  // even i = 2h: i-1 in this cell's slot, i+1 the next slot
+            xm = ov[m][t];
+            xp = t + 1 < CPL ? ov[m][t + 1 < CPL ? t + 1 : 0] : nb;
+            if (!perx && i == nx) xp = (C)0;
+          } else {     // odd i = 2h-1: i+1 in this cell's slot, i-1 the previous slot
+            xp = ov[m][t];
+            xm = t > 0 ? ov[m][t > 0 ? t - 1 : 0] : nb;
+            if (!perx && i == 1) xm = (C)0;
+          }
+          CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+          C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] +
+                c.a[5] * zm[m][t];
+          buf[i] = ef[m][t] - (c.diag * ec[m][t] - S);
+          buf[par ? i + 1 : i - 1] = (C)0;  // the black neighbour of the same pair
+        }
       }
     } else {
     const int trips = VPL > 0 ? 1 : (cnt + 63) / 64;
@@ -629,7 +648,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(Lv Cl, Lv F, const T* __rest
 // order (coalesced loads of the split rows), and each fine cell reads its
 // eight coarse values there. Needs col >= 0, ny % 4 == 0 and a half row of
 // 64 VPL cells; same arithmetic, in the same order, as k_mg_prolong.
-template <typename T, int VPL>
+template <typename T, int VPL, int CPL = 1>
 __global__ __launch_bounds__(256) void k_mg_prolong_lds(Lv Cl, Lv F, const T* __restrict__ ec,
                                                         T* __restrict__ ef, int col) {
   using C = typename St<T>::C;
@@ -671,8 +690,11 @@ __global__ __launch_bounds__(256) void k_mg_prolong_lds(Lv Cl, Lv F, const T* __
       if (q <= NC) stg[r][u] = St<T>::ld(ec, crow + q);
     }
   }
+  // the lane's fine cells: h = 1 + CPL*lane + 64*CPL*mm + t, CPL consecutive
+  // cells read and written as one aligned vector (CPL > 1: rows checked aligned)
+  constexpr int NM = VPL / CPL;
 #pragma unroll
-  for (int m = 0; m < VPL; m++) fv[m] = St<T>::ld(ef, rb + off + 1 + lane + 64 * m);
+  for (int mm = 0; mm < NM; mm++) Vec<T, CPL>::ld(ef + rb + off + 1 + CPL * lane + 64 * CPL * mm, fv + mm * CPL);
 #pragma unroll
   for (int r = 0; r < 8; r++)
 #pragma unroll
@@ -682,25 +704,30 @@ __global__ __launch_bounds__(256) void k_mg_prolong_lds(Lv Cl, Lv F, const T* __
     }
   __syncthreads();
 #pragma unroll
-  for (int m = 0; m < VPL; m++) {
-    const int h = 1 + lane + 64 * m;
-    const int pp = off + h;
-    const int i = pp <= nE ? 2 * pp : 2 * (pp - nE) - 1;
-    int ix[2];
-    double wx[2];
-    pweights(Cl.per[0], Cl.n[0], i, ix, wx);
-    C acc = 0;
+  for (int mm = 0; mm < NM; mm++) {
+    C out[CPL];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int t = 0; t < CPL; t++) {
+      const int h = 1 + CPL * lane + 64 * CPL * mm + t;
+      const int pp = off + h;
+      const int i = pp <= nE ? 2 * pp : 2 * (pp - nE) - 1;
+      int ix[2];
+      double wx[2];
+      pweights(Cl.per[0], Cl.n[0], i, ix, wx);
+      C acc = 0;
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        const C* rowp = cs + (a * 4 + sy[b]) * NC;
-        C w0 = (C)(wz[a] * wy[b] * wx[0]);
-        acc += w0 * rowp[ix[0] - 1];
-        C w1 = (C)(wz[a] * wy[b] * wx[1]);
-        acc += w1 * rowp[ix[1] - 1];
-      }
-    St<T>::st(ef, rb + pp, fv[m] + acc);
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          const C* rowp = cs + (a * 4 + sy[b]) * NC;
+          C w0 = (C)(wz[a] * wy[b] * wx[0]);
+          acc += w0 * rowp[ix[0] - 1];
+          C w1 = (C)(wz[a] * wy[b] * wx[1]);
+          acc += w1 * rowp[ix[1] - 1];
+        }
+      out[t] = fv[mm * CPL + t] + acc;
+    }
+    Vec<T, CPL>::st(ef + rb + off + 1 + CPL * lane + 64 * CPL * mm, out);
   }
 }
 
@@ -822,15 +849,18 @@ void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typena
   const int nx = F.n[0];
   const int vpl = row_vpl(nx / 2);
   const bool fast = red && vpl >= 1 && nx == 128 * vpl && F.n[1] % 4 == 0;
+  const bool vec = fast && rows_aligned(F, e) && rows_aligned(F, f);
 #define SMAC_RX(V, P, R, FA) \
   hipLaunchKernelGGL((k_mg_resid_rx<T, P, V, R, FA>), lrows_grid(F), ROWB, shm, st, F, C, e, f, tb)
+#define SMAC_RXV(V, P) \
+  hipLaunchKernelGGL((k_mg_resid_rx<T, P, V, true, true, vec_cpl<T>(P)>), lrows_grid(F), ROWB, shm, st, F, C, e, f, tb)
 #define SMAC_RX_V(V)                                                         \
   if (fast) {                                                                \
     switch (vpl) {                                                           \
       case 1: SMAC_RX(V, 1, true, true); break;                              \
-      case 2: SMAC_RX(V, 2, true, true); break;                              \
-      case 4: SMAC_RX(V, 4, true, true); break;                              \
-      default: SMAC_RX(V, 8, true, true); break;                             \
+      case 2: if (vec) SMAC_RXV(V, 2); else SMAC_RX(V, 2, true, true); break; \
+      case 4: if (vec) SMAC_RXV(V, 4); else SMAC_RX(V, 4, true, true); break; \
+      default: if (vec) SMAC_RXV(V, 8); else SMAC_RX(V, 8, true, true); break; \
     }                                                                        \
   } else if (red) {                                                          \
     SMAC_VPL_DISPATCH(nx / 2, SMAC_RX(V, VPL, true, false))                  \
@@ -843,6 +873,7 @@ void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typena
     SMAC_RX_V(false)
   }
 #undef SMAC_RX_V
+#undef SMAC_RXV
 #undef SMAC_RX
   hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
 }
@@ -851,14 +882,18 @@ void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hi
   const int cntmax = col < 0 ? F.n[0] : F.n[0] - (F.n[0] >> 1);
   const int vpl = row_vpl(cntmax);
   if (col >= 0 && vpl >= 1 && F.n[0] == 128 * vpl && F.n[1] % 4 == 0) {
+    const bool vec = rows_aligned(F, ef);
 #define SMAC_PROLONG_LDS(P) \
   hipLaunchKernelGGL((k_mg_prolong_lds<T, P>), lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col)
+#define SMAC_PROLONG_LDSV(P) \
+  hipLaunchKernelGGL((k_mg_prolong_lds<T, P, vec_cpl<T>(P)>), lrows_grid(F), ROWB, 0, st, C, F, ec, ef, col)
     switch (vpl) {
       case 1: SMAC_PROLONG_LDS(1); break;
-      case 2: SMAC_PROLONG_LDS(2); break;
-      case 4: SMAC_PROLONG_LDS(4); break;
-      default: SMAC_PROLONG_LDS(8); break;
+      case 2: if (vec) SMAC_PROLONG_LDSV(2); else SMAC_PROLONG_LDS(2); break;
+      case 4: if (vec) SMAC_PROLONG_LDSV(4); else SMAC_PROLONG_LDS(4); break;
+      default: if (vec) SMAC_PROLONG_LDSV(8); else SMAC_PROLONG_LDS(8); break;
     }
+#undef SMAC_PROLONG_LDSV
 #undef SMAC_PROLONG_LDS
     return;
   }

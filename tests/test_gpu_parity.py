@@ -178,7 +178,11 @@ def test_mixed_precision_vs_fp64_oracle(hip, oracle_mod, precision, kind):
         # anisotropic cells of cavity_wide (153 against 108 in FP64)
         slack = max(6, so.iters // 2) if precision == "bf16" else 6
         assert sg.iters <= so.iters + slack, (sg.iters, so.iters)
-    errs = {f: rel_err(g.download(f), o.download(f)) for f in ("u", "v", "w")}
+    # velocity errors against the velocity scale of the whole field (a component
+    # that is nearly zero, w in the 256x8x8 cavity, would inflate its own ratio)
+    vel = {f: (g.download(f), o.download(f)) for f in ("u", "v", "w")}
+    vscale = max(float(np.max(np.abs(b))) for _, b in vel.values())
+    errs = {f: float(np.max(np.abs(a - b))) / vscale for f, (a, b) in vel.items()}
     errs["p"] = rel_err_mod_mean(g.download("p"), o.download("p"), desc["n"])
     print(f"mixed {precision} {kind}: error vs FP64 oracle {errs}, iters {sg.iters} vs {so.iters}", flush=True)
     for f, e in errs.items():
