@@ -1071,17 +1071,20 @@ __global__ __launch_bounds__(256) void k_cvt(Geo g, const double* __restrict__ r
 // Reduce up to four row-partial arrays with the canonical 1024-lane tree
 // (kind 0 = sum, 1 = max) into S[out[m]], then apply one scalar op.
 // lane t of the canonical tree accumulates rows t, t+1024, ... in order;
-// loads are issued 16 at a time so their latencies overlap (the summation
-// order is unchanged)
+// loads are issued in batches (64 in the multi-block first stage, 16 in the
+// single-block kernel) so their latencies overlap; the summation order is
+// unchanged
+// B: loads in flight per lane (the adds stay sequential)
+template <int B>
 __device__ __forceinline__ double fin_lane(const double* __restrict__ part, int kind, int nrows, int t) {
   double acc = 0.0;
   int r = t;
-  for (; r + 15 * 1024 < nrows; r += 16 * 1024) {
-    double v[16];
+  for (; r + (B - 1) * 1024 < nrows; r += B * 1024) {
+    double v[B];
 #pragma unroll
-    for (int u = 0; u < 16; u++) v[u] = part[r + u * 1024];
+    for (int u = 0; u < B; u++) v[u] = part[r + u * 1024];
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
+    for (int u = 0; u < B; u++) {
       if (kind == 0) {
         acc += v[u];
       } else if (v[u] > acc || v[u] != v[u]) {
@@ -1105,7 +1108,7 @@ __device__ __forceinline__ double fin_lane(const double* __restrict__ part, int 
 __global__ __launch_bounds__(64) void k_fin_lanes(FinalizeArgs fa) {
   const int m = blockIdx.y;
   const int t = (int)blockIdx.x * 64 + (int)threadIdx.x;
-  fa.stage[m * 1024 + t] = fin_lane(fa.part[m], fa.kind[m], fa.nrows, t);
+  fa.stage[m * 1024 + t] = fin_lane<64>(fa.part[m], fa.kind[m], fa.nrows, t);
 }
 
 __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs fa, double* __restrict__ S) {
@@ -1113,7 +1116,7 @@ __global__ __launch_bounds__(1024) void k_finalize(FinalizeArgs fa, double* __re
   const int t = threadIdx.x;
   for (int m = 0; m < fa.count; m++) {
     const int kind = fa.kind[m];
-    double acc = fa.stage ? fa.stage[m * 1024 + t] : fin_lane(fa.part[m], kind, fa.nrows, t);
+    double acc = fa.stage ? fa.stage[m * 1024 + t] : fin_lane<16>(fa.part[m], kind, fa.nrows, t);
     lds[t] = acc;
     __syncthreads();
     for (int s = 512; s >= 64; s >>= 1) {
