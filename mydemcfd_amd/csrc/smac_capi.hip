@@ -209,7 +209,11 @@ static int balloc(void** p, size_t bytes, size_t esize = 8) {
   const size_t off = 128 - esize;
   *p = nullptr;
   if (hipMalloc(&raw, bytes + 256) != hipSuccess) return SMAC_ERR_OOM;
-  if (hipMemset(raw, 0, bytes + 256) != hipSuccess) {
+  // the fill runs on the null stream, which does not order against the
+  // context's non-blocking stream: wait for it before any kernel or copy of
+  // that stream can touch the buffer (buffers are also allocated lazily, right
+  // before their first kernel)
+  if (hipMemset(raw, 0, bytes + 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
     (void)hipFree(raw);
     return SMAC_ERR_HIP;
   }
@@ -866,7 +870,10 @@ static int mg_coeffs(smac_ctx* c) {
     if (rc) return rc;
   }
   // row-major: the wave of row r reads the row's entries at consecutive addresses
-  HIPOK(hipMemcpy(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+  // (on the context's stream, waited for: h is freed on return and the coarse
+  // solve kernels run on that stream)
+  HIPOK(hipMemcpyAsync(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPOK(hipStreamSynchronize(c->st));
   c->cinv_n = n;
   c->mg_ver = c->props_ver;
   return SMAC_OK;

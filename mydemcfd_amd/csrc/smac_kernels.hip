@@ -815,105 +815,126 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z
   if (lane == 0) part0[row] = acc;
 }
 
-// Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124).
-// Lanes own split positions: lane l holds q = 1+l, 1+l+64, ... of the row
-// (contiguous loads and stores), and the row sum of d*Ad is accumulated per
-// lane in that order (natural cell i = 2q for q <= nx/2, 2(q - nx/2) - 1
-// above).
-// FAST (nx = 64 VPL, VPL even): the row is loaded once; the x neighbours of a
-// cell are the other half's value in the same lane and, one lane over, a
-// rotate shuffle (wall neighbours are the zero ghost value). LDSY (FAST and
-// ny % 4 == 0): the four waves of a block own rows j0..j0+3 of one plane and
-// share them, with rows j0-1 and j0+4, through LDS for the y neighbours.
-template <bool VAR, int VPL, bool FAST, bool LDSY>
+// Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124), the dot in
+// pair order (lane l: split positions 1+2l+128m, then +1; the oracle's
+// RED_DOT_S2). NM = nx/128 in {2, 4}, rows aligned (FAST): the lane holds its
+// pairs of both half rows as double2; the x neighbours of a cell are the other
+// half's pair in the same lane and, across a pair edge, one rotate shuffle
+// (wall neighbours are the zero ghost value). LDSY (ny % 4 == 0): the four
+// waves of a block own rows j0..j0+3 of one plane and share them, with rows
+// j0-1 and j0+4, through LDS for the y neighbours. NM = 0: any nx, loads per
+// neighbour, same order.
+template <bool VAR, int NM, bool LDSY>
 __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double* __restrict__ d,
                                                   double* __restrict__ Ad, double* __restrict__ part0) {
-  constexpr int NV = VPL > 0 ? VPL : 1;
-  __shared__ double ysm[LDSY ? 6 * 64 * NV : 1];
+  constexpr int NMC = NM > 0 ? NM : 1;
+  __shared__ double ysm[LDSY ? 6 * 128 * NMC : 1];
   ROW_SETUP
   const int nx = g.n[0], nh = nx >> 1;
   const size_t ypr = idx(g, 0, W(g, 1, j + 1), k), ymr = idx(g, 0, W(g, 1, j - 1), k);
   const size_t zpr = idx(g, 0, j, W(g, 2, k + 1)), zmr = idx(g, 0, j, W(g, 2, k - 1));
   double acc = 0.0;
-  if constexpr (!FAST) {
-    for (int q = 1 + lane; q <= nx; q += 64) {
-      const int i = q <= nh ? 2 * q : 2 * (q - nh) - 1;
-      Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
-      double xc = d[rb + q];
-      double xp = d[rb + spos(W(g, 0, i + 1), nx)], xm = d[rb + spos(W(g, 0, i - 1), nx)];
-      double yp = d[ypr + q], ym = d[ymr + q], zp = d[zpr + q], zm = d[zmr + q];
-      double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
-      double v = c.diag * xc - s;
-      Ad[rb + q] = v;
-      acc += xc * v;
+  if constexpr (NM == 0) {
+    for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int q = q0 + t;
+        if (q > nx) continue;
+        const int i = q <= nh ? 2 * q : 2 * (q - nh) - 1;
+        Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+        double xc = d[rb + q];
+        double xp = d[rb + spos(W(g, 0, i + 1), nx)], xm = d[rb + spos(W(g, 0, i - 1), nx)];
+        double yp = d[ypr + q], ym = d[ymr + q], zp = d[zpr + q], zm = d[zmr + q];
+        double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp + c.a[3] * ym + c.a[4] * zp + c.a[5] * zm;
+        double v = c.diag * xc - s;
+        Ad[rb + q] = v;
+        acc += xc * v;
+      }
     }
   } else {
-    constexpr int HV = NV / 2;  // lane slots per half row
-    double xc[NV], yp[NV], ym[NV], zp[NV], zm[NV];
+    constexpr int NH = NM / 2;  // pair chunks per half row
+    constexpr int NX = 128 * NM;
+    double xc[NM][2], yp[NM][2], ym[NM][2], zp[NM][2], zm[NM][2];
     if constexpr (LDSY) {
       const int w = (int)(threadIdx.x >> 6);
-      double hv[NV];
+      double hv[NM][2];
       const size_t hr = w == 0 ? ymr : ypr;  // wave 0: row j0-1, wave 3: row j0+4
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int q = 1 + lane + 64 * m;
-        xc[m] = d[rb + q];
-        zp[m] = d[zpr + q];
-        zm[m] = d[zmr + q];
-        if (w == 0 || w == 3) hv[m] = d[hr + q];
+      for (int m = 0; m < NM; m++) {
+        const int q = 1 + 2 * lane + 128 * m;
+        Vec<double, 2>::ld(d + rb + q, xc[m]);
+        Vec<double, 2>::ld(d + zpr + q, zp[m]);
+        Vec<double, 2>::ld(d + zmr + q, zm[m]);
+        if (w == 0 || w == 3) Vec<double, 2>::ld(d + hr + q, hv[m]);
       }
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int o = lane + 64 * m;
-        ysm[(w + 1) * 64 * NV + o] = xc[m];
-        if (w == 0) ysm[o] = hv[m];
-        if (w == 3) ysm[5 * 64 * NV + o] = hv[m];
-      }
+      for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          const int o = 2 * lane + 128 * m + t;
+          ysm[(w + 1) * NX + o] = xc[m][t];
+          if (w == 0) ysm[o] = hv[m][t];
+          if (w == 3) ysm[5 * NX + o] = hv[m][t];
+        }
       __syncthreads();
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int o = lane + 64 * m;
-        ym[m] = ysm[w * 64 * NV + o];
-        yp[m] = ysm[(w + 2) * 64 * NV + o];
-      }
+      for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          const int o = 2 * lane + 128 * m + t;
+          ym[m][t] = ysm[w * NX + o];
+          yp[m][t] = ysm[(w + 2) * NX + o];
+        }
     } else {
 #pragma unroll
-      for (int m = 0; m < NV; m++) {
-        const int q = 1 + lane + 64 * m;
-        xc[m] = d[rb + q];
-        yp[m] = d[ypr + q];
-        ym[m] = d[ymr + q];
-        zp[m] = d[zpr + q];
-        zm[m] = d[zmr + q];
+      for (int m = 0; m < NM; m++) {
+        const int q = 1 + 2 * lane + 128 * m;
+        Vec<double, 2>::ld(d + rb + q, xc[m]);
+        Vec<double, 2>::ld(d + ypr + q, yp[m]);
+        Vec<double, 2>::ld(d + ymr + q, ym[m]);
+        Vec<double, 2>::ld(d + zpr + q, zp[m]);
+        Vec<double, 2>::ld(d + zmr + q, zm[m]);
       }
     }
     const bool perx = g.per[0] != 0;
     const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-      const int q = 1 + lane + 64 * m;
-      const int i = m < HV ? 2 * q : 2 * (q - nh) - 1;
-      double xp, xm;
-      if (m < HV) {
-        // even i = 2q: i-1 is the other half's q (this lane), i+1 its q+1
-        // (next lane; lane 63 takes lane 0 of the next slot, i = nx wraps to i = 1)
-        const double src = lane == 0 ? xc[m + HV + 1 < NV ? m + HV + 1 : HV] : xc[m + HV];
-        xm = xc[m + HV];
-        xp = __shfl(src, up);
-        if (!perx && i == nx) xp = 0.0;
+    for (int m = 0; m < NM; m++) {
+      // the x neighbour across the pair edge: for an even i (first half) the
+      // other half's first value of the next lane (lane 63: lane 0 of the
+      // next chunk, wrapping from i = nx to i = 1), for an odd i (second half)
+      // the other half's second value of the previous lane
+      double nb;
+      if (m < NH) {
+        const double src = lane == 0 ? xc[(m + 1) % NH + NH][0] : xc[m + NH][0];
+        nb = __shfl(src, up);
       } else {
-        // odd i: i+1 is the other half's slot (this lane), i-1 one lane down
-        // (lane 0 takes lane 63 of the previous slot, i = 1 wraps to i = nx)
-        const double src = lane == 63 ? xc[m - HV - 1 >= 0 ? m - HV - 1 : HV - 1] : xc[m - HV];
-        xp = xc[m - HV];
-        xm = __shfl(src, dn);
-        if (!perx && i == 1) xm = 0.0;
+        const double src = lane == 63 ? xc[(m - NH + NH - 1) % NH][1] : xc[m - NH][1];
+        nb = __shfl(src, dn);
       }
-      Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
-      double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m] + c.a[3] * ym[m] + c.a[4] * zp[m] + c.a[5] * zm[m];
-      double v = c.diag * xc[m] - s;
-      Ad[rb + q] = v;
-      acc += xc[m] * v;
+      double av[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int q = 1 + 2 * lane + 128 * m + t;
+        const int i = m < NH ? 2 * q : 2 * (q - nh) - 1;
+        double xp, xm;
+        if (m < NH) {  // even i = 2q: i-1 = other half, same slot; i+1 the next slot
+          xm = xc[m + NH][t];
+          xp = t == 0 ? xc[m + NH][1] : nb;
+          if (!perx && i == nx) xp = 0.0;
+        } else {       // odd i: i+1 = other half, same slot; i-1 the previous slot
+          xp = xc[m - NH][t];
+          xm = t == 1 ? xc[m - NH][0] : nb;
+          if (!perx && i == 1) xm = 0.0;
+        }
+        Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+        double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] +
+                   c.a[5] * zm[m][t];
+        double v = c.diag * xc[m][t] - s;
+        av[t] = v;
+        acc += xc[m][t] * v;
+      }
+      Vec<double, 2>::st(Ad + rb + 1 + 2 * lane + 128 * m, av);
     }
   }
   acc = wave_tree_sum(acc);
@@ -921,46 +942,66 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
 }
 
 
+// Row sums over the solver's split vectors in pair order: lane l accumulates the
+// split positions q = 1+2l+128m, 1+2l+128m+1 (m = 0, 1, ...; q <= nx) in that
+// order (the oracle's RED_*_S2 sums). NM > 0 (nx = 128 NM): the pairs are
+// read and written as aligned double2 (rows start on 128-byte lines);
+// NM = 0: any nx, one element at a time in the same order.
+
 // r -= alpha Ad (the residual half of k_update_p_r, G_PCGSolver.cu:66-85);
-// part0 <- sum r^2, part1 <- max |r|. The dp += alpha d half is deferred to
-// the next k_xpay, which reads d anyway, or to k_dp_flush after the last
-// iteration (the same dp + alpha d per cell).
+// part0 <- sum r^2 (pair order), part1 <- max |r|. The dp += alpha d half is
+// deferred to the next k_xpay, which reads d anyway, or to k_dp_flush after
+// the last iteration (the same dp + alpha d per cell).
 // F: multigrid storage; when f0 is given it receives (F) r (the fine-level rhs
 // of the next V-cycle in the mixed forms)
-template <typename F, int VPL>
+template <typename F, int NM>
 __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ r, const double* __restrict__ Ad,
                                                 const double* __restrict__ S, int slot_alpha,
                                                 double* __restrict__ part0, double* __restrict__ part1,
                                                 F* __restrict__ f0) {
+  using C = typename St<F>::C;
   ROW_SETUP
   const int nx = g.n[0];
   const double alpha = S[slot_alpha];
   double a0 = 0.0, mx = 0.0;
-  constexpr int NV = VPL > 0 ? VPL : 1;
-  const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
-  for (int t = 0; t < trips; t++) {
-    // VPL > 0: one trip covering the row, loads first; VPL == 0: one cell per trip
-    double vr[NV], vA[NV];
+  if constexpr (NM > 0) {
+    double vr[NM][2], vA[NM][2];
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
-      if (i <= nx) {
-        const size_t o = rb + spos(i, nx);
-        vr[m] = r[o];
-        vA[m] = Ad[o];
-      }
+    for (int m = 0; m < NM; m++) {
+      const size_t o = rb + 1 + 2 * lane + 128 * m;
+      Vec<double, 2>::ld(r + o, vr[m]);
+      Vec<double, 2>::ld(Ad + o, vA[m]);
     }
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
-      if (i <= nx) {
-        const size_t o = rb + spos(i, nx);
-        double rn = vr[m] - alpha * vA[m];
-        r[o] = rn;
-        if (f0) St<F>::st(f0, o, (typename St<F>::C)rn);
-        a0 += rn * rn;
-        double ar = fabs(rn);
+    for (int m = 0; m < NM; m++) {
+      const size_t o = rb + 1 + 2 * lane + 128 * m;
+      double rn[2];
+      C fv[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        rn[t] = vr[m][t] - alpha * vA[m][t];
+        fv[t] = (C)rn[t];
+        a0 += rn[t] * rn[t];
+        double ar = fabs(rn[t]);
         if (ar > mx || ar != ar) mx = ar;
+      }
+      Vec<double, 2>::st(r + o, rn);
+      if (f0) Vec<F, 2>::st(f0 + o, fv);
+    }
+  } else {
+    for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int q = q0 + t;
+        if (q <= nx) {
+          const size_t o = rb + q;
+          double rn = r[o] - alpha * Ad[o];
+          r[o] = rn;
+          if (f0) St<F>::st(f0, o, (C)rn);
+          a0 += rn * rn;
+          double ar = fabs(rn);
+          if (ar > mx || ar != ar) mx = ar;
+        }
       }
     }
   }
@@ -981,34 +1022,43 @@ __global__ __launch_bounds__(256) void k_dp_flush(Geo g, double* __restrict__ dp
   for (int i = 1 + lane; i <= g.n[0]; i += 64) dp[rb + i] = dp[rb + i] + alpha * d[rb + spos(i, g.n[0])];
 }
 
-// part0 <- sum r*z; part1 <- sum z*Ad (flexible only)
-template <typename Z, int VPL>
+// part0 <- sum r*z; part1 <- sum z*Ad (flexible only), both in pair order
+template <typename Z, int NM>
 __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ r, const Z* __restrict__ z,
                                               const double* __restrict__ Ad, int flexible,
                                               double* __restrict__ part0, double* __restrict__ part1) {
+  using C = typename St<Z>::C;
   ROW_SETUP
   const int nx = g.n[0];
   double a0 = 0.0, a1 = 0.0;
-  constexpr int NV = VPL > 0 ? VPL : 1;
-  const int trips = VPL > 0 ? 1 : (nx + 63) / 64;
-  for (int t = 0; t < trips; t++) {
-    double vr[NV], vz[NV], vA[NV];
+  if constexpr (NM > 0) {
+    double vr[NM][2], vA[NM][2];
+    C vz[NM][2];
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
-      if (i <= nx) {
-        const size_t o = rb + spos(i, nx);
-        vr[m] = r[o];
-        vz[m] = (double)St<Z>::ld(z, o);
-        vA[m] = flexible ? Ad[o] : 0.0;
-      }
+    for (int m = 0; m < NM; m++) {
+      const size_t o = rb + 1 + 2 * lane + 128 * m;
+      Vec<double, 2>::ld(r + o, vr[m]);
+      Vec<Z, 2>::ld(z + o, vz[m]);
+      if (flexible) Vec<double, 2>::ld(Ad + o, vA[m]);
     }
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-      const int i = 1 + lane + 64 * (VPL > 0 ? m : t);
-      if (i <= nx) {
-        a0 += vr[m] * vz[m];
-        if (flexible) a1 += vz[m] * vA[m];
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        a0 += vr[m][t] * (double)vz[m][t];
+        if (flexible) a1 += (double)vz[m][t] * vA[m][t];
+      }
+  } else {
+    for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int q = q0 + t;
+        if (q <= nx) {
+          const size_t o = rb + q;
+          const double zz = (double)St<Z>::ld(z, o);
+          a0 += r[o] * zz;
+          if (flexible) a1 += zz * Ad[o];
+        }
       }
     }
   }
@@ -1418,23 +1468,30 @@ template <typename Z>
 void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st) {
   hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0);
 }
+// pair-order kernels: the vector form when a row is 128 NM cells (NM = 1..4)
+// and the split vectors start every row on a 128-byte line
+static inline int pair_nm(const Geo& g, const void* a) {
+  const int nx = g.n[0];
+  if (nx % 128 != 0 || nx > 512 || g.px % 64 != 0) return 0;
+  const uintptr_t e1 = (uintptr_t)a + sizeof(double);  // element 1 of the box
+  return e1 % 128 == 0 ? nx / 128 : 0;
+}
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st) {
-  const int vpl = row_vpl(g.n[0]);
-  const bool fast = vpl >= 2 && g.n[0] == 64 * vpl;
-  const bool ldsy = fast && g.n[1] % 4 == 0;
-#define SMAC_SPMV(V, P, F, L) \
-  hipLaunchKernelGGL((k_spmv_dot<V, P, F, L>), rows_grid(g), ROWB, 0, st, g, bf ? *bf : BetaF{}, d, Ad, p0)
-#define SMAC_SPMV_V(V)                                   \
-  if (!fast) SMAC_SPMV(V, 0, false, false);              \
-  else if (vpl == 2) {                                   \
-    if (ldsy) SMAC_SPMV(V, 2, true, true);               \
-    else SMAC_SPMV(V, 2, true, false);                   \
-  } else if (vpl == 4) {                                 \
-    if (ldsy) SMAC_SPMV(V, 4, true, true);               \
-    else SMAC_SPMV(V, 4, true, false);                   \
-  } else {                                               \
-    if (ldsy) SMAC_SPMV(V, 8, true, true);               \
-    else SMAC_SPMV(V, 8, true, false);                   \
+  int nm = pair_nm(g, d);
+  if (nm && pair_nm(g, Ad) != nm) nm = 0;
+  if (nm != 2 && nm != 4) nm = 0;  // both half rows in whole pair chunks
+  const bool ldsy = nm && g.n[1] % 4 == 0;
+#define SMAC_SPMV(V, N, L) \
+  hipLaunchKernelGGL((k_spmv_dot<V, N, L>), rows_grid(g), ROWB, 0, st, g, bf ? *bf : BetaF{}, d, Ad, p0)
+#define SMAC_SPMV_V(V)                                 \
+  if (nm == 2) {                                       \
+    if (ldsy) SMAC_SPMV(V, 2, true);                   \
+    else SMAC_SPMV(V, 2, false);                       \
+  } else if (nm == 4) {                                \
+    if (ldsy) SMAC_SPMV(V, 4, true);                   \
+    else SMAC_SPMV(V, 4, false);                       \
+  } else {                                             \
+    SMAC_SPMV(V, 0, false);                            \
   }
   if (bf) {
     SMAC_SPMV_V(true)
@@ -1447,8 +1504,19 @@ void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad,
 template <typename F>
 void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
                    double* p1, F* f0, hipStream_t st) {
-  SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_update<F, VPL>), rows_grid(g), ROWB, 0, st, g, r, Ad, S, slot_alpha,
-                                               p0, p1, f0))
+  int nm = pair_nm(g, r);
+  if (nm && pair_nm(g, Ad) != nm) nm = 0;
+  if (nm && f0 && ((uintptr_t)(f0 + 1) % 128) != 0) nm = 0;
+#define SMAC_UPD(N) \
+  hipLaunchKernelGGL((k_update<F, N>), rows_grid(g), ROWB, 0, st, g, r, Ad, S, slot_alpha, p0, p1, f0)
+  switch (nm) {
+    case 1: SMAC_UPD(1); break;
+    case 2: SMAC_UPD(2); break;
+    case 3: SMAC_UPD(3); break;
+    case 4: SMAC_UPD(4); break;
+    default: SMAC_UPD(0); break;
+  }
+#undef SMAC_UPD
 }
 void launch_dp_flush(const Geo& g, double* dp, const double* d, const double* S, int slot_alpha, hipStream_t st) {
   hipLaunchKernelGGL(k_dp_flush, rows_grid(g), ROWB, 0, st, g, dp, d, S, slot_alpha);
@@ -1456,8 +1524,19 @@ void launch_dp_flush(const Geo& g, double* dp, const double* d, const double* S,
 template <typename Z>
 void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, int flexible, double* p0,
                  double* p1, hipStream_t st) {
-  SMAC_VPL_DISPATCH(g.n[0],
-                    hipLaunchKernelGGL((k_dot2<Z, VPL>), rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1))
+  int nm = pair_nm(g, r);
+  if (nm && flexible && pair_nm(g, Ad) != nm) nm = 0;
+  if (nm && ((uintptr_t)(z + 1) % 128) != 0) nm = 0;
+#define SMAC_D2(N) \
+  hipLaunchKernelGGL((k_dot2<Z, N>), rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1)
+  switch (nm) {
+    case 1: SMAC_D2(1); break;
+    case 2: SMAC_D2(2); break;
+    case 3: SMAC_D2(3); break;
+    case 4: SMAC_D2(4); break;
+    default: SMAC_D2(0); break;
+  }
+#undef SMAC_D2
 }
 template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* S, int slot_alpha, int slot_beta,

@@ -457,27 +457,34 @@ static double final_reduce(const double* part, size_t nrows) {
 }
 
 /* generic dot-like sum: sum_c x[c]*y[c] (y may be NULL: sum x; sq: x*x).
- * RED_DOT_SPLIT: the dot in the order of a row held in the row-split layout
- * (even i first, then odd i): lane l accumulates split positions
- * q = 1+l, 1+l+64, ... (cell i = 2q for q <= nx/2, else 2(q - nx/2) - 1). */
-enum { RED_SUM = 0, RED_DOT = 1, RED_SQ = 2, RED_DOT_SPLIT = 3 };
+ * Natural modes: lane l accumulates cells i = 1+l, 1+l+64, ...
+ * Pair modes (RED_*_S2): the order of a row held in the row-split layout
+ * (even i first, then odd i) and read two positions per lane: lane l
+ * accumulates split positions q = 1+2l, 2+2l, 1+2l+128, 2+2l+128, ...
+ * (cell i = 2q for q <= nx/2, else 2(q - nx/2) - 1). */
+enum { RED_SUM = 0, RED_DOT = 1, RED_SQ = 2, RED_DOT_S2 = 3, RED_SQ_S2 = 4 };
 static double csum(const struct smac_ctx* c, int mode, const double* x, const double* y) {
     int nx = c->n[0], ny = c->n[1], nz = c->n[2];
     size_t nrows = (size_t)ny * (size_t)nz;
     double* part = (double*)malloc(nrows * sizeof(double));
+    const int pair = mode == RED_DOT_S2 || mode == RED_SQ_S2;
+    const int step = pair ? 128 : 64, width = pair ? 2 : 1;
 #pragma omp parallel for schedule(static)
     for (long r = 0; r < (long)nrows; r++) {
         int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
         double lanes[64];
         for (int l = 0; l < 64; l++) {
             double acc = 0.0;
-            for (int q = 1 + l; q <= nx; q += 64) {
-                int i = q;
-                if (mode == RED_DOT_SPLIT) i = q <= nx / 2 ? 2 * q : 2 * (q - nx / 2) - 1;
-                size_t id = IDX(c, i, j, k);
-                double v = mode == RED_SUM ? x[id] : (mode == RED_SQ ? x[id] * x[id] : x[id] * y[id]);
-                acc += v;
-            }
+            for (int q0 = 1 + width * l; q0 <= nx; q0 += step)
+                for (int t = 0; t < width; t++) {
+                    int q = q0 + t, i = q;
+                    if (q > nx) break;
+                    if (pair) i = q <= nx / 2 ? 2 * q : 2 * (q - nx / 2) - 1;
+                    size_t id = IDX(c, i, j, k);
+                    double v = mode == RED_SUM ? x[id]
+                               : (mode == RED_SQ || mode == RED_SQ_S2 ? x[id] * x[id] : x[id] * y[id]);
+                    acc += v;
+                }
             lanes[l] = acc;
         }
         part[r] = tree(lanes, 64);
@@ -1455,12 +1462,15 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
     int iter = 0, conv = 0;
     double rr = bb, linf = 0.0;
     if (bb == 0.0) conv = 1;
+    /* |r|^2 after an update: pair order from the update itself, natural order
+     * where the mean of r is removed after it */
+    const int rr_mode = pure && o.precond == SMAC_PRECOND_REF_GMG ? RED_SQ : RED_SQ_S2;
     for (iter = 0; iter < o.max_iter && !conv; ++iter) {
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= ny; j++)
                 for (int i = 1; i <= nx; i++) c->Ad[IDX(c, i, j, k)] = apply_A(c, c->dvec, i, j, k);
-        double pAp = csum(c, RED_DOT_SPLIT, c->dvec, c->Ad);
+        double pAp = csum(c, RED_DOT_S2, c->dvec, c->Ad);
         /* stationary: x += z, r -= A z (a fixed linear map of b for tol = 0) */
         double alpha = o.stationary ? 1.0 : rz / pAp;
 #pragma omp parallel for collapse(2) schedule(static)
@@ -1476,15 +1486,15 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
         if (pure && o.precond == SMAC_PRECOND_REF_GMG)
             add_scalar_interior(c, c->r, -1. * size_inv, csum(c, RED_SUM, c->r, NULL));
         precond(c, &o);
-        double rz_new = csum(c, RED_DOT, c->r, c->z);
+        double rz_new = csum(c, RED_DOT_S2, c->r, c->z);
         double zAd = 0.0;
-        if (o.flexible) zAd = csum(c, RED_DOT, c->z, c->Ad);
+        if (o.flexible) zAd = csum(c, RED_DOT_S2, c->z, c->Ad);
         if (iter % o.check_every == 0) {
             if (o.stop_rule == SMAC_STOP_REF_LINF_DT) {
                 linf = cmaxabs(c, c->r) * dt;
                 if (linf < o.tol) { iter++; conv = 1; break; }
             } else {
-                rr = csum(c, RED_SQ, c->r, NULL);
+                rr = csum(c, rr_mode, c->r, NULL);
                 if (sqrt(rr) < o.tol * bnorm) { iter++; conv = 1; break; }
             }
         }
@@ -1504,7 +1514,7 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
         st->converged = conv;
         st->pure_neumann = pure;
         st->mg_levels = o.precond != SMAC_PRECOND_JACOBI ? L : 0;
-        rr = csum(c, RED_SQ, c->r, NULL);
+        if (iter > 0) rr = csum(c, rr_mode, c->r, NULL);
         st->bnorm = bnorm;
         st->rel_res = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
         st->linf_res_dt = cmaxabs(c, c->r) * dt;
