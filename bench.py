@@ -82,6 +82,7 @@ def run_form(name, n, steps, warmup, fields, re, cfl, tol, levels):
     spec = FORMS[name]
     desc = cases.periodic_desc(n, re=re)
     ctx = mydemcfd_amd.SMACContext(**desc)
+    torch.cuda.synchronize()  # the context's stream is not ordered with torch's
     for fname, t in zip(("u", "v", "w"), fields):
         ctx.upload_device(fname, t.data_ptr(), t.numel())
     ctx.sync()
@@ -138,6 +139,7 @@ def field_errors(ctx_a, ctx_b, n):
         shape = _abi.field_shape((n, n, n), _abi.FIELD_NAMES[f])
         ta = torch.empty(shape, dtype=torch.float64, device="cuda")
         tb = torch.empty_like(ta)
+        torch.cuda.synchronize()
         ctx_a.download_device(f, ta.data_ptr(), ta.numel())
         ctx_b.download_device(f, tb.data_ptr(), tb.numel())
         ia = ta[1:n + 1, 1:n + 1, 1:n + 1]

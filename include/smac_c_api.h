@@ -183,7 +183,11 @@ int smac_set_fluid(smac_ctx* ctx, const smac_fluid_desc* fluid);
  * field in the reference layout (above). Synchronous. */
 int smac_upload(smac_ctx* ctx, int32_t field, const double* host, size_t count);
 int smac_download(smac_ctx* ctx, int32_t field, double* host, size_t count);
-/* Same, from/to a device buffer (e.g. a torch tensor) in the reference layout. */
+/* Same, from/to a device buffer (e.g. a torch tensor) in the reference layout.
+ * The copy runs on the context's non-blocking stream, which is not ordered with
+ * any other stream: finish the other streams' work on the buffer first (for a
+ * torch tensor, torch.cuda.synchronize(); the caching allocator hands a freed
+ * block to a new tensor while kernels queued on its stream may still read it). */
 int smac_upload_device(smac_ctx* ctx, int32_t field, const double* dev, size_t count);
 int smac_download_device(smac_ctx* ctx, int32_t field, double* dev, size_t count);
 /* Element count of a field in the reference layout. */
