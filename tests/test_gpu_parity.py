@@ -239,14 +239,16 @@ def test_cpp_facade_driver(hip, oracle_mod, tmp_path, mode):
         assert err <= TOL, (f, err)
 
 
-def test_periodic_mg_wide_rows(hip, oracle_mod):
-    """Anisotropic periodic box with 256-cell rows: the shuffle/LDS paths of
-    the smoother, residual, SpMV and prolongation at four cells per lane, and
-    the tiled predictor."""
+@pytest.mark.parametrize("n", [(256, 16, 8), (512, 8, 8), (384, 8, 8)])
+def test_periodic_mg_wide_rows(hip, oracle_mod, n):
+    """Anisotropic periodic boxes with 256- and 512-cell rows: the shuffle/LDS
+    paths of the smoother, residual, SpMV and prolongation at four and eight
+    cells per lane, the pair-order PCG kernels with two and four chunks per row
+    (the x wrap across chunk edges), and the tiled predictor; 384-cell rows
+    take the pair-order kernels with three chunks and the general SpMV."""
     from mydemcfd_amd import cases
 
-    print("case: TG-like periodic 256x16x8 x3 mg", flush=True)
-    n = (256, 16, 8)
+    print(f"case: TG-like periodic {n} x3 mg", flush=True)
     desc = cases.periodic_desc(n)
     g, o = _pair(hip, oracle_mod, desc)
     rng = np.random.default_rng(21)
@@ -275,11 +277,12 @@ def test_cavity_mg_tiled(hip, oracle_mod):
     _compare(g, o, n)
 
 
-@pytest.mark.parametrize("n", [(128, 16, 12), (256, 8, 8)])
+@pytest.mark.parametrize("n", [(128, 16, 12), (256, 8, 8), (512, 8, 8)])
 def test_cavity_mg_wide_rows(hip, oracle_mod, n):
     """Walled box whose fine level takes the shuffle/LDS paths of the
-    smoother, red residual, SpMV and prolongation (rows of 128 and 256 cells,
-    ny % 4 == 0): the wall ghosts of the x shuffles and the LDS halo rows."""
+    smoother, red residual, SpMV and prolongation (rows of 128, 256 and 512
+    cells, ny % 4 == 0): the wall ghosts of the x shuffles and the LDS halo
+    rows."""
     from mydemcfd_amd import cases
 
     print(f"case: cavity {n} x3 mg", flush=True)
