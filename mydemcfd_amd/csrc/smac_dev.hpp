@@ -281,12 +281,11 @@ struct St<float> {
   static __device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
   static __device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
 };
-// float -> bf16 bits, round to nearest even (inf stays inf, NaN stays NaN)
+// float -> bf16 bits, round to nearest even (inf stays inf, NaN stays NaN):
+// the language conversion, which gfx950 does in one instruction
+// (v_cvt_pk_bf16_f32) instead of an integer rounding sequence
 __device__ __forceinline__ unsigned bf16_bits(float v) {
-  unsigned u = __float_as_uint(v);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
+  return (unsigned)__builtin_bit_cast(unsigned short, (__bf16)v);
 }
 template <>
 struct St<bf16s> {

@@ -337,6 +337,112 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
   }
 }
 
+// a / b in the smoother: IEEE division in FP64 (the oracle's arithmetic),
+// reciprocal times a in FP32 (within 1.5 ulp; the mixed forms store the
+// result in FP32 or BF16 and their V-cycle is a preconditioner only)
+__device__ __forceinline__ double sm_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float sm_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// Half sweep with two rows per wave for rows of 64 CPL cells, CPL = the
+// elements of one 16-byte vector (BF16: 512-cell rows, FP32: 256, FP64: 128):
+// lanes 0-31 and 32-63 each own a row and lane l of a half the cells
+// h = 1 + CPL*l .. CPL*(l+1) of the half row, so every access is one 16-byte
+// vector even where a full wave per row would move 8 bytes per lane. The four
+// waves of a block own rows j0..j0+7 of one plane (ny % 8 == 0) and share
+// them, with rows j0-1 and j0+8, through 10 LDS rows for the y neighbours;
+// the x neighbour across a lane edge is one rotate shuffle within the 32
+// lanes of the row. The arithmetic per cell is that of k_mg_rbgs.
+template <typename T, bool VB, int CPL>
+__global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
+                                                    int zero_start) {
+  using C = typename St<T>::C;
+  constexpr int NH = 32 * CPL;  // cells per half row
+  __shared__ C ysm[10 * NH];
+  const int lane = threadIdx.x & 63;
+  const int w = (int)(threadIdx.x >> 6);
+  const int sub = lane >> 5, l = lane & 31;
+  const int r8 = 2 * w + sub;  // row of the block, 0..7
+  const int vrow = xcd_block((int)blockIdx.x, (int)gridDim.x) * 8 + r8;
+  int j, k;
+  band_jk(vrow, L.n[1], L.n[2], j, k);
+  const size_t rb = lrow(L, j, k);
+  const int nx = L.n[0], nE = nx >> 1;
+  const int par = (col + j + k) & 1;  // parity of the active i
+  const int off = par ? nE : 0;
+  const int oo = par ? 0 : nE;  // the other half
+  const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
+  const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+  const int p = 1 + CPL * l;
+  const bool perx = L.per[0] != 0;
+  // uniform coefficients: every cell of the row off the x walls has those of
+  // cell i = 2 (computed once per row, the same values cell by cell)
+  const CoefC<C> crow = level_coef<C, false>(L, 2, j, k);
+  auto coef = [&](int i) -> CoefC<C> {
+    if (VB || (!perx && (i == 1 || i == nx))) return level_coef<C, VB>(L, i, j, k);
+    return crow;
+  };
+  C fv[CPL];
+  Vec<T, CPL>::ld(f + rb + off + p, fv);
+  if (zero_start) {
+    C ev[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; t++) {
+      const int h = p + t;
+      const int i = par ? 2 * h - 1 : 2 * h;
+      CoefC<C> c = coef(i);
+      ev[t] = sm_div(fv[t], c.diag);
+    }
+    Vec<T, CPL>::st(e + rb + off + p, ev);
+    return;
+  }
+  C ov[CPL], yp[CPL], ym[CPL], zp[CPL], zm[CPL], hv[CPL];
+  const bool lo = r8 == 0, hi = r8 == 7;  // rows that also load the halo rows j0-1 and j0+8
+  Vec<T, CPL>::ld(e + rb + oo + p, ov);
+  Vec<T, CPL>::ld(e + zpr + off + p, zp);
+  Vec<T, CPL>::ld(e + zmr + off + p, zm);
+  if (lo || hi) Vec<T, CPL>::ld(e + (lo ? ymr : ypr) + off + p, hv);
+#pragma unroll
+  for (int t = 0; t < CPL; t++) {
+    const int o = CPL * l + t;
+    ysm[(r8 + 1) * NH + o] = ov[t];
+    if (lo) ysm[o] = hv[t];
+    if (hi) ysm[9 * NH + o] = hv[t];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < CPL; t++) {
+    const int o = CPL * l + t;
+    ym[t] = ysm[r8 * NH + o];
+    yp[t] = ysm[(r8 + 2) * NH + o];
+  }
+  // the x neighbour across the lane edge: for even i the other half's first
+  // cell of the next lane (lane 31 of the row: lane 0, i = nx wraps to i = 1),
+  // for odd i its last cell of the previous lane
+  C nb;
+  if (!par) nb = __shfl(ov[0], (sub << 5) | ((l + 1) & 31));
+  else nb = __shfl(ov[CPL - 1], (sub << 5) | ((l + 31) & 31));
+  C ev[CPL];
+#pragma unroll
+  for (int t = 0; t < CPL; t++) {
+    const int h = p + t;
+    const int i = par ? 2 * h - 1 : 2 * h;
+    C xp, xm;
+    if (!par) {
+      xm = ov[t];
+      xp = t + 1 < CPL ? ov[t + 1 < CPL ? t + 1 : 0] : nb;
+      if (!perx && i == nx) xp = (C)0;
+    } else {
+      xp = ov[t];
+      xm = t > 0 ? ov[t > 0 ? t - 1 : 0] : nb;
+      if (!perx && i == 1) xm = (C)0;
+    }
+    CoefC<C> c = coef(i);
+    C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[t] + c.a[3] * ym[t] + c.a[4] * zp[t] + c.a[5] * zm[t];
+    ev[t] = sm_div(fv[t] + S, c.diag);
+  }
+  Vec<T, CPL>::st(e + rb + off + p, ev);
+}
+
 // 1-D restriction stencil of coarse index I (transpose of cell-centred linear
 // prolongation; at a wall the mirrored weight folds onto the boundary cell)
 __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], double w[4]) {
@@ -432,6 +538,8 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
         }
       const bool perx = L.per[0] != 0;
       const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
+      // uniform coefficients: the cells off the x walls share those of i = 2
+      const CoefC<C> crow = level_coef<C, false>(L, 2, j, k);
 #pragma unroll
       for (int m = 0; m < NM; m++) {
         // the x neighbour across the chunk boundary (as in k_mg_rbgs_vec)
@@ -457,7 +565,8 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
             xm = t > 0 ? ov[m][t > 0 ? t - 1 : 0] : nb;
             if (!perx && i == 1) xm = (C)0;
           }
-          CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+          const CoefC<C> c =
+              (VB || (!perx && (i == 1 || i == nx))) ? level_coef<C, VB>(L, i, j, k) : crow;
           C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] +
                 c.a[5] * zm[m][t];
           buf[i] = ef[m][t] - (c.diag * ec[m][t] - S);
@@ -703,6 +812,17 @@ __global__ __launch_bounds__(256) void k_mg_prolong_lds(Lv Cl, Lv F, const T* __
       if (q <= NC) cs[r * NC + (q <= NC / 2 ? 2 * q : 2 * (q - NC / 2) - 1) - 1] = stg[r][u];
     }
   __syncthreads();
+  // the eight weights wz*wy*wx are the same for every cell of the row (wx is
+  // 3/4 on the cell's own coarse column and 1/4 on the neighbour, walls only
+  // clamp the index): products formed once, in the per-cell order
+  C wq[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      wq[a][b][0] = (C)(wz[a] * wy[b] * 0.75);
+      wq[a][b][1] = (C)(wz[a] * wy[b] * 0.25);
+    }
 #pragma unroll
   for (int mm = 0; mm < NM; mm++) {
     C out[CPL];
@@ -720,10 +840,8 @@ __global__ __launch_bounds__(256) void k_mg_prolong_lds(Lv Cl, Lv F, const T* __
 #pragma unroll
         for (int b = 0; b < 2; b++) {
           const C* rowp = cs + (a * 4 + sy[b]) * NC;
-          C w0 = (C)(wz[a] * wy[b] * wx[0]);
-          acc += w0 * rowp[ix[0] - 1];
-          C w1 = (C)(wz[a] * wy[b] * wx[1]);
-          acc += w1 * rowp[ix[1] - 1];
+          acc += wq[a][b][0] * rowp[ix[0] - 1];
+          acc += wq[a][b][1] * rowp[ix[1] - 1];
         }
       out[t] = fv[mm * CPL + t] + acc;
     }
@@ -802,6 +920,13 @@ void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipS
   const int vpl = row_vpl(half);
   const bool fast = vpl >= 1 && L.n[0] == 128 * vpl;
   const bool ldsy = fast && L.n[1] % 4 == 0;
+  constexpr int C16 = (int)(16 / sizeof(T));  // elements of a 16-byte vector
+  if (L.n[0] == 64 * C16 && L.n[1] % 8 == 0 && rows_aligned(L, e) && rows_aligned(L, f)) {
+    const dim3 grid((unsigned)(L.n[1] * L.n[2] / 8));
+    if (L.b[0]) hipLaunchKernelGGL((k_mg_rbgs_v2<T, true, C16>), grid, ROWB, 0, st, L, e, f, col, zero_start);
+    else hipLaunchKernelGGL((k_mg_rbgs_v2<T, false, C16>), grid, ROWB, 0, st, L, e, f, col, zero_start);
+    return;
+  }
   if (fast && vpl >= 2 && rows_aligned(L, e) && rows_aligned(L, f)) {
 #define SMAC_RV(V, P, Y) \
   hipLaunchKernelGGL((k_mg_rbgs_vec<T, P, V, vec_cpl<T>(P), Y>), lrows_grid(L), ROWB, 0, st, L, e, f, col, zero_start)
