@@ -612,13 +612,29 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
   __syncthreads();
   if (!valid) return;
   const size_t tr = ((size_t)k * (size_t)L.py + (size_t)j) * (size_t)(Cl.n[0] + 3);
-  for (int I = 1 + lane; I <= Cl.n[0]; I += 64) {
-    int ix[4];
-    double wx[4];
-    int mx = rweights(Cl.per[0], Cl.n[0], I, ix, wx);
-    C tx = 0;
-    for (int c = 0; c < mx; c++) tx += (C)wx[c] * buf[ix[c]];
-    tb[tr + I] = tx;
+  if (Cl.per[0]) {
+    // periodic: the four taps of rweights with its constant weights, in its
+    // order (only the end taps wrap)
+    const int nf = L.n[0];
+    const C q = (C)0.25, h = (C)0.75;
+    for (int I = 1 + lane; I <= Cl.n[0]; I += 64) {
+      const int i0 = I == 1 ? nf : 2 * I - 2, i3 = 2 * I + 1 > nf ? 1 : 2 * I + 1;
+      C tx = 0;
+      tx += q * buf[i0];
+      tx += h * buf[2 * I - 1];
+      tx += h * buf[2 * I];
+      tx += q * buf[i3];
+      tb[tr + I] = tx;
+    }
+  } else {
+    for (int I = 1 + lane; I <= Cl.n[0]; I += 64) {
+      int ix[4];
+      double wx[4];
+      int mx = rweights(Cl.per[0], Cl.n[0], I, ix, wx);
+      C tx = 0;
+      for (int c = 0; c < mx; c++) tx += (C)wx[c] * buf[ix[c]];
+      tb[tr + I] = tx;
+    }
   }
 }
 
