@@ -898,6 +898,8 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
     }
     const bool perx = g.per[0] != 0;
     const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
+    // uniform coefficients: the cells off the x walls share those of i = 2
+    const Coef crow = coef_uniform(g.n, g.per, g.inv_d2, g.beta0, 2, j, k);
 #pragma unroll
     for (int m = 0; m < NM; m++) {
       // the x neighbour across the pair edge: for an even i (first half) the
@@ -927,7 +929,8 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
           xm = t == 1 ? xc[m - NH][0] : nb;
           if (!perx && i == 1) xm = 0.0;
         }
-        Coef c = VAR ? coef_var(g, bf.b, i, j, k) : coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k);
+        const Coef c = VAR ? coef_var(g, bf.b, i, j, k)
+                           : ((!perx && (i == 1 || i == nx)) ? coef_uniform(g.n, g.per, g.inv_d2, g.beta0, i, j, k) : crow);
         double s = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] +
                    c.a[5] * zm[m][t];
         double v = c.diag * xc[m][t] - s;

@@ -239,6 +239,13 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
   const int oo = par ? 0 : nE;  // the other half
   const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
   const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+  const bool perx = L.per[0] != 0;
+  // uniform coefficients: the cells off the x walls share those of i = 2
+  const CoefC<C> crow = level_coef<C, false>(L, 2, j, k);
+  auto coef = [&](int i) -> CoefC<C> {
+    if (VB || (!perx && (i == 1 || i == nx))) return level_coef<C, VB>(L, i, j, k);
+    return crow;
+  };
   C fv[NM][CPL];
 #pragma unroll
   for (int m = 0; m < NM; m++) Vec<T, CPL>::ld(f + rb + off + 1 + CPL * lane + 64 * CPL * m, fv[m]);
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
       for (int t = 0; t < CPL; t++) {
         const int h = 1 + CPL * lane + 64 * CPL * m + t;
         const int i = par ? 2 * h - 1 : 2 * h;
-        CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+        CoefC<C> c = coef(i);
         ev[t] = fv[m][t] / c.diag;
       }
       Vec<T, CPL>::st(e + rb + off + 1 + CPL * lane + 64 * CPL * m, ev);
@@ -299,7 +306,6 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
       Vec<T, CPL>::ld(e + zmr + off + p, zm[m]);
     }
   }
-  const bool perx = L.per[0] != 0;
   const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
 #pragma unroll
   for (int m = 0; m < NM; m++) {
@@ -329,7 +335,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
         xm = t > 0 ? ov[m][t > 0 ? t - 1 : 0] : nb;
         if (!perx && i == 1) xm = (C)0;
       }
-      CoefC<C> c = level_coef<C, VB>(L, i, j, k);
+      CoefC<C> c = coef(i);
       C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] + c.a[5] * zm[m][t];
       ev[t] = (fv[m][t] + S) / c.diag;
     }
