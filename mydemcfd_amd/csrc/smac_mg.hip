@@ -349,26 +349,29 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
 __device__ __forceinline__ double sm_div(double a, double b) { return a / b; }
 __device__ __forceinline__ float sm_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
-// Half sweep with two rows per wave for rows of 64 CPL cells, CPL = the
-// elements of one 16-byte vector (BF16: 512-cell rows, FP32: 256, FP64: 128):
-// lanes 0-31 and 32-63 each own a row and lane l of a half the cells
-// h = 1 + CPL*l .. CPL*(l+1) of the half row, so every access is one 16-byte
-// vector even where a full wave per row would move 8 bytes per lane. The four
-// waves of a block own rows j0..j0+7 of one plane (ny % 8 == 0) and share
-// them, with rows j0-1 and j0+8, through 10 LDS rows for the y neighbours;
-// the x neighbour across a lane edge is one rotate shuffle within the 32
-// lanes of the row. The arithmetic per cell is that of k_mg_rbgs.
-template <typename T, bool VB, int CPL>
+// Half sweep with RPW = 2 or 4 rows per wave for rows of 128 CPL / RPW cells,
+// CPL = the elements of one 16-byte vector (RPW 2: BF16 512-cell rows, FP32
+// 256, FP64 128; RPW 4: half those): each group of LPR = 64 / RPW lanes owns a
+// row and lane l of a group the cells h = 1 + CPL*l .. CPL*(l+1) of the half
+// row, so every access is one 16-byte vector even where a full wave per row
+// would move 8 bytes or less per lane. The four waves of a block own rows
+// j0..j0+4 RPW-1 of one plane (ny % (4 RPW) == 0) and share them, with the
+// rows just below and above, through LDS for the y neighbours; the x
+// neighbour across a lane edge is one rotate shuffle within the row's lanes.
+// The arithmetic per cell is that of k_mg_rbgs.
+template <typename T, bool VB, int CPL, int RPW>
 __global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, const T* __restrict__ f, int col,
                                                     int zero_start) {
   using C = typename St<T>::C;
-  constexpr int NH = 32 * CPL;  // cells per half row
-  __shared__ C ysm[10 * NH];
+  constexpr int LPR = 64 / RPW;     // lanes per row
+  constexpr int RB = 4 * RPW;       // rows per block
+  constexpr int NH = LPR * CPL;     // cells per half row
+  __shared__ C ysm[(RB + 2) * NH];
   const int lane = threadIdx.x & 63;
   const int w = (int)(threadIdx.x >> 6);
-  const int sub = lane >> 5, l = lane & 31;
-  const int r8 = 2 * w + sub;  // row of the block, 0..7
-  const int vrow = xcd_block((int)blockIdx.x, (int)gridDim.x) * 8 + r8;
+  const int sub = lane / LPR, l = lane % LPR;
+  const int r8 = RPW * w + sub;  // row of the block, 0..RB-1
+  const int vrow = xcd_block((int)blockIdx.x, (int)gridDim.x) * RB + r8;
   int j, k;
   band_jk(vrow, L.n[1], L.n[2], j, k);
   const size_t rb = lrow(L, j, k);
@@ -402,7 +405,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, con
     return;
   }
   C ov[CPL], yp[CPL], ym[CPL], zp[CPL], zm[CPL], hv[CPL];
-  const bool lo = r8 == 0, hi = r8 == 7;  // rows that also load the halo rows j0-1 and j0+8
+  const bool lo = r8 == 0, hi = r8 == RB - 1;  // rows that also load the halo rows j0-1 and j0+RB
   Vec<T, CPL>::ld(e + rb + oo + p, ov);
   Vec<T, CPL>::ld(e + zpr + off + p, zp);
   Vec<T, CPL>::ld(e + zmr + off + p, zm);
@@ -412,7 +415,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, con
     const int o = CPL * l + t;
     ysm[(r8 + 1) * NH + o] = ov[t];
     if (lo) ysm[o] = hv[t];
-    if (hi) ysm[9 * NH + o] = hv[t];
+    if (hi) ysm[(RB + 1) * NH + o] = hv[t];
   }
   __syncthreads();
 #pragma unroll
@@ -422,11 +425,11 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, con
     yp[t] = ysm[(r8 + 2) * NH + o];
   }
   // the x neighbour across the lane edge: for even i the other half's first
-  // cell of the next lane (lane 31 of the row: lane 0, i = nx wraps to i = 1),
-  // for odd i its last cell of the previous lane
+  // cell of the next lane (the row's last lane: its first, i = nx wraps to
+  // i = 1), for odd i its last cell of the previous lane
   C nb;
-  if (!par) nb = __shfl(ov[0], (sub << 5) | ((l + 1) & 31));
-  else nb = __shfl(ov[CPL - 1], (sub << 5) | ((l + 31) & 31));
+  if (!par) nb = __shfl(ov[0], sub * LPR + (l + 1) % LPR);
+  else nb = __shfl(ov[CPL - 1], sub * LPR + (l + LPR - 1) % LPR);
   C ev[CPL];
 #pragma unroll
   for (int t = 0; t < CPL; t++) {
@@ -943,10 +946,17 @@ void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipS
   const bool fast = vpl >= 1 && L.n[0] == 128 * vpl;
   const bool ldsy = fast && L.n[1] % 4 == 0;
   constexpr int C16 = (int)(16 / sizeof(T));  // elements of a 16-byte vector
-  if (L.n[0] == 64 * C16 && L.n[1] % 8 == 0 && rows_aligned(L, e) && rows_aligned(L, f)) {
+  const bool al = rows_aligned(L, e) && rows_aligned(L, f);
+  if (al && L.n[0] == 64 * C16 && L.n[1] % 8 == 0) {
     const dim3 grid((unsigned)(L.n[1] * L.n[2] / 8));
-    if (L.b[0]) hipLaunchKernelGGL((k_mg_rbgs_v2<T, true, C16>), grid, ROWB, 0, st, L, e, f, col, zero_start);
-    else hipLaunchKernelGGL((k_mg_rbgs_v2<T, false, C16>), grid, ROWB, 0, st, L, e, f, col, zero_start);
+    if (L.b[0]) hipLaunchKernelGGL((k_mg_rbgs_v2<T, true, C16, 2>), grid, ROWB, 0, st, L, e, f, col, zero_start);
+    else hipLaunchKernelGGL((k_mg_rbgs_v2<T, false, C16, 2>), grid, ROWB, 0, st, L, e, f, col, zero_start);
+    return;
+  }
+  if (al && L.n[0] == 32 * C16 && L.n[1] % 16 == 0) {
+    const dim3 grid((unsigned)(L.n[1] * L.n[2] / 16));
+    if (L.b[0]) hipLaunchKernelGGL((k_mg_rbgs_v2<T, true, C16, 4>), grid, ROWB, 0, st, L, e, f, col, zero_start);
+    else hipLaunchKernelGGL((k_mg_rbgs_v2<T, false, C16, 4>), grid, ROWB, 0, st, L, e, f, col, zero_start);
     return;
   }
   if (fast && vpl >= 2 && rows_aligned(L, e) && rows_aligned(L, f)) {
