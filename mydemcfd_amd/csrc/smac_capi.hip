@@ -1212,17 +1212,18 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     launch_finalize(fa, c->S, s);
   }
   double size_inv = 1. / Ncell;
-  if (pure) launch_add_scalar<double>(g, c->r, -1. * size_inv, c->S, S_SUMR, s);
-  if (f0) launch_cvt<T>(g, c->r, f0, s);
+  // r -= mean(r) (pure Neumann), then f0 = (T) r: one pass when both apply
+  if (f0) launch_cvt<T>(g, c->r, f0, s, -1. * size_inv, c->S, pure ? S_SUMR : -1);
+  else if (pure) launch_add_scalar<double>(g, c->r, -1. * size_inv, c->S, S_SUMR, s);
   precond();
   if (pure) {
     launch_sum<T>(g, z, P[0], s);
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_SUMZ);
     launch_finalize(fa, c->S, s);
-    launch_add_scalar<T>(g, z, -1. * size_inv, c->S, S_SUMZ, s);
   }
-  launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s);
+  // z -= mean(z) (pure Neumann) in the same pass as d = z and r.z
+  launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s, -1. * size_inv, c->S, pure ? S_SUMZ : -1);
   {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_RZ);

@@ -164,7 +164,9 @@ void launch_sum(const Geo& g, const Z* x, double* p0, hipStream_t st);
 // bf == nullptr: one-phase operator (beta0); else face coefficients bf->b
 void launch_jacobi(const Geo& g, const BetaF* bf, const double* r, double* z, hipStream_t st);
 template <typename Z>
-void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st);
+// d = z, part0 <- sum r*z (natural order); slot >= 0: z += a S[slot] first
+void launch_copy_dot(const Geo& g, Z* z, double* d, const double* r, double* p0, hipStream_t st, double a = 0.0,
+                     const double* S = nullptr, int slot = -1);
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st);
 template <typename F>
 void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
@@ -177,7 +179,9 @@ template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* S, int slot_alpha, int slot_beta,
                  hipStream_t st);
 template <typename F>
-void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st);
+// f0 = (F) r; slot >= 0: r += a S[slot] first
+void launch_cvt(const Geo& g, double* r, F* f0, hipStream_t st, double a = 0.0, const double* S = nullptr,
+                int slot = -1);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
 void launch_scalar_set(double* S, int dst, double v, hipStream_t st);

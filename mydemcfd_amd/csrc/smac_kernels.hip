@@ -801,12 +801,17 @@ __global__ __launch_bounds__(256) void k_jacobi(Geo g, BetaF bf, const double* _
 
 // d = z; part0 <- sum r*z
 template <typename Z>
-__global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z, double* __restrict__ d,
-                                                  const double* __restrict__ r, double* __restrict__ part0) {
+__global__ __launch_bounds__(256) void k_copy_dot(Geo g, Z* __restrict__ z, double* __restrict__ d,
+                                                  const double* __restrict__ r, double* __restrict__ part0,
+                                                  double a, const double* __restrict__ S, int slot) {
   ROW_SETUP
   double acc = 0.0;
+  const double s = slot >= 0 ? S[slot] : 0.0;
   for (int i = 1 + lane; i <= g.n[0]; i += 64) {
     const size_t o = rb + spos(i, g.n[0]);
+    // slot >= 0: z += a S[slot] first (k_add_scalar's arithmetic), then the
+    // stored value is read back, as a separate pass would read it
+    if (slot >= 0) St<Z>::st(z, o, (typename St<Z>::C)((double)St<Z>::ld(z, o) + a * s));
     double zz = (double)St<Z>::ld(z, o);
     d[o] = zz;
     acc += r[o] * zz;
@@ -1114,10 +1119,20 @@ __global__ __launch_bounds__(256) void k_xpay(Geo g, const Z* __restrict__ z, do
 }
 
 // f0 = (F) r on interior cells (fine-level rhs of the mixed V-cycle)
+// slot >= 0: r += a S[slot] first (k_add_scalar's arithmetic, one pass for both)
 template <typename F>
-__global__ __launch_bounds__(256) void k_cvt(Geo g, const double* __restrict__ r, F* __restrict__ f0) {
+__global__ __launch_bounds__(256) void k_cvt(Geo g, double* __restrict__ r, F* __restrict__ f0, double a,
+                                             const double* __restrict__ S, int slot) {
   ROW_SETUP
-  for (int p = 1 + lane; p <= g.n[0]; p += 64) St<F>::st(f0, rb + p, (typename St<F>::C)r[rb + p]);
+  const double s = slot >= 0 ? S[slot] : 0.0;
+  for (int p = 1 + lane; p <= g.n[0]; p += 64) {
+    double v = r[rb + p];
+    if (slot >= 0) {
+      v = v + a * s;
+      r[rb + p] = v;
+    }
+    St<F>::st(f0, rb + p, (typename St<F>::C)v);
+  }
 }
 
 // --------------------------------------------------------------- finalize
@@ -1468,8 +1483,9 @@ void launch_jacobi(const Geo& g, const BetaF* bf, const double* r, double* z, hi
   else hipLaunchKernelGGL(k_jacobi<false>, rows_grid(g), ROWB, 0, st, g, BetaF{}, r, z);
 }
 template <typename Z>
-void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st) {
-  hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0);
+void launch_copy_dot(const Geo& g, Z* z, double* d, const double* r, double* p0, hipStream_t st, double a,
+                     const double* S, int slot) {
+  hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0, a, S, slot);
 }
 // pair-order kernels: the vector form when a row is 128 NM cells (NM = 1..4)
 // and the split vectors start every row on a 128-byte line
@@ -1548,19 +1564,20 @@ void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* 
                                                slot_alpha, slot_beta))
 }
 template <typename F>
-void launch_cvt(const Geo& g, const double* r, F* f0, hipStream_t st) {
-  hipLaunchKernelGGL(k_cvt<F>, rows_grid(g), ROWB, 0, st, g, r, f0);
+void launch_cvt(const Geo& g, double* r, F* f0, hipStream_t st, double a, const double* S, int slot) {
+  hipLaunchKernelGGL(k_cvt<F>, rows_grid(g), ROWB, 0, st, g, r, f0, a, S, slot);
 }
 #define SMAC_INST_Z(Z)                                                                                     \
   template void launch_add_scalar<Z>(const Geo&, Z*, double, const double*, int, hipStream_t);            \
   template void launch_sum<Z>(const Geo&, const Z*, double*, hipStream_t);                                \
-  template void launch_copy_dot<Z>(const Geo&, const Z*, double*, const double*, double*, hipStream_t);   \
+  template void launch_copy_dot<Z>(const Geo&, Z*, double*, const double*, double*, hipStream_t, double,  \
+                                   const double*, int);                                                   \
   template void launch_update<Z>(const Geo&, double*, const double*, const double*, int, double*, double*, \
                                  Z*, hipStream_t);                                                        \
   template void launch_dot2<Z>(const Geo&, const double*, const Z*, const double*, int, double*, double*,  \
                                hipStream_t);                                                              \
   template void launch_xpay<Z>(const Geo&, const Z*, double*, double*, const double*, int, int, hipStream_t); \
-  template void launch_cvt<Z>(const Geo&, const double*, Z*, hipStream_t);
+  template void launch_cvt<Z>(const Geo&, double*, Z*, hipStream_t, double, const double*, int);
 SMAC_INST_Z(double)
 SMAC_INST_Z(float)
 SMAC_INST_Z(bf16s)
