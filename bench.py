@@ -22,7 +22,8 @@ width of the V-cycle):
          spmv_dot 16, update 24 (r, Ad; +W_s: the mixed forms also write
          (T) r), dot2 8 + W_z (+8 flexible), xpay 32 + W_z (z, d, dp: the
          dp += alpha d of the previous update rides on the direction update),
-         dp_flush 24 (that dp update after the last iteration),
+         dp_flush 24 (that dp update after the last iteration; a step
+         leaves it to the correction, which then also reads d: +8 there),
          mg_rbgs_l0 1.5*W_s (a half sweep), mg_restrict_l0 2.125*W_s
          (residual + restriction), mg_prolong_l0 1.125*W_s (red cells),
          mg_fill_l0 W_s, predict 56, rhs 32, correct 80 (u* and dp read, u
@@ -58,7 +59,7 @@ KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (
     "mg_fill_l0": lambda W, flex: 1.0 * W,
     "predict": lambda W, flex: 56.0,
     "rhs": lambda W, flex: 32.0,
-    "correct": lambda W, flex: 80.0,
+    "correct": lambda W, flex: 88.0,
 }
 
 FORMS = {

@@ -66,6 +66,9 @@ struct smac_ctx {
   double* us[3] = {nullptr, nullptr, nullptr};
   double *p = nullptr, *dp = nullptr, *rhs = nullptr, *alpha = nullptr;
   double *r = nullptr, *z = nullptr, *dv = nullptr, *Ad = nullptr, *res0 = nullptr;
+  // smac_step lets the solve leave its last dp += alpha d to the correction
+  // (defer_dp), which then adds it cell by cell (dp_deferred)
+  bool defer_dp = false, dp_deferred = false;
   double* part[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int nrows = 0;
   double* S = nullptr;
@@ -1323,7 +1326,10 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     if (!stat) launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }
   // the last update's dp += alpha d (S_ALPHA still holds its alpha)
-  if (dp_pending) KT(c, "dp_flush", launch_dp_flush(g, c->dp, c->dv, c->S, S_ALPHA, s));
+  if (dp_pending) {
+    if (c->defer_dp) c->dp_deferred = true;
+    else KT(c, "dp_flush", launch_dp_flush(g, c->dp, c->dv, c->S, S_ALPHA, s));
+  }
   HIPOK(hipGetLastError());
   if (st) {
     memset(st, 0, sizeof(*st));
@@ -1398,7 +1404,9 @@ extern "C" int smac_correct(smac_ctx* c, double dt) {
   int rc = ensure_props(c);
   if (rc) return rc;
   const BetaF bf = beta_of(c);
-  KT(c, "correct", launch_correct_all(c->g, var_props(c) ? &bf : nullptr, c->us, c->dp, c->u, c->p, dt, c->st));
+  KT(c, "correct", launch_correct_all(c->g, var_props(c) ? &bf : nullptr, c->us, c->dp, c->u, c->p, dt, c->st,
+                                      c->dv, c->S, c->dp_deferred ? S_ALPHA : -1));
+  c->dp_deferred = false;
   HIPOK(hipGetLastError());
   return SMAC_OK;
 }
@@ -1418,11 +1426,23 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
   if (rc) return rc;
   double t_pred = prof_end(c, 0);
   prof_begin(c, 2);
+  c->defer_dp = true;
   rc = smac_pressure_solve(c, dt, o, sp);
-  if (rc) return rc;
+  c->defer_dp = false;
+  if (rc) {
+    if (c->dp_deferred) {  // leave dp complete on an error return
+      launch_dp_flush(c->g, c->dp, c->dv, c->S, S_ALPHA, c->st);
+      c->dp_deferred = false;
+    }
+    return rc;
+  }
   double t_solve = prof_end(c, 2);
   prof_begin(c, 4);
   rc = smac_correct(c, dt);
+  if (c->dp_deferred) {  // the correction returned early: leave dp complete
+    launch_dp_flush(c->g, c->dp, c->dv, c->S, S_ALPHA, c->st);
+    c->dp_deferred = false;
+  }
   double t_corr = prof_end(c, 4);
   sp->ms_predictor = t_pred;
   sp->ms_solve = t_solve;

@@ -190,7 +190,8 @@ void launch_props(const Geo& g, const double* alpha, double* rho, double* rho_ol
                   hipStream_t st);
 // the correction of all components, p += dp, then dp = 0 (one pass + a clear)
 void launch_correct_all(const Geo& g, const BetaF* bf, double* const us[3], double* dp, double* u[3], double* p,
-                        double dt, hipStream_t st);
+                        double dt, hipStream_t st, const double* dv = nullptr,
+                        const double* S = nullptr, int slot_alpha = -1);
 // inv_rho == nullptr: one-phase viscous CFL `visc`; else (mu/rho) dt sum 1/h^2 per cell
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, const double* inv_rho, const double* mu,
                 double* p0, double* p1, hipStream_t st);

@@ -1237,21 +1237,31 @@ __global__ __launch_bounds__(256) void k_correct_all(Geo g, BetaF bf, int var, c
                                                      const double* __restrict__ us1,
                                                      const double* __restrict__ us2, const double* __restrict__ dp,
                                                      double* __restrict__ u0, double* __restrict__ u1,
-                                                     double* __restrict__ u2, double* __restrict__ p, double dt) {
+                                                     double* __restrict__ u2, double* __restrict__ p, double dt,
+                                                     const double* __restrict__ dv, const double* __restrict__ S,
+                                                     int slot_alpha) {
   const int i = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
   const int j = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
   const int k = 1 + (int)blockIdx.z;
   if (i > g.n[0] || j > g.n[1]) return;
   const I3 q = mk(i, j, k);
   const size_t id = IX(g, q);
-  const double d0 = dp[id];
+  // slot_alpha >= 0: the solve's last dp += alpha d (d split, per row) is
+  // still pending and is added here, cell by cell, with the flush's arithmetic
+  const double alpha = slot_alpha >= 0 ? S[slot_alpha] : 0.0;
+  auto dpv = [&](const I3& c, size_t ic) {
+    double v = dp[ic];
+    if (slot_alpha >= 0) v = v + alpha * dv[ic - (size_t)c.v[0] + (size_t)spos(c.v[0], g.n[0])];
+    return v;
+  };
+  const double d0 = dpv(q, id);
   const double* const us[3] = {us0, us1, us2};
   double* const u[3] = {u0, u1, u2};
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     if (!g.per[a] && q.v[a] < 2) continue;  // the wall face 1 is not interior
     const I3 cm = wr(g, sh(q, a, -1));
-    const double gr = (d0 - dp[IX(g, cm)]) * g.inv_d[a];
+    const double gr = (d0 - dpv(cm, IX(g, cm))) * g.inv_d[a];
     const double ir = var ? bf.b[a][id] : g.beta0;  // 1/rho_f on interior faces
     u[a][id] = us[a][id] + ir * dt * (0.0 - gr);
   }
@@ -1599,10 +1609,10 @@ void launch_scalar_set(double* S, int dst, double v, hipStream_t st) {
   hipLaunchKernelGGL(k_scalar_set, dim3(1), dim3(1), 0, st, S, dst, v);
 }
 void launch_correct_all(const Geo& g, const BetaF* bf, double* const us[3], double* dp, double* u[3], double* p,
-                        double dt, hipStream_t st) {
+                        double dt, hipStream_t st, const double* dv, const double* S, int slot_alpha) {
   dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4), (unsigned)g.n[2]);
   hipLaunchKernelGGL(k_correct_all, grid, CELLB, 0, st, g, bf ? *bf : BetaF{}, bf ? 1 : 0, us[0], us[1], us[2], dp,
-                     u[0], u[1], u[2], p, dt);
+                     u[0], u[1], u[2], p, dt, dv, S, slot_alpha);
   hipLaunchKernelGGL(k_zero_dp, rows_grid(g), ROWB, 0, st, g, dp);
 }
 void launch_cfl(const Geo& g, const Vel& V, double dt, double visc, const double* inv_rho, const double* mu,
