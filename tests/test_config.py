@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import config_ref
+from cpp import drivers as cpp_drivers
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
@@ -21,11 +22,7 @@ REF_CONFIG = "/root/reference/CFD/config.json"
 
 @pytest.fixture(scope="module")
 def tool():
-    from mydemcfd_amd import build
-
-    if not os.path.exists(build.CONFIG_TOOL):
-        build.build_config_tool()
-    return build.CONFIG_TOOL
+    return cpp_drivers.config_tool()
 
 
 def _run(tool, cfg, tmp_path):

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_err, rel_err_mod_mean
+from cpp import drivers as cpp_drivers
 
 pytestmark = pytest.mark.gpu
 
@@ -216,13 +217,12 @@ def test_cpp_facade_driver(hip, oracle_mod, tmp_path, mode):
     import os
     import subprocess
 
-    from mydemcfd_amd import build, cases
+    from mydemcfd_amd import cases
 
-    if not os.path.exists(build.DRIVER):
-        pytest.skip("C++ facade driver not built")
+    driver = cpp_drivers.facade_driver()
     n = 24
     print(f"case: C++ facade cavity {n}^3 x5 {mode}", flush=True)
-    args = [build.DRIVER, str(n), "5", str(tmp_path), mode]
+    args = [driver, str(n), "5", str(tmp_path), mode]
     out = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     o = oracle_mod.OracleContext(**cases.cavity_desc(n, re=100.0))
@@ -580,14 +580,13 @@ def test_config_driver_dam_break(hip, oracle_mod, tmp_path):
     import subprocess
 
     import config_ref
-    from mydemcfd_amd import build, cfdtime
+    from mydemcfd_amd import cfdtime
 
-    if not os.path.exists(build.CONFIG_DRIVER):
-        pytest.skip("C++ config driver not built")
+    driver = cpp_drivers.config_driver()
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dam_break_config.json")
     steps = 8
     print(f"case: config driver dam break 32x16x16 x{steps}", flush=True)
-    out = subprocess.run([build.CONFIG_DRIVER, path, str(tmp_path), str(steps)], capture_output=True, text=True,
+    out = subprocess.run([driver, path, str(tmp_path), str(steps)], capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 0, out.stderr
     rows = [ln.split() for ln in out.stdout.splitlines() if ln.startswith("step ")]
