@@ -7,11 +7,18 @@ field. One step = predictor -> RHS -> MG-PCG (RBGS V(2,2) preconditioner,
 ||r||_2/||b||_2 < 1e-8) -> correction. Inputs are resident in HBM before the
 timed region; each form is timed over --steps steps after --warmup steps.
 
-Prints ONE JSON line. `roofline` is for the dominant kernel (largest total
-device time in the timed region, HIP events on the context stream), with
-algorithmic bytes per launch from the byte model below. `cpu_baseline` is the
-CPU FP64 oracle (a restatement of the reference algorithm; the reference's own
-CPU SMACSolver has no step) timed on this host on a bounded sample.
+Prints ONE JSON line. The headline (value, ms_per_step) is timed with kernel
+profiling off. A second pass of --prof-steps further steps records HIP events
+around every launch on the context stream (the stream the kernels run on);
+`roofline` is the dominant kernel of that pass (largest total device time) and
+`roofline_kernels` the six largest, each with algorithmic bytes per launch from
+the byte model below. `cpu_baseline` is the CPU FP64 oracle (a restatement of
+the reference's GPU algorithm; the reference's own CPU SMACSolver has no step),
+compiled on this host with -O3 -march=native -ffp-contract=off and timed here
+on a bounded sample: the reference algorithm (REF mode: reference GMG-PCG,
+weighted-Jacobi V(4,4) + 30 coarse sweeps, 4 levels) and the build's algorithm
+(NS mode: RBGS-MG-PCG V(2,2)), each 1 warm-up step then the median of 3 steps,
+at --cpu-n^3 of the same workload.
 
 Canonical byte model (bytes per interior cell, FP64 word = 8 B; W_s = storage
 width of the V-cycle):
@@ -24,7 +31,10 @@ width of the V-cycle):
          dp += alpha d of the previous update rides on the direction update),
          dp_flush 24 (that dp update after the last iteration; a step
          leaves it to the correction, which then also reads d: +8 there),
-         mg_rbgs_l0 1.5*W_s (a half sweep), mg_restrict_l0 2.125*W_s
+         mg_rbgs_l0 1.5*W_s (a half sweep: other colour, own rhs, own
+         colour written), mg_rbgs_l0_zs 1.0*W_s (the first red half sweep of
+         a V-cycle starts from zero: rhs read, red cells written),
+         mg_restrict_l0 2.125*W_s
          (residual + restriction), mg_prolong_l0 1.125*W_s (red cells),
          mg_fill_l0 W_s, predict 56, rhs 32, correct 80 (u* and dp read, u
          written, p += dp, then dp cleared)
@@ -54,6 +64,7 @@ KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (
     "xpay": lambda W, flex: 32.0 + W,
     "dp_flush": lambda W, flex: 24.0,
     "mg_rbgs_l0": lambda W, flex: 1.5 * W,
+    "mg_rbgs_l0_zs": lambda W, flex: 1.0 * W,
     "mg_restrict_l0": lambda W, flex: 2.125 * W,
     "mg_prolong_l0": lambda W, flex: 1.125 * W,
     "mg_fill_l0": lambda W, flex: 1.0 * W,
@@ -74,7 +85,7 @@ def step_bytes(W, n_it_per_step, flexible):
     return 160.0 + (17.43 * W + 32.0) + n_it_per_step * b_it
 
 
-def run_form(name, n, steps, warmup, fields, re, cfl, tol, levels):
+def run_form(name, n, steps, warmup, prof_steps, fields, re, cfl, tol, levels):
     import torch
 
     import mydemcfd_amd
@@ -94,8 +105,7 @@ def run_form(name, n, steps, warmup, fields, re, cfl, tol, levels):
         st = ctx.step(dt, opts)
         if not st.converged:
             raise RuntimeError(f"{name}: warmup step did not converge: {st.as_dict()}")
-    ctx.set_profiling(True)
-    ctx.reset_kernel_times()
+    ctx.set_profiling(False)
     ctx.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -111,6 +121,16 @@ def run_form(name, n, steps, warmup, fields, re, cfl, tol, levels):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
+    # kernel breakdown: a second pass with HIP events around every launch
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    prof_iters = []
+    t2 = time.perf_counter()
+    for _ in range(prof_steps):
+        st = ctx.step(dt, opts)
+        prof_iters.append(st.iters)
+    ctx.sync()
+    t3 = time.perf_counter()
     kt = {}
     for k in ctx.kernel_names():
         cnt, ms = ctx.kernel_time(k)
@@ -118,7 +138,8 @@ def run_form(name, n, steps, warmup, fields, re, cfl, tol, levels):
     ctx.set_profiling(False)
     out = {"form": name, "dt": dt, "wall_s": wall, "ms_per_step": 1e3 * wall / steps,
            "cell_steps_per_s": n ** 3 * steps / wall, "iters": iters, "rel_res_max": max(res),
-           "kernels": kt, "W": spec["W"], "flexible": spec["flexible"]}
+           "kernels": kt, "prof_steps": prof_steps, "prof_iters": prof_iters,
+           "prof_ms_per_step": 1e3 * (t3 - t2) / max(prof_steps, 1), "W": spec["W"], "flexible": spec["flexible"]}
     n_it = sum(iters) / len(iters)
     bstep = step_bytes(spec["W"], n_it, spec["flexible"])
     out["bytes_per_cell_step"] = bstep
@@ -171,46 +192,85 @@ def measured_traffic(n, form, kernel):
     return None
 
 
-def dominant_roofline(form, n):
-    best = None
-    for k, (cnt, ms) in form["kernels"].items():
-        if cnt == 0 or k not in KERNEL_BYTES:
-            continue
-        if best is None or ms > best[2]:
-            best = (k, cnt, ms)
-    if best is None:
-        return None
-    k, cnt, ms = best
+def kernel_roofline(form, n, k):
+    cnt, ms = form["kernels"][k]
     per_launch = KERNEL_BYTES[k](form["W"], form["flexible"]) * n ** 3
     avg_s = ms / cnt / 1e3
     achieved = per_launch / avg_s / 1e9
     return {"bound": "hbm", "kernel": k, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, form["form"], k),
-            "launches": cnt,
-            "avg_us": round(1e6 * avg_s, 2), "alg_bytes_per_launch": per_launch}
+            "launches": cnt, "avg_us": round(1e6 * avg_s, 2), "alg_bytes_per_launch": per_launch,
+            "share_of_step": round(ms / (form["prof_ms_per_step"] * form["prof_steps"]), 4)}
+
+
+def top_kernels(form, count):
+    """The `count` kernels of the byte model with the largest total device time."""
+    ks = [k for k, (cnt, ms) in form["kernels"].items() if cnt > 0 and k in KERNEL_BYTES]
+    ks.sort(key=lambda k: -form["kernels"][k][1])
+    return ks[:count]
+
+
+def dominant_roofline(form, n):
+    ks = top_kernels(form, 1)
+    return kernel_roofline(form, n, ks[0]) if ks else None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(n_cpu, re, cfl, tol):
-    """CPU FP64 oracle: one SMAC step of the same workload type at n_cpu^3."""
-    import numpy as np
+    """The CPU FP64 oracle, compiled here with -O3 -march=native
+    -ffp-contract=off, on n_cpu^3 of the workload: 1 warm-up step, then the
+    median of 3 steps, in REF mode (the reference algorithm) and NS mode."""
+    import statistics
+    import tempfile
 
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from mydemcfd_amd import cases
     from oracle import oracle
 
-    u, v, w = cases.tg_fields(n_cpu, seed=0x5EED0003, pert="k53")
-    ctx = oracle.OracleContext(**cases.periodic_desc(n_cpu, re=re))
-    for f, a in zip(("u", "v", "w"), (u, v, w)):
-        ctx.upload(f, a)
-    dt = cases.cfl_dt(ctx, cfl)
-    opts = ctx.opts(precond="mg", stop_rule="rel_l2", tol=tol)
-    ctx.apply_mg(np.zeros(ctx.shape("p")))  # multigrid setup (dense coarse inverse) outside the sample
-    t0 = time.perf_counter()
-    st = ctx.step(dt, opts)
-    t1 = time.perf_counter()
-    del np
-    return {"value": n_cpu ** 3 / (t1 - t0), "unit": "cell-steps/s", "cores": oracle.threads(), "kind": "port",
-            "sample": f"1 step, {n_cpu}^3 periodic TG+k^-5/3, FP64 RBGS-MG-PCG to 1e-8 "
-                      f"({st.iters} iterations, {t1 - t0:.1f} s)"}
+    tmp = tempfile.mkdtemp(prefix="smac_cpu_baseline_")
+    lib = oracle.native_lib(tmp)
+    fields = cases.tg_fields(n_cpu, seed=0x5EED0003, pert="k53")
+    modes = {"ref": dict(precond="ref_gmg", mg_levels=4, stop_rule="rel_l2", tol=tol),
+             "ns": dict(precond="mg", stop_rule="rel_l2", tol=tol)}
+    out = {}
+    for mode, kw in modes.items():
+        ctx = oracle.OracleContext(lib_=lib, **cases.periodic_desc(n_cpu, re=re))
+        for f, a in zip(("u", "v", "w"), fields):
+            ctx.upload(f, a)
+        dt = cases.cfl_dt(ctx, cfl)
+        ctx.step(dt, **kw)  # warm-up (multigrid set-up included)
+        ts, its = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            st = ctx.step(dt, **kw)
+            ts.append(time.perf_counter() - t0)
+            its.append(st.iters)
+        med = statistics.median(ts)
+        out[mode] = {"value": n_cpu ** 3 / med, "median_s": med, "runs_s": ts, "iters": its}
+        ctx.close()
+        print(f"# cpu baseline {mode}: {n_cpu}^3 median {med:.2f} s/step, iters {its}", file=sys.stderr, flush=True)
+    threads = oracle.threads(lib)
+    return {"value": out["ref"]["value"], "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n_cpu}^3 periodic TG+k^-5/3 (same workload type), REF mode = the reference algorithm "
+                      f"(reference GMG-PCG, 4 levels, to rel-res {tol:g}): 1 warm-up step + median of 3 steps "
+                      f"({out['ref']['median_s']:.2f} s/step, iterations {out['ref']['iters']})",
+            "ns_mode": {"value": out["ns"]["value"], "median_s": out["ns"]["median_s"],
+                        "iters": out["ns"]["iters"], "what": "the build's RBGS-MG-PCG V(2,2) on the CPU"},
+            "ref_mode_runs_s": out["ref"]["runs_s"], "ns_mode_runs_s": out["ns"]["runs_s"],
+            "cpu_model": cpu_model(), "threads": threads,
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "build": " ".join(["gcc"] + oracle.NATIVE_FLAGS)}
 
 
 def copy_calibration(nbytes, reps=4):
@@ -242,6 +302,7 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--levels", type=int, default=0)
+    ap.add_argument("--prof-steps", type=int, default=3, help="steps of the profiled (kernel breakdown) pass")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES")
@@ -268,7 +329,8 @@ def main():
     t_ic = time.perf_counter() - t_ic
     results = {}
     for f in forms:
-        results[f] = run_form(f, args.n, args.steps, args.warmup, fields, args.re, args.cfl, args.tol, args.levels)
+        results[f] = run_form(f, args.n, args.steps, args.warmup, args.prof_steps, fields, args.re, args.cfl,
+                              args.tol, args.levels)
         print(f"# {f}: {results[f]['ms_per_step']:.2f} ms/step, iters {results[f]['iters']}", file=sys.stderr,
               flush=True)
     del fields
@@ -291,6 +353,9 @@ def main():
                                f"CFL {args.cfl:g}, RBGS-MG-PCG V(2,2) to rel-res {args.tol:g}, form {headline}",
                    "n": args.n, "form": headline},
         "roofline": dominant_roofline(h, args.n),
+        "roofline_kernels": [kernel_roofline(h, args.n, k) for k in top_kernels(h, 6)],
+        "kernel_pass": {"steps": h["prof_steps"], "iters": h["prof_iters"],
+                        "ms_per_step_with_events": h["prof_ms_per_step"]},
         "step_model_gbps": h["step_model_gbps"],
         "step_roofline_frac": h["step_roofline_frac"],
         "bytes_per_cell_step": h["bytes_per_cell_step"],

@@ -28,35 +28,64 @@ def build(force: bool = False) -> str:
     return LIB
 
 
+def _bind_hooks(l):
+    _abi.bind(l)
+    l.smac_oracle_apply_A.restype = C.c_int
+    l.smac_oracle_apply_A.argtypes = [C.c_void_p, _abi._D, _abi._D]
+    l.smac_oracle_apply_mg.restype = C.c_int
+    l.smac_oracle_apply_mg.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), _abi._D, _abi._D]
+    l.smac_oracle_csum.restype = C.c_double
+    l.smac_oracle_csum.argtypes = [C.c_void_p, _abi._D]
+    l.smac_oracle_rg_beta.restype = C.c_int
+    l.smac_oracle_rg_beta.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), C.c_int, C.c_int,
+                                      C.POINTER(C.c_int * 3), _abi._D, C.c_size_t]
+    l.smac_oracle_vofmath.restype = C.c_int
+    l.smac_oracle_vofmath.argtypes = [C.c_int, _abi._D, _abi._D, C.c_size_t]
+    l.smac_oracle_threads.restype = C.c_int
+    l.smac_oracle_threads.argtypes = []
+    return l
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        l = C.CDLL(LIB, mode=C.RTLD_LOCAL)
-        _abi.bind(l)
-        l.smac_oracle_apply_A.restype = C.c_int
-        l.smac_oracle_apply_A.argtypes = [C.c_void_p, _abi._D, _abi._D]
-        l.smac_oracle_apply_mg.restype = C.c_int
-        l.smac_oracle_apply_mg.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), _abi._D, _abi._D]
-        l.smac_oracle_csum.restype = C.c_double
-        l.smac_oracle_csum.argtypes = [C.c_void_p, _abi._D]
-        l.smac_oracle_rg_beta.restype = C.c_int
-        l.smac_oracle_rg_beta.argtypes = [C.c_void_p, C.POINTER(_abi.SolverOpts), C.c_int, C.c_int,
-                                          C.POINTER(C.c_int * 3), _abi._D, C.c_size_t]
-        l.smac_oracle_vofmath.restype = C.c_int
-        l.smac_oracle_vofmath.argtypes = [C.c_int, _abi._D, _abi._D, C.c_size_t]
-        l.smac_oracle_threads.restype = C.c_int
-        l.smac_oracle_threads.argtypes = []
-        _lib = l
+        _lib = _bind_hooks(C.CDLL(LIB, mode=C.RTLD_LOCAL))
     return _lib
 
 
-class OracleContext(_abi.Context):
-    """A smac_ctx of the CPU oracle."""
+NATIVE_FLAGS = ["-O3", "-march=native", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math"]
 
-    def __init__(self, *a, **kw):
-        super().__init__(lib(), *a, **kw)
+
+def native_lib(outdir: str):
+    """The oracle built for the host it runs on (-O3 -march=native, no FMA
+    contraction) into outdir: the CPU baseline's build. It is compiled where it
+    runs, since -march=native code need not run on another host."""
+    out = os.path.join(outdir, "libsmac_oracle_native.so")
+    subprocess.run(["gcc"] + NATIVE_FLAGS + ["-shared", "-Wl,-Bsymbolic", "-o", out,
+                                             os.path.join(HERE, "smac_oracle.c"), "-lm"], check=True)
+    return _bind_hooks(C.CDLL(out, mode=C.RTLD_LOCAL))
+
+
+def mutant_lib(mutant: int, outdir: str):
+    """The oracle built with planted mistake `mutant` (SMAC_ORACLE_MUTANT in
+    smac_oracle.c) into outdir, loaded and bound. Only the tests that show the
+    closed-form pins catch such mistakes use it."""
+    out = os.path.join(outdir, f"libsmac_oracle_mutant{int(mutant)}.so")
+    if not os.path.exists(out):
+        subprocess.run(["gcc", "-O1", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+                        f"-DSMAC_ORACLE_MUTANT={int(mutant)}", "-shared", "-Wl,-Bsymbolic", "-o", out,
+                        os.path.join(HERE, "smac_oracle.c"), "-lm"], check=True)
+    return _bind_hooks(C.CDLL(out, mode=C.RTLD_LOCAL))
+
+
+class OracleContext(_abi.Context):
+    """A smac_ctx of the CPU oracle (`lib_`: another build of it, e.g. a
+    mutant_lib)."""
+
+    def __init__(self, *a, lib_=None, **kw):
+        super().__init__(lib_ if lib_ is not None else lib(), *a, **kw)
 
     def apply_A(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -94,8 +123,8 @@ class OracleContext(_abi.Context):
         return self.lib.smac_oracle_csum(self.h, x.ctypes.data_as(_abi._D))
 
 
-def threads() -> int:
-    return lib().smac_oracle_threads()
+def threads(l=None) -> int:
+    return (l or lib()).smac_oracle_threads()
 
 
 def vofmath(fn, x):

@@ -42,6 +42,20 @@
 
 enum { F_INTERIOR = 0, F_BOUNDARY = 1, F_GHOST = 2 };   /* CFD/Enums.h:3-8 */
 
+/* Planted mistakes, for the tests that show the closed-form pins catch them
+ * (tests/test_oracle_pins.py builds this file with -DSMAC_ORACLE_MUTANT=n into
+ * a scratch directory; the default build, and everything that checks the
+ * product, has n = 0):
+ *   1  wall mirror drops the wall velocity (-v_in instead of 2 v_wall - v_in)
+ *   2  the two wall faces' velocities summed, not averaged
+ *   3  slip wall mirrored like a no-slip wall
+ *   4  inflow face value taken as zero
+ *   5  convective CFL from the smaller of a cell's two face speeds
+ *   6  viscous CFL from mu instead of mu/rho */
+#ifndef SMAC_ORACLE_MUTANT
+#define SMAC_ORACLE_MUTANT 0
+#endif
+
 #define MAXL 12
 
 typedef struct level {
@@ -155,7 +169,7 @@ static double get_face(const struct smac_ctx* c, int a, ivec q) {
     if (t == F_BOUNDARY) {
         int bid = face_bcid(c, a, q);
         int bt = c->bctype[bid];
-        if (bt == SMAC_BC_INFLOW) return (1. - 0.0) * c->bcv[bid][a];
+        if (bt == SMAC_BC_INFLOW) return SMAC_ORACLE_MUTANT == 4 ? 0. : (1. - 0.0) * c->bcv[bid][a];
         if (bt == SMAC_BC_NOSLIP) return 0.;
         if (bt == SMAC_BC_OUTLET) return c->vel[a][IX(c, q)];
         if (bt == SMAC_BC_SLIP && a != 0) return c->vel[a][IX(c, q)];
@@ -182,9 +196,10 @@ static double get_dir(const struct smac_ctx* c, int a, int b, ivec q, int s) {
         vw += c->bcv[face_bcid(c, b, qf2)][a];
         count++;
     }
-    if (count > 0) vw /= (double)count;
+    if (count > 0 && SMAC_ORACLE_MUTANT != 2) vw /= (double)count;
     int bt = c->bctype[face_bcid(c, b, qf)];
-    if (bt == SMAC_BC_SLIP || bt == SMAC_BC_OUTLET) return vin;
+    if ((bt == SMAC_BC_SLIP && SMAC_ORACLE_MUTANT != 3) || bt == SMAC_BC_OUTLET) return vin;
+    if (SMAC_ORACLE_MUTANT == 1) return -vin;
     return 2.0 * vw - vin;
 }
 
@@ -1582,11 +1597,12 @@ int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
                 for (int a = 0; a < 3; a++) {
                     double vp = fabs(c->vel[a][IX(c, wr(c, add(q, a, 1)))]);
                     double vm = fabs(c->vel[a][IX(c, q)]);
-                    m[a] = vp > vm ? vp : vm;
+                    m[a] = (SMAC_ORACLE_MUTANT == 5) ? (vp < vm ? vp : vm) : (vp > vm ? vp : vm);
                 }
                 double cf = dt * (m[0] * c->inv_d[0] + m[1] * c->inv_d[1] + m[2] * c->inv_d[2]);
                 size_t id = IX(c, q);
-                double cvv = c->inv_rho[id] * c->mu[id] * dt * (c->inv_d2[0] + c->inv_d2[1] + c->inv_d2[2]);
+                double cvv = (SMAC_ORACLE_MUTANT == 6 ? 1.0 : c->inv_rho[id]) * c->mu[id] * dt *
+                             (c->inv_d2[0] + c->inv_d2[1] + c->inv_d2[2]);
                 if (cf > mc) mc = cf;
                 if (cvv > mv) mv = cvv;
             }
