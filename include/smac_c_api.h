@@ -258,6 +258,22 @@ int smac_step(smac_ctx* ctx, double dt, const smac_solver_opts* opts, smac_solve
 /* G_SMACSolver::calc_cfl (CFD/G_SMACSolver.cu:1365-1430): convective and viscous
  * CFL numbers for time step dt. */
 int smac_calc_cfl(smac_ctx* ctx, double dt, double* cfl_conv, double* cfl_visc);
+/* smac_calc_cfl that allocates no device memory. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same kernels
+ * and the same result as smac_calc_cfl, bit for bit. Its scratch (two row
+ * partials of ny*nz doubles, the lane sums of the two-stage reduction and the
+ * two result slots, 8 MiB + 16.25 KiB) is one static device array per GPU,
+ * resolved once per device by smac_create and held until the process ends.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other threads, other contexts' streams) race; the array keeps the last
+ * call's contents and nothing in it needs to start at zero (every slot a call
+ * reads is written first by that call). Returns SMAC_ERR_INVALID_GRID when
+ * ny*nz > SMAC_NOALLOC_MAX_ROWS, and SMAC_ERR_UNSUPPORTED for a two-phase
+ * context whose properties are not built yet (they would be allocated: call
+ * smac_update_properties or take a step first). Synchronous. */
+#define SMAC_NOALLOC_MAX_ROWS (1 << 19)
+int smac_calc_cfl_noalloc(smac_ctx* ctx, double dt, double* cfl_conv, double* cfl_visc);
 
 /* ---- after the step (CFD/main.cu:316-317) -----------------------------------
  * G_SMACSolver::update_boundary_ghost (CFD/G_SMACSolver.cu:1345-1362):

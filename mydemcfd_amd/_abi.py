@@ -100,6 +100,7 @@ SIGNATURES = {
     "smac_correct": (C.c_int, [_P, C.c_double]),
     "smac_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "smac_calc_cfl": (C.c_int, [_P, C.c_double, _D, _D]),
+    "smac_calc_cfl_noalloc": (C.c_int, [_P, C.c_double, _D, _D]),
     "smac_update_boundary_ghost": (C.c_int, [_P, C.c_double]),
     "smac_make_face_gradp": (C.c_int, [_P]),
     "smac_cell_output": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),

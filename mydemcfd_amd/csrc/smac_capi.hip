@@ -61,6 +61,7 @@ struct smac_ctx {
   smac_fluid_desc fl;
   bool alpha_nonzero = false;
   hipStream_t st = nullptr;
+  int dev = -1;  // device current at smac_create
   size_t tot = 0;
   double* u[3] = {nullptr, nullptr, nullptr};
   double* us[3] = {nullptr, nullptr, nullptr};
@@ -238,6 +239,44 @@ static void dfree(void* p) {
   }
   (void)hipFree(raw);
 }
+// Scratch of smac_calc_cfl_noalloc: one static device array per GPU holding
+// the two row-partial arrays (SMAC_NOALLOC_MAX_ROWS doubles each), the lane
+// sums of the two-stage reduction (2 x 1024) and 32 scalar slots. Host code
+// never passes the symbol itself to a kernel or a copy (on the host it is a
+// stand-in, not a device address): smac_create resolves the array's address
+// on each device once, with that device current, and every use goes through
+// that address.
+namespace {
+constexpr size_t SCR_PART1 = (size_t)SMAC_NOALLOC_MAX_ROWS;
+constexpr size_t SCR_STAGE = 2 * (size_t)SMAC_NOALLOC_MAX_ROWS;
+constexpr size_t SCR_S = SCR_STAGE + 2 * 1024;
+constexpr size_t SCR_WORDS = SCR_S + 32;
+constexpr int SCR_MAX_DEV = 64;
+}  // namespace
+static __device__ __align__(256) double scratch[SCR_WORDS];
+static std::mutex g_scr_mu;
+static double* g_scr_ptr[SCR_MAX_DEV] = {};
+
+// Address of `scratch` on device dev (the current device), resolved once.
+static int scratch_resolve(int dev) {
+  if (dev < 0 || dev >= SCR_MAX_DEV) return SMAC_ERR_NO_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if (g_scr_ptr[dev]) return SMAC_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fprintf(stderr, "smac: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev, hipGetErrorString(e));
+    return SMAC_ERR_HIP;
+  }
+  g_scr_ptr[dev] = (double*)p;
+  return SMAC_OK;
+}
+static double* scratch_ptr(int dev) {
+  if (dev < 0 || dev >= SCR_MAX_DEV) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  return g_scr_ptr[dev];
+}
+
 // row pitch of box arrays: a multiple of 64 elements (128 bytes in BF16,
 // 256 in FP32, 512 in FP64)
 static int box_pitch(int n) { return ((n + 3 + 63) / 64) * 64; }
@@ -367,7 +406,9 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
   c->tot = (size_t)c->g.px * (size_t)c->g.py * (size_t)(n[2] + 3);
   c->nrows = n[1] * n[2];
   int rc = SMAC_OK;
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) rc = SMAC_ERR_HIP;
+  if (hipGetDevice(&c->dev) != hipSuccess) rc = SMAC_ERR_HIP;
+  if (rc == SMAC_OK) rc = scratch_resolve(c->dev);
+  if (rc == SMAC_OK && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) rc = SMAC_ERR_HIP;
   double** arrs[] = {&c->u[0], &c->u[1], &c->u[2], &c->us[0], &c->us[1], &c->us[2], &c->p, &c->dp, &c->rhs,
                      &c->r, &c->z, &c->dv, &c->Ad, &c->res0};
   for (auto a : arrs)
@@ -1608,6 +1649,41 @@ extern "C" int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
   HIPOK(hipGetLastError());
   rc = read_scalars(c);
   if (rc) return rc;
+  if (cc) *cc = c->Sh[S_CFLC];
+  if (cv) *cv = c->Sh[S_CFLV];
+  return SMAC_OK;
+}
+
+// smac_calc_cfl with its row partials, lane sums and result slots in the
+// device's static scratch array instead of the context's buffers; no
+// allocation (the result comes back through the context's pinned host slots,
+// allocated by smac_create).
+extern "C" int smac_calc_cfl_noalloc(smac_ctx* c, double dt, double* cc, double* cv) {
+  if (!c) return SMAC_ERR_INVALID_ARG;
+  if (c->nrows > SMAC_NOALLOC_MAX_ROWS) return SMAC_ERR_INVALID_GRID;
+  const bool var = var_props(c);
+  if (!c->props_ready) {
+    if (var) return SMAC_ERR_UNSUPPORTED;  // building them would allocate
+    int rc = smac_update_properties(c, 1);  // one-phase: sets constants only
+    if (rc) return rc;
+  }
+  static_assert(S_CFLV == S_CFLC + 1, "the two CFL slots are copied as one pair");
+  double* scr = scratch_ptr(c->dev);
+  if (!scr) return SMAC_ERR_HIP;
+  Vel V;
+  for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
+  double visc = (1. / c->fl.rho0) * c->fl.mu0 * dt * (c->g.inv_d2[0] + c->g.inv_d2[1] + c->g.inv_d2[2]);
+  launch_cfl(c->g, V, dt, visc, var ? c->inv_rho : nullptr, var ? c->mu : nullptr, scr, scr + SCR_PART1, c->st);
+  FinalizeArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  fa.nrows = c->nrows;
+  fa.stage = scr + SCR_STAGE;
+  fin_add(fa, scr, 1, 0);
+  fin_add(fa, scr + SCR_PART1, 1, 1);
+  launch_finalize(fa, scr + SCR_S, c->st);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(c->Sh + S_CFLC, scr + SCR_S, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPOK(hipStreamSynchronize(c->st));
   if (cc) *cc = c->Sh[S_CFLC];
   if (cv) *cv = c->Sh[S_CFLV];
   return SMAC_OK;

@@ -1611,6 +1611,15 @@ int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
     return SMAC_OK;
 }
 
+/* The no-allocation variant of the C-ABI (no counterpart in the reference):
+ * on the CPU it is smac_calc_cfl behind the same argument checks. */
+int smac_calc_cfl_noalloc(smac_ctx* c, double dt, double* cc, double* cv) {
+    if (!c) return SMAC_ERR_INVALID_ARG;
+    if ((long)c->n[1] * (long)c->n[2] > SMAC_NOALLOC_MAX_ROWS) return SMAC_ERR_INVALID_GRID;
+    if (!c->props_ready && (c->fl.rho0 != c->fl.rho1 || c->fl.mu0 != c->fl.mu1)) return SMAC_ERR_UNSUPPORTED;
+    return smac_calc_cfl(c, dt, cc, cv);
+}
+
 /* ------------------------------------------------ VoF transport */
 #define THINC_BETA 2.0              /* BETA, CFD/hardCodedParameters.h:6 */
 #define VOF_EPS 1.E-12              /* EPS, CFD/hardCodedParameters.h:7 */
