@@ -170,7 +170,12 @@ typedef struct smac_ctx smac_ctx;
  * (CFD/G_SMACSolver.h:25,84,90-100; CFD/G_SMACSolver.cu:1451-1496) and the host
  * setup SMACSolver::set_face_type / set_cell_type / set_face_internal_direction
  * (CFD/SMACSolver.cpp:37-248). All fields start at zero (the reference
- * calloc's its host arrays, CFD/SMACSolver.cpp:423-440). */
+ * calloc's its host arrays, CFD/SMACSolver.cpp:423-440). Side kinds: no-slip,
+ * slip, inflow and outlet; an outlet side is a Dirichlet face of the pressure
+ * (its face coefficient 2 f_b of the inner face, update_boundary_faces,
+ * CFD/G_SMACSolver_VoF.cu:1798-1950), so such a grid is not pure Neumann.
+ * SMAC_BC_OUTFLOW, which no reference kernel handles, returns
+ * SMAC_ERR_UNSUPPORTED. */
 int smac_create(const smac_grid_desc* grid, const smac_fluid_desc* fluid, smac_ctx** out);
 /* Replaces G_SMACSolver::solver_free (CFD/G_SMACSolver.cu:1433-1449). */
 void smac_destroy(smac_ctx* ctx);
@@ -283,8 +288,10 @@ int smac_calc_cfl_noalloc(smac_ctx* ctx, double dt, double* cfl_conv, double* cf
  *  - tangential velocity ghosts by d_get_velocity_ghost_value with the bc id of the
  *    face one row inside, and edge ghosts as the mean of the two adjacent ghosts
  *    (k_update_v{x,y,z}_ghost[_corner], CFD/G_BoundaryFunctions.cu:257-540);
- *  - outlet faces (k_update_v{x,y,z}_outlet, CFD/G_BoundaryFunctions.cu:143-255)
- *    have no case: smac_create rejects outlet sides.
+ *  - outlet faces first (k_update_v{x,y,z}_outlet, CFD/G_BoundaryFunctions.cu:143-255):
+ *    u = u* + s beta_f dt dp(c)/h on the faces of an outlet side, c the inner
+ *    cell, s = +1 on the high side and -1 on the low side (after smac_correct,
+ *    which clears dp as the reference's memset does, the face keeps u*);
  * Periodic directions (the extension) get periodic images instead. dt is the
  * outlet update's time step. */
 int smac_update_boundary_ghost(smac_ctx* ctx, double dt);

@@ -60,6 +60,10 @@ struct smac_ctx {
   std::vector<smac_bc_entry> bc;
   smac_fluid_desc fl;
   bool alpha_nonzero = false;
+  // an outlet side: its faces carry the Dirichlet coefficient 2 beta of the
+  // inner face (p = 0 at the face), so the Poisson operator is regular and
+  // the face coefficients live in arrays
+  bool has_outlet = false;
   hipStream_t st = nullptr;
   int dev = -1;  // device current at smac_create
   size_t tot = 0;
@@ -293,7 +297,7 @@ static int pure_neumann(const smac_ctx* c) {
 }
 
 static bool uniform_props(const smac_ctx* c) {
-  return c->fl.rho0 == c->fl.rho1 && c->fl.mu0 == c->fl.mu1 && !c->alpha_nonzero;
+  return c->fl.rho0 == c->fl.rho1 && c->fl.mu0 == c->fl.mu1 && !c->alpha_nonzero && !c->has_outlet;
 }
 
 static void set_props(smac_ctx* c) {
@@ -368,13 +372,16 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
   c->bc.assign(256, smac_bc_entry{SMAC_BC_NOSLIP, 0.0, 0.0, 0.0});
   for (int b = 0; b < gd->num_bc; b++) c->bc[b] = gd->bc_table[b];
   memcpy(c->side_bcid, gd->side_bcid, 6);
+  // BC_OUTFLOW has no case in the reference's kernels (and no config
+  // spelling, CFD/config/ConfigLoader.cpp:242-267): refused
   for (int s = 0; s < 6; s++) {
     if (gd->periodic[s / 2]) continue;
     int bt = c->bc[c->side_bcid[s]].type;
-    if (bt == SMAC_BC_OUTLET || bt == SMAC_BC_OUTFLOW) {
+    if (bt == SMAC_BC_OUTFLOW || bt < SMAC_BC_NOSLIP || bt > SMAC_BC_OUTLET) {
       delete c;
       return SMAC_ERR_UNSUPPORTED;
     }
+    if (bt == SMAC_BC_OUTLET) c->has_outlet = true;
   }
   memset(&c->g, 0, sizeof(c->g));
   for (int d = 0; d < 3; d++) {
@@ -908,7 +915,9 @@ static int mg_coeffs(smac_ctx* c) {
   }
   int n = C.n[0] * C.n[1] * C.n[2];
   std::vector<double> h((size_t)n * n);
-  dense_pinv(C, var ? hb : nullptr, h.data(), 1);
+  // an outlet makes the operator regular: a plain inverse, else the
+  // pseudo-inverse of the singular pure-Neumann operator
+  dense_pinv(C, var ? hb : nullptr, h.data(), c->has_outlet ? 0 : 1);
   if (!c->cinv) {
     int rc = dalloc(&c->cinv, (size_t)n * n);
     if (rc) return rc;
@@ -1538,11 +1547,12 @@ extern "C" int smac_transport_alpha(smac_ctx* c, double dt, int32_t substeps, in
 
 // ------------------------------------------------------------------ after the step
 extern "C" int smac_update_boundary_ghost(smac_ctx* c, double dt) {
-  (void)dt;  // the outlet update's; outlet sides are rejected at create
   if (!c) return SMAC_ERR_INVALID_ARG;
   int rc = ensure_props(c);
   if (rc) return rc;
   KT(c, "ghost_pressure", launch_ghost_pressure(c->g, c->d, c->p, var_props(c) ? c->rho : nullptr, c->st));
+  // the outlet faces first: the velocity ghosts read them as their inner rows
+  if (c->has_outlet) KT(c, "outlet_faces", launch_outlet_faces(c->g, c->us, c->beta, c->dp, c->u, dt, c->st));
   KT(c, "ghost_velocity", launch_ghost_velocity(c->g, c->u, c->st));
   HIPOK(hipGetLastError());
   return SMAC_OK;
@@ -1587,6 +1597,8 @@ extern "C" int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts*
   if (c->vof_armed) return SMAC_ERR_UNSUPPORTED;
   // the adjoint differentiates the one-fluid step: no volume fraction terms
   if (c->eps || c->bpois_on) return SMAC_ERR_UNSUPPORTED;
+  // outlet sides (the star-BC copy and the Dirichlet faces) are outside it
+  if (c->has_outlet) return SMAC_ERR_UNSUPPORTED;
   int rc = ensure_props(c);
   if (rc) return rc;
   for (int m = 0; m < 11 && !rc; m++)

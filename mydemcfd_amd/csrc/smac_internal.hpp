@@ -208,6 +208,8 @@ struct Spacing {
 // ghost cells of p; rho == nullptr: uniform density g.rho0
 void launch_ghost_pressure(const Geo& g, const double h[3], double* p, const double* rho, hipStream_t st);
 void launch_ghost_velocity(const Geo& g, double* const u[3], hipStream_t st);
+void launch_outlet_faces(const Geo& g, const double* const us[3], const double* const beta[3], const double* dp,
+                         double* const u[3], double dt, hipStream_t st);
 void launch_face_gradp(const Geo& g, const double* p, double* const gp[3], hipStream_t st);
 // VTI cell data: p[n], alpha[n] (al == nullptr: 0), velocity[3n] as float
 void launch_pack_cells(const Geo& g, const double* p, const double* al, const double* const u[3], float* out,

@@ -1365,7 +1365,16 @@ __global__ __launch_bounds__(256) void k_props_faces(Geo g, const double* __rest
   } else if (t == F_BOUNDARY) {
     const int inner = q.v[A] == 1 ? 0 : -1;
     muf[id] = mu[IX(g, sh(q, A, inner))];
-    beta[id] = 0.;
+    // k_update_{x,y,z}_face_boundary_properties (CFD/G_SMACSolver_VoF.cu:1798-1941):
+    // an outlet face takes 2 beta of the adjacent interior face (p = 0 at the
+    // face), formed here with that face's own arithmetic; other boundary faces 0
+    double b = 0.;
+    if (btype_of(g, face_slot(g, A, q)) == BC_OUTLET) {
+      const I3 qi = sh(q, A, inner < 0 ? -1 : 1);
+      const double rfi = 0.5 * (rho[IX(g, qi)] + rho[IX(g, sh(qi, A, -1))]);
+      b = 2.0 * (1. / rfi);
+    }
+    beta[id] = b;
   } else {
     beta[id] = 0.;
   }

@@ -100,6 +100,41 @@ __global__ __launch_bounds__(256) void k_ghost_planes(Geo g, Plane Q, Spacing H,
   f[IX(g, q)] = val;
 }
 
+// k_update_v{x,y,z}_outlet (CFD/G_BoundaryFunctions.cu:143-255) on the faces
+// of one outlet side (side = 2*axis + high): u = u* + s*beta_f*dt*dp(c)/h with
+// c the inner cell, s = +1 on the high side and -1 on the low side, beta_f the
+// face's Dirichlet coefficient. (In the reference loop the correction has
+// cleared dp before this runs, so the face keeps u*.) Faces with the other
+// two indices in 1..N, as the reference's F_BOUNDARY test selects.
+__global__ __launch_bounds__(256) void k_outlet_face(Geo g, int side, const double* __restrict__ us,
+                                                     const double* __restrict__ beta, const double* __restrict__ dp,
+                                                     double* __restrict__ u, double dt) {
+  const int a = side >> 1;
+  const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+  const int t1 = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int t2 = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+  if (t1 > g.n[b1] || t2 > g.n[b2]) return;
+  I3 q;
+  q.v[a] = (side & 1) ? g.n[a] + 1 : 1;
+  q.v[b1] = t1;
+  q.v[b2] = t2;
+  const int shift = (side & 1) ? -1 : 0;  // f_{x,y,z}internal_id_
+  const double s = (side & 1) ? 1.0 : -1.0;
+  const size_t id = IX(g, q);
+  u[id] = us[id] + s * beta[id] * dt * dp[IX(g, sh(q, a, shift))] * g.inv_d[a];
+}
+
+void launch_outlet_faces(const Geo& g, const double* const us[3], const double* const beta[3], const double* dp,
+                         double* const u[3], double dt, hipStream_t st) {
+  for (int side = 0; side < 6; side++) {
+    const int a = side >> 1;
+    if (g.per[a] || g.btype[side] != BC_OUTLET) continue;
+    const int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+    dim3 grid((unsigned)((g.n[b1] + 63) / 64), (unsigned)((g.n[b2] + 3) / 4), 1);
+    hipLaunchKernelGGL(k_outlet_face, grid, dim3(64, 4), 0, st, g, side, us[a], beta[a], dp, u[a], dt);
+  }
+}
+
 // f_gradp_{x,y,z} = (p(c) - p(c-1))/h on faces with all indices in 1..N+1
 __global__ __launch_bounds__(256) void k_face_gradp(Geo g, const double* __restrict__ p, double* __restrict__ gx,
                                                     double* __restrict__ gy, double* __restrict__ gz) {

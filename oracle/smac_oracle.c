@@ -51,7 +51,15 @@ enum { F_INTERIOR = 0, F_BOUNDARY = 1, F_GHOST = 2 };   /* CFD/Enums.h:3-8 */
  *   3  slip wall mirrored like a no-slip wall
  *   4  inflow face value taken as zero
  *   5  convective CFL from the smaller of a cell's two face speeds
- *   6  viscous CFL from mu instead of mu/rho */
+ *   6  viscous CFL from mu instead of mu/rho
+ *   7  outlet face coefficient beta of the inner face instead of 2 beta
+ *   8  outlet face coefficient 2/rho of the inner cell instead of 2 beta of
+ *      the inner face
+ *   9  star value of an outlet face taken as zero (no copy of the inner face)
+ *  10  outlet face value taken as zero in the predictor's getter
+ *  11  outlet velocity ghost copies the inner row (bval) instead of 2 bval - inval
+ *  12  an outlet side counted as pure Neumann (mean removal of the rhs)
+ *  13  outlet face update with the sign of the dp term flipped */
 #ifndef SMAC_ORACLE_MUTANT
 #define SMAC_ORACLE_MUTANT 0
 #endif
@@ -74,6 +82,7 @@ struct smac_ctx {
     int per[3];
     uint8_t side_bcid[6];
     int nbc;
+    int has_outlet;
     int bctype[256];
     double bcv[256][3];
     smac_fluid_desc fl;
@@ -171,7 +180,7 @@ static double get_face(const struct smac_ctx* c, int a, ivec q) {
         int bt = c->bctype[bid];
         if (bt == SMAC_BC_INFLOW) return SMAC_ORACLE_MUTANT == 4 ? 0. : (1. - 0.0) * c->bcv[bid][a];
         if (bt == SMAC_BC_NOSLIP) return 0.;
-        if (bt == SMAC_BC_OUTLET) return c->vel[a][IX(c, q)];
+        if (bt == SMAC_BC_OUTLET) return SMAC_ORACLE_MUTANT == 10 ? 0. : c->vel[a][IX(c, q)];
         if (bt == SMAC_BC_SLIP && a != 0) return c->vel[a][IX(c, q)];
     }
     return 0.;
@@ -392,6 +401,7 @@ static double get_star_boundary(const struct smac_ctx* c, int a, ivec q) {
     if (bt == SMAC_BC_NOSLIP) return 0.;
     if (bt == SMAC_BC_OUTLET) {
         int internal = (q.v[a] == 1) ? 0 : -1; /* set_face_internal_direction */
+        if (SMAC_ORACLE_MUTANT == 9) return 0.;
         return internal < 0 ? c->vst[a][IX(c, add(q, a, -1))] : c->vst[a][IX(c, add(q, a, +1))];
     }
     return 0.;
@@ -579,7 +589,19 @@ int smac_update_properties(smac_ctx* c, int32_t initial) {
                         ivec ci = add(q, a, inner);
                         c->rhof[a][id] = c->rho[IX(c, ci)];
                         c->muf[a][id] = c->mu[IX(c, ci)];
-                        c->beta[a][id] = 0.;
+                        /* an outlet face: 2 f_b of the adjacent interior face
+                         * ("pressure is assumed to be zero at the boundary",
+                         * k_update_{x,y,z}_face_boundary_properties); the
+                         * interior face's value is formed here, as its own
+                         * branch forms it (the loop may not have reached it) */
+                        double b = 0.;
+                        if (c->bctype[face_bcid(c, a, q)] == SMAC_BC_OUTLET) {
+                            ivec qi = add(q, a, inner < 0 ? -1 : 1);
+                            double rfi = 0.5 * (c->rho[IX(c, qi)] + c->rho[IX(c, add(qi, a, -1))]);
+                            if (SMAC_ORACLE_MUTANT == 8) rfi = c->rho[IX(c, ci)];
+                            b = (SMAC_ORACLE_MUTANT == 7 ? 1.0 : 2.0) * (1. / rfi);
+                        }
+                        c->beta[a][id] = b;
                         c->inv_rhof[a][id] = 0.;
                     } else {
                         c->beta[a][id] = 0.;
@@ -903,7 +925,7 @@ static int pure_neumann(const struct smac_ctx* c) {
         if (c->per[d]) continue;
         for (int s = 0; s < 2; s++) {
             int bt = c->bctype[c->side_bcid[2 * d + s]];
-            if (bt == SMAC_BC_OUTLET || bt == SMAC_BC_INFLOW) return 0;
+            if ((bt == SMAC_BC_OUTLET && SMAC_ORACLE_MUTANT != 12) || bt == SMAC_BC_INFLOW) return 0;
         }
     }
     return 1;
@@ -936,8 +958,8 @@ static int mg_setup(struct smac_ctx* c, int L) {
     int n = C->n[0] * C->n[1] * C->n[2];
     c->cinv = (double*)malloc((size_t)n * n * sizeof(double));
     c->cinv_n = n;
-    /* no Dirichlet (outlet) faces are supported, so A is always singular */
-    dense_pinv(c, C, c->cinv, 1);
+    /* outlet faces (Dirichlet) make A regular; without them it is singular */
+    dense_pinv(c, C, c->cinv, c->has_outlet ? 0 : 1);
     c->mg_ready = 1;
     return SMAC_OK;
 }
@@ -1621,7 +1643,7 @@ int smac_calc_cfl_noalloc(smac_ctx* c, double dt, double* cc, double* cv) {
 }
 
 /* ------------------------------------------------ VoF transport */
-#define THINC_BETA 2.0              /* BETA, CFD/hardCodedParameters.h:6 */
+#define THINC_BETA 2.0             /* BETA, CFD/hardCodedParameters.h:6 */
 #define VOF_EPS 1.E-12              /* EPS, CFD/hardCodedParameters.h:7 */
 #define VOF_PI 3.14159265358979323846
 
@@ -1896,7 +1918,7 @@ int smac_transport_alpha(smac_ctx* c, double dt, int32_t substeps, int32_t step)
 static double ghost_value(int bt, double bv, double bval, double inval) {
     if (bt == SMAC_BC_INFLOW) return 2.0 * (1.0 - 0.0) * bv - inval;
     if (bt == SMAC_BC_NOSLIP) return -inval;
-    if (bt == SMAC_BC_OUTLET) return 2.0 * bval - inval;
+    if (bt == SMAC_BC_OUTLET) return SMAC_ORACLE_MUTANT == 11 ? bval : 2.0 * bval - inval;
     if (bt == SMAC_BC_SLIP) return inval;
     return bval;
 }
@@ -1938,7 +1960,6 @@ static double ghost_vel(const struct smac_ctx* c, int a, ivec q) {
 
 /* G_SMACSolver::update_boundary_ghost (CFD/G_SMACSolver.cu:1345-1362) */
 int smac_update_boundary_ghost(smac_ctx* c, double dt) {
-    (void)dt; /* the outlet update's; outlet sides are rejected at create */
     if (!c) return SMAC_ERR_INVALID_ARG;
     if (!c->props_ready) smac_update_properties(c, 1);
     const double gv[3] = {c->fl.gx, c->fl.gy, c->fl.gz};
@@ -1959,6 +1980,30 @@ int smac_update_boundary_ghost(smac_ctx* c, double dt) {
                 size_t is = IX(c, s);
                 c->p_[IX(c, q)] = c->p_[is] + c->rho[is] * (r[0] * gv[0] + r[1] * gv[1] + r[2] * gv[2]);
             }
+    /* k_update_v{x,y,z}_outlet (CFD/G_BoundaryFunctions.cu:143-255), before
+     * the velocity ghosts, which read these faces: on the boundary a-faces of
+     * an outlet side, u = u* + s*f_b*dt*dp(c)/h with c the inner cell
+     * (internal id 0 on the low side, -1 on the high side) and s = +1 when the
+     * id is negative, else -1 */
+    for (int a = 0; a < 3; a++) {
+        if (c->per[a]) continue;
+        int b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+        for (int side = 0; side < 2; side++) {
+            if (c->bctype[c->side_bcid[2 * a + side]] != SMAC_BC_OUTLET) continue;
+            int shift = side ? -1 : 0;
+            int sgn = (shift < 0) == (SMAC_ORACLE_MUTANT != 13) ? 1 : -1;
+            for (int t2 = 1; t2 <= c->n[b2]; t2++)
+                for (int t1 = 1; t1 <= c->n[b1]; t1++) {
+                    ivec q;
+                    q.v[a] = side ? c->n[a] + 1 : 1;
+                    q.v[b1] = t1;
+                    q.v[b2] = t2;
+                    size_t id = IX(c, q);
+                    c->vel[a][id] = c->vst[a][id] + sgn * c->beta[a][id] * dt * c->dp[IX(c, add(q, a, shift))] *
+                                                        c->inv_d[a];
+                }
+        }
+    }
     /* velocity ghosts: every position of the face array with a ghost index
      * along a tangential axis, or along its own axis when that is periodic;
      * faces 0 and n+2 along a wall axis are not touched (the kernels run
@@ -2167,12 +2212,16 @@ int smac_create(const smac_grid_desc* g, const smac_fluid_desc* f, smac_ctx** ou
         c->bcv[b][1] = g->bc_table[b].vy;
         c->bcv[b][2] = g->bc_table[b].vz;
     }
+    /* BC_OUTFLOW has no case in the reference's kernels (and no config
+     * spelling): refused. An outlet side makes the operator regular. */
     for (int s = 0; s < 6; s++) {
         if (c->per[s / 2]) continue;
-        if (c->bctype[c->side_bcid[s]] == SMAC_BC_OUTLET || c->bctype[c->side_bcid[s]] == SMAC_BC_OUTFLOW) {
+        int bt = c->bctype[c->side_bcid[s]];
+        if (bt == SMAC_BC_OUTFLOW || bt < SMAC_BC_NOSLIP || bt > SMAC_BC_OUTLET) {
             free(c);
             return SMAC_ERR_UNSUPPORTED;
         }
+        if (bt == SMAC_BC_OUTLET) c->has_outlet = 1;
     }
     if (f) c->fl = *f;
     else { c->fl.rho0 = c->fl.rho1 = 1.0; c->fl.mu0 = c->fl.mu1 = 0.0; }

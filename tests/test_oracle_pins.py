@@ -343,12 +343,16 @@ def test_invalid_grid_and_args(oracle_mod):
     c = oracle_mod.OracleContext(**cases.cavity_desc(8))
     with pytest.raises(ValueError):
         c.upload("u", np.zeros((3, 3, 3)))
+    # BC_OUTFLOW has no case in the reference's kernels: refused; an outlet
+    # side is accepted
     desc = cases.cavity_desc(8)
-    desc["bc_table"] = ((_abi.BC_OUTLET, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0))
+    desc["bc_table"] = ((_abi.BC_OUTFLOW, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0))
     desc["side_bcid"] = (0, 1, 1, 1, 1, 1)
     with pytest.raises(_abi.SmacError) as e:
         oracle_mod.OracleContext(**desc)
     assert e.value.status == _abi.SMAC_ERR_UNSUPPORTED
+    desc["bc_table"] = ((_abi.BC_OUTLET, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0), (_abi.BC_NOSLIP, 0, 0, 0))
+    oracle_mod.OracleContext(**desc).close()
 
 
 # ------------------------------------------ the reference's GMG (REF_GMG)
