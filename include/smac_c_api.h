@@ -192,7 +192,9 @@ int smac_download(smac_ctx* ctx, int32_t field, double* host, size_t count);
  * The copy runs on the context's non-blocking stream, which is not ordered with
  * any other stream: finish the other streams' work on the buffer first (for a
  * torch tensor, torch.cuda.synchronize(); the caching allocator hands a freed
- * block to a new tensor while kernels queued on its stream may still read it). */
+ * block to a new tensor while kernels queued on its stream may still read it).
+ * A device upload of SMAC_FIELD_ALPHA is not scanned for zeros: the grid then
+ * keeps its properties in arrays, as a two-phase grid. */
 int smac_upload_device(smac_ctx* ctx, int32_t field, const double* dev, size_t count);
 int smac_download_device(smac_ctx* ctx, int32_t field, double* dev, size_t count);
 /* Element count of a field in the reference layout. */
@@ -323,6 +325,27 @@ int smac_cell_output(smac_ctx* ctx, float* host_out, size_t count);
  * callers through a torch.autograd.Function (mydemcfd_amd/autograd.py). */
 int smac_adjoint_step(smac_ctx* ctx, double dt, const smac_solver_opts* opts, const double* const u_n[3],
                       double* const ubar[3], double* pbar, smac_solve_stats* stats);
+
+/* Adjoint (vector-Jacobian product) of one step of the VoF loop -- the
+ * transport smac_transport_alpha(ctx, dt, substeps, step) followed by
+ * smac_step(ctx, dt, opts), the order of CFD/main.cu:283-317 -- taken from the
+ * state (alpha_n, u_n, p_n) (device arrays, reference layout). On entry
+ * alpha_bar (cells), ubar[3] (faces) and pbar (cells) hold the cotangents of
+ * the step's output (alpha^{n+1}, u^{n+1}, p^{n+1}); on return those of its
+ * input. Covers the split THINC/WLIC transport (the branch the forward took:
+ * upwind donor, THINC/WLIC or first order, the clip), the property map
+ * alpha -> (rho, mu, beta = 1/rho_f) of alpha^{n+1} and rho_old = rho(alpha^n),
+ * the alpha-flux mass flux of the predictor, the alpha-dependent Poisson
+ * operator (a second solve with opts) and the correction. The step is taken
+ * again from the given state (the context's fields are overwritten: alpha
+ * ends at alpha^{n+1}); statistics of the second solve go to stats. Entries
+ * that are not state are ignored on entry and zero on return.
+ * SMAC_ERR_UNSUPPORTED with volume fractions or outlet sides. No counterpart
+ * in the reference (it has no adjoint); it serves gradient-based callers of the
+ * VoF loop through mydemcfd_amd/autograd.py (vof_rollout). */
+int smac_adjoint_vof_step(smac_ctx* ctx, double dt, int32_t substeps, int32_t step, const smac_solver_opts* opts,
+                          const double* alpha_n, const double* const u_n[3], const double* p_n, double* alpha_bar,
+                          double* const ubar[3], double* pbar, smac_solve_stats* stats);
 
 /* Fill opts with the defaults (RBGS multigrid, FP64, REL_L2, 1e-8). */
 void smac_default_opts(smac_solver_opts* opts);

@@ -106,6 +106,8 @@ SIGNATURES = {
     "smac_cell_output": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "smac_adjoint_step": (C.c_int, [_P, C.c_double, C.POINTER(SolverOpts), C.POINTER(_P * 3), C.POINTER(_P * 3), _P,
                                     C.POINTER(SolveStats)]),
+    "smac_adjoint_vof_step": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, C.POINTER(SolverOpts), _P,
+                                        C.POINTER(_P * 3), _P, _P, C.POINTER(_P * 3), _P, C.POINTER(SolveStats)]),
     "smac_default_opts": (None, [C.POINTER(SolverOpts)]),
     "smac_set_profiling": (C.c_int, [_P, C.c_int32]),
     "smac_kernel_time": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int64), _D]),
@@ -274,6 +276,20 @@ class Context:
         ub = (_P * 3)(*[int(x) for x in ubar])
         self._check(self.lib.smac_adjoint_step(self.h, dt, C.byref(o), C.byref(un), C.byref(ub), int(pbar),
                                                C.byref(st)), "smac_adjoint_step")
+        return st
+
+    def adjoint_vof_step(self, dt, substeps, step, alpha_n, u_n, p_n, alpha_bar, ubar, pbar, opts=None, **kw):
+        """Adjoint of transport_alpha(dt, substeps, step) + step(dt) from the
+        state (alpha_n, u_n, p_n) (device pointers, the reference layout):
+        alpha_bar, ubar (three) and pbar hold the cotangents of the output on
+        entry and of the input on return."""
+        o = opts if opts is not None else default_opts(self.lib, **kw)
+        st = SolveStats()
+        un = (_P * 3)(*[int(x) for x in u_n])
+        ub = (_P * 3)(*[int(x) for x in ubar])
+        self._check(self.lib.smac_adjoint_vof_step(self.h, dt, int(substeps), int(step), C.byref(o), int(alpha_n),
+                                                   C.byref(un), int(p_n), int(alpha_bar), C.byref(ub), int(pbar),
+                                                   C.byref(st)), "smac_adjoint_vof_step")
         return st
 
     def calc_cfl(self, dt):

@@ -21,8 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OUT = os.path.join(LIBDIR, "libsmac_hip.so")
-SOURCES = ["smac_kernels.hip", "smac_mg.hip", "smac_refgmg.hip", "smac_adjoint.hip", "smac_post.hip", "smac_vof.hip",
-           "smac_capi.hip"]
+SOURCES = ["smac_kernels.hip", "smac_mg.hip", "smac_refgmg.hip", "smac_adjoint.hip", "smac_adjoint_vof.hip",
+           "smac_post.hip", "smac_vof.hip", "smac_capi.hip"]
 ARCH = os.environ.get("SMAC_OFFLOAD_ARCH", "gfx950")
 
 

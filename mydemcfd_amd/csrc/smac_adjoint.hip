@@ -20,6 +20,7 @@
 // scatter are therefore order-dependent in the last bits; every other stage is
 // a deterministic gather.
 #include "smac_internal.hpp"
+#include "smac_predict.hpp"
 
 namespace smac {
 
@@ -38,8 +39,10 @@ struct TV {
 __device__ __forceinline__ TV tconst(double v) { return TV{v, -1, 0, 0.0}; }
 __device__ __forceinline__ TV tmem(const Vel& V, int c, size_t i) { return TV{V.c[c][i], c, i, 1.0}; }
 
-// tracked forms of the boundary getters (same values as get_face / get_dir)
+// tracked forms of the boundary getters (same values as get_face / get_dir);
+// Fwd is the forward policy with the same getters
 struct WallT {
+  using Fwd = WallBC;
   static __device__ __forceinline__ TV face(const Geo& g, const Vel& V, int a, I3 q) {
     q = wr(g, q);
     int t = face_type(g, a, q);
@@ -83,6 +86,7 @@ struct WallT {
   }
 };
 struct PeriodicT {
+  using Fwd = PeriodicBC;
   static __device__ __forceinline__ I3 w3(const Geo& g, I3 q) {
     q.v[0] = wrapi(q.v[0], g.n[0]);
     q.v[1] = wrapi(q.v[1], g.n[1]);
@@ -101,12 +105,39 @@ struct PeriodicT {
 struct Acc {
   double* ub[3];
 };
+// Cotangent sinks of the VoF adjoint (the step after a transport): Fb = the
+// accumulated alpha flux of each axis (faces), anp1 = alpha^{n+1} (the
+// properties rho, mu, beta of the step), an = alpha^n (rho_old); the
+// property derivatives d rho / d alpha = drho, d mu / d alpha = dmu.
+struct AccV {
+  double* Fb[3];
+  double* anp1;
+  double* an;
+  double drho, dmu;
+};
+__device__ __forceinline__ void aadd(double* base, size_t i, double v) {
+  if (v != 0.0) unsafeAtomicAdd(base + i, v);
+}
 __device__ __forceinline__ void scat(const Acc& U, const TV& t, double c) {
   if (t.c >= 0 && c != 0.0) {
     // select chain: a dynamic index into the by-value argument would force a scratch copy
     double* base = t.c == 0 ? U.ub[0] : (t.c == 1 ? U.ub[1] : U.ub[2]);
     unsafeAtomicAdd(base + t.i, t.f * c);
   }
+}
+// mass-flux face value of conv_term (MfRho0 or MfVof, the same expression)
+template <bool VOF, int E>
+__device__ __forceinline__ double mflux(const Geo& g, const VofF& M, I3 r, const TV& f) {
+  if constexpr (VOF) return g.rho0 * f.v + M.drho * (M.F[E][IX(g, wr(g, r))] * M.hbydt[E]);
+  return g.rho0 * f.v;
+}
+// cotangent c of a mass-flux face value: rho0 c to the velocity, (for VOF)
+// drho h/dt c to the accumulated alpha flux at r
+template <bool VOF, int E>
+__device__ __forceinline__ void mflux_bar(const Geo& g, const VofF& M, const AccV& AV, const Acc& U, I3 r,
+                                          const TV& f, double c) {
+  scat(U, f, g.rho0 * c);
+  if constexpr (VOF) aadd(AV.Fb[E], IX(g, wr(g, r)), M.drho * M.hbydt[E] * c);
 }
 
 // branch of minmod(dpv, dmv): 0 -> 0, 1 -> dpv, 2 -> dmv (d_minmod,
@@ -117,11 +148,12 @@ __device__ __forceinline__ int minmod_sel(double dp, double dm) {
 }
 
 // adjoint of one side of the convective flux: M = m * (uw + sgn(m) 0.5 s),
-// m = 0.5 (rho0 f0 + rho0 f1); cM = cotangent of M
-template <class BC>
-__device__ __forceinline__ void flux_vjp(const Acc& U, double rho0, const TV& f0, const TV& f1, const TV& uw,
-                                         bool slope, const TV& sa, const TV& sb, const TV& sc, double cM) {
-  const double m = 0.5 * (rho0 * f0.v + rho0 * f1.v);
+// m = 0.5 (mf(f0) + mf(f1)) with the mass-flux faces at r0, r1; cM = cotangent of M
+template <bool VOF, int E>
+__device__ __forceinline__ void flux_vjp(const Geo& g, const VofF& M, const AccV& AV, const Acc& U, I3 r0,
+                                         const TV& f0, I3 r1, const TV& f1, const TV& uw, bool slope, const TV& sa,
+                                         const TV& sb, const TV& sc, double cM) {
+  const double m = 0.5 * (mflux<VOF, E>(g, M, r0, f0) + mflux<VOF, E>(g, M, r1, f1));
   double s = 0.0;
   int sel = 0;
   if (slope) {
@@ -131,9 +163,9 @@ __device__ __forceinline__ void flux_vjp(const Acc& U, double rho0, const TV& f0
   }
   const int dir = sgn(m);
   const double up = uw.v + dir * 0.5 * s;
-  // dM/dm = up; dm/df0 = dm/df1 = 0.5 rho0
-  scat(U, f0, 0.5 * rho0 * up * cM);
-  scat(U, f1, 0.5 * rho0 * up * cM);
+  // dM/dm = up; dm/dmf0 = dm/dmf1 = 0.5
+  mflux_bar<VOF, E>(g, M, AV, U, r0, f0, 0.5 * up * cM);
+  mflux_bar<VOF, E>(g, M, AV, U, r1, f1, 0.5 * up * cM);
   // dM/duw = m; dM/ds = m sgn(m) 0.5
   scat(U, uw, m * cM);
   const double cs = m * dir * 0.5 * cM;
@@ -146,19 +178,18 @@ __device__ __forceinline__ void flux_vjp(const Acc& U, double rho0, const TV& f0
   }
 }
 
-// adjoint of conv_term<BC, A, E> scaled by cot (cotangent of the term)
-template <class BC, int A, int E>
-__device__ __forceinline__ void conv_vjp(const Geo& g, const Vel& V, const Acc& U, I3 q, const TV& v0, const TV& vp1,
-                                         const TV& vm1, double cot) {
-  const double rho0 = g.rho0;
+// adjoint of conv_term<BC, MF, A, E> scaled by cot (cotangent of the term)
+template <class BC, bool VOF, int A, int E>
+__device__ __forceinline__ void conv_vjp(const Geo& g, const Vel& V, const VofF& M, const AccV& AV, const Acc& U, I3 q,
+                                         const TV& v0, const TV& vp1, const TV& vm1, double cot) {
   const double cE = cot * g.inv_d[E];  // d(term)/dMp = inv_d, d/dMm = -inv_d
   if (E == A) {
-    const TV fa0 = BC::face(g, V, A, q), fap = BC::face(g, V, A, sh(q, A, +1)),
-             fam = BC::face(g, V, A, sh(q, A, -1));
-    const double mp = 0.5 * (rho0 * fa0.v + rho0 * fap.v);
-    const double mm = 0.5 * (rho0 * fam.v + rho0 * fa0.v);
+    const I3 qp = sh(q, A, +1), qm = sh(q, A, -1);
+    const TV fa0 = BC::face(g, V, A, q), fap = BC::face(g, V, A, qp), fam = BC::face(g, V, A, qm);
+    const double mp = 0.5 * (mflux<VOF, A>(g, M, q, fa0) + mflux<VOF, A>(g, M, qp, fap));
+    const double mm = 0.5 * (mflux<VOF, A>(g, M, qm, fam) + mflux<VOF, A>(g, M, q, fa0));
     {
-      const I3 up = mp > 0. ? q : sh(q, A, +1);
+      const I3 up = mp > 0. ? q : qp;
       const bool in = BC::interior(g, A, up);
       const TV uw = mp > 0. ? v0 : vp1;
       TV sa = tconst(0.), sb = tconst(0.), sc = tconst(0.);
@@ -167,10 +198,10 @@ __device__ __forceinline__ void conv_vjp(const Geo& g, const Vel& V, const Acc& 
         sb = BC::face(g, V, A, up);
         sc = BC::face(g, V, A, sh(up, A, -1));
       }
-      flux_vjp<BC>(U, rho0, fa0, fap, uw, in, sa, sb, sc, cE);
+      flux_vjp<VOF, A>(g, M, AV, U, q, fa0, qp, fap, uw, in, sa, sb, sc, cE);
     }
     {
-      const I3 up = mm > 0. ? sh(q, A, -1) : q;
+      const I3 up = mm > 0. ? qm : q;
       const bool in = BC::interior(g, A, up);
       const TV uw = mm > 0. ? vm1 : v0;
       TV sa = tconst(0.), sb = tconst(0.), sc = tconst(0.);
@@ -179,14 +210,15 @@ __device__ __forceinline__ void conv_vjp(const Geo& g, const Vel& V, const Acc& 
         sb = BC::face(g, V, A, up);
         sc = BC::face(g, V, A, sh(up, A, -1));
       }
-      flux_vjp<BC>(U, rho0, fam, fa0, uw, in, sa, sb, sc, -cE);
+      flux_vjp<VOF, A>(g, M, AV, U, qm, fam, q, fa0, uw, in, sa, sb, sc, -cE);
     }
   } else {
     const TV vcm = BC::dir(g, V, A, E, q, -1), vcp = BC::dir(g, V, A, E, q, +1);
-    const TV e1 = BC::face(g, V, E, sh(sh(q, E, +1), A, -1)), e2 = BC::face(g, V, E, sh(q, E, +1));
-    const TV e3 = BC::face(g, V, E, q), e4 = BC::face(g, V, E, sh(q, A, -1));
-    const double mp = 0.5 * (rho0 * e1.v + rho0 * e2.v);
-    const double mm = 0.5 * (rho0 * e3.v + rho0 * e4.v);
+    const I3 r1 = sh(sh(q, E, +1), A, -1), r2 = sh(q, E, +1), r3 = sh(q, A, -1);
+    const TV e1 = BC::face(g, V, E, r1), e2 = BC::face(g, V, E, r2);
+    const TV e3 = BC::face(g, V, E, q), e4 = BC::face(g, V, E, r3);
+    const double mp = 0.5 * (mflux<VOF, E>(g, M, r1, e1) + mflux<VOF, E>(g, M, r2, e2));
+    const double mm = 0.5 * (mflux<VOF, E>(g, M, q, e3) + mflux<VOF, E>(g, M, r3, e4));
     {
       const I3 upw = wr(g, mp > 0. ? q : sh(q, E, +1));
       const bool in = BC::interior(g, A, upw);
@@ -197,7 +229,7 @@ __device__ __forceinline__ void conv_vjp(const Geo& g, const Vel& V, const Acc& 
         sa = BC::dir(g, V, A, E, upw, +1);
         sc = BC::dir(g, V, A, E, upw, -1);
       }
-      flux_vjp<BC>(U, rho0, e1, e2, uw, in, sa, sb, sc, cE);
+      flux_vjp<VOF, E>(g, M, AV, U, r1, e1, r2, e2, uw, in, sa, sb, sc, cE);
     }
     {
       const I3 upw = wr(g, mm > 0. ? sh(q, E, -1) : q);
@@ -209,15 +241,30 @@ __device__ __forceinline__ void conv_vjp(const Geo& g, const Vel& V, const Acc& 
         sa = BC::dir(g, V, A, E, upw, +1);
         sc = BC::dir(g, V, A, E, upw, -1);
       }
-      flux_vjp<BC>(U, rho0, e3, e4, uw, in, sa, sb, sc, -cE);
+      flux_vjp<VOF, E>(g, M, AV, U, q, e3, r3, e4, uw, in, sa, sb, sc, -cE);
     }
   }
 }
 
-// adjoint of visc_cross<BC, PR, A, B> scaled by cot
-template <class BC, class PR, int A, int B>
-__device__ __forceinline__ void visc_vjp(const Geo& g, const Props& P, const Vel& V, const Acc& U, I3 q, const TV& v0,
-                                         double cot) {
+// cotangent c of the face viscosity muf[B] at f (VOF): to the cells it is
+// formed from (k_props_faces: the mean of its two cells on an interior face,
+// the inner cell on a boundary face), then to alpha^{n+1} with dmu
+__device__ __forceinline__ void mu_face_bar(const Geo& g, const AccV& AV, int B, I3 f, double c) {
+  f = wr(g, f);
+  const int t = face_type(g, B, f);
+  if (t == F_INTERIOR) {
+    aadd(AV.anp1, IX(g, f), AV.dmu * (0.5 * c));
+    aadd(AV.anp1, IX(g, wr(g, sh(f, B, -1))), AV.dmu * (0.5 * c));
+  } else if (t == F_BOUNDARY) {
+    aadd(AV.anp1, IX(g, f.v[B] == 1 ? f : sh(f, B, -1)), AV.dmu * c);
+  }
+}
+
+// adjoint of visc_cross<BC, PR, A, B> scaled by cot (VOF: and of its edge
+// viscosities)
+template <class BC, class PR, bool VOF, int A, int B>
+__device__ __forceinline__ void visc_vjp(const Geo& g, const Props& P, const Vel& V, const Acc& U, const AccV& AV, I3 q,
+                                         const TV& v0, double cot) {
   const TV vbm = BC::dir(g, V, A, B, q, -1), vbp = BC::dir(g, V, A, B, q, +1);
   const TV wpp = BC::face(g, V, B, sh(q, B, +1)), wmp = BC::face(g, V, B, sh(sh(q, B, +1), A, -1));
   const TV wp0 = BC::face(g, V, B, q), wm0 = BC::face(g, V, B, sh(q, A, -1));
@@ -232,12 +279,24 @@ __device__ __forceinline__ void visc_vjp(const Geo& g, const Props& P, const Vel
   scat(U, wm0, c * mu_m * g.inv_d[A]);
   scat(U, v0, -c * mu_m * g.inv_d[B]);
   scat(U, vbm, c * mu_m * g.inv_d[B]);
+  if constexpr (VOF) {
+    // t_p = mu_p X_p with mu_p = 0.5 (muf(f1) + muf(f2)); t_m likewise
+    const double xp = (wpp.v - wmp.v) * g.inv_d[A] + (vbp.v - v0.v) * g.inv_d[B];
+    const double xm = (wp0.v - wm0.v) * g.inv_d[A] + (v0.v - vbm.v) * g.inv_d[B];
+    mu_face_bar(g, AV, B, sh(q, B, +1), 0.5 * c * xp);
+    mu_face_bar(g, AV, B, sh(sh(q, B, +1), A, -1), 0.5 * c * xp);
+    mu_face_bar(g, AV, B, q, -0.5 * c * xm);
+    mu_face_bar(g, AV, B, sh(q, A, -1), -0.5 * c * xm);
+  }
 }
 
-// (d u*_A(q) / d u^n)^T w, scattered into U (the pressure term is separate)
-template <class BC, class PR, int A>
-__device__ void predict_face_vjp(const Geo& g, const Props& P, const Vel& V, const Acc& U, I3 q, double dt,
-                                 double w) {
+// (d u*_A(q) / d u^n)^T w, scattered into U (the pressure term is separate).
+// VOF: the step after a VoF transport -- mass flux with the accumulated alpha
+// flux (its cotangent to AV.Fb) and the property cotangents: beta = 1/rho_f
+// and mu of alpha^{n+1}, rho_old of alpha^n
+template <class BC, class PR, bool VOF, int A>
+__device__ void predict_face_vjp(const Geo& g, const Props& P, const Vel& V, const VofF& M, const double* p,
+                                 const Acc& U, const AccV& AV, I3 q, double dt, double w) {
   constexpr int B1 = A == 0 ? 1 : 0;
   constexpr int B2 = A == 2 ? 1 : 2;
   const TV v0 = tmem(V, A, IX(g, q));
@@ -254,16 +313,32 @@ __device__ void predict_face_vjp(const Geo& g, const Props& P, const Vel& V, con
   scat(U, vp1, k2 * muq);
   scat(U, v0, -k2 * muq - k2 * mucm);
   scat(U, vm1, k2 * mucm);
-  visc_vjp<BC, PR, A, B1>(g, P, V, U, q, v0, W);
-  visc_vjp<BC, PR, A, B2>(g, P, V, U, q, v0, W);
-  conv_vjp<BC, A, 0>(g, V, U, q, v0, vp1, vm1, -W);
-  conv_vjp<BC, A, 1>(g, V, U, q, v0, vp1, vm1, -W);
-  conv_vjp<BC, A, 2>(g, V, U, q, v0, vp1, vm1, -W);
+  visc_vjp<BC, PR, VOF, A, B1>(g, P, V, U, AV, q, v0, W);
+  visc_vjp<BC, PR, VOF, A, B2>(g, P, V, U, AV, q, v0, W);
+  conv_vjp<BC, VOF, A, 0>(g, V, M, AV, U, q, v0, vp1, vm1, -W);
+  conv_vjp<BC, VOF, A, 1>(g, V, M, AV, U, q, v0, vp1, vm1, -W);
+  conv_vjp<BC, VOF, A, 2>(g, V, M, AV, U, q, v0, vp1, vm1, -W);
+  if constexpr (VOF) {
+    const size_t iq = IX(g, q), ic = IX(g, cm);
+    // beta = 1/(0.5 (rho_q + rho_cm)): d beta / d rho_c = -0.5 beta^2
+    const double tmp = predict_tmp<typename BC::Fwd, PR, MfVof, A>(g, P, V, M, p, q);
+    const double bb = w * (v0.v * rho_old_f + dt * tmp);
+    const double kr = AV.drho * (-0.5 * beta * beta * bb);
+    aadd(AV.anp1, iq, kr);
+    aadd(AV.anp1, ic, kr);
+    // rho_old_f = 0.5 (rho_old_q + rho_old_cm)
+    const double ko = AV.drho * (0.5 * (w * beta * v0.v));
+    aadd(AV.an, iq, ko);
+    aadd(AV.an, ic, ko);
+    // own-axis stresses mu_q (vp1 - v0) and mu_cm (v0 - vm1)
+    aadd(AV.anp1, iq, AV.dmu * (k2 * (vp1.v - v0.v)));
+    aadd(AV.anp1, ic, AV.dmu * (-k2 * (v0.v - vm1.v)));
+  }
 }
 
-template <class BC, class PR, int A>
-__global__ __launch_bounds__(256) void k_predict_vjp(Geo g, Props P, Vel V, const double* __restrict__ wbar, Acc U,
-                                                     double dt) {
+template <class BC, class PR, bool VOF, int A>
+__global__ __launch_bounds__(256) void k_predict_vjp(Geo g, Props P, Vel V, VofF M, const double* __restrict__ p,
+                                                     const double* __restrict__ wbar, Acc U, AccV AV, double dt) {
   int lo[3] = {1, 1, 1}, hi[3] = {g.n[0], g.n[1], g.n[2]};
   lo[A] = g.per[A] ? 1 : 2;
   int i = lo[0] + (int)(blockIdx.x * 64 + threadIdx.x);
@@ -273,7 +348,7 @@ __global__ __launch_bounds__(256) void k_predict_vjp(Geo g, Props P, Vel V, cons
   I3 q = mk(i, j, k);
   const double w = wbar[IX(g, q)];
   if (w == 0.0) return;
-  predict_face_vjp<BC, PR, A>(g, P, V, U, q, dt, w);
+  predict_face_vjp<BC, PR, VOF, A>(g, P, V, M, p, U, AV, q, dt, w);
 }
 
 // out_c = pin_c + s * sum over the interior faces f of cell c of +-beta_f ub_f / h
@@ -348,18 +423,21 @@ __global__ __launch_bounds__(256) void k_rowsum_nat(Geo g, const double* __restr
   if (lane == 0) part0[row] = acc;
 }
 
-template <class BC, class PR>
-void predict_vjp_components(const Geo& g, const Props& P, const Vel& V, double* const wbar[3], const Acc& U,
-                            double dt, hipStream_t st) {
+template <class BC, class PR, bool VOF>
+void predict_vjp_components(const Geo& g, const Props& P, const Vel& V, const VofF& M, const double* p,
+                            double* const wbar[3], const Acc& U, const AccV& AV, double dt, hipStream_t st) {
   for (int a = 0; a < 3; a++) {
     int lo = g.per[a] ? 1 : 2;
     int nf[3] = {g.n[0], g.n[1], g.n[2]};
     nf[a] = g.n[a] - lo + 1;
     if (nf[0] <= 0 || nf[1] <= 0 || nf[2] <= 0) continue;
     dim3 cg((unsigned)((nf[0] + 63) / 64), (unsigned)((nf[1] + 3) / 4), (unsigned)nf[2]);
-    if (a == 0) hipLaunchKernelGGL((k_predict_vjp<BC, PR, 0>), cg, CELLB_A, 0, st, g, P, V, wbar[0], U, dt);
-    if (a == 1) hipLaunchKernelGGL((k_predict_vjp<BC, PR, 1>), cg, CELLB_A, 0, st, g, P, V, wbar[1], U, dt);
-    if (a == 2) hipLaunchKernelGGL((k_predict_vjp<BC, PR, 2>), cg, CELLB_A, 0, st, g, P, V, wbar[2], U, dt);
+    if (a == 0)
+      hipLaunchKernelGGL((k_predict_vjp<BC, PR, VOF, 0>), cg, CELLB_A, 0, st, g, P, V, M, p, wbar[0], U, AV, dt);
+    if (a == 1)
+      hipLaunchKernelGGL((k_predict_vjp<BC, PR, VOF, 1>), cg, CELLB_A, 0, st, g, P, V, M, p, wbar[1], U, AV, dt);
+    if (a == 2)
+      hipLaunchKernelGGL((k_predict_vjp<BC, PR, VOF, 2>), cg, CELLB_A, 0, st, g, P, V, M, p, wbar[2], U, AV, dt);
   }
 }
 
@@ -369,14 +447,34 @@ void launch_predict_vjp(const Geo& g, const Props* P, const Vel& V, double* cons
                         hipStream_t st) {
   Acc U;
   for (int d = 0; d < 3; d++) U.ub[d] = ub[d];
+  const VofF M0{};
+  const AccV AV0{};
   const bool allper = g.per[0] && g.per[1] && g.per[2];
   if (P) {
-    if (allper) predict_vjp_components<PeriodicT, VarP>(g, *P, V, wbar, U, dt, st);
-    else predict_vjp_components<WallT, VarP>(g, *P, V, wbar, U, dt, st);
+    if (allper) predict_vjp_components<PeriodicT, VarP, false>(g, *P, V, M0, nullptr, wbar, U, AV0, dt, st);
+    else predict_vjp_components<WallT, VarP, false>(g, *P, V, M0, nullptr, wbar, U, AV0, dt, st);
   } else {
-    if (allper) predict_vjp_components<PeriodicT, UniP>(g, Props{}, V, wbar, U, dt, st);
-    else predict_vjp_components<WallT, UniP>(g, Props{}, V, wbar, U, dt, st);
+    if (allper) predict_vjp_components<PeriodicT, UniP, false>(g, Props{}, V, M0, nullptr, wbar, U, AV0, dt, st);
+    else predict_vjp_components<WallT, UniP, false>(g, Props{}, V, M0, nullptr, wbar, U, AV0, dt, st);
   }
+}
+
+void launch_predict_vjp_vof(const Geo& g, const Props& P, const Vel& V, const VofF& M, const double* p,
+                            double* const wbar[3], double* ub[3], double* const Fb[3], double* anp1, double* an,
+                            double drho, double dmu, double dt, hipStream_t st) {
+  Acc U;
+  AccV AV;
+  for (int d = 0; d < 3; d++) {
+    U.ub[d] = ub[d];
+    AV.Fb[d] = Fb[d];
+  }
+  AV.anp1 = anp1;
+  AV.an = an;
+  AV.drho = drho;
+  AV.dmu = dmu;
+  const bool allper = g.per[0] && g.per[1] && g.per[2];
+  if (allper) predict_vjp_components<PeriodicT, VarP, true>(g, P, V, M, p, wbar, U, AV, dt, st);
+  else predict_vjp_components<WallT, VarP, true>(g, P, V, M, p, wbar, U, AV, dt, st);
 }
 
 void launch_adj_div(const Geo& g, const BetaF* bf, double* const ub[3], const double* pin, double* out, double s,

@@ -105,6 +105,12 @@ struct smac_ctx {
   // adjoint work arrays (box layout, allocated on first use): cotangents of
   // u^{n+1}, u*, u^n (3 faces each), of p^{n+1} and of p^n
   double* adj[11] = {nullptr};
+  // work arrays of smac_adjoint_vof_step (box layout): the input alpha of
+  // every transport sweep (3 S + 1), each substep's flux per axis (3 S), three alpha
+  // cotangent buffers, the rho_old part, the accumulated-flux cotangents (3)
+  // and the forward dp; vadj_S = the substep count they are sized for
+  std::vector<double*> vadj;
+  int vadj_S = 0;
   double* gradp[3] = {nullptr, nullptr, nullptr};  // f_gradp_{x,y,z}, allocated on first use
   // VoF transport (allocated on first use): alpha_new_, f_F{x,y,z}_,
   // f_F{x,y,z}_accum_, correct_coeff_; vof_armed = the accumulated flux
@@ -488,6 +494,7 @@ extern "C" void smac_destroy(smac_ctx* c) {
   for (auto a : arrs) dfree(a);
   for (int m = 0; m < 5; m++) dfree(c->part[m]);
   for (int m = 0; m < 11; m++) dfree(c->adj[m]);
+  for (auto a : c->vadj) dfree(a);
   dfree(c->vcoeff);
   dfree(c->cellout);
   if (c->Sh) (void)hipHostFree(c->Sh);
@@ -642,12 +649,19 @@ extern "C" int smac_download(smac_ctx* c, int32_t f, double* h, size_t count) {
 }
 
 extern "C" int smac_upload_device(smac_ctx* c, int32_t f, const double* dsrc, size_t count) {
-  if (!c || !dsrc || f < 0 || f >= SMAC_FIELD_COUNT || f == SMAC_FIELD_ALPHA) return SMAC_ERR_INVALID_ARG;
+  if (!c || !dsrc || f < 0 || f >= SMAC_FIELD_COUNT) return SMAC_ERR_INVALID_ARG;
   if (count != smac_field_count(c, f)) return SMAC_ERR_INVALID_ARG;
   double** slot = field_slot(c, f);
   if (!*slot) {
     int rc = dalloc(slot, c->tot);
     if (rc) return rc;
+  }
+  if (f == SMAC_FIELD_ALPHA) {
+    // not scanned on the device: the grid takes the property arrays (the
+    // same results as the uniform path when alpha is zero)
+    c->alpha_nonzero = true;
+    c->props_ready = false;
+    c->props_ver++;
   }
   return copy_field(c, f, *slot, dsrc, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
 }
@@ -1643,6 +1657,127 @@ extern "C" int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts*
     if (rc) return rc;
   }
   return copy_field(c, SMAC_FIELD_P, nullptr, nullptr, false, hipMemcpyDeviceToDevice, pbar, pbo);
+}
+
+// Adjoint of one step of the VoF loop (smac_transport_alpha(dt, S, step) then
+// smac_step(dt, opts)) from the state (alpha^n, u^n, p^n): the step is taken
+// again from that state with the transport's sweeps recorded, then the
+// cotangents run back through the correction, the solve (a second solve), the
+// predictor, the properties and the transport (smac_adjoint_vof.hip).
+extern "C" int smac_adjoint_vof_step(smac_ctx* c, double dt, int32_t S, int32_t step, const smac_solver_opts* opts,
+                                     const double* alpha_n, const double* const u_n[3], const double* p_n,
+                                     double* alpha_bar, double* const ubar[3], double* pbar, smac_solve_stats* st) {
+  if (!c || !(dt > 0) || S < 1 || S > 64 || step < 0 || !alpha_n || !u_n || !p_n || !alpha_bar || !ubar || !pbar)
+    return SMAC_ERR_INVALID_ARG;
+  for (int a = 0; a < 3; a++)
+    if (!u_n[a] || !ubar[a]) return SMAC_ERR_INVALID_ARG;
+  if (c->eps || c->bpois_on || c->has_outlet) return SMAC_ERR_UNSUPPORTED;
+  int rc = vof_alloc(c);
+  for (int m = 0; m < 11 && !rc; m++)
+    if (!c->adj[m]) rc = dalloc(&c->adj[m], c->tot);
+  if (rc) return rc;
+  const int nrec = 3 * S;
+  if (c->vadj_S != S) {
+    for (auto a : c->vadj) dfree(a);
+    c->vadj.assign((size_t)(2 * nrec + 9), nullptr);
+    c->vadj_S = 0;
+    for (auto& a : c->vadj)
+      if (!rc) rc = dalloc(&a, c->tot);
+    if (rc) return rc;
+    c->vadj_S = S;
+  }
+  double** ral = c->vadj.data();          // alpha before sweep r (r = 0..3S)
+  double** rF = c->vadj.data() + nrec + 1;  // flux of axis a in substep t: rF[3 t + a]
+  double* X = c->vadj[2 * nrec + 1];
+  double* Y = c->vadj[2 * nrec + 2];
+  double* Z = c->vadj[2 * nrec + 3];
+  double* anx = c->vadj[2 * nrec + 4];
+  double* Fb[3] = {c->vadj[2 * nrec + 5], c->vadj[2 * nrec + 6], c->vadj[2 * nrec + 7]};
+  double* dpf = c->vadj[2 * nrec + 8];
+  double* ubn[3] = {c->adj[0], c->adj[1], c->adj[2]};
+  double* ubs[3] = {c->adj[3], c->adj[4], c->adj[5]};
+  double* un[3] = {c->adj[6], c->adj[7], c->adj[8]};
+  double* pb = c->adj[9];
+  double* pbo = c->adj[10];
+  const Geo& g = c->g;
+  hipStream_t s = c->st;
+  const size_t bytes = c->tot * sizeof(double);
+  // ---- the step again, from (alpha^n, u^n, p^n)
+  rc = copy_field(c, SMAC_FIELD_ALPHA, ral[0], alpha_n, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+  for (int a = 0; a < 3 && !rc; a++)
+    rc = copy_field(c, SMAC_FIELD_U + a, c->u[a], u_n[a], true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+  if (!rc) rc = copy_field(c, SMAC_FIELD_P, c->p, p_n, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+  if (rc) return rc;
+  HIPOK(hipMemcpyAsync(c->alpha, ral[0], bytes, hipMemcpyDeviceToDevice, s));
+  c->alpha_nonzero = true;  // the property arrays of a two-phase grid
+  c->props_ver++;           // new face coefficients: the multigrid rebuilds its coarse ones
+  rc = smac_update_properties(c, 1);  // rho_old of the step = rho(alpha^n)
+  if (rc) return rc;
+  for (int a = 0; a < 3; a++) HIPOK(hipMemsetAsync(c->vfacc[a], 0, bytes, s));
+  Vel V;
+  for (int a = 0; a < 3; a++) V.c[a] = c->u[a];
+  const double sdt = dt / (double)S;
+  for (int t = 0; t < S; t++)
+    KT(c, "vof_rec", launch_vof_substep_rec(g, c->d, V, ral + 3 * t, rF + 3 * t, c->vfacc, c->vcoeff, sdt, step, s));
+  HIPOK(hipMemcpyAsync(c->alpha, ral[nrec], bytes, hipMemcpyDeviceToDevice, s));
+  c->vof_armed = true;
+  c->props_ver++;
+  rc = smac_update_properties(c, 0);
+  if (!rc) rc = smac_predictor(c, dt);
+  if (!rc) rc = pressure_solve(c, dt, opts, st, false);
+  if (rc) return rc;
+  HIPOK(hipMemcpyAsync(dpf, c->dp, bytes, hipMemcpyDeviceToDevice, s));
+  // ---- cotangents of the step's output, box layout
+  for (int a = 0; a < 3 && !rc; a++)
+    rc = copy_field(c, SMAC_FIELD_U + a, ubn[a], ubar[a], true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+  if (!rc) rc = copy_field(c, SMAC_FIELD_P, pb, pbar, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+  if (!rc) rc = copy_field(c, SMAC_FIELD_ALPHA, X, alpha_bar, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+  if (rc) return rc;
+  launch_rg_zero(anx, c->tot, s);
+  for (int a = 0; a < 3; a++) launch_rg_zero(Fb[a], c->tot, s);
+  const BetaF bf = beta_of(c);
+  const double drho = c->fl.rho1 - c->fl.rho0, dmu = c->fl.mu1 - c->fl.mu0;
+  // correction and solve: dp cotangent pb' + dt D_i(beta ub'), second solve
+  KT(c, "adj_div", launch_adj_div(g, &bf, ubn, pb, c->rhs, dt, s));
+  launch_rg_zero(c->dp, c->tot, s);
+  rc = pressure_solve(c, dt, opts, st, true);
+  if (rc) return rc;
+  // beta cotangents of the correction and the solve -> alpha^{n+1}
+  KT(c, "adj_beta", launch_adj_beta_cells(g, bf, ubn, c->dp, dpf, dt, drho, X, s));
+  KT(c, "adj_ustar", launch_adj_ustar(g, ubn, c->dp, ubs, un, 1. / dt, s));
+  KT(c, "adj_div", launch_adj_div(g, &bf, ubs, pb, pbo, dt, s));
+  // predictor: velocity, accumulated alpha flux and property cotangents
+  {
+    const Props P = props_of(c);
+    VofF M{};
+    for (int a = 0; a < 3; a++) {
+      M.F[a] = c->vfacc[a];
+      M.hbydt[a] = c->d[a] * (1. / dt);
+    }
+    M.drho = drho;
+    KT(c, "predict_vjp", launch_predict_vjp_vof(g, P, V, M, c->p, ubs, un, Fb, X, anx, drho, dmu, dt, s));
+  }
+  // transport, sweeps in reverse
+  const int first = step % 3;
+  for (int t = S - 1; t >= 0; t--) {
+    launch_vof_coeff(g, ral[3 * t], c->vcoeff, s);
+    for (int k = 2; k >= 0; k--) {
+      const int a = (first + k) % 3;
+      const int r = 3 * t + k;
+      launch_rg_zero(Z, c->tot, s);  // the ghost entries of the result stay zero
+      KT(c, "vof_adj", launch_vof_adj_sweep(g, c->d, V, a, ral[r], rF[3 * t + a], c->vcoeff, X, Y, Z, Fb[a], un[a],
+                                            sdt, k == 2, s));
+      std::swap(X, Z);
+    }
+  }
+  launch_add_box(X, anx, c->tot, s);
+  launch_rg_zero(c->dp, c->tot, s);
+  HIPOK(hipGetLastError());
+  for (int a = 0; a < 3 && !rc; a++)
+    rc = copy_field(c, SMAC_FIELD_U + a, nullptr, nullptr, false, hipMemcpyDeviceToDevice, ubar[a], un[a]);
+  if (!rc) rc = copy_field(c, SMAC_FIELD_P, nullptr, nullptr, false, hipMemcpyDeviceToDevice, pbar, pbo);
+  if (!rc) rc = copy_field(c, SMAC_FIELD_ALPHA, nullptr, nullptr, false, hipMemcpyDeviceToDevice, alpha_bar, X);
+  return rc;
 }
 
 extern "C" int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {

@@ -216,6 +216,21 @@ void launch_pack_cells(const Geo& g, const double* p, const double* al, const do
                        hipStream_t st);
 // one substep of the split VoF transport (smac_vof.hip); swaps *al and *an
 // once per direction, accumulates the face fluxes into Facc
+// the same substep recording each sweep's input alpha (al[0..3]) and flux
+// (F[axis]) for the adjoint
+void launch_vof_substep_rec(const Geo& g, const double h[3], const Vel& V, double* const al[4], double* const F[3],
+                            double* const Facc[3], unsigned char* coeff, double dt, int step, hipStream_t st);
+void launch_vof_coeff(const Geo& g, const double* al, unsigned char* coeff, hipStream_t st);
+// adjoint of the VoF loop's step (smac_adjoint.hip, smac_adjoint_vof.hip)
+void launch_predict_vjp_vof(const Geo& g, const Props& P, const Vel& V, const VofF& M, const double* p,
+                            double* const wbar[3], double* ub[3], double* const Fb[3], double* anp1, double* an,
+                            double drho, double dmu, double dt, hipStream_t st);
+void launch_adj_beta_cells(const Geo& g, const BetaF& bf, double* const ub[3], const double* bb, const double* dp,
+                           double dt, double drho, double* abar, hipStream_t st);
+void launch_vof_adj_sweep(const Geo& g, const double h[3], const Vel& V, int axis, const double* al, const double* F,
+                          const unsigned char* coeff, const double* aout, double* apre, double* acur,
+                          const double* Fbacc, double* ubar, double dt, bool clip, hipStream_t st);
+void launch_add_box(double* x, const double* y, size_t n, hipStream_t st);
 void launch_vof_substep(const Geo& g, const double h[3], const Vel& V, double** al, double** an, double* const F[3],
                         double* const Facc[3], unsigned char* coeff, double dt, int step, hipStream_t st);
 // multigrid (smac_mg.hip), storage T in {double, float, bf16s}; the restriction

