@@ -161,6 +161,26 @@ typedef struct smac_solve_stats {
     double linf_res_dt;          /* max|r|*dt at exit                         */
     double bnorm;                /* ||b - mean b||_2                          */
     double ms_predictor, ms_solve, ms_correct; /* device time, when profiling */
+    int32_t nan_detected;        /* 1: ||b||^2 or a checked residual was NaN/Inf;
+                                    the solve stopped there and the call returned
+                                    SMAC_ERR_NAN (the reference has no check:
+                                    check_nan_all is declared, never defined,
+                                    CFD/G_SMACSolver.h) */
+    int32_t reserved;
+    double bytes_model;          /* algorithmic HBM bytes of the call (whole
+                                    arrays read or written once per pass, per
+                                    interior cell x cells): RHS 32 (8 for a
+                                    given RHS, +16 with the volume fraction),
+                                    PCG start 32 + P, per iteration
+                                    104 + P (+16 flexible, +24 face
+                                    coefficients); P = one V-cycle, 17.43 W_s
+                                    (+17.14 W_s face coefficients), W_s the
+                                    smoother width 8/4/2; Jacobi P = 16 (+24);
+                                    the reference GMG: 8 sweeps x 3 words +
+                                    4.25 words transfers per level, 90 words
+                                    coarsest, geometric 1/8 per level. smac_step
+                                    adds predictor 56 and correction 72 (+24
+                                    each with face coefficients). */
 } smac_solve_stats;
 
 typedef struct smac_ctx smac_ctx;
@@ -199,6 +219,28 @@ int smac_upload_device(smac_ctx* ctx, int32_t field, const double* dev, size_t c
 int smac_download_device(smac_ctx* ctx, int32_t field, double* dev, size_t count);
 /* Element count of a field in the reference layout. */
 size_t smac_field_count(const smac_ctx* ctx, int32_t field);
+/* Device layout of every field: a box of planes x pitch_y x pitch_x doubles,
+ * element (i, j, k) of the reference index at (k*pitch_y + j)*pitch_x + i
+ * (pitch_x >= Nx+3, rows 128-byte aligned; pitch_y = Ny+3; planes = Nz+3). The
+ * reference-layout extent of a field is the sub-box from the origin; the rest is
+ * padding. Extension (the reference keeps MyArray's dense layout). */
+int smac_device_layout(const smac_ctx* ctx, int64_t* pitch_x, int64_t* pitch_y, int64_t* planes);
+/* Zero-copy binding (e.g. of a torch tensor): from this call the context reads
+ * and writes `dev` -- device memory of the context's GPU, 256-byte aligned,
+ * count = pitch_x*pitch_y*planes doubles in the device layout above -- as
+ * field U, V, W, P or ALPHA in place of its own array, and takes its contents
+ * as the field's values. Borrowed: the context never frees it; the caller keeps
+ * it alive and orders its own streams' work on it with the context stream, as
+ * for smac_upload_device. dev = NULL (count 0) unbinds: the context copies the
+ * bound contents into its own array and uses that again. smac_destroy drops
+ * bindings without copying. Binding ALPHA marks the grid two-phase (as a device
+ * upload of ALPHA); the VoF transport writes its result into the bound array.
+ * Periodic images and ghost entries are the context's working values (a host
+ * download fills periodic images; the bound array is not rewritten for them).
+ * Returns SMAC_ERR_UNSUPPORTED for other fields, SMAC_ERR_INVALID_ARG for a
+ * wrong count, misaligned or non-device pointer. Extension: the reference's
+ * cpuTogpu/gpuTocpu (CFD/G_SMACSolver.cu:1507-1535) always copy. */
+int smac_bind_device(smac_ctx* ctx, int32_t field, double* dev, size_t count);
 
 
 /* ---- the step ---------------------------------------------------------------

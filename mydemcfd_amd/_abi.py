@@ -69,7 +69,8 @@ class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("pure_neumann", C.c_int32),
                 ("mg_levels", C.c_int32), ("rel_res", C.c_double), ("linf_res_dt", C.c_double),
                 ("bnorm", C.c_double), ("ms_predictor", C.c_double), ("ms_solve", C.c_double),
-                ("ms_correct", C.c_double)]
+                ("ms_correct", C.c_double), ("nan_detected", C.c_int32), ("reserved", C.c_int32),
+                ("bytes_model", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -88,6 +89,8 @@ SIGNATURES = {
     "smac_upload_device": (C.c_int, [_P, C.c_int32, _P, C.c_size_t]),
     "smac_download_device": (C.c_int, [_P, C.c_int32, _P, C.c_size_t]),
     "smac_field_count": (C.c_size_t, [_P, C.c_int32]),
+    "smac_device_layout": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "smac_bind_device": (C.c_int, [_P, C.c_int32, _P, C.c_size_t]),
     "smac_update_properties": (C.c_int, [_P, C.c_int32]),
     "smac_compute_mass_flux": (C.c_int, [_P, C.c_double]),
     "smac_transport_alpha": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32]),
@@ -133,6 +136,7 @@ class SmacError(RuntimeError):
         msg = lib.smac_strerror(status).decode() if lib is not None else str(status)
         super().__init__(f"{what} failed: {msg} ({status})")
         self.status = status
+        self.stats = None  # the SolveStats of a failed solve or step (e.g. nan_detected)
 
 
 def default_opts(lib, **kw) -> SolverOpts:
@@ -190,9 +194,11 @@ class Context:
         self.size = tuple(size)
         self.periodic = tuple(int(v) for v in periodic)
 
-    def _check(self, rc, what):
+    def _check(self, rc, what, stats=None):
         if rc != SMAC_OK:
-            raise SmacError(self.lib, rc, what)
+            e = SmacError(self.lib, rc, what)
+            e.stats = stats
+            raise e
 
     def close(self):
         if getattr(self, "h", None):
@@ -231,6 +237,19 @@ class Context:
     def download_device(self, f, ptr: int, count: int):
         self._check(self.lib.smac_download_device(self.h, self._field(f), ptr, count), "smac_download_device")
 
+    def device_layout(self):
+        """(planes, pitch_y, pitch_x): the device box every field lives in; the
+        reference-layout field is the sub-box [:sz, :sy, :sx] of shape(f)."""
+        px, py, pz = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.smac_device_layout(self.h, C.byref(px), C.byref(py), C.byref(pz)), "smac_device_layout")
+        return pz.value, py.value, px.value
+
+    def bind_device(self, f, ptr, count: int):
+        """Zero-copy: the context uses the device buffer at `ptr` (device_layout
+        box, `count` doubles) as field f (u, v, w, p or alpha) until
+        bind_device(f, None, 0)."""
+        self._check(self.lib.smac_bind_device(self.h, self._field(f), ptr, count), "smac_bind_device")
+
     def set_fluid(self, rho=(1.0, 1.0), mu=(0.0, 0.0), g=(0.0, 0.0, 0.0)):
         self.fluid = FluidDesc(rho[0], rho[1], mu[0], mu[1], g[0], g[1], g[2])
         self._check(self.lib.smac_set_fluid(self.h, C.byref(self.fluid)), "smac_set_fluid")
@@ -254,7 +273,7 @@ class Context:
     def pressure_solve(self, dt, opts=None, **kw):
         o = opts if opts is not None else default_opts(self.lib, **kw)
         st = SolveStats()
-        self._check(self.lib.smac_pressure_solve(self.h, dt, C.byref(o), C.byref(st)), "smac_pressure_solve")
+        self._check(self.lib.smac_pressure_solve(self.h, dt, C.byref(o), C.byref(st)), "smac_pressure_solve", st)
         return st
 
     def correct(self, dt):
@@ -263,7 +282,7 @@ class Context:
     def step(self, dt, opts=None, **kw):
         o = opts if opts is not None else default_opts(self.lib, **kw)
         st = SolveStats()
-        self._check(self.lib.smac_step(self.h, dt, C.byref(o), C.byref(st)), "smac_step")
+        self._check(self.lib.smac_step(self.h, dt, C.byref(o), C.byref(st)), "smac_step", st)
         return st
 
     def adjoint_step(self, dt, u_n, ubar, pbar, opts=None, **kw):

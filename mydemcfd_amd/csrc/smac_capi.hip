@@ -111,6 +111,10 @@ struct smac_ctx {
   // and the forward dp; vadj_S = the substep count they are sized for
   std::vector<double*> vadj;
   int vadj_S = 0;
+  // zero-copy bindings (smac_bind_device): the borrowed array of a field, and
+  // the context's own array it stands in for (restored on unbind)
+  double* bound[SMAC_FIELD_COUNT] = {};
+  double* own[SMAC_FIELD_COUNT] = {};
   double* gradp[3] = {nullptr, nullptr, nullptr};  // f_gradp_{x,y,z}, allocated on first use
   // VoF transport (allocated on first use): alpha_new_, f_F{x,y,z}_,
   // f_F{x,y,z}_accum_, correct_coeff_; vof_armed = the accumulated flux
@@ -479,9 +483,13 @@ static void rg_free(smac_ctx* c) {
   c->rgt = nullptr;
 }
 
+static double** field_slot(smac_ctx* c, int f);
+
 extern "C" void smac_destroy(smac_ctx* c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  for (int f = 0; f < SMAC_FIELD_COUNT; f++)  // borrowed arrays are the caller's
+    if (c->bound[f]) *field_slot(c, f) = c->own[f];
   mg_free(c);
   rg_free(c);
   double* arrs[] = {c->u[0],   c->u[1],    c->u[2],    c->us[0],   c->us[1],  c->us[2],  c->p,      c->dp,
@@ -664,6 +672,53 @@ extern "C" int smac_upload_device(smac_ctx* c, int32_t f, const double* dsrc, si
     c->props_ver++;
   }
   return copy_field(c, f, *slot, dsrc, true, hipMemcpyDeviceToDevice, nullptr, nullptr);
+}
+
+extern "C" int smac_device_layout(const smac_ctx* c, int64_t* pitch_x, int64_t* pitch_y, int64_t* planes) {
+  if (!c || !pitch_x || !pitch_y || !planes) return SMAC_ERR_INVALID_ARG;
+  *pitch_x = c->g.px;
+  *pitch_y = c->g.py;
+  *planes = c->n[2] + 3;
+  return SMAC_OK;
+}
+
+extern "C" int smac_bind_device(smac_ctx* c, int32_t f, double* dev, size_t count) {
+  if (!c || f < 0 || f >= SMAC_FIELD_COUNT) return SMAC_ERR_INVALID_ARG;
+  if (f != SMAC_FIELD_U && f != SMAC_FIELD_V && f != SMAC_FIELD_W && f != SMAC_FIELD_P && f != SMAC_FIELD_ALPHA)
+    return SMAC_ERR_UNSUPPORTED;
+  double** slot = field_slot(c, f);
+  if (!dev) {  // unbind: the bound contents move into the context's own array
+    if (count != 0) return SMAC_ERR_INVALID_ARG;
+    if (!c->bound[f]) return SMAC_OK;
+    if (!c->own[f]) {
+      int rc = dalloc(&c->own[f], c->tot);
+      if (rc) return rc;
+    }
+    HIPOK(hipMemcpyAsync(c->own[f], c->bound[f], c->tot * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    HIPOK(hipStreamSynchronize(c->st));
+    *slot = c->own[f];
+    c->own[f] = nullptr;
+    c->bound[f] = nullptr;
+    return SMAC_OK;
+  }
+  if (count != c->tot || (reinterpret_cast<uintptr_t>(dev) & 255u) != 0) return SMAC_ERR_INVALID_ARG;
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return SMAC_ERR_INVALID_ARG;
+  }
+  if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != c->dev)
+    return SMAC_ERR_INVALID_ARG;
+  HIPOK(hipStreamSynchronize(c->st));  // work queued on the old array ends first
+  if (!c->bound[f]) c->own[f] = *slot;
+  c->bound[f] = dev;
+  *slot = dev;
+  if (f == SMAC_FIELD_ALPHA) {
+    c->alpha_nonzero = true;
+    c->props_ready = false;
+    c->props_ver++;
+  }
+  return SMAC_OK;
 }
 
 extern "C" int smac_download_device(smac_ctx* c, int32_t f, double* ddst, size_t count) {
@@ -1234,6 +1289,28 @@ extern "C" void smac_default_opts(smac_solver_opts* o) {
 // T, r, d, Ad and dp in FP64. For T = double z is c->z and the fine level works
 // on r directly; otherwise the fine-level rhs f0 = (T) r is refreshed by the
 // update kernel.
+// Algorithmic HBM bytes per interior cell of one pressure solve: each array
+// read or written once per pass (the canonical model of the step; halo re-reads
+// and the implementation's extra passes are not counted). W = smoother width.
+static double solve_bytes_model(const smac_ctx* c, const smac_solver_opts& o, int L, int iters, bool rhs_given) {
+  const bool var = pois_var(c);
+  const double W = o.precision == SMAC_PREC_MIXED_FP32 ? 4.0 : o.precision == SMAC_PREC_MIXED_BF16 ? 2.0 : 8.0;
+  double P;  // one application of the preconditioner
+  if (o.precond == SMAC_PRECOND_MG) {
+    P = 17.43 * W + (var ? (15.0 * 8.0 / 7.0) * W : 0.0);
+  } else if (o.precond == SMAC_PRECOND_JACOBI) {
+    P = 16.0 + (var ? 24.0 : 0.0);
+  } else {  // the reference V-cycle: 8 sweeps of 3 words + 4.25 words of transfers per level, 30 coarsest sweeps
+    double words = 0.0, f = 1.0;
+    for (int l = 0; l < L - 1; l++, f /= 8.0) words += f * ((8.0 * 3.0 + 4.25) + (var ? 8.0 * 3.0 + 3.0 : 0.0));
+    words += f * 30.0 * (var ? 6.0 : 3.0);
+    P = words * 8.0;
+  }
+  const double rhs = rhs_given ? 8.0 : 32.0 + ((c->eps && c->eps_old) ? 16.0 : 0.0);
+  const double it = 104.0 + P + (o.flexible ? 16.0 : 0.0) + (var ? 24.0 : 0.0);
+  return rhs + (32.0 + P) + iters * it;
+}
+
 template <typename T>
 static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve_stats* st, int L, bool rhs_given) {
   const Geo& g = c->g;
@@ -1307,6 +1384,8 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
   int iter = 0, conv = 0;
   double rr = bb, linf = 0.0;
   if (bb == 0.0) conv = 1;
+  // a NaN/Inf right-hand side (a NaN or overflow in u*) stops the solve at once
+  bool nan = !std::isfinite(bb);
   bool have_update = false, dp_pending = false;
   // stationary iteration: alpha = 1, beta = 0 (d = z) throughout
   const bool stat = o.stationary != 0;
@@ -1314,7 +1393,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     launch_scalar_set(c->S, S_ALPHA, 1.0, s);
     launch_scalar_set(c->S, S_BETA, 0.0, s);
   }
-  for (iter = 0; iter < o.max_iter && !conv; ++iter) {
+  for (iter = 0; iter < o.max_iter && !conv && !nan; ++iter) {
     KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
     if (!stat) {
       FinalizeArgs fa = fin(c);
@@ -1372,6 +1451,11 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       HIPOK(hipGetLastError());
       rc = read_scalars(c);
       if (rc) return rc;
+      if (!std::isfinite(c->Sh[S_RR]) || !std::isfinite(c->Sh[S_LINF])) {
+        iter++;
+        nan = true;
+        break;
+      }
       if (o.stop_rule == SMAC_STOP_REF_LINF_DT) {
         linf = c->Sh[S_LINF] * dt;
         if (linf < o.tol) {
@@ -1419,8 +1503,10 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       st->linf_res_dt = 0.0;
     }
     st->rel_res = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
+    st->nan_detected = nan || !(rr == rr);
+    st->bytes_model = solve_bytes_model(c, o, L, iter, rhs_given) * Ncell;
   }
-  if (!(bb == bb) || !(rr == rr)) return SMAC_ERR_NAN;
+  if (nan || !(bb == bb) || !(rr == rr)) return SMAC_ERR_NAN;
   return SMAC_OK;
 }
 
@@ -1514,6 +1600,8 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
   sp->ms_predictor = t_pred;
   sp->ms_solve = t_solve;
   sp->ms_correct = t_corr;
+  // predictor 56 and correction 72 bytes per cell (+24 each with property arrays)
+  sp->bytes_model += (128.0 + (var_props(c) ? 48.0 : 0.0)) * (double)c->n[0] * (double)c->n[1] * (double)c->n[2];
   return rc;
 }
 
@@ -1546,6 +1634,11 @@ extern "C" int smac_alpha_flux_split(smac_ctx* c, double sub_dt, int32_t step) {
   KT(c, "vof_substep", launch_vof_substep(c->g, c->d, V, &c->alpha, &c->alpha_new, c->vf, c->vfacc, c->vcoeff,
                                           sub_dt, step, c->st));
   HIPOK(hipGetLastError());
+  if (c->bound[SMAC_FIELD_ALPHA] && c->alpha != c->bound[SMAC_FIELD_ALPHA]) {
+    // the sweeps ping-pong alpha and its scratch array: a bound alpha takes the result back
+    HIPOK(hipMemcpyAsync(c->alpha_new, c->alpha, c->tot * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    std::swap(c->alpha, c->alpha_new);
+  }
   c->vof_armed = true;
   c->props_ver++;  // the face coefficients follow the new alpha at the next property update
   return SMAC_OK;

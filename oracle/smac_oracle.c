@@ -1448,6 +1448,34 @@ void smac_default_opts(smac_solver_opts* o) {
     o->stationary = 0;
 }
 
+/* Algorithmic HBM bytes per interior cell of one solve, the model the HIP
+ * library reports in smac_solve_stats.bytes_model (each array read or written
+ * once per pass; W = smoother width). */
+static int var_props(const smac_ctx* c) {
+    if (c->fl.rho0 != c->fl.rho1 || c->fl.mu0 != c->fl.mu1 || c->has_outlet) return 1;
+    for (size_t t = 0; t < c->tot; t++)
+        if (c->alpha[t] != 0.0) return 1;
+    return 0;
+}
+static double solve_bytes_model(const smac_ctx* c, const smac_solver_opts* o, int L, int iters) {
+    int var = c->bpois_on || var_props(c);
+    double W = o->precision == SMAC_PREC_MIXED_FP32 ? 4.0 : o->precision == SMAC_PREC_MIXED_BF16 ? 2.0 : 8.0;
+    double P;
+    if (o->precond == SMAC_PRECOND_MG) {
+        P = 17.43 * W + (var ? (15.0 * 8.0 / 7.0) * W : 0.0);
+    } else if (o->precond == SMAC_PRECOND_JACOBI) {
+        P = 16.0 + (var ? 24.0 : 0.0);
+    } else {
+        double words = 0.0, f = 1.0;
+        for (int l = 0; l < L - 1; l++, f /= 8.0) words += f * ((8.0 * 3.0 + 4.25) + (var ? 8.0 * 3.0 + 3.0 : 0.0));
+        words += f * 30.0 * (var ? 6.0 : 3.0);
+        P = words * 8.0;
+    }
+    double rhs = 32.0 + ((c->eps && c->eps_old) ? 16.0 : 0.0);
+    double it = 104.0 + P + (o->flexible ? 16.0 : 0.0) + (var ? 24.0 : 0.0);
+    return rhs + (32.0 + P) + iters * it;
+}
+
 int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, smac_solve_stats* st) {
     if (!c || !(dt > 0)) return SMAC_ERR_INVALID_ARG;
     smac_solver_opts o;
@@ -1499,10 +1527,13 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
     int iter = 0, conv = 0;
     double rr = bb, linf = 0.0;
     if (bb == 0.0) conv = 1;
+    /* a NaN/Inf right-hand side stops the solve at once (the reference has no
+     * check: check_nan_all is declared, never defined, CFD/G_SMACSolver.h:88) */
+    int nan = !isfinite(bb);
     /* |r|^2 after an update: pair order from the update itself, natural order
      * where the mean of r is removed after it */
     const int rr_mode = pure && o.precond == SMAC_PRECOND_REF_GMG ? RED_SQ : RED_SQ_S2;
-    for (iter = 0; iter < o.max_iter && !conv; ++iter) {
+    for (iter = 0; iter < o.max_iter && !conv && !nan; ++iter) {
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= ny; j++)
@@ -1527,6 +1558,7 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
         double zAd = 0.0;
         if (o.flexible) zAd = csum(c, RED_DOT_S2, c->z, c->Ad);
         if (iter % o.check_every == 0) {
+            if (!isfinite(csum(c, rr_mode, c->r, NULL))) { iter++; nan = 1; break; }
             if (o.stop_rule == SMAC_STOP_REF_LINF_DT) {
                 linf = cmaxabs(c, c->r) * dt;
                 if (linf < o.tol) { iter++; conv = 1; break; }
@@ -1555,7 +1587,10 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
         st->bnorm = bnorm;
         st->rel_res = bnorm > 0 ? sqrt(rr) / bnorm : 0.0;
         st->linf_res_dt = cmaxabs(c, c->r) * dt;
+        st->nan_detected = nan || !(rr == rr);
+        st->bytes_model = solve_bytes_model(c, &o, L, iter) * Ncell;
     }
+    if (nan || !(bb == bb) || !(rr == rr)) return SMAC_ERR_NAN;
     return SMAC_OK;
 }
 
@@ -1595,6 +1630,8 @@ int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac_solve_stat
     else if (c->fl.rho0 != c->fl.rho1 || c->fl.mu0 != c->fl.mu1) smac_update_properties(c, 0);
     if ((rc = smac_predictor(c, dt))) return rc;
     if ((rc = smac_pressure_solve(c, dt, o, st))) return rc;
+    /* predictor 56 and correction 72 bytes per cell (+24 each with property arrays) */
+    if (st) st->bytes_model += (128.0 + (var_props(c) ? 48.0 : 0.0)) * (double)c->n[0] * c->n[1] * c->n[2];
     return smac_correct(c, dt);
 }
 
@@ -2184,6 +2221,18 @@ int smac_upload_device(smac_ctx* c, int32_t f, const double* d, size_t n) {
     return SMAC_ERR_UNSUPPORTED;
 }
 int smac_download_device(smac_ctx* c, int32_t f, double* d, size_t n) {
+    (void)c; (void)f; (void)d; (void)n;
+    return SMAC_ERR_UNSUPPORTED;
+}
+/* The oracle's arrays are host boxes of (Nx+3)(Ny+3)(Nz+3); nothing binds. */
+int smac_device_layout(const smac_ctx* c, int64_t* px, int64_t* py, int64_t* planes) {
+    if (!c || !px || !py || !planes) return SMAC_ERR_INVALID_ARG;
+    *px = c->n[0] + 3;
+    *py = c->n[1] + 3;
+    *planes = c->n[2] + 3;
+    return SMAC_OK;
+}
+int smac_bind_device(smac_ctx* c, int32_t f, double* d, size_t n) {
     (void)c; (void)f; (void)d; (void)n;
     return SMAC_ERR_UNSUPPORTED;
 }
