@@ -787,17 +787,40 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
 
 // Row sums over the solver's split vectors in pair order: lane l accumulates the
 // split positions q = 1+2l+128m, 1+2l+128m+1 (m = 0, 1, ...; q <= nx) in that
-// order (the oracle's RED_*_S2 sums). NM > 0 (nx = 128 NM): the pairs are
-// read and written as aligned double2 (rows start on 128-byte lines);
-// NM = 0: any nx, one element at a time in the same order.
+// order (the oracle's RED_*_S2 sums), then the canonical tree over the lanes.
+//
+// Chunk form (rows of 128 NM cells, NM = 1..4, element 1 of every row on a
+// 128-byte line): a wave owns one 128-position chunk c of one row -- lane l the
+// split positions 1+2l+128c and +1, one 16-byte access per double array -- and
+// a block owns RB whole rows (RB = 4/NM; NM = 3: one row, 192 threads), blocks
+// in plane order. A wave walking a whole row with NM accesses per lane per
+// array, in the band order of the stencil kernels, streams 20-25% slower. The
+// row partials keep the pair order: each chunk wave leaves its lane's two
+// values in LDS and the row's chunk-0 wave adds them chunk by chunk.
+template <int NM>
+struct ChunkGeom {
+  static constexpr int RB = NM == 3 ? 1 : 4 / NM;  // rows per block
+  static constexpr int THREADS = 64 * NM * RB;
+};
+#define CHUNK_SETUP(NM)                                                                    \
+  constexpr int RB_ = ChunkGeom<NM>::RB;                                                   \
+  const int lane = threadIdx.x & 63;                                                       \
+  const int wv = (int)(threadIdx.x >> 6);                                                  \
+  const int rib = wv / (NM), ch = wv % (NM);                                               \
+  const int row = (int)blockIdx.x * RB_ + rib;                                             \
+  const bool live = row < g.n[1] * g.n[2];                                                 \
+  const int j = live ? row % g.n[1] + 1 : 1, k = live ? row / g.n[1] + 1 : 1;              \
+  const size_t rb = ((size_t)k * (size_t)g.py + (size_t)j) * (size_t)g.px;                 \
+  const size_t o = rb + 1 + 2 * lane + 128 * ch;                                           \
+  (void)o;
 
 // r -= alpha Ad (the residual half of k_update_p_r, G_PCGSolver.cu:66-85);
 // part0 <- sum r^2 (pair order), part1 <- max |r|. The dp += alpha d half is
 // deferred to the next k_xpay, which reads d anyway, or to k_dp_flush after
 // the last iteration (the same dp + alpha d per cell).
 // F: multigrid storage; when f0 is given it receives (F) r (the fine-level rhs
-// of the next V-cycle in the mixed forms)
-template <typename F, int NM>
+// of the next V-cycle in the mixed forms). Row form: any nx.
+template <typename F>
 __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ r, const double* __restrict__ Ad,
                                                 const double* __restrict__ S, int slot_alpha,
                                                 double* __restrict__ part0, double* __restrict__ part1,
@@ -807,44 +830,18 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ r, c
   const int nx = g.n[0];
   const double alpha = S[slot_alpha];
   double a0 = 0.0, mx = 0.0;
-  if constexpr (NM > 0) {
-    double vr[NM][2], vA[NM][2];
+  for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
 #pragma unroll
-    for (int m = 0; m < NM; m++) {
-      const size_t o = rb + 1 + 2 * lane + 128 * m;
-      Vec<double, 2>::ld(r + o, vr[m]);
-      Vec<double, 2>::ld(Ad + o, vA[m]);
-    }
-#pragma unroll
-    for (int m = 0; m < NM; m++) {
-      const size_t o = rb + 1 + 2 * lane + 128 * m;
-      double rn[2];
-      C fv[2];
-#pragma unroll
-      for (int t = 0; t < 2; t++) {
-        rn[t] = vr[m][t] - alpha * vA[m][t];
-        fv[t] = (C)rn[t];
-        a0 += rn[t] * rn[t];
-        double ar = fabs(rn[t]);
+    for (int t = 0; t < 2; t++) {
+      const int q = q0 + t;
+      if (q <= nx) {
+        const size_t o = rb + q;
+        double rn = r[o] - alpha * Ad[o];
+        r[o] = rn;
+        if (f0) St<F>::st(f0, o, (C)rn);
+        a0 += rn * rn;
+        double ar = fabs(rn);
         if (ar > mx || ar != ar) mx = ar;
-      }
-      Vec<double, 2>::st(r + o, rn);
-      if (f0) Vec<F, 2>::st(f0 + o, fv);
-    }
-  } else {
-    for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
-#pragma unroll
-      for (int t = 0; t < 2; t++) {
-        const int q = q0 + t;
-        if (q <= nx) {
-          const size_t o = rb + q;
-          double rn = r[o] - alpha * Ad[o];
-          r[o] = rn;
-          if (f0) St<F>::st(f0, o, (C)rn);
-          a0 += rn * rn;
-          double ar = fabs(rn);
-          if (ar > mx || ar != ar) mx = ar;
-        }
       }
     }
   }
@@ -853,6 +850,53 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ r, c
   if (lane == 0) {
     part0[row] = a0;
     part1[row] = mx;
+  }
+}
+
+// chunk form of k_update (same arithmetic and sums)
+template <typename F, int NM>
+__global__ __launch_bounds__(256) void k_update_ch(Geo g, double* __restrict__ r, const double* __restrict__ Ad,
+                                                   const double* __restrict__ S, int slot_alpha,
+                                                   double* __restrict__ part0, double* __restrict__ part1,
+                                                   F* __restrict__ f0) {
+  using C = typename St<F>::C;
+  __shared__ double rv[ChunkGeom<NM>::RB][NM][64][2];
+  CHUNK_SETUP(NM)
+  double rn[2] = {0.0, 0.0};
+  if (live) {
+    const double alpha = S[slot_alpha];
+    double vr[2], vA[2];
+    Vec<double, 2>::ld(r + o, vr);
+    Vec<double, 2>::ld(Ad + o, vA);
+    C fv[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      rn[t] = vr[t] - alpha * vA[t];
+      fv[t] = (C)rn[t];
+    }
+    Vec<double, 2>::st(r + o, rn);
+    if (f0) Vec<F, 2>::st(f0 + o, fv);
+  }
+  rv[rib][ch][lane][0] = rn[0];
+  rv[rib][ch][lane][1] = rn[1];
+  __syncthreads();
+  if (live && ch == 0) {
+    double a0 = 0.0, mx = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const double v = rv[rib][m][lane][t];
+        a0 += v * v;
+        const double ar = fabs(v);
+        if (ar > mx || ar != ar) mx = ar;
+      }
+    a0 = wave_tree_sum(a0);
+    mx = wave_max(mx);
+    if (lane == 0) {
+      part0[row] = a0;
+      part1[row] = mx;
+    }
   }
 }
 
@@ -865,43 +909,24 @@ __global__ __launch_bounds__(256) void k_dp_flush(Geo g, double* __restrict__ dp
   for (int i = 1 + lane; i <= g.n[0]; i += 64) dp[rb + i] = dp[rb + i] + alpha * d[rb + spos(i, g.n[0])];
 }
 
-// part0 <- sum r*z; part1 <- sum z*Ad (flexible only), both in pair order
-template <typename Z, int NM>
+// part0 <- sum r*z; part1 <- sum z*Ad (flexible only), both in pair order.
+// Row form: any nx.
+template <typename Z>
 __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ r, const Z* __restrict__ z,
                                               const double* __restrict__ Ad, int flexible,
                                               double* __restrict__ part0, double* __restrict__ part1) {
-  using C = typename St<Z>::C;
   ROW_SETUP
   const int nx = g.n[0];
   double a0 = 0.0, a1 = 0.0;
-  if constexpr (NM > 0) {
-    double vr[NM][2], vA[NM][2];
-    C vz[NM][2];
+  for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
 #pragma unroll
-    for (int m = 0; m < NM; m++) {
-      const size_t o = rb + 1 + 2 * lane + 128 * m;
-      Vec<double, 2>::ld(r + o, vr[m]);
-      Vec<Z, 2>::ld(z + o, vz[m]);
-      if (flexible) Vec<double, 2>::ld(Ad + o, vA[m]);
-    }
-#pragma unroll
-    for (int m = 0; m < NM; m++)
-#pragma unroll
-      for (int t = 0; t < 2; t++) {
-        a0 += vr[m][t] * (double)vz[m][t];
-        if (flexible) a1 += (double)vz[m][t] * vA[m][t];
-      }
-  } else {
-    for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
-#pragma unroll
-      for (int t = 0; t < 2; t++) {
-        const int q = q0 + t;
-        if (q <= nx) {
-          const size_t o = rb + q;
-          const double zz = (double)St<Z>::ld(z, o);
-          a0 += r[o] * zz;
-          if (flexible) a1 += zz * Ad[o];
-        }
+    for (int t = 0; t < 2; t++) {
+      const int q = q0 + t;
+      if (q <= nx) {
+        const size_t o = rb + q;
+        const double zz = (double)St<Z>::ld(z, o);
+        a0 += r[o] * zz;
+        if (flexible) a1 += zz * Ad[o];
       }
     }
   }
@@ -910,6 +935,51 @@ __global__ __launch_bounds__(256) void k_dot2(Geo g, const double* __restrict__ 
   if (lane == 0) {
     part0[row] = a0;
     if (flexible) part1[row] = a1;
+  }
+}
+
+// chunk form of k_dot2 (the same products, summed in the same order)
+template <typename Z, int NM>
+__global__ __launch_bounds__(256) void k_dot2_ch(Geo g, const double* __restrict__ r, const Z* __restrict__ z,
+                                                 const double* __restrict__ Ad, int flexible,
+                                                 double* __restrict__ part0, double* __restrict__ part1) {
+  using C = typename St<Z>::C;
+  __shared__ double pr[ChunkGeom<NM>::RB][NM][64][2];
+  __shared__ double pa[ChunkGeom<NM>::RB][NM][64][2];
+  CHUNK_SETUP(NM)
+  double t0[2] = {0.0, 0.0}, t1[2] = {0.0, 0.0};
+  if (live) {
+    double vr[2], vA[2];
+    C vz[2];
+    Vec<double, 2>::ld(r + o, vr);
+    Vec<Z, 2>::ld(z + o, vz);
+    if (flexible) Vec<double, 2>::ld(Ad + o, vA);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      t0[t] = vr[t] * (double)vz[t];
+      if (flexible) t1[t] = (double)vz[t] * vA[t];
+    }
+  }
+  pr[rib][ch][lane][0] = t0[0];
+  pr[rib][ch][lane][1] = t0[1];
+  pa[rib][ch][lane][0] = t1[0];
+  pa[rib][ch][lane][1] = t1[1];
+  __syncthreads();
+  if (live && ch == 0) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        a0 += pr[rib][m][lane][t];
+        if (flexible) a1 += pa[rib][m][lane][t];
+      }
+    a0 = wave_tree_sum(a0);
+    if (flexible) a1 = wave_tree_sum(a1);
+    if (lane == 0) {
+      part0[row] = a0;
+      if (flexible) part1[row] = a1;
+    }
   }
 }
 
@@ -951,6 +1021,30 @@ __global__ __launch_bounds__(256) void k_xpay(Geo g, const Z* __restrict__ z, do
       }
     }
   }
+}
+
+// chunk form of k_xpay: wave c of a row owns the natural cells i = 128c+1+2l
+// and +1 (lane l; dp as one 16-byte access), d and z at their split positions
+// (the odd cell in the second half, the even one in the first)
+template <typename Z, int NM>
+__global__ __launch_bounds__(256) void k_xpay_ch(Geo g, const Z* __restrict__ z, double* __restrict__ d,
+                                                 double* __restrict__ dp, const double* __restrict__ S,
+                                                 int slot_alpha, int slot_beta) {
+  CHUNK_SETUP(NM)
+  if (!live) return;
+  const double alpha = S[slot_alpha];
+  const double beta = S[slot_beta];
+  const int i0 = 128 * ch + 1 + 2 * lane;             // odd; i0 + 1 even
+  const size_t so = rb + (g.n[0] >> 1) + (i0 + 1) / 2;  // spos(i0)
+  const size_t se = rb + (i0 + 1) / 2;                  // spos(i0 + 1)
+  double vdp[2];
+  Vec<double, 2>::ld(dp + rb + i0, vdp);
+  const double dO = d[so], dE = d[se];
+  const double zO = (double)St<Z>::ld(z, so), zE = (double)St<Z>::ld(z, se);
+  const double np[2] = {vdp[0] + alpha * dO, vdp[1] + alpha * dE};
+  Vec<double, 2>::st(dp + rb + i0, np);
+  d[so] = zO + beta * dO;
+  d[se] = zE + beta * dE;
 }
 
 // f0 = (F) r on interior cells (fine-level rhs of the mixed V-cycle)
@@ -1374,21 +1468,32 @@ void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad,
 #undef SMAC_SPMV_V
 #undef SMAC_SPMV
 }
+// grid and block of the chunk form: RB rows per block
+template <int NM>
+static inline dim3 chunk_grid(const Geo& g) {
+  return dim3((unsigned)((g.n[1] * g.n[2] + ChunkGeom<NM>::RB - 1) / ChunkGeom<NM>::RB));
+}
+#define SMAC_CHUNK_SWITCH(NMV, LAUNCH) \
+  switch (NMV) {                       \
+    case 1: LAUNCH(1); break;          \
+    case 2: LAUNCH(2); break;          \
+    case 3: LAUNCH(3); break;          \
+    default: LAUNCH(4); break;         \
+  }
 template <typename F>
 void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
                    double* p1, F* f0, hipStream_t st) {
   int nm = pair_nm(g, r);
   if (nm && pair_nm(g, Ad) != nm) nm = 0;
   if (nm && f0 && ((uintptr_t)(f0 + 1) % 128) != 0) nm = 0;
-#define SMAC_UPD(N) \
-  hipLaunchKernelGGL((k_update<F, N>), rows_grid(g), ROWB, 0, st, g, r, Ad, S, slot_alpha, p0, p1, f0)
-  switch (nm) {
-    case 1: SMAC_UPD(1); break;
-    case 2: SMAC_UPD(2); break;
-    case 3: SMAC_UPD(3); break;
-    case 4: SMAC_UPD(4); break;
-    default: SMAC_UPD(0); break;
+  if (!nm) {
+    hipLaunchKernelGGL(k_update<F>, rows_grid(g), ROWB, 0, st, g, r, Ad, S, slot_alpha, p0, p1, f0);
+    return;
   }
+#define SMAC_UPD(N)                                                                                         \
+  hipLaunchKernelGGL((k_update_ch<F, N>), chunk_grid<N>(g), dim3(ChunkGeom<N>::THREADS), 0, st, g, r, Ad, S, \
+                     slot_alpha, p0, p1, f0)
+  SMAC_CHUNK_SWITCH(nm, SMAC_UPD)
 #undef SMAC_UPD
 }
 void launch_dp_flush(const Geo& g, double* dp, const double* d, const double* S, int slot_alpha, hipStream_t st) {
@@ -1400,20 +1505,29 @@ void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, in
   int nm = pair_nm(g, r);
   if (nm && flexible && pair_nm(g, Ad) != nm) nm = 0;
   if (nm && ((uintptr_t)(z + 1) % 128) != 0) nm = 0;
-#define SMAC_D2(N) \
-  hipLaunchKernelGGL((k_dot2<Z, N>), rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1)
-  switch (nm) {
-    case 1: SMAC_D2(1); break;
-    case 2: SMAC_D2(2); break;
-    case 3: SMAC_D2(3); break;
-    case 4: SMAC_D2(4); break;
-    default: SMAC_D2(0); break;
+  if (!nm) {
+    hipLaunchKernelGGL(k_dot2<Z>, rows_grid(g), ROWB, 0, st, g, r, z, Ad, flexible, p0, p1);
+    return;
   }
+#define SMAC_D2(N)                                                                                         \
+  hipLaunchKernelGGL((k_dot2_ch<Z, N>), chunk_grid<N>(g), dim3(ChunkGeom<N>::THREADS), 0, st, g, r, z, Ad, \
+                     flexible, p0, p1)
+  SMAC_CHUNK_SWITCH(nm, SMAC_D2)
 #undef SMAC_D2
 }
 template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* S, int slot_alpha, int slot_beta,
                  hipStream_t st) {
+  int nm = pair_nm(g, d);
+  if (nm && pair_nm(g, dp) != nm) nm = 0;  // dp: natural rows, 16-byte pairs from element 1
+  if (nm) {
+#define SMAC_XP(N)                                                                                        \
+  hipLaunchKernelGGL((k_xpay_ch<Z, N>), chunk_grid<N>(g), dim3(ChunkGeom<N>::THREADS), 0, st, g, z, d, dp, S, \
+                     slot_alpha, slot_beta)
+    SMAC_CHUNK_SWITCH(nm, SMAC_XP)
+#undef SMAC_XP
+    return;
+  }
   SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_xpay<Z, VPL>), rows_grid(g), ROWB, 0, st, g, z, d, dp, S,
                                                slot_alpha, slot_beta))
 }
