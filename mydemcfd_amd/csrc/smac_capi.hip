@@ -71,6 +71,10 @@ struct smac_ctx {
   double* us[3] = {nullptr, nullptr, nullptr};
   double *p = nullptr, *dp = nullptr, *rhs = nullptr, *alpha = nullptr;
   double *r = nullptr, *z = nullptr, *dv = nullptr, *Ad = nullptr, *res0 = nullptr;
+  // second direction array: the fused direction update + SpMV writes the new
+  // direction here while neighbouring blocks still read the old one (dv and
+  // dv2 swap after every such pass; dv holds the current direction between calls)
+  double* dv2 = nullptr;
   // smac_step lets the solve leave its last dp += alpha d to the correction
   // (defer_dp), which then adds it cell by cell (dp_deferred)
   bool defer_dp = false, dp_deferred = false;
@@ -493,7 +497,7 @@ extern "C" void smac_destroy(smac_ctx* c) {
   mg_free(c);
   rg_free(c);
   double* arrs[] = {c->u[0],   c->u[1],    c->u[2],    c->us[0],   c->us[1],  c->us[2],  c->p,      c->dp,
-                    c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->Ad,     c->res0,
+                    c->rhs,    c->alpha,   c->r,       c->z,       c->dv,     c->dv2,    c->Ad,     c->res0,
                     c->S,      c->rho,     c->rho_old, c->inv_rho, c->mu,     c->beta[0], c->beta[1], c->beta[2],
                     c->muf[0], c->muf[1],  c->muf[2], c->fstage, c->gradp[0], c->gradp[1], c->gradp[2],
                     c->alpha_new, c->vf[0], c->vf[1], c->vf[2], c->vfacc[0], c->vfacc[1], c->vfacc[2],
@@ -1393,11 +1397,21 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     launch_scalar_set(c->S, S_ALPHA, 1.0, s);
     launch_scalar_set(c->S, S_BETA, 0.0, s);
   }
+  // the direction update of an iteration rides on the next iteration's SpMV
+  // where the rows have the pair form; pAp partials then come in P[4]
+  if (!c->dv2 && !nan && !conv) {
+    rc = dalloc(&c->dv2, c->tot);
+    if (rc) return rc;
+  }
+  const bool fuse = c->dv2 && dir_spmv_applies<T>(g, z, c->dv, c->dv2, c->dp, c->Ad);
+  bool ad_ready = false;
   for (iter = 0; iter < o.max_iter && !conv && !nan; ++iter) {
-    KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
+    if (!ad_ready) KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
+    double* pap = ad_ready ? P[4] : P[0];
+    ad_ready = false;
     if (!stat) {
       FinalizeArgs fa = fin(c);
-      fin_add(fa, P[0], 0, S_PAP);
+      fin_add(fa, pap, 0, S_PAP);
       fa.op = OP_DIV;
       fa.dst = S_ALPHA;
       fa.sa = S_RZ;
@@ -1472,7 +1486,13 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
         }
       }
     }
-    KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->dp, c->S, S_ALPHA, S_BETA, s));
+    if (fuse && iter + 1 < o.max_iter) {
+      KT(c, "dir_spmv", launch_dir_spmv<T>(g, bfp, z, c->dv, c->dv2, c->dp, c->Ad, P[4], c->S, S_ALPHA, S_BETA, s));
+      std::swap(c->dv, c->dv2);
+      ad_ready = true;
+    } else {
+      KT(c, "xpay", launch_xpay<T>(g, z, c->dv, c->dp, c->S, S_ALPHA, S_BETA, s));
+    }
     dp_pending = false;
     if (!stat) launch_scalar_copy(c->S, S_RZ, S_RZN, s);
   }

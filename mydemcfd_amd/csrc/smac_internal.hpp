@@ -168,6 +168,15 @@ template <typename Z>
 void launch_copy_dot(const Geo& g, Z* z, double* d, const double* r, double* p0, hipStream_t st, double a = 0.0,
                      const double* S = nullptr, int slot = -1);
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st);
+// dp += alpha d_old, dnew = z + beta d_old and Ad = A dnew with its pair-order
+// row partials in one pass (slots of S hold alpha and beta), for rows of the
+// pair form (nx = 256 or 512, every array's rows aligned): dir_spmv_applies
+template <typename Z>
+bool dir_spmv_applies(const Geo& g, const Z* z, const double* dold, const double* dnew, const double* dp,
+                      const double* Ad);
+template <typename Z>
+void launch_dir_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* dold, double* dnew, double* dp,
+                     double* Ad, double* p0, const double* S, int slot_alpha, int slot_beta, hipStream_t st);
 template <typename F>
 void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
                    double* p1, F* f0, hipStream_t st);

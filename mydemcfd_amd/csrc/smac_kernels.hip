@@ -655,6 +655,21 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, Z* __restrict__ z, doub
   if (lane == 0) part0[row] = acc;
 }
 
+// The direction update fused into the next SpMV (DIR): d = z + beta d_old
+// (k_update_dir, G_PCGSolver.cu:276-288) is formed on the fly for the cell and
+// its neighbours from z and d_old, stored to a second array (a neighbouring
+// block may still read d_old), and dp += alpha d_old (the deferred half of
+// k_update_p_r) rides along; the lane's pairs of both half rows are four
+// consecutive natural cells, two 16-byte dp pairs.
+template <typename Z>
+struct DirArgs {
+  const Z* z;
+  const double* dold;
+  double* dp;
+  const double* S;
+  int slot_alpha, slot_beta;
+};
+
 // Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124), the dot in
 // pair order (lane l: split positions 1+2l+128m, then +1; the oracle's
 // RED_DOT_S2). NM = nx/128 in {2, 4}, rows aligned (FAST): the lane holds its
@@ -663,10 +678,13 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, Z* __restrict__ z, doub
 // (wall neighbours are the zero ghost value). LDSY (ny % 4 == 0): the four
 // waves of a block own rows j0..j0+3 of one plane and share them, with rows
 // j0-1 and j0+4, through LDS for the y neighbours. NM = 0: any nx, loads per
-// neighbour, same order.
-template <bool VAR, int NM, bool LDSY>
-__global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double* __restrict__ d,
-                                                  double* __restrict__ Ad, double* __restrict__ part0) {
+// neighbour, same order. DIR (NM > 0 only): d is written, not read; its values
+// come from da as described above.
+template <bool VAR, int NM, bool LDSY, typename Z = double, bool DIR = false>
+__global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, double* __restrict__ d,
+                                                  double* __restrict__ Ad, double* __restrict__ part0,
+                                                  DirArgs<Z> da) {
+  static_assert(!DIR || NM > 0, "the fused direction update needs the pair form");
   constexpr int NMC = NM > 0 ? NM : 1;
   __shared__ double ysm[LDSY ? 6 * 128 * NMC : 1];
   ROW_SETUP
@@ -695,6 +713,49 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
     constexpr int NH = NM / 2;  // pair chunks per half row
     constexpr int NX = 128 * NM;
     double xc[NM][2], yp[NM][2], ym[NM][2], zp[NM][2], zm[NM][2];
+    const double beta = DIR ? da.S[da.slot_beta] : 0.0;
+    // a pair of d at split position a: loaded, or (DIR) z + beta d_old
+    auto ld2 = [&](size_t a, double* o2) {
+      if constexpr (DIR) {
+        typename St<Z>::C zz[2];
+        double dd[2];
+        Vec<Z, 2>::ld(da.z + a, zz);
+        Vec<double, 2>::ld(da.dold + a, dd);
+        o2[0] = (double)zz[0] + beta * dd[0];
+        o2[1] = (double)zz[1] + beta * dd[1];
+      } else {
+        Vec<double, 2>::ld(d + a, o2);
+      }
+    };
+    if constexpr (DIR) {
+      // the row itself: d = z + beta d_old stored, dp += alpha d_old on the
+      // natural cells 4l+256m+1..+4 (odd cells from the second half row)
+      const double alpha = da.S[da.slot_alpha];
+      double dold[NM][2];
+#pragma unroll
+      for (int m = 0; m < NM; m++) {
+        const size_t a = rb + 1 + 2 * lane + 128 * m;
+        typename St<Z>::C zz[2];
+        Vec<Z, 2>::ld(da.z + a, zz);
+        Vec<double, 2>::ld(da.dold + a, dold[m]);
+        xc[m][0] = (double)zz[0] + beta * dold[m][0];
+        xc[m][1] = (double)zz[1] + beta * dold[m][1];
+        Vec<double, 2>::st(d + a, xc[m]);
+      }
+#pragma unroll
+      for (int m = 0; m < NH; m++) {
+        double* pa = da.dp + rb + 1 + 4 * lane + 256 * m;
+        double v0[2], v1[2];
+        Vec<double, 2>::ld(pa, v0);
+        Vec<double, 2>::ld(pa + 2, v1);
+        v0[0] = v0[0] + alpha * dold[m + NH][0];
+        v0[1] = v0[1] + alpha * dold[m][0];
+        v1[0] = v1[0] + alpha * dold[m + NH][1];
+        v1[1] = v1[1] + alpha * dold[m][1];
+        Vec<double, 2>::st(pa, v0);
+        Vec<double, 2>::st(pa + 2, v1);
+      }
+    }
     if constexpr (LDSY) {
       const int w = (int)(threadIdx.x >> 6);
       double hv[NM][2];
@@ -702,10 +763,10 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
 #pragma unroll
       for (int m = 0; m < NM; m++) {
         const int q = 1 + 2 * lane + 128 * m;
-        Vec<double, 2>::ld(d + rb + q, xc[m]);
-        Vec<double, 2>::ld(d + zpr + q, zp[m]);
-        Vec<double, 2>::ld(d + zmr + q, zm[m]);
-        if (w == 0 || w == 3) Vec<double, 2>::ld(d + hr + q, hv[m]);
+        if constexpr (!DIR) Vec<double, 2>::ld(d + rb + q, xc[m]);
+        ld2(zpr + q, zp[m]);
+        ld2(zmr + q, zm[m]);
+        if (w == 0 || w == 3) ld2(hr + q, hv[m]);
       }
 #pragma unroll
       for (int m = 0; m < NM; m++)
@@ -729,11 +790,11 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, const double*
 #pragma unroll
       for (int m = 0; m < NM; m++) {
         const int q = 1 + 2 * lane + 128 * m;
-        Vec<double, 2>::ld(d + rb + q, xc[m]);
-        Vec<double, 2>::ld(d + ypr + q, yp[m]);
-        Vec<double, 2>::ld(d + ymr + q, ym[m]);
-        Vec<double, 2>::ld(d + zpr + q, zp[m]);
-        Vec<double, 2>::ld(d + zmr + q, zm[m]);
+        if constexpr (!DIR) Vec<double, 2>::ld(d + rb + q, xc[m]);
+        ld2(ypr + q, yp[m]);
+        ld2(ymr + q, ym[m]);
+        ld2(zpr + q, zp[m]);
+        ld2(zmr + q, zm[m]);
       }
     }
     const bool perx = g.per[0] != 0;
@@ -1448,8 +1509,9 @@ void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad,
   if (nm && pair_nm(g, Ad) != nm) nm = 0;
   if (nm != 2 && nm != 4) nm = 0;  // both half rows in whole pair chunks
   const bool ldsy = nm && g.n[1] % 4 == 0;
-#define SMAC_SPMV(V, N, L) \
-  hipLaunchKernelGGL((k_spmv_dot<V, N, L>), rows_grid(g), ROWB, 0, st, g, bf ? *bf : BetaF{}, d, Ad, p0)
+#define SMAC_SPMV(V, N, L)                                                                               \
+  hipLaunchKernelGGL((k_spmv_dot<V, N, L>), rows_grid(g), ROWB, 0, st, g, bf ? *bf : BetaF{}, const_cast<double*>(d), \
+                     Ad, p0, DirArgs<double>{})
 #define SMAC_SPMV_V(V)                                 \
   if (nm == 2) {                                       \
     if (ldsy) SMAC_SPMV(V, 2, true);                   \
@@ -1480,6 +1542,38 @@ static inline dim3 chunk_grid(const Geo& g) {
     case 3: LAUNCH(3); break;          \
     default: LAUNCH(4); break;         \
   }
+template <typename Z>
+bool dir_spmv_applies(const Geo& g, const Z* z, const double* dold, const double* dnew, const double* dp,
+                      const double* Ad) {
+  int nm = pair_nm(g, dold);
+  if (nm && (pair_nm(g, dnew) != nm || pair_nm(g, Ad) != nm || pair_nm(g, dp) != nm)) nm = 0;
+  if (nm && ((uintptr_t)(z + 1) % 128) != 0) nm = 0;
+  return nm == 2 || nm == 4;
+}
+template <typename Z>
+void launch_dir_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* dold, double* dnew, double* dp,
+                     double* Ad, double* p0, const double* S, int slot_alpha, int slot_beta, hipStream_t st) {
+  const int nm = pair_nm(g, dold);
+  const bool ldsy = g.n[1] % 4 == 0;
+  const DirArgs<Z> da{z, dold, dp, S, slot_alpha, slot_beta};
+#define SMAC_DSP(V, N, L) \
+  hipLaunchKernelGGL((k_spmv_dot<V, N, L, Z, true>), rows_grid(g), ROWB, 0, st, g, bf ? *bf : BetaF{}, dnew, Ad, p0, da)
+#define SMAC_DSP_V(V)                  \
+  if (nm == 2) {                       \
+    if (ldsy) SMAC_DSP(V, 2, true);    \
+    else SMAC_DSP(V, 2, false);        \
+  } else {                             \
+    if (ldsy) SMAC_DSP(V, 4, true);    \
+    else SMAC_DSP(V, 4, false);        \
+  }
+  if (bf) {
+    SMAC_DSP_V(true)
+  } else {
+    SMAC_DSP_V(false)
+  }
+#undef SMAC_DSP_V
+#undef SMAC_DSP
+}
 template <typename F>
 void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
                    double* p1, F* f0, hipStream_t st) {
@@ -1545,6 +1639,10 @@ void launch_cvt(const Geo& g, double* r, F* f0, hipStream_t st, double a, const 
   template void launch_dot2<Z>(const Geo&, const double*, const Z*, const double*, int, double*, double*,  \
                                hipStream_t);                                                              \
   template void launch_xpay<Z>(const Geo&, const Z*, double*, double*, const double*, int, int, hipStream_t); \
+  template bool dir_spmv_applies<Z>(const Geo&, const Z*, const double*, const double*, const double*,     \
+                                    const double*);                                                       \
+  template void launch_dir_spmv<Z>(const Geo&, const BetaF*, const Z*, const double*, double*, double*, double*, \
+                                   double*, const double*, int, int, hipStream_t);                        \
   template void launch_cvt<Z>(const Geo&, double*, Z*, hipStream_t, double, const double*, int);
 SMAC_INST_Z(double)
 SMAC_INST_Z(float)
