@@ -75,6 +75,9 @@ struct smac_ctx {
   // direction here while neighbouring blocks still read the old one (dv and
   // dv2 swap after every such pass; dv holds the current direction between calls)
   double* dv2 = nullptr;
+  // dp is zero (as the correction leaves it, and at creation): the next solve
+  // need not clear it
+  bool dp_zero = true;
   // smac_step lets the solve leave its last dp += alpha d to the correction
   // (defer_dp), which then adds it cell by cell (dp_deferred)
   bool defer_dp = false, dp_deferred = false;
@@ -631,6 +634,7 @@ static void fill_periodic_images(const smac_ctx* c, int f, double* h) {
 extern "C" int smac_upload(smac_ctx* c, int32_t f, const double* h, size_t count) {
   if (!c || !h || f < 0 || f >= SMAC_FIELD_COUNT) return SMAC_ERR_INVALID_ARG;
   if (count != smac_field_count(c, f)) return SMAC_ERR_INVALID_ARG;
+  if (f == SMAC_FIELD_DP) c->dp_zero = false;
   double** slot = field_slot(c, f);
   if (!*slot) {
     int rc = dalloc(slot, c->tot);
@@ -663,6 +667,7 @@ extern "C" int smac_download(smac_ctx* c, int32_t f, double* h, size_t count) {
 extern "C" int smac_upload_device(smac_ctx* c, int32_t f, const double* dsrc, size_t count) {
   if (!c || !dsrc || f < 0 || f >= SMAC_FIELD_COUNT) return SMAC_ERR_INVALID_ARG;
   if (count != smac_field_count(c, f)) return SMAC_ERR_INVALID_ARG;
+  if (f == SMAC_FIELD_DP) c->dp_zero = false;
   double** slot = field_slot(c, f);
   if (!*slot) {
     int rc = dalloc(slot, c->tot);
@@ -1355,17 +1360,22 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     fa.c = Ncell;
     launch_finalize(fa, c->S, s);
   }
-  launch_init_r(g, c->rhs, c->r, c->dp, c->S, pure ? S_MEANB : -1, P[0], P[1], s);
+  // r = b - mean b; |r|^2 and sum r; then (pure Neumann) r -= mean r, and
+  // f0 = (T) r. dp is zeroed only when the last correction has not cleared it.
+  double size_inv = 1. / Ncell;
+  const int zero_dp = c->dp_zero ? 0 : 1;
+  c->dp_zero = false;
+  launch_init_r<T>(g, c->rhs, c->r, f0, c->dp, zero_dp, c->S, pure ? S_MEANB : -1, 0.0, -1, P[0], P[1], 1,
+                   pure ? 0 : 1, s);
   {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_BB);
     fin_add(fa, P[1], 0, S_SUMR);
     launch_finalize(fa, c->S, s);
   }
-  double size_inv = 1. / Ncell;
-  // r -= mean(r) (pure Neumann), then f0 = (T) r: one pass when both apply
-  if (f0) launch_cvt<T>(g, c->r, f0, s, -1. * size_inv, c->S, pure ? S_SUMR : -1);
-  else if (pure) launch_add_scalar<double>(g, c->r, -1. * size_inv, c->S, S_SUMR, s);
+  if (pure)
+    launch_init_r<T>(g, c->rhs, c->r, f0, c->dp, zero_dp, c->S, S_MEANB, -1. * size_inv, S_SUMR, P[0], P[1], 0, 1,
+                     s);
   precond();
   if (pure) {
     launch_sum<T>(g, z, P[0], s);
@@ -1580,6 +1590,7 @@ extern "C" int smac_correct(smac_ctx* c, double dt) {
   KT(c, "correct", launch_correct_all(c->g, var_props(c) ? &bf : nullptr, c->us, c->dp, c->u, c->p, dt, c->st,
                                       c->dv, c->S, c->dp_deferred ? S_ALPHA : -1));
   c->dp_deferred = false;
+  c->dp_zero = true;
   HIPOK(hipGetLastError());
   return SMAC_OK;
 }
@@ -1717,6 +1728,7 @@ extern "C" int smac_cell_output(smac_ctx* c, float* h, size_t count) {
 extern "C" int smac_adjoint_step(smac_ctx* c, double dt, const smac_solver_opts* opts, const double* const u_n[3],
                                  double* const ubar[3], double* pbar, smac_solve_stats* st) {
   if (!c || !(dt > 0) || !u_n || !ubar || !pbar) return SMAC_ERR_INVALID_ARG;
+  c->dp_zero = false;  // the adjoint's solves and copies leave dp as they please
   for (int a = 0; a < 3; a++)
     if (!u_n[a] || !ubar[a]) return SMAC_ERR_INVALID_ARG;
   // the adjoint holds alpha fixed (mass flux rho0 u): a pending VoF flux is
@@ -1782,6 +1794,7 @@ extern "C" int smac_adjoint_vof_step(smac_ctx* c, double dt, int32_t S, int32_t 
                                      double* alpha_bar, double* const ubar[3], double* pbar, smac_solve_stats* st) {
   if (!c || !(dt > 0) || S < 1 || S > 64 || step < 0 || !alpha_n || !u_n || !p_n || !alpha_bar || !ubar || !pbar)
     return SMAC_ERR_INVALID_ARG;
+  c->dp_zero = false;  // the adjoint's solves and copies leave dp as they please
   for (int a = 0; a < 3; a++)
     if (!u_n[a] || !ubar[a]) return SMAC_ERR_INVALID_ARG;
   if (c->eps || c->bpois_on || c->has_outlet) return SMAC_ERR_UNSUPPORTED;

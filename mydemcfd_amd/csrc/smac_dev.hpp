@@ -296,6 +296,18 @@ struct St<bf16s> {
   static __device__ __forceinline__ void st(bf16s* p, size_t i, float v) { p[i].u = (uint16_t)bf16_bits(v); }
 };
 
+// the value v has once stored as Z (from its compute type) and read back
+template <typename Z>
+__device__ __forceinline__ double stored_value(double v);
+template <>
+__device__ __forceinline__ double stored_value<double>(double v) { return v; }
+template <>
+__device__ __forceinline__ double stored_value<float>(double v) { return (double)(float)v; }
+template <>
+__device__ __forceinline__ double stored_value<bf16s>(double v) {
+  return (double)__uint_as_float(bf16_bits((float)v) << 16);
+}
+
 // N consecutive storage elements in one aligned access (N * sizeof(T) <= 16,
 // the address a multiple of N * sizeof(T)), converted to / from the compute type
 template <typename T, int N>

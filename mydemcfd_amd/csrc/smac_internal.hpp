@@ -155,8 +155,12 @@ void launch_pois_two_way(const Geo& g, double* const beta[3], const double* eps,
 void launch_predict(const Geo& g, const Props* P, const Vel& V, const double* p, double* us[3], double dt,
                     hipStream_t st, const VofF* M = nullptr);
 void launch_rhs(const Geo& g, double* const us[3], double* rhs, double inv_dt, double* part0, hipStream_t st);
-void launch_init_r(const Geo& g, const double* rhs, double* r, double* dp, const double* S, int slot_mean,
-                   double* p0, double* p1, hipStream_t st);
+// r = rhs - S[slot_mean] (+ a S[slot_shift]), f0 = (F) r, dp = 0 when zero_dp
+// (write); part0/part1 <- row sums of v^2 and v of v = rhs - S[slot_mean] (sums)
+template <typename F>
+void launch_init_r(const Geo& g, const double* rhs, double* r, F* f0, double* dp, int zero_dp, const double* S,
+                   int slot_mean, double a, int slot_shift, double* p0, double* p1, int sums, int write,
+                   hipStream_t st);
 template <typename Z>
 void launch_add_scalar(const Geo& g, Z* x, double a, const double* S, int slot, hipStream_t st);
 template <typename Z>
@@ -164,9 +168,9 @@ void launch_sum(const Geo& g, const Z* x, double* p0, hipStream_t st);
 // bf == nullptr: one-phase operator (beta0); else face coefficients bf->b
 void launch_jacobi(const Geo& g, const BetaF* bf, const double* r, double* z, hipStream_t st);
 template <typename Z>
-// d = z, part0 <- sum r*z (natural order); slot >= 0: z += a S[slot] first
-void launch_copy_dot(const Geo& g, Z* z, double* d, const double* r, double* p0, hipStream_t st, double a = 0.0,
-                     const double* S = nullptr, int slot = -1);
+// d = z, part0 <- sum r*z (natural order); slot >= 0: z + a S[slot] first, as stored in Z
+void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st,
+                     double a = 0.0, const double* S = nullptr, int slot = -1);
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st);
 // dp += alpha d_old, dnew = z + beta d_old and Ad = A dnew with its pair-order
 // row partials in one pass (slots of S hold alpha and beta), for rows of the
@@ -187,10 +191,6 @@ void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, in
 template <typename Z>
 void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* S, int slot_alpha, int slot_beta,
                  hipStream_t st);
-template <typename F>
-// f0 = (F) r; slot >= 0: r += a S[slot] first
-void launch_cvt(const Geo& g, double* r, F* f0, hipStream_t st, double a = 0.0, const double* S = nullptr,
-                int slot = -1);
 void launch_finalize(const FinalizeArgs& fa, double* S, hipStream_t st);
 void launch_scalar_copy(double* S, int dst, int src, hipStream_t st);
 void launch_scalar_set(double* S, int dst, double v, hipStream_t st);

@@ -579,26 +579,48 @@ __global__ __launch_bounds__(256) void k_rhs_eps(Geo g, const double* __restrict
   if (lane == 0) part0[row] = acc;
 }
 
-// r = b - mean_b (split), dp = 0 (natural); part0 <- sum r^2, part1 <- sum r
+// The start of the PCG, r = b - mean b (pure Neumann: mean = S[slot_mean]) on
+// the interior cells, r in the split layout (G_PCGSolver.cu:619-640 with
+// dp = 0: r = b - A dp = b). SUMS: part0 <- sum v^2, part1 <- sum v of
+// v = b - mean (lanes over natural i, the canonical row order). WRITE: r = v,
+// or v + a S[slot_shift] (slot_shift >= 0: the second mean removal,
+// subtract_cell_mean, G_PressureSolverBase.cu:142-155), f0 = (F) r when f0 is
+// given (the fine-level rhs of the mixed V-cycle), dp = 0 when zero_dp (dp is
+// otherwise left as the last correction cleared it). The pure-Neumann start
+// is two passes, sums then writes (the shift needs the sum of v); a regular
+// operator's start is one.
+template <typename F, bool SUMS, bool WRITE>
 __global__ __launch_bounds__(256) void k_init_r(Geo g, const double* __restrict__ rhs, double* __restrict__ r,
-                                                double* __restrict__ dp, const double* __restrict__ S, int slot_mean,
-                                                double* __restrict__ part0, double* __restrict__ part1) {
+                                                F* __restrict__ f0, double* __restrict__ dp, int zero_dp,
+                                                const double* __restrict__ S, int slot_mean, double a,
+                                                int slot_shift, double* __restrict__ part0,
+                                                double* __restrict__ part1) {
   ROW_SETUP
   const int nx = g.n[0];
   const double mb = slot_mean >= 0 ? S[slot_mean] : 0.0;
+  const double sh = slot_shift >= 0 ? S[slot_shift] : 0.0;
   double a0 = 0.0, a1 = 0.0;
   for (int i = 1 + lane; i <= nx; i += 64) {
     double v = rhs[rb + i] - mb;
-    r[rb + spos(i, nx)] = v;
-    dp[rb + i] = 0.0;
-    a0 += v * v;
-    a1 += v;
+    if (SUMS) {
+      a0 += v * v;
+      a1 += v;
+    }
+    if (WRITE) {
+      if (slot_shift >= 0) v = v + a * sh;
+      const size_t o = rb + spos(i, nx);
+      r[o] = v;
+      if (f0) St<F>::st(f0, o, (typename St<F>::C)v);
+      if (zero_dp) dp[rb + i] = 0.0;
+    }
   }
-  a0 = wave_tree_sum(a0);
-  a1 = wave_tree_sum(a1);
-  if (lane == 0) {
-    part0[row] = a0;
-    part1[row] = a1;
+  if (SUMS) {
+    a0 = wave_tree_sum(a0);
+    a1 = wave_tree_sum(a1);
+    if (lane == 0) {
+      part0[row] = a0;
+      part1[row] = a1;
+    }
   }
 }
 
@@ -634,9 +656,12 @@ __global__ __launch_bounds__(256) void k_jacobi(Geo g, BetaF bf, const double* _
   }
 }
 
-// d = z; part0 <- sum r*z
+// d = z; part0 <- sum r*z. slot >= 0: z + a S[slot] first (k_add_scalar's
+// arithmetic) rounded to the storage of z, the value a separate pass would
+// store and read back (z itself is not written: the next V-cycle overwrites it
+// before anything reads it)
 template <typename Z>
-__global__ __launch_bounds__(256) void k_copy_dot(Geo g, Z* __restrict__ z, double* __restrict__ d,
+__global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z, double* __restrict__ d,
                                                   const double* __restrict__ r, double* __restrict__ part0,
                                                   double a, const double* __restrict__ S, int slot) {
   ROW_SETUP
@@ -644,10 +669,8 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, Z* __restrict__ z, doub
   const double s = slot >= 0 ? S[slot] : 0.0;
   for (int i = 1 + lane; i <= g.n[0]; i += 64) {
     const size_t o = rb + spos(i, g.n[0]);
-    // slot >= 0: z += a S[slot] first (k_add_scalar's arithmetic), then the
-    // stored value is read back, as a separate pass would read it
-    if (slot >= 0) St<Z>::st(z, o, (typename St<Z>::C)((double)St<Z>::ld(z, o) + a * s));
     double zz = (double)St<Z>::ld(z, o);
+    if (slot >= 0) zz = stored_value<Z>((double)(typename St<Z>::C)(zz + a * s));
     d[o] = zz;
     acc += r[o] * zz;
   }
@@ -1108,23 +1131,6 @@ __global__ __launch_bounds__(256) void k_xpay_ch(Geo g, const Z* __restrict__ z,
   d[se] = zE + beta * dE;
 }
 
-// f0 = (F) r on interior cells (fine-level rhs of the mixed V-cycle)
-// slot >= 0: r += a S[slot] first (k_add_scalar's arithmetic, one pass for both)
-template <typename F>
-__global__ __launch_bounds__(256) void k_cvt(Geo g, double* __restrict__ r, F* __restrict__ f0, double a,
-                                             const double* __restrict__ S, int slot) {
-  ROW_SETUP
-  const double s = slot >= 0 ? S[slot] : 0.0;
-  for (int p = 1 + lane; p <= g.n[0]; p += 64) {
-    double v = r[rb + p];
-    if (slot >= 0) {
-      v = v + a * s;
-      r[rb + p] = v;
-    }
-    St<F>::st(f0, rb + p, (typename St<F>::C)v);
-  }
-}
-
 // --------------------------------------------------------------- finalize
 // Reduce up to four row-partial arrays with the canonical 1024-lane tree
 // (kind 0 = sum, 1 = max) into S[out[m]], then apply one scalar op.
@@ -1475,9 +1481,19 @@ void launch_pois_two_way(const Geo& g, double* const beta[3], const double* eps,
 void launch_rhs(const Geo& g, double* const us[3], double* rhs, double inv_dt, double* part0, hipStream_t st) {
   hipLaunchKernelGGL(k_rhs, rows_grid(g), ROWB, 0, st, g, us[0], us[1], us[2], rhs, inv_dt, part0);
 }
-void launch_init_r(const Geo& g, const double* rhs, double* r, double* dp, const double* S, int slot_mean,
-                   double* p0, double* p1, hipStream_t st) {
-  hipLaunchKernelGGL(k_init_r, rows_grid(g), ROWB, 0, st, g, rhs, r, dp, S, slot_mean, p0, p1);
+template <typename F>
+void launch_init_r(const Geo& g, const double* rhs, double* r, F* f0, double* dp, int zero_dp, const double* S,
+                   int slot_mean, double a, int slot_shift, double* p0, double* p1, int sums, int write,
+                   hipStream_t st) {
+  if (sums && write)
+    hipLaunchKernelGGL((k_init_r<F, true, true>), rows_grid(g), ROWB, 0, st, g, rhs, r, f0, dp, zero_dp, S,
+                       slot_mean, a, slot_shift, p0, p1);
+  else if (sums)
+    hipLaunchKernelGGL((k_init_r<F, true, false>), rows_grid(g), ROWB, 0, st, g, rhs, r, f0, dp, zero_dp, S,
+                       slot_mean, a, slot_shift, p0, p1);
+  else
+    hipLaunchKernelGGL((k_init_r<F, false, true>), rows_grid(g), ROWB, 0, st, g, rhs, r, f0, dp, zero_dp, S,
+                       slot_mean, a, slot_shift, p0, p1);
 }
 template <typename Z>
 void launch_add_scalar(const Geo& g, Z* x, double a, const double* S, int slot, hipStream_t st) {
@@ -1492,7 +1508,7 @@ void launch_jacobi(const Geo& g, const BetaF* bf, const double* r, double* z, hi
   else hipLaunchKernelGGL(k_jacobi<false>, rows_grid(g), ROWB, 0, st, g, BetaF{}, r, z);
 }
 template <typename Z>
-void launch_copy_dot(const Geo& g, Z* z, double* d, const double* r, double* p0, hipStream_t st, double a,
+void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st, double a,
                      const double* S, int slot) {
   hipLaunchKernelGGL(k_copy_dot<Z>, rows_grid(g), ROWB, 0, st, g, z, d, r, p0, a, S, slot);
 }
@@ -1625,15 +1641,13 @@ void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* 
   SMAC_VPL_DISPATCH(g.n[0], hipLaunchKernelGGL((k_xpay<Z, VPL>), rows_grid(g), ROWB, 0, st, g, z, d, dp, S,
                                                slot_alpha, slot_beta))
 }
-template <typename F>
-void launch_cvt(const Geo& g, double* r, F* f0, hipStream_t st, double a, const double* S, int slot) {
-  hipLaunchKernelGGL(k_cvt<F>, rows_grid(g), ROWB, 0, st, g, r, f0, a, S, slot);
-}
 #define SMAC_INST_Z(Z)                                                                                     \
   template void launch_add_scalar<Z>(const Geo&, Z*, double, const double*, int, hipStream_t);            \
   template void launch_sum<Z>(const Geo&, const Z*, double*, hipStream_t);                                \
-  template void launch_copy_dot<Z>(const Geo&, Z*, double*, const double*, double*, hipStream_t, double,  \
-                                   const double*, int);                                                   \
+  template void launch_copy_dot<Z>(const Geo&, const Z*, double*, const double*, double*, hipStream_t,    \
+                                   double, const double*, int);                                           \
+  template void launch_init_r<Z>(const Geo&, const double*, double*, Z*, double*, int, const double*, int, \
+                                 double, int, double*, double*, int, int, hipStream_t);                   \
   template void launch_update<Z>(const Geo&, double*, const double*, const double*, int, double*, double*, \
                                  Z*, hipStream_t);                                                        \
   template void launch_dot2<Z>(const Geo&, const double*, const Z*, const double*, int, double*, double*,  \
@@ -1642,8 +1656,7 @@ void launch_cvt(const Geo& g, double* r, F* f0, hipStream_t st, double a, const 
   template bool dir_spmv_applies<Z>(const Geo&, const Z*, const double*, const double*, const double*,     \
                                     const double*);                                                       \
   template void launch_dir_spmv<Z>(const Geo&, const BetaF*, const Z*, const double*, double*, double*, double*, \
-                                   double*, const double*, int, int, hipStream_t);                        \
-  template void launch_cvt<Z>(const Geo&, double*, Z*, hipStream_t, double, const double*, int);
+                                   double*, const double*, int, int, hipStream_t);
 SMAC_INST_Z(double)
 SMAC_INST_Z(float)
 SMAC_INST_Z(bf16s)

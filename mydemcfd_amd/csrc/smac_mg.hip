@@ -471,6 +471,41 @@ __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], doubl
   return m;
 }
 
+// rweights padded to four taps (weight 0 on the last valid index, last),
+// written with static indices so that the taps stay in registers
+__device__ __forceinline__ void rtaps4(int per, int nc, int I, int ix[4], double w[4]) {
+  const int nf = 2 * nc;
+  if (per) {
+    ix[0] = wrapi(2 * I - 2, nf); w[0] = 0.25;
+    ix[1] = 2 * I - 1; w[1] = 0.75;
+    ix[2] = 2 * I; w[2] = 0.75;
+    ix[3] = wrapi(2 * I + 1, nf); w[3] = 0.25;
+    return;
+  }
+  const bool lo = I == 1, hi = I == nc;
+  if (!lo && !hi) {
+    ix[0] = 2 * I - 2; w[0] = 0.25;
+    ix[1] = 2 * I - 1; w[1] = 0.75;
+    ix[2] = 2 * I; w[2] = 0.75;
+    ix[3] = 2 * I + 1; w[3] = 0.25;
+  } else if (lo && !hi) {
+    ix[0] = 1; w[0] = 1.0;
+    ix[1] = 2; w[1] = 0.75;
+    ix[2] = 3; w[2] = 0.25;
+    ix[3] = 3; w[3] = 0.0;
+  } else if (!lo && hi) {
+    ix[0] = 2 * I - 2; w[0] = 0.25;
+    ix[1] = 2 * I - 1; w[1] = 0.75;
+    ix[2] = 2 * I; w[2] = 1.0;
+    ix[3] = 2 * I; w[3] = 0.0;
+  } else {
+    ix[0] = 1; w[0] = 1.0;
+    ix[1] = 2; w[1] = 1.0;
+    ix[2] = 2; w[2] = 0.0;
+    ix[3] = 2; w[3] = 0.0;
+  }
+}
+
 // Restriction, separable: coarse f = (1/8) sum_z wz sum_y wy sum_x wx res over
 // the folded 4x4x4 fine neighbourhood. Pass 1 forms the residual
 // res = f - (diag e - S) of one fine row in LDS and restricts it along x into
@@ -650,52 +685,65 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_mg_ryz(Lv F, Lv Cl, const typename St<T>::C* __restrict__ tb,
-                                                T* __restrict__ fc) {
+                                                T* __restrict__ fc, int KC) {
   using C = typename St<T>::C;
+  // a thread owns the coarse column (I, J) over the coarse planes K0..K0+KC-1;
   // blocks in x-fastest order, remapped so that each XCD takes a contiguous
-  // range of coarse planes (the fine rows of tb are shared by neighbouring
-  // coarse rows and planes, and each XCD has its own L2)
+  // range of chunks (neighbouring coarse rows and chunks share fine rows of tb)
   const int nbx = (int)gridDim.x, nby = (int)gridDim.y;
   const int lin = xcd_block((int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)),
                             (int)(gridDim.x * gridDim.y * gridDim.z));
   const int bx = lin % nbx, by = (lin / nbx) % nby, bz = lin / (nbx * nby);
   const int I = 1 + bx * 64 + (int)threadIdx.x;
   const int J = 1 + by * 4 + (int)threadIdx.y;
-  const int K = 1 + bz;
-  if (I > Cl.n[0] || J > Cl.n[1] || K > Cl.n[2]) return;
-  int iy[4], iz[4];
-  double wy[4], wz[4];
-  const int my = rweights(Cl.per[1], Cl.n[1], J, iy, wy);
-  const int mz = rweights(Cl.per[2], Cl.n[2], K, iz, wz);
-  // pad to 4 x 4 with weight 0 on a valid index: the padded terms come last
-  // and add exact zeros, so the sums are those of the 3- or 4-term stencil;
-  // all 16 loads are issued before the arithmetic
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    if (m >= my) {
-      iy[m] = iy[my - 1];
-      wy[m] = 0.0;
-    }
-    if (m >= mz) {
-      iz[m] = iz[mz - 1];
-      wz[m] = 0.0;
-    }
-  }
+  const int K0 = 1 + bz * KC;
+  if (I > Cl.n[0] || J > Cl.n[1]) return;
+  // four taps (padded with weight 0 on a valid index: the padded terms come
+  // last and add exact zeros, so the sums are those of the 3- or 4-term stencil)
+  int iy[4];
+  double wy[4];
+  rtaps4(Cl.per[1], Cl.n[1], J, iy, wy);
   const size_t pxT = (size_t)(Cl.n[0] + 3);
-  C v[4][4];
+  // the y-restricted value of fine plane z, ty(z) = sum_b wy[b] tb(I, iy[b], z),
+  // computed once per plane: the last four planes are kept
+  // (static register indices only: the newest plane enters at the end)
+  int cz[4] = {-1, -1, -1, -1};
+  C cv[4] = {(C)0, (C)0, (C)0, (C)0};
+  auto ty_of = [&](int z) -> C {
+    C hit = (C)0;
+    bool found = false;
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+    for (int u = 0; u < 4; u++)
+      if (!found && cz[u] == z) {
+        hit = cv[u];
+        found = true;
+      }
+    if (found) return hit;
+    C v[4];
 #pragma unroll
-    for (int b = 0; b < 4; b++) v[a][b] = tb[((size_t)iz[a] * (size_t)F.py + (size_t)iy[b]) * pxT + I];
-  C tz = 0;
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
+    for (int b = 0; b < 4; b++) v[b] = tb[((size_t)z * (size_t)F.py + (size_t)iy[b]) * pxT + I];
     C ty = 0;
 #pragma unroll
-    for (int b = 0; b < 4; b++) ty += (C)wy[b] * v[a][b];
-    tz += (C)wz[a] * ty;
+    for (int b = 0; b < 4; b++) ty += (C)wy[b] * v[b];
+#pragma unroll
+    for (int u = 0; u < 3; u++) {
+      cz[u] = cz[u + 1];
+      cv[u] = cv[u + 1];
+    }
+    cz[3] = z;
+    cv[3] = ty;
+    return ty;
+  };
+  const int K1 = min(K0 + KC, Cl.n[2] + 1);
+  for (int K = K0; K < K1; K++) {
+    int iz[4];
+    double wz[4];
+    rtaps4(Cl.per[2], Cl.n[2], K, iz, wz);
+    C tz = 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) tz += (C)wz[a] * ty_of(iz[a]);
+    St<T>::st(fc, lrow(Cl, J, K) + spos(I, Cl.n[0]), (C)0.125 * tz);
   }
-  St<T>::st(fc, lrow(Cl, J, K) + spos(I, Cl.n[0]), (C)0.125 * tz);
 }
 
 __device__ __forceinline__ void pweights(int per, int nc, int i, int ix[2], double w[2]) {
@@ -1032,7 +1080,11 @@ void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typena
 #undef SMAC_RX_V
 #undef SMAC_RXV
 #undef SMAC_RX
-  hipLaunchKernelGGL(k_mg_ryz<T>, cell_grid(C), CELLB, 0, st, F, C, tb, fc);
+  // coarse planes per thread: 8 on large levels (half the loads), 1 on small
+  // ones, whose launch needs every thread it can get
+  const int KC = (size_t)C.n[0] * C.n[1] * C.n[2] >= ((size_t)1 << 21) ? 8 : 1;
+  const dim3 grid((unsigned)((C.n[0] + 63) / 64), (unsigned)((C.n[1] + 3) / 4), (unsigned)((C.n[2] + KC - 1) / KC));
+  hipLaunchKernelGGL(k_mg_ryz<T>, grid, CELLB, 0, st, F, C, tb, fc, KC);
 }
 template <typename T>
 void launch_mg_prolong(const Lv& C, const Lv& F, const T* ec, T* ef, int col, hipStream_t st) {
