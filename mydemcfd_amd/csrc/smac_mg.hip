@@ -1133,18 +1133,32 @@ __global__ __launch_bounds__(256) void k_mg_coarsen_beta(Lv F, Lv C, int d, doub
     return;
   }
   const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
-  int fq[3];
-  fq[d] = 2 * q[d] - 1;
-  double v[2][2];
+  // layer sums S(m) of the 2x2 fine faces 2q-1+m along d (m = -1, 0, 1;
+  // periodic: wrapped); a face on a wall or outlet side takes its own layer,
+  // an interior face the weights 1/4, 1/2, 1/4 along the normal (the coupling
+  // of the two coarse cells, the fine faces inside them included: with a
+  // density jump inside a coarse cell, the coplanar layer alone leaves the
+  // V-cycle four times weaker at 1000:1)
+  const bool bnd = !C.per[d] && (q[d] == 1 || q[d] == C.n[d] + 1);
+  double S3[3] = {0., 0., 0.};
 #pragma unroll
-  for (int s2 = 0; s2 < 2; s2++)
+  for (int m = -1; m <= 1; m++) {
+    if (bnd && m != 0) continue;
+    int fq[3];
+    fq[d] = 2 * q[d] - 1 + m;
+    if (m != 0 && C.per[d]) fq[d] = wrapi(fq[d], F.n[d]);
+    double v[2][2];
 #pragma unroll
-    for (int s1 = 0; s1 < 2; s1++) {
-      fq[t1] = 2 * q[t1] - 1 + s1;
-      fq[t2] = 2 * q[t2] - 1 + s2;
-      v[s2][s1] = F.b[d][lrow(F, fq[1], fq[2]) + fq[0]];
-    }
-  cb[id] = ((v[0][0] + v[0][1]) + (v[1][0] + v[1][1])) * 0.25;
+    for (int s2 = 0; s2 < 2; s2++)
+#pragma unroll
+      for (int s1 = 0; s1 < 2; s1++) {
+        fq[t1] = 2 * q[t1] - 1 + s1;
+        fq[t2] = 2 * q[t2] - 1 + s2;
+        v[s2][s1] = F.b[d][lrow(F, fq[1], fq[2]) + fq[0]];
+      }
+    S3[m + 1] = (v[0][0] + v[0][1]) + (v[1][0] + v[1][1]);
+  }
+  cb[id] = bnd ? S3[1] * 0.25 : ((0.25 * S3[0] + 0.5 * S3[1]) + 0.25 * S3[2]) * 0.25;
 }
 
 void launch_mg_coarsen_beta(const Lv& F, const Lv& C, int d, double* cb, hipStream_t st) {

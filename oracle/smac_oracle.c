@@ -824,16 +824,27 @@ static void mg_coarsen_beta(const struct smac_ctx* c, const level* F, level* C) 
                         if (q[e] < 1 || q[e] > hi) ok = 0;
                     }
                     if (!ok) { C->b[d][LIDX(C, I, J, K)] = 0.; continue; }
-                    int fq[3];
-                    fq[d] = 2 * q[d] - 1;
-                    double v[2][2];
-                    for (int s2 = 0; s2 < 2; s2++)
-                        for (int s1 = 0; s1 < 2; s1++) {
-                            fq[t1] = 2 * q[t1] - 1 + s1;
-                            fq[t2] = 2 * q[t2] - 1 + s2;
-                            v[s2][s1] = F->b[d][LIDX(F, fq[0], fq[1], fq[2])];
-                        }
-                    C->b[d][LIDX(C, I, J, K)] = ((v[0][0] + v[0][1]) + (v[1][0] + v[1][1])) * 0.25;
+                    /* layer sums S(m) of the 2x2 fine faces 2q-1+m along d (m = -1, 0, 1;
+                     * periodic: wrapped); a face on a wall or outlet side takes its own
+                     * layer, an interior face the weights 1/4, 1/2, 1/4 along the normal */
+                    const int bnd = !c->per[d] && (q[d] == 1 || q[d] == C->n[d] + 1);
+                    double S3[3] = {0., 0., 0.};
+                    for (int m = -1; m <= 1; m++) {
+                        if (bnd && m != 0) continue;
+                        int fq[3];
+                        fq[d] = 2 * q[d] - 1 + m;
+                        if (m != 0 && c->per[d]) fq[d] = wrap(fq[d], F->n[d]);
+                        double v[2][2];
+                        for (int s2 = 0; s2 < 2; s2++)
+                            for (int s1 = 0; s1 < 2; s1++) {
+                                fq[t1] = 2 * q[t1] - 1 + s1;
+                                fq[t2] = 2 * q[t2] - 1 + s2;
+                                v[s2][s1] = F->b[d][LIDX(F, fq[0], fq[1], fq[2])];
+                            }
+                        S3[m + 1] = (v[0][0] + v[0][1]) + (v[1][0] + v[1][1]);
+                    }
+                    C->b[d][LIDX(C, I, J, K)] =
+                        bnd ? S3[1] * 0.25 : ((0.25 * S3[0] + 0.5 * S3[1]) + 0.25 * S3[2]) * 0.25;
                 }
     }
 }
