@@ -152,6 +152,105 @@ def run_form(name, n, steps, warmup, prof_steps, fields, re, cfl, tol, levels):
     return out
 
 
+def run_c5(steps, warmup, tol, adjoint_steps):
+    """The variable-coefficient workload: the VoF dam break on 256x128x128
+    ([0,2]x[0,1]x[0,1], heavy column x < 0.25, z < 0.5, rho 1000:1, mu 100:1,
+    gravity -z, no-slip walls), dt = 1e-3, each loop step = the split THINC/WLIC
+    transport (two substeps) + the SMAC step with mixed FP32 RBGS-MG-PCG to
+    rel-res tol. Timed: `steps` loop steps after `warmup`; then the adjoint of
+    the column-front position x_f = (h_x/n_y) sum alpha over the floor layer
+    w.r.t. the initial VoF field over `adjoint_steps` steps (forward with
+    per-step checkpoints + backward). The byte model is the library's
+    bytes_model (variable coefficients: +48 B fixed, +24 B per iteration, +(15 x
+    8/7) W per V-cycle); the transport has no term in it, so the roofline
+    fraction is taken over the step time without the transport kernels
+    (measured in a profiled pass)."""
+    import torch
+
+    import mydemcfd_amd
+    from mydemcfd_amd import autograd, cases
+
+    n = (256, 128, 128)
+    cells = n[0] * n[1] * n[2]
+    dt, sub = 1e-3, 2
+    desc = cases.dam_break_desc(n)
+    ctx = mydemcfd_amd.SMACContext(**desc)
+    alpha0 = cases.dam_break_alpha(n)
+    ctx.upload("alpha", alpha0)
+    ctx.update_properties(True)
+    opts = ctx.opts(precond="mg", precision="fp32", flexible=1, stop_rule="rel_l2", tol=tol, max_iter=3000)
+    k = 0
+    for _ in range(warmup):
+        ctx.transport_alpha(dt, sub, k)
+        ctx.step(dt, opts)
+        k += 1
+    ctx.set_profiling(False)
+    ctx.sync()
+    t0 = time.perf_counter()
+    iters, bmodel = [], 0.0
+    for _ in range(steps):
+        ctx.transport_alpha(dt, sub, k)
+        st = ctx.step(dt, opts)
+        if not st.converged:
+            raise RuntimeError(f"c5: step did not converge: {st.as_dict()}")
+        iters.append(st.iters)
+        bmodel += st.bytes_model
+        k += 1
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    # transport share: a profiled pass of three loop steps
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    for _ in range(3):
+        ctx.transport_alpha(dt, sub, k)
+        ctx.step(dt, opts)
+        k += 1
+    ctx.sync()
+    kt = {name: ctx.kernel_time(name) for name in ctx.kernel_names()}
+    ctx.set_profiling(False)
+    total_ms = sum(ms for _, ms in kt.values())
+    vof_ms = sum(ms for name, (_, ms) in kt.items() if "vof" in name)
+    vof_share = vof_ms / total_ms if total_ms > 0 else 0.0
+    step_s = wall * (1.0 - vof_share)
+    achieved = bmodel / step_s / 1e9
+    out = {"workload": "VoF dam break 256x128x128 ([0,2]x[0,1]x[0,1]), rho 1000:1, mu 100:1, gravity -z, "
+                       "dt 1e-3, transport (2 substeps) + step, mixed FP32 RBGS-MG-PCG V(2,2) to rel-res "
+                       f"{tol:g}",
+           "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * wall / steps,
+           "cell_steps_per_s": cells * steps / wall, "iters": iters,
+           "transport_share": round(vof_share, 4),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_model_per_step": bmodel / steps,
+                        "basis": "bytes_model of the steps / (loop time x (1 - transport share))"},
+           "kernels_ms": {name: round(ms / 3, 4) for name, (_, ms) in kt.items()}}
+    del ctx
+    torch.cuda.empty_cache()
+    if adjoint_steps > 0:
+        actx = mydemcfd_amd.SMACContext(**desc)
+        leaves = {"alpha": torch.tensor(alpha0, dtype=torch.float64, device="cuda").requires_grad_()}
+        for f in ("u", "v", "w", "p"):
+            leaves[f] = torch.zeros(actx.shape(f), dtype=torch.float64, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = autograd.vof_rollout(actx, leaves["alpha"], leaves["u"], leaves["v"], leaves["w"], leaves["p"], dt,
+                                    adjoint_steps, sub, opts)
+        front = outs[0][1, 1:n[1] + 1, 1:n[0] + 1].sum() * ((2.0 / n[0]) / n[1])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        front.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        g = leaves["alpha"].grad
+        out["adjoint"] = {"steps": adjoint_steps, "x_front": float(front.detach()),
+                          "forward_ms_per_step": 1e3 * (t1 - t0) / adjoint_steps,
+                          "backward_ms_per_step": 1e3 * (t2 - t1) / adjoint_steps,
+                          "grad_norm": float(torch.linalg.vector_norm(g)), "grad_finite": bool(torch.isfinite(g).all())}
+        del outs, leaves, actx
+        torch.cuda.empty_cache()
+    return out
+
+
 def field_errors(ctx_a, ctx_b, n):
     """max|a-b|/max|b| per field on the GPU (p modulo its interior mean)."""
     import torch
@@ -308,6 +407,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--c5-steps", type=int, default=50, help="steps of the VoF dam-break workload (0: skip)")
+    ap.add_argument("--c5-adjoint-steps", type=int, default=50, help="steps of its adjoint (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU (z-slab) decomposition "
@@ -378,6 +479,10 @@ def main():
             line["speedup_vs_fp64"] = results[headline]["cell_steps_per_s"] / results["fp64"]["cell_steps_per_s"]
     for r in results.values():
         r.pop("_ctx", None)
+    if args.c5_steps > 0:
+        line["c5"] = run_c5(args.c5_steps, 2, args.tol, args.c5_adjoint_steps)
+        print(f"# c5: {line['c5']['ms_per_step']:.2f} ms/step, iters {line['c5']['iters'][:5]}...", file=sys.stderr,
+              flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.re, args.cfl, args.tol)
     else:
