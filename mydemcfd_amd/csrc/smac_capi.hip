@@ -154,6 +154,7 @@ struct smac_ctx {
   struct Rec {
     int name;
     hipEvent_t a, b;
+    int launches;  // launches between the two events
   };
   std::vector<Rec> pending;
   std::map<std::string, int> name_id;
@@ -164,7 +165,10 @@ struct smac_ctx {
 static hipEvent_t pool_event(smac_ctx* c) {
   if (c->pool_used == c->pool.size()) {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    // timing only (read after the stream is synchronized): a device-scope
+    // release, without the system-scope cache writeback that would lengthen
+    // the launch it brackets
+    if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) return nullptr;
     c->pool.push_back(e);
   }
   return c->pool[c->pool_used++];
@@ -175,7 +179,7 @@ static void prof_flush(smac_ctx* c) {
   for (auto& r : c->pending) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
-      c->totals[r.name].n += 1;
+      c->totals[r.name].n += r.launches;
       c->totals[r.name].ms += (double)ms;
     }
   }
@@ -199,11 +203,12 @@ static int kbeg(smac_ctx* c, const char* name) {
   hipEvent_t a = pool_event(c), b = pool_event(c);
   if (!a || !b) return -1;
   (void)hipEventRecord(a, c->st);
-  c->pending.push_back({id, a, b});
+  c->pending.push_back({id, a, b, 1});
   return (int)c->pending.size() - 1;
 }
-static void kend(smac_ctx* c, int rec) {
+static void kend(smac_ctx* c, int rec, int launches = 1) {
   if (rec < 0) return;
+  c->pending[rec].launches = launches;
   (void)hipEventRecord(c->pending[rec].b, c->st);
 }
 #define KT(c, name, launch)      \
@@ -211,6 +216,14 @@ static void kend(smac_ctx* c, int rec) {
     int r_ = kbeg((c), (name));  \
     launch;                      \
     kend((c), r_);               \
+  } while (0)
+// one event pair around `n` consecutive launches of one kernel: the gap
+// before a launch that a pair around each launch would add counts once
+#define KT_N(c, name, n, launches) \
+  do {                             \
+    int r_ = kbeg((c), (name));    \
+    launches;                      \
+    kend((c), r_, (n));            \
   } while (0)
 
 #define HIPOK(x)                         \
@@ -1126,12 +1139,25 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   // the first red half sweep starts from e = 0 and writes every red cell; the
   // black half sweep that follows reads only red cells, so no fill is needed
   if (nu1 == 0) KT(c, fine ? "mg_fill_l0" : "mg_coarse_levels", launch_fill<T>(V.lv, e, c->st));
-  for (int s = 0; s < nu1; s++) {
+  if (fine) {
     // the first red half sweep starts from zero (reads f, writes e: one word
-    // less per cell than the others), timed under its own name
-    KT(c, fine ? (s == 0 ? "mg_rbgs_l0_zs" : "mg_rbgs_l0") : "mg_coarse_levels",
-       launch_mg_rbgs<T>(V.lv, e, f, 0, s == 0, c->st));
-    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
+    // less per cell than the others), timed under its own name; the other
+    // pre-smoothing half sweeps run back to back and are timed together
+    if (nu1 > 0) {
+      KT(c, "mg_rbgs_l0_zs", launch_mg_rbgs<T>(V.lv, e, f, 0, 1, c->st));
+      KT_N(c, "mg_rbgs_l0", 2 * nu1 - 1, {
+        launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st);
+        for (int s = 1; s < nu1; s++) {
+          launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st);
+          launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st);
+        }
+      });
+    }
+  } else {
+    for (int s = 0; s < nu1; s++) {
+      KT(c, "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 0, s == 0, c->st));
+      KT(c, "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
+    }
   }
   Level& C = c->lev[l + 1];
   KT(c, fine ? "mg_restrict_l0" : "mg_coarse_levels",
@@ -1141,9 +1167,19 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   // before they are read, so only the red cells take the correction
   KT(c, fine ? "mg_prolong_l0" : "mg_coarse_levels",
      launch_mg_prolong<T>(C.lv, V.lv, (const T*)C.e, e, nu2 > 0 ? 0 : -1, c->st));
-  for (int s = 0; s < nu2; s++) {
-    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
-    KT(c, fine ? "mg_rbgs_l0" : "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st));
+  if (fine) {
+    if (nu2 > 0)
+      KT_N(c, "mg_rbgs_l0", 2 * nu2, {
+        for (int s = 0; s < nu2; s++) {
+          launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st);
+          launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st);
+        }
+      });
+  } else {
+    for (int s = 0; s < nu2; s++) {
+      KT(c, "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
+      KT(c, "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st));
+    }
   }
 }
 
