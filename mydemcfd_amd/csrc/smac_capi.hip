@@ -102,7 +102,7 @@ struct smac_ctx {
   bool props_ready = false;
   uint64_t props_ver = 1;     // bumped whenever the face coefficients may change
   uint64_t mg_ver = ~0ull;    // props_ver the multigrid hierarchy was built for
-  std::vector<double*> mg_b;  // coarse-level face coefficients, 3 per level >= 1
+  std::vector<double*> mg_b;  // coarse-level face coefficients, 3 per level >= 1, and the FP32 copies of every level
   // the V-cycle below the fine level, captured once as a HIP graph and
   // replayed (its kernels are small and launch-bound); key = nu1, nu2
   hipGraphExec_t cg_exec = nullptr;
@@ -993,6 +993,13 @@ static int mg_coeffs(smac_ctx* c) {
   for (int l = 1; l < L && var; l++)
     for (int d = 0; d < 3; d++)
       launch_mg_coarsen_beta(c->lev[l - 1].lv, c->lev[l].lv, d, (double*)c->lev[l].lv.b[d], c->st);
+  // the mixed forms' FP32 copies of every level's coefficients
+  for (int l = 0; l < L && var; l++)
+    for (int d = 0; d < 3; d++)
+      if (c->lev[l].lv.bs[d]) {
+        const Lv& V = c->lev[l].lv;
+        launch_d2f(V.b[d], (float*)V.bs[d], (size_t)V.px * V.py * (size_t)(V.n[2] + 3), c->st);
+      }
   HIPOK(hipGetLastError());
   const Lv& C = c->lev[L - 1].lv;
   std::vector<double> hb[3];
@@ -1077,6 +1084,14 @@ static int mg_setup(smac_ctx* c, int L, int prec) {
           if (!rc) {
             c->mg_b.push_back(b);
             V.lv.b[d] = b;
+          }
+        }
+        if (!rc && prec != SMAC_PREC_FP64) {
+          void* bs = nullptr;
+          rc = balloc(&bs, t * sizeof(float), sizeof(float));
+          if (!rc) {
+            c->mg_b.push_back((double*)bs);
+            V.lv.bs[d] = (const float*)bs;
           }
         }
       }

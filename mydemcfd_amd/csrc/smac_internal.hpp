@@ -16,6 +16,9 @@ struct Lv {
   double inv_d2[3];
   double beta0;
   const double* b[3];
+  // the face coefficients in FP32 for the mixed V-cycles (the compute type
+  // of their smoothers), or nullptr: half the coefficient bytes per sweep
+  const float* bs[3];
 };
 
 // level of the reference's multigrid (natural layout, pitches n+3)
@@ -246,6 +249,8 @@ void launch_vof_substep(const Geo& g, const double h[3], const Vel& V, double** 
 // buffer tb has the compute type of T
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st);
+// dst = (float) src over n elements (the FP32 copies of the face coefficients)
+void launch_d2f(const double* src, float* dst, size_t n, hipStream_t st);
 // red != 0: the residual is 0 on black cells (the pre-smoothing ended with a
 // black half sweep) and formed on the red cells only
 template <typename T>

@@ -38,8 +38,14 @@ __device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k)
     for (int d = 0; d < 3; d++) {
       int qp[3] = {i, j, k};
       qp[d] = L.per[d] ? wrapi(q[d] + 1, L.n[d]) : q[d] + 1;
-      c.a[2 * d] = L.b[d][lrow(L, qp[1], qp[2]) + qp[0]] * L.inv_d2[d];
-      c.a[2 * d + 1] = L.b[d][lrow(L, j, k) + i] * L.inv_d2[d];
+      const size_t op = lrow(L, qp[1], qp[2]) + qp[0], om = lrow(L, j, k) + i;
+      if (sizeof(C) == 4 && L.bs[d]) {  // the mixed V-cycles: FP32 copies
+        c.a[2 * d] = (double)L.bs[d][op] * L.inv_d2[d];
+        c.a[2 * d + 1] = (double)L.bs[d][om] * L.inv_d2[d];
+      } else {
+        c.a[2 * d] = L.b[d][op] * L.inv_d2[d];
+        c.a[2 * d + 1] = L.b[d][om] * L.inv_d2[d];
+      }
     }
     c.diag = c.a[0] + c.a[1] + c.a[2] + c.a[3] + c.a[4] + c.a[5];
 #pragma unroll
@@ -1159,6 +1165,14 @@ __global__ __launch_bounds__(256) void k_mg_coarsen_beta(Lv F, Lv C, int d, doub
     S3[m + 1] = (v[0][0] + v[0][1]) + (v[1][0] + v[1][1]);
   }
   cb[id] = bnd ? S3[1] * 0.25 : ((0.25 * S3[0] + 0.5 * S3[1]) + 0.25 * S3[2]) * 0.25;
+}
+
+__global__ __launch_bounds__(256) void k_d2f(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) dst[t] = (float)src[t];
+}
+void launch_d2f(const double* src, float* dst, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
 }
 
 void launch_mg_coarsen_beta(const Lv& F, const Lv& C, int d, double* cb, hipStream_t st) {
