@@ -27,7 +27,8 @@ width of the V-cycle):
   kernels (per launch, per cell of the level it runs on; W_z = width of z,
   8 in FP64, W_s in the mixed forms):
          spmv_dot 16, update 24 (r, Ad; +W_s: the mixed forms also write
-         (T) r), dot2 8 + W_z (+8 flexible), xpay 32 + W_z (z, d, dp: the
+         (T) r), update_zs the same + 0.5*W_s (it also does the next
+         V-cycle's zero-start red half sweep, e = f/diag), dot2 8 + W_z (+8 flexible), xpay 32 + W_z (z, d, dp: the
          dp += alpha d of the previous update rides on the direction update),
          dir_spmv 40 + W_z (xpay fused into the next SpMV: z, d_old, dp read,
          dp, d, Ad written), dp_flush 24 (that dp update after the last iteration; a step
@@ -61,6 +62,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (< 8: a mixed form)
     "spmv_dot": lambda W, flex: 16.0,
     "update": lambda W, flex: 24.0 + (W if W < 8.0 else 0.0),
+    "update_zs": lambda W, flex: 24.0 + (W if W < 8.0 else 0.0) + 0.5 * W,
     "dot2": lambda W, flex: 8.0 + W + (8.0 if flex else 0.0),
     "xpay": lambda W, flex: 32.0 + W,
     "dir_spmv": lambda W, flex: 40.0 + W,

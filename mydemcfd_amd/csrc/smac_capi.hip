@@ -78,6 +78,9 @@ struct smac_ctx {
   // dp is zero (as the correction leaves it, and at creation): the next solve
   // need not clear it
   bool dp_zero = true;
+  // the residual update has done level 0's zero-start half sweep: the next
+  // V-cycle starts with the black half sweep
+  bool zs_done = false;
   // smac_step lets the solve leave its last dp += alpha d to the correction
   // (defer_dp), which then adds it cell by cell (dp_deferred)
   bool defer_dp = false, dp_deferred = false;
@@ -1156,10 +1159,12 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   if (nu1 == 0) KT(c, fine ? "mg_fill_l0" : "mg_coarse_levels", launch_fill<T>(V.lv, e, c->st));
   if (fine) {
     // the first red half sweep starts from zero (reads f, writes e: one word
-    // less per cell than the others), timed under its own name; the other
-    // pre-smoothing half sweeps run back to back and are timed together
+    // less per cell than the others), timed under its own name, unless the
+    // residual update has done it; the other pre-smoothing half sweeps run
+    // back to back and are timed together
     if (nu1 > 0) {
-      KT(c, "mg_rbgs_l0_zs", launch_mg_rbgs<T>(V.lv, e, f, 0, 1, c->st));
+      if (!c->zs_done) KT(c, "mg_rbgs_l0_zs", launch_mg_rbgs<T>(V.lv, e, f, 0, 1, c->st));
+      c->zs_done = false;
       KT_N(c, "mg_rbgs_l0", 2 * nu1 - 1, {
         launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st);
         for (int s = 1; s < nu1; s++) {
@@ -1465,6 +1470,12 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     if (rc) return rc;
   }
   const bool fuse = c->dv2 && dir_spmv_applies<T>(g, z, c->dv, c->dv2, c->dp, c->Ad);
+  // the zero-start red half sweep of level 0 rides on the residual update
+  // (uniform coefficients: pointwise f/diag)
+  const int zs_mode = (mg && !var && o.nu1 >= 1)
+                          ? mg_zs_division<T>(c->lev[0].lv, (const T*)c->lev[0].e, (const T*)c->lev[0].f)
+                          : 0;
+  c->zs_done = false;
   bool ad_ready = false;
   for (iter = 0; iter < o.max_iter && !conv && !nan; ++iter) {
     if (!ad_ready) KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
@@ -1479,7 +1490,11 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       fa.sb = S_PAP;
       launch_finalize(fa, c->S, s);
     }
-    KT(c, "update", launch_update<T>(g, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
+    if (zs_mode)
+      KT(c, "update_zs",
+         c->zs_done = launch_update<T>(g, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s, z, zs_mode, &c->lev[0].lv));
+    else
+      KT(c, "update", launch_update<T>(g, c->r, c->Ad, c->S, S_ALPHA, P[0], P[1], f0, s));
     have_update = true;
     dp_pending = true;  // dp += alpha d waits for the next xpay (or the flush)
     if (refg && pure) {

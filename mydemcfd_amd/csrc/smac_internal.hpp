@@ -184,9 +184,17 @@ bool dir_spmv_applies(const Geo& g, const Z* z, const double* dold, const double
 template <typename Z>
 void launch_dir_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* dold, double* dnew, double* dp,
                      double* Ad, double* p0, const double* S, int slot_alpha, int slot_beta, hipStream_t st);
+// r -= alpha Ad with |r|^2, max|r| and f0 = (F) r; e0 != nullptr and zs != 0:
+// also the zero-start red half sweep of level 0 (uniform coefficients;
+// zs = 1: IEEE division, 2: the FP32 reciprocal). Returns whether it did that
+// (the chunk form only)
 template <typename F>
-void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
-                   double* p1, F* f0, hipStream_t st);
+bool launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
+                   double* p1, F* f0, hipStream_t st, F* e0 = nullptr, int zs = 0, const Lv* L0 = nullptr);
+// the division the zero-start half sweep of level L uses (mg_rbgs's dispatch):
+// 2 for the FP32 reciprocal, 1 for IEEE division
+template <typename T>
+int mg_zs_division(const Lv& L, const T* e, const T* f);
 void launch_dp_flush(const Geo& g, double* dp, const double* d, const double* S, int slot_alpha, hipStream_t st);
 template <typename Z>
 void launch_dot2(const Geo& g, const double* r, const Z* z, const double* Ad, int flexible, double* p0,

@@ -937,12 +937,17 @@ __global__ __launch_bounds__(256) void k_update(Geo g, double* __restrict__ r, c
   }
 }
 
-// chunk form of k_update (same arithmetic and sums)
+// chunk form of k_update (same arithmetic and sums). zs != 0 (uniform
+// coefficients): the first half sweep of the V-cycle that follows rides
+// along -- it starts from e = 0, so on the red cells ((i+j+k) even)
+// e = f/diag, pointwise, with f the value just stored (rounded to F) and the
+// division of the level-0 smoother (zs = 1: IEEE division, 2: the FP32
+// reciprocal); e0 is level 0's e, L0 its geometry
 template <typename F, int NM>
 __global__ __launch_bounds__(256) void k_update_ch(Geo g, double* __restrict__ r, const double* __restrict__ Ad,
                                                    const double* __restrict__ S, int slot_alpha,
                                                    double* __restrict__ part0, double* __restrict__ part1,
-                                                   F* __restrict__ f0) {
+                                                   F* __restrict__ f0, F* __restrict__ e0, int zs, Lv L0) {
   using C = typename St<F>::C;
   __shared__ double rv[ChunkGeom<NM>::RB][NM][64][2];
   CHUNK_SETUP(NM)
@@ -960,6 +965,23 @@ __global__ __launch_bounds__(256) void k_update_ch(Geo g, double* __restrict__ r
     }
     Vec<double, 2>::st(r + o, rn);
     if (f0) Vec<F, 2>::st(f0 + o, fv);
+    if (zs) {
+      // the lane's pair: split positions q, q+1 of one half, cells of one colour
+      const int nh = g.n[0] >> 1;
+      const int q = 1 + 2 * lane + 128 * ch;
+      const int i0 = q <= nh ? 2 * q : 2 * (q - nh) - 1;
+      if (((i0 + j + k) & 1) == 0) {
+        C ev[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          const C fs = (C)stored_value<F>(rn[t]);
+          const C cd = (C)coef_uniform(L0.n, L0.per, L0.inv_d2, L0.beta0, i0 + 2 * t, j, k).diag;
+          if constexpr (sizeof(C) == 4) ev[t] = zs == 2 ? fs * __builtin_amdgcn_rcpf((float)cd) : fs / cd;
+          else ev[t] = fs / cd;
+        }
+        Vec<F, 2>::st(e0 + o, ev);
+      }
+    }
   }
   rv[rib][ch][lane][0] = rn[0];
   rv[rib][ch][lane][1] = rn[1];
@@ -1591,20 +1613,24 @@ void launch_dir_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* do
 #undef SMAC_DSP
 }
 template <typename F>
-void launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
-                   double* p1, F* f0, hipStream_t st) {
+bool launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
+                   double* p1, F* f0, hipStream_t st, F* e0, int zs, const Lv* L0) {
   int nm = pair_nm(g, r);
   if (nm && pair_nm(g, Ad) != nm) nm = 0;
   if (nm && f0 && ((uintptr_t)(f0 + 1) % 128) != 0) nm = 0;
+  if (nm && e0 && ((uintptr_t)(e0 + 1) % 128) != 0) nm = 0;
   if (!nm) {
     hipLaunchKernelGGL(k_update<F>, rows_grid(g), ROWB, 0, st, g, r, Ad, S, slot_alpha, p0, p1, f0);
-    return;
+    return false;
   }
+  if (!e0 || !L0) zs = 0;
+  const Lv lz = L0 ? *L0 : Lv{};
 #define SMAC_UPD(N)                                                                                         \
   hipLaunchKernelGGL((k_update_ch<F, N>), chunk_grid<N>(g), dim3(ChunkGeom<N>::THREADS), 0, st, g, r, Ad, S, \
-                     slot_alpha, p0, p1, f0)
+                     slot_alpha, p0, p1, f0, e0, zs, lz)
   SMAC_CHUNK_SWITCH(nm, SMAC_UPD)
 #undef SMAC_UPD
+  return zs != 0;
 }
 void launch_dp_flush(const Geo& g, double* dp, const double* d, const double* S, int slot_alpha, hipStream_t st) {
   hipLaunchKernelGGL(k_dp_flush, rows_grid(g), ROWB, 0, st, g, dp, d, S, slot_alpha);
@@ -1648,8 +1674,8 @@ void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* 
                                    double, const double*, int);                                           \
   template void launch_init_r<Z>(const Geo&, const double*, double*, Z*, double*, int, const double*, int, \
                                  double, int, double*, double*, int, int, hipStream_t);                   \
-  template void launch_update<Z>(const Geo&, double*, const double*, const double*, int, double*, double*, \
-                                 Z*, hipStream_t);                                                        \
+  template bool launch_update<Z>(const Geo&, double*, const double*, const double*, int, double*, double*, \
+                                 Z*, hipStream_t, Z*, int, const Lv*);                                    \
   template void launch_dot2<Z>(const Geo&, const double*, const Z*, const double*, int, double*, double*,  \
                                hipStream_t);                                                              \
   template void launch_xpay<Z>(const Geo&, const Z*, double*, double*, const double*, int, int, hipStream_t); \

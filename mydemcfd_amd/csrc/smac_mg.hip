@@ -994,6 +994,16 @@ static constexpr int vec_cpl(int vpl) {
 }
 
 template <typename T>
+int mg_zs_division(const Lv& L, const T* e, const T* f) {
+  constexpr int C16 = (int)(16 / sizeof(T));
+  if (sizeof(T) == 8) return 1;
+  const bool al = rows_aligned(L, e) && rows_aligned(L, f);
+  // the two- and four-rows-per-wave forms divide with sm_div (the reciprocal)
+  if (al && L.n[0] == 64 * C16 && L.n[1] % 8 == 0) return 2;
+  if (al && L.n[0] == 32 * C16 && L.n[1] % 16 == 0) return 2;
+  return 1;
+}
+template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st) {
   const int half = L.n[0] - (L.n[0] >> 1);  // longest active half row
   const int vpl = row_vpl(half);
@@ -1187,6 +1197,7 @@ void launch_fill(const Lv& L, T* x, hipStream_t st) {
 
 #define SMAC_INST_MG(T)                                                                                 \
   template void launch_mg_rbgs<T>(const Lv&, T*, const T*, int, int, hipStream_t);                     \
+  template int mg_zs_division<T>(const Lv&, const T*, const T*);                                         \
   template void launch_mg_restrict<T>(const Lv&, const Lv&, const T*, const T*, typename St<T>::C*, T*, int, \
                                       hipStream_t);                                                    \
   template void launch_mg_prolong<T>(const Lv&, const Lv&, const T*, T*, int, hipStream_t);            \
