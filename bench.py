@@ -376,6 +376,94 @@ def cpu_baseline(n_cpu, re, cfl, tol):
             "build": " ".join(["gcc"] + oracle.NATIVE_FLAGS)}
 
 
+def protocol(tol):
+    """The CPU-baseline protocol in full, beside the GPU on the same steps (a
+    few minutes; not part of the default run): C1 the 64^3 lid-driven cavity
+    (Re 100, dt 1e-3, 100 steps, Jacobi-PCG), C2 the 256^3 periodic TG + curl
+    perturbation (seed 0x5EED0002, RMS 1e-3, Re 1600, CFL 0.3, 20 steps), C3
+    the 512^3 TG + k^-5/3 field (2 steps on the CPU). CPU: the oracle built
+    here with -O3 -march=native in REF mode (the reference algorithm: Jacobi-PCG
+    for C1, the reference GMG-PCG for C2 and C3) and NS mode (RBGS-MG-PCG
+    V(2,2)), one warm-up step, then every step of the run timed (C3: the
+    median of its two). GPU: the same steps in FP64 and the mixed forms, with
+    the byte model of the library (bytes_model) for the roofline fraction.
+    All solves to rel-res tol. Prints one JSON line."""
+    import statistics
+    import tempfile
+
+    import torch
+
+    import mydemcfd_amd
+    from mydemcfd_amd import cases
+    from oracle import oracle
+
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    lib = oracle.native_lib(tempfile.mkdtemp(prefix="smac_protocol_"))
+    mydemcfd_amd.load()
+    c2_fields = None
+
+    def setup(ctx, cfg, n):
+        if cfg == "C1":
+            return 1e-3
+        fields = c2_fields if cfg == "C2" else cases.tg_fields(n, seed=0x5EED0003, pert="k53")
+        for f, a in zip(("u", "v", "w"), fields):
+            ctx.upload(f, a)
+        return cases.cfl_dt(ctx, 0.3)
+
+    def run(make, cfg, n, steps, kw, gpu):
+        ctx = make()
+        dt = setup(ctx, cfg, n)
+        ctx.step(dt, **kw)  # warm-up (multigrid set-up included)
+        if gpu:
+            ctx.sync()
+        ts, its, by = [], [], 0.0
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            st = ctx.step(dt, **kw)
+            if gpu:
+                ctx.sync()
+            ts.append(time.perf_counter() - t0)
+            its.append(st.iters)
+            by += st.bytes_model
+        ctx.close()
+        cells = n ** 3
+        t = statistics.median(ts) * steps if cfg == "C3" and not gpu else sum(ts)
+        out = {"cell_steps_per_s": cells * steps / t, "s": t, "iters_mean": sum(its) / len(its)}
+        if gpu:
+            out["roofline_frac"] = by / t / 1e9 / HBM_PEAK_GBS
+        print(f"# protocol {cfg} {'gpu' if gpu else 'cpu'} {kw}: {out}", file=sys.stderr, flush=True)
+        return out
+
+    rows = {}
+    cfgs = [("C1", 64, 100), ("C2", 256, 20), ("C3", 512, 2)]
+    for cfg, n, steps in cfgs:
+        if cfg == "C2":
+            c2_fields = cases.tg_fields(256, seed=0x5EED0002, pert="white", pert_rms=1e-3)
+        desc = cases.cavity_desc(n) if cfg == "C1" else cases.periodic_desc(n, re=1600.0)
+        ref = dict(precond="jacobi", tol=tol, max_iter=5000) if cfg == "C1" else \
+            dict(precond="ref_gmg", mg_levels=4, tol=tol)
+        ns = dict(precond="mg", tol=tol)
+        row = {"n": n, "steps": steps}
+        row["cpu_ref"] = run(lambda: oracle.OracleContext(lib_=lib, **desc), cfg, n, steps, ref, False)
+        if cfg != "C1":
+            row["cpu_ns"] = run(lambda: oracle.OracleContext(lib_=lib, **desc), cfg, n, steps, ns, False)
+        gsteps = steps if cfg != "C3" else 10
+        if cfg == "C1":
+            row["gpu_fp64"] = run(lambda: mydemcfd_amd.SMACContext(**desc), cfg, n, gsteps, ref, True)
+        else:
+            for form, spec in FORMS.items():
+                kw = dict(precond="mg", precision=spec["precision"], flexible=spec["flexible"], tol=tol)
+                row[f"gpu_{form}"] = run(lambda: mydemcfd_amd.SMACContext(**desc), cfg, n, gsteps, kw, True)
+        rows[cfg] = row
+        torch.cuda.empty_cache()
+    line = {"protocol": "CPU baseline in full beside the GPU", "cpu_model": cpu_model(),
+            "threads": oracle.threads(lib), "build": " ".join(["gcc"] + oracle.NATIVE_FLAGS),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "tol": tol, "configs": rows}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes, reps=4):
     """A plain device-to-device copy of nbytes (reads nbytes, writes nbytes),
     the reference point for the HBM byte counters."""
@@ -409,6 +497,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--protocol", action="store_true",
+                    help="run the CPU-baseline protocol in full beside the GPU instead (minutes)")
     ap.add_argument("--c5-steps", type=int, default=50, help="steps of the VoF dam-break workload (0: skip)")
     ap.add_argument("--c5-adjoint-steps", type=int, default=50, help="steps of its adjoint (0: skip)")
     args = ap.parse_args()
@@ -420,6 +510,9 @@ def main():
 
     if args.copy_calib:
         copy_calibration(args.copy_calib)
+        return
+    if args.protocol:
+        protocol(args.tol)
         return
 
     import mydemcfd_amd
