@@ -200,9 +200,13 @@ def _energy(out, masks, h3):
     return 0.5 * h3 * sum(float((out[f] ** 2)[masks[f]].sum()) for f in ("u", "v", "w"))
 
 
-def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
+def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6, solver=None):
     """Gradient of J = 1/2 sum |u_N|^2 dV with respect to u_0 through the
-    autograd binding, against central differences along ndir directions."""
+    autograd binding, against central differences along ndir directions.
+    solver: the options of the rollout whose gradient is checked (default the
+    stationary multigrid of the differences); the differences always run the
+    stationary multigrid, whose forward equals that of an MG-PCG to 1e-8 up to
+    that tolerance (checked on J) while being a linear map of the RHS."""
     torch = _torch()
     from mydemcfd_amd import autograd, cases
 
@@ -212,7 +216,8 @@ def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
     u, v, w = cases.adjoint_fields(n)
     st0 = _state(ctx, {"u": u, "v": v, "w": w})
     dt = 2e-3
-    opts = ctx.opts(**dict(TIGHT, precision=precision, flexible=0 if precision == "fp64" else 1))
+    sol = dict(TIGHT) if solver is None else dict(solver)
+    opts = ctx.opts(**dict(sol, precision=precision, flexible=0 if precision == "fp64" else 1))
     h3 = (2 * math.pi / n) ** 3
     x = {f: t.clone().requires_grad_(f != "p") for f, t in st0.items()}
     out = autograd.rollout(ctx, x["u"], x["v"], x["w"], x["p"], dt, nsteps, opts)
@@ -220,29 +225,40 @@ def _rollout_grad_check(hip, n, nsteps, ndir, precision="fp64", tol_rel=1e-6):
     J.backward()
     grad = {f: x[f].grad for f in ("u", "v", "w")}
     fd_opts = ctx.opts(**TIGHT)
-    fds, ads = [], []
+    if solver is not None:
+        # the rollout checked and the one differenced reach the same state
+        o0 = _step_out(ctx, st0, dt, fd_opts, nsteps)
+        J0 = 0.5 * h3 * sum(float(((o0[f]) ** 2)[masks[f]].sum()) for f in ("u", "v", "w"))
+        print(f"  J: rollout {float(J):.12e}, stationary forward {J0:.12e}", flush=True)
+        assert abs(float(J) - J0) <= 1e-8 * abs(J0)
+    def quotient(d, eps):
+        op = _step_out(ctx, {f: st0[f] + eps * d[f] for f in st0}, dt, fd_opts, nsteps)
+        om = _step_out(ctx, {f: st0[f] - eps * d[f] for f in st0}, dt, fd_opts, nsteps)
+        # J(+) - J(-) = h^3/2 sum (a - b)(a + b): formed from the difference, not from two large sums
+        dj = 0.5 * h3 * sum(float(((op[f] - om[f]) * (op[f] + om[f]))[masks[f]].sum()) for f in ("u", "v", "w"))
+        return dj / (2 * eps)
+
+    eps = 1e-6
+    fails = []
     for k in range(ndir):
         # smooth random directions (seeds 0x5EED0014 + k): k^-5/3 fields with |k| <= 4
         du, dv, dw = cases.adjoint_fields(n, seed=0x5EED0014 + k, kmax=4)
         d = {f: torch.tensor(a, device="cuda") * masks[f] for f, a in zip(("u", "v", "w"), (du, dv, dw))}
         d["p"] = torch.zeros_like(st0["p"])
-        eps = 1e-6
-        op = _step_out(ctx, {f: st0[f] + eps * d[f] for f in st0}, dt, fd_opts, nsteps)
-        om = _step_out(ctx, {f: st0[f] - eps * d[f] for f in st0}, dt, fd_opts, nsteps)
-        # J(+) - J(-) = h^3/2 sum (a - b)(a + b): formed from the difference, not from two large sums
-        dj = 0.5 * h3 * sum(float(((op[f] - om[f]) * (op[f] + om[f]))[masks[f]].sum()) for f in ("u", "v", "w"))
-        fd = dj / (2 * eps)
+        fd = quotient(d, eps)
         ad = sum(float((grad[f] * d[f]).sum()) for f in ("u", "v", "w"))
-        fds.append(fd)
-        ads.append(ad)
-        print(f"  direction: FD {fd:.10e} adjoint {ad:.10e} rel {abs(fd - ad) / abs(fd):.2e}", flush=True)
-    # relative to each directional derivative, floored at a tenth of the
-    # largest: the difference quotient carries an absolute noise (~5e-7 here,
-    # the same for every direction), so a direction nearly orthogonal to the
-    # gradient would otherwise measure that noise against a tiny derivative
-    floor = 0.1 * max(abs(f) for f in fds)
-    errs = [abs(f - a) / max(abs(f), floor) for f, a in zip(fds, ads)]
-    assert max(errs) <= tol_rel, errs
+        # The quotient's own resolution: its truncation error (eps^2/6 times
+        # the third derivative) is below 1e-12 relative at these eps, so the
+        # change of the quotient between eps/2, eps and 2 eps is the rounding
+        # of the 16-step forwards (about 1e-14 of J, i.e. ~5e-7 absolute in
+        # the quotient here), which no difference at eps = 1e-6 resolves.
+        res = max(abs(quotient(d, e) - fd) for e in (0.5 * eps, 2 * eps))
+        rel = abs(fd - ad) / abs(fd)
+        print(f"  direction {k}: FD {fd:.10e} adjoint {ad:.10e} rel {rel:.2e} "
+              f"(quotient resolution {res:.1e}, {res / abs(fd):.1e} relative)", flush=True)
+        if abs(fd - ad) > tol_rel * abs(fd) + res:
+            fails.append((k, fd, ad, res))
+    assert not fails, fails
     return grad
 
 
@@ -251,14 +267,50 @@ def test_rollout_gradient_vs_finite_differences(hip):
     _rollout_grad_check(hip, 32, 4, 4)
 
 
+PCG_1E8 = dict(precond="mg", stop_rule="rel_l2", tol=1e-8, max_iter=400)
+
+
 def test_c4_adjoint_128_16_steps(hip):
     """The adjoint configuration: 128^3 periodic box, k^-5/3 initial velocity
     with unit RMS (seed 0x5EED0004), Re = 500, dt = 2e-3, 16 steps with
-    per-step checkpoints; gradient of the final kinetic energy along 8 seeded
-    smooth directions (seeds 0x5EED0014 + k, |k| <= 4) against central
-    differences, eps = 1e-6, through the stationary-multigrid forward."""
-    print("case: C4 adjoint, periodic 128^3 x16 steps, 8 directions", flush=True)
-    _rollout_grad_check(hip, 128, 16, 8)
+    per-step checkpoints; the gradient of the final kinetic energy of the
+    rollout with MG-PCG to 1e-8, along 8 seeded smooth directions (seeds
+    0x5EED0014 + k, |k| <= 4), against central differences, eps = 1e-6, of the
+    stationary-multigrid forward."""
+    print("case: C4 adjoint, periodic 128^3 x16 steps, 8 directions, MG-PCG to 1e-8", flush=True)
+    _rollout_grad_check(hip, 128, 16, 8, solver=PCG_1E8)
+
+
+def test_c4_mixed_adjoint_vs_fp64(hip):
+    """C4 at its size: the gradients of the 128^3 x 16-step rollout with the
+    mixed FP32 and BF16 V-cycles (flexible MG-PCG) against the FP64 one, all to
+    rel-res 1e-12."""
+    torch = _torch()
+    from mydemcfd_amd import autograd, cases
+
+    print("case: mixed vs FP64 adjoint gradient, periodic 128^3 x16", flush=True)
+    n = 128
+    desc = cases.periodic_desc(n, re=500.0)
+    grads = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        ctx = hip.SMACContext(**desc)
+        masks = _masks(ctx, desc, torch)
+        u, v, w = cases.adjoint_fields(n)
+        st0 = _state(ctx, {"u": u, "v": v, "w": w})
+        opts = ctx.opts(precond="mg", stop_rule="rel_l2", tol=1e-12, max_iter=400, precision=prec,
+                        flexible=0 if prec == "fp64" else 1)
+        x = {f: t.clone().requires_grad_(f != "p") for f, t in st0.items()}
+        out = autograd.rollout(ctx, x["u"], x["v"], x["w"], x["p"], 2e-3, 16, opts)
+        J = 0.5 * sum(((o ** 2) * masks[f]).sum() for o, f in zip(out[:3], ("u", "v", "w")))
+        J.backward()
+        grads[prec] = torch.cat([x[f].grad[masks[f]] for f in ("u", "v", "w")])
+        del out, x, ctx
+        torch.cuda.empty_cache()
+    ref = grads["fp64"]
+    for prec in ("fp32", "bf16"):
+        err = float((grads[prec] - ref).abs().max() / ref.abs().max())
+        print(f"  {prec}: max rel diff vs FP64 gradient {err:.2e}", flush=True)
+        assert err <= 1e-6, (prec, err)
 
 
 def test_mixed_adjoint_vs_fp64(hip):
