@@ -328,6 +328,70 @@ __device__ __forceinline__ double predict_face_pu(const Geo& g, const Vel& V, co
   return predict_face_gen<A>(g, ld, dt);
 }
 
+constexpr int PT_X = 64, PT_Y = 4, PT_W = PT_X + 4, PT_PL = (PT_X + 4) * (PT_Y + 4);
+
+// The + side of the face of component A along direction E (periodic, one
+// phase): the convective flux through it (MUSCL-minmod, mass flux from the E
+// component on the edge) and the viscous stress there (the normal stress when
+// E == A, the cross stress otherwise). The - side of a face is the + side of
+// the face one cell down along E with the very same operands (only the two
+// terms of the mass-flux sum swap places), so face_assemble applied to + sides
+// computed here and - sides taken from the neighbours gives predict_face_gen's
+// result bit for bit.
+template <int A, int E, class LD>
+__device__ __forceinline__ void plus_side(const Geo& g, const LD& ld, double& M, double& T) {
+  const double rho0 = g.rho0, mu0 = g.mu0;
+  auto at = [&](int c, int sE, int sA) {
+    int o[3] = {0, 0, 0};
+    o[E] += sE;
+    o[A] += sA;
+    return ld(c, o[0], o[1], o[2]);
+  };
+  double c[5];
+  c[0] = 0.0;  // flux_plus reads c[1..4]
+#pragma unroll
+  for (int s = -1; s <= 2; s++) c[s + 2] = at(A, s, 0);
+  double mp;
+  if constexpr (E == A) {
+    mp = 0.5 * (rho0 * c[2] + rho0 * c[3]);
+    T = mu0 * (c[3] - c[2]);
+  } else {
+    // the E component on the edge: at q + e_E and q + e_E - e_A
+    const double bpp = at(E, 1, 0), bmp = at(E, 1, -1);
+    mp = 0.5 * (rho0 * bmp + rho0 * bpp);
+    const double mu_p = 0.5 * (mu0 + mu0);
+    T = mu_p * ((bpp - bmp) * g.inv_d[A] + (c[3] - c[2]) * g.inv_d[E]);
+  }
+  M = flux_plus(mp, c);
+}
+
+// predict_face_gen's sum from the + and - sides (index E) of the face of
+// component A, its own old value v0 and the pressure on both sides.
+template <int A>
+__device__ __forceinline__ double face_assemble(const Geo& g, const double Mp[3], const double Tp[3],
+                                                const double Mm[3], const double Tm[3], double v0, double pq,
+                                                double pm, double dt) {
+  constexpr int B1 = A == 0 ? 1 : 0;
+  constexpr int B2 = A == 2 ? 1 : 2;
+  const double rho0 = g.rho0;
+  double tmp = 2. * (Tp[A] - Tm[A]) * g.inv_d2[A];
+  tmp += (Tp[B1] - Tm[B1]) * g.inv_d[B1];
+  tmp += (Tp[B2] - Tm[B2]) * g.inv_d[B2];
+#pragma unroll
+  for (int E = 0; E < 3; E++) tmp -= (Mp[E] - Mm[E]) * g.inv_d[E];
+  const double beta = g.beta0;
+  const double rho_old_f = 0.5 * (rho0 + rho0);
+  tmp -= (pq - pm) * g.inv_d[A];
+  return beta * (v0 * rho_old_f) + dt * (beta * tmp + g.g[A]);
+}
+
+template <int A, class LD>
+__device__ __forceinline__ void plus_sides(const Geo& g, const LD& ld, double M[3], double T[3]) {
+  plus_side<A, 0>(g, ld, M[0], T[0]);
+  plus_side<A, 1>(g, ld, M[1], T[1]);
+  plus_side<A, 2>(g, ld, M[2], T[2]);
+}
+
 // The fused periodic predictor as a z-march over a 64 x 4 column tile: the
 // block keeps planes k-1 and k of u and v, k and k+1 of w and k of p (each
 // with a two-cell x/y halo) in LDS, and every thread its own column's u, v, w
@@ -336,15 +400,30 @@ __device__ __forceinline__ double predict_face_pu(const Geo& g, const Vel& V, co
 // down for u and v; all x/y offsets stay in plane k but those of w at k+1
 // and u, v at k-1, which are within one cell). A plane step writes the next
 // planes (loaded during the previous step) into the ring slots the step no
-// longer reads, then issues the loads for the following step before the
-// arithmetic. Needs nx % 64 == 0, ny % 4 == 0 and nz >= 3 (one periodic
-// wrap reaches k-2 and k+3); the per-face arithmetic is
-// predict_face_gen, so the results are those of k_predict_fused bit for bit.
-constexpr int PT_X = 64, PT_Y = 4, PT_W = PT_X + 4, PT_PL = (PT_X + 4) * (PT_Y + 4);
+// longer reads, then issues the loads for the following step.
+//
+// Each face side is evaluated once (plus_side): a thread computes the + sides
+// of its three faces; the - sides come from the cell one plane down
+// (registers carried along the march; on a column's first plane they are
+// evaluated there), from the lane one cell down in x (a lane shuffle; lane 0
+// reads the x halo) and from the row one cell down in y (LDS; row 0 reads the
+// y halo). Wave c evaluates component c's halo sides from the LDS planes: along
+// y on the row j0 - 1 (all lanes) and along x on the column i0 - 1 (lanes 0..3).
+// Plane k is evaluated after the second barrier of step k and assembled after
+// the first barrier of step k + 1. Per cell this is about 480 VALU
+// instructions against 640 when each face evaluates both of its sides.
+//
+// Needs nx % 64 == 0, ny % 4 == 0 and nz >= 3 (one periodic wrap reaches k-2
+// and k+3); the results are those of predict_face_gen (k_predict_fused) bit
+// for bit.
 __global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const double* __restrict__ p,
                                                       double* __restrict__ us0, double* __restrict__ us1,
                                                       double* __restrict__ us2, double dt, int KC) {
   __shared__ double Ush[2 * PT_PL], Vsh[2 * PT_PL], Wsh[2 * PT_PL], Psh[PT_PL];
+  // + sides along y of rows j0-1 .. j0+2 (index row - j0 + 1), quantity 2A (M) / 2A+1 (T)
+  __shared__ double PY[4 * 6 * PT_X];
+  // + sides along x of the column i0 - 1, rows j0 .. j0+3
+  __shared__ double PX[PT_Y * 6];
   const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
   const int t = ty * PT_X + tx;
   const int i0 = 1 + (int)blockIdx.x * PT_X, j0 = 1 + (int)blockIdx.y * PT_Y;
@@ -353,7 +432,6 @@ __global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const dou
   const size_t px = (size_t)g.px, plane = (size_t)g.px * (size_t)g.py;
   const int k0 = 1 + (int)blockIdx.z * KC;
   const int k1 = min(k0 + KC - 1, nz);
-  // tile entries of this thread: e = t, t+256, t+512 (< PT_PL)
   size_t toff[3];
   bool tval[3];
 #pragma unroll
@@ -380,7 +458,6 @@ __global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const dou
   const double* ua = V.c[0];
   const double* va = V.c[1];
   const double* wa = V.c[2];
-  // prologue: LDS u, v at k0-1, w at k0; own windows u, v, w at k0-2..k0+1, p at k0-1
   double uz[5], vz[5], wz[5], pz[2];
   {
     double tu[3], tv[3], tw[3];
@@ -399,7 +476,6 @@ __global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const dou
     tile_st(Vsh + ((k0 - 1) & 1) * PT_PL, tv);
     tile_st(Wsh + (k0 & 1) * PT_PL, tw);
   }
-  // prefetch of the first step: u, v, p at k0, w at k0+1; own u, v, w at k0+2
   double nu[3], nv[3], nw[3], np[3], ou, ov, ow;
   tile_ld(ua, k0, nu);
   tile_ld(va, k0, nv);
@@ -411,8 +487,58 @@ __global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const dou
     ov = va[o];
     ow = wa[o];
   }
-  for (int k = k0; k <= k1; k++) {
-    __syncthreads();  // the previous step no longer reads the slots written here
+  // - sides along z of the plane being evaluated (the + sides of the plane below)
+  double mzM[3], mzT[3];
+  // + sides of the pending plane k - 1, its pressure one cell down in x and y
+  double Mu[3], Tu[3], Mv[3], Tv[3], Mw[3], Tw[3], pm0 = 0.0, pm1 = 0.0;
+  // plane k is evaluated after the second barrier of step k and assembled
+  // after the first barrier of step k + 1, which also orders the exchange
+  // through PY / PX (written after the second barrier, overwritten only after
+  // the second barrier of the next step)
+  for (int k = k0; k <= k1 + 1; k++) {
+    __syncthreads();  // plane k - 1's exchange is written; the ring slots below are free
+    if (k > k0) {
+      double myM[3], myT[3], mxM[3], mxT[3];
+      const double* py = PY + ty * 6 * PT_X + tx;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        myM[a] = py[(2 * a) * PT_X];
+        myT[a] = py[(2 * a + 1) * PT_X];
+      }
+      const double xs[6] = {Mu[0], Tu[0], Mv[0], Tv[0], Mw[0], Tw[0]};
+      double xm[6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        const double s = __shfl_up(xs[q], 1, 64);
+        xm[q] = tx == 0 ? PX[ty * 6 + q] : s;
+      }
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        mxM[a] = xm[2 * a];
+        mxT[a] = xm[2 * a + 1];
+      }
+      const size_t id = kpl(k - 1) + own_off;
+      const double pq = pz[1];
+      {
+        const double Mm[3] = {mxM[0], myM[0], mzM[0]}, Tm[3] = {mxT[0], myT[0], mzT[0]};
+        us0[id] = face_assemble<0>(g, Mu, Tu, Mm, Tm, uz[2], pq, pm0, dt);
+      }
+      {
+        const double Mm[3] = {mxM[1], myM[1], mzM[1]}, Tm[3] = {mxT[1], myT[1], mzT[1]};
+        us1[id] = face_assemble<1>(g, Mv, Tv, Mm, Tm, vz[2], pq, pm1, dt);
+      }
+      {
+        const double Mm[3] = {mxM[2], myM[2], mzM[2]}, Tm[3] = {mxT[2], myT[2], mzT[2]};
+        us2[id] = face_assemble<2>(g, Mw, Tw, Mm, Tm, wz[2], pq, pz[0], dt);
+      }
+      mzM[0] = Mu[2];
+      mzT[0] = Tu[2];
+      mzM[1] = Mv[2];
+      mzT[1] = Tv[2];
+      mzM[2] = Mw[2];
+      mzT[2] = Tw[2];
+    }
+    if (k > k1) break;
     double* U0 = Ush + (k & 1) * PT_PL;
     double* V0 = Vsh + (k & 1) * PT_PL;
     double* W1 = Wsh + ((k + 1) & 1) * PT_PL;
@@ -432,6 +558,25 @@ __global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const dou
     pz[0] = pz[1];
     __syncthreads();
     pz[1] = Psh[own];
+    const double* U1 = Ush + ((k - 1) & 1) * PT_PL;
+    const double* V1 = Vsh + ((k - 1) & 1) * PT_PL;
+    const double* W0 = Wsh + (k & 1) * PT_PL;
+    // LDS planes at LDS index o: plane k of every field, k + 1 of w, k - 1 of u and v
+    auto lds = [&](int c, int o, int dk) -> double {
+      if (dk == 0) return c == 0 ? U0[o] : (c == 1 ? V0[o] : (c == 2 ? W0[o] : Psh[o]));
+      if (dk == 1) return W1[o];
+      return c == 0 ? U1[o] : V1[o];
+    };
+    auto ld = [&](int c, int di, int dj, int dk) -> double {
+      if (di == 0 && dj == 0) {
+        if (c == 0) return uz[dk + 2];
+        if (c == 1) return vz[dk + 2];
+        if (c == 2) return wz[dk + 2];
+        return pz[dk + 1];
+      }
+      return lds(c, own + dj * PT_W + di, dk);
+    };
+    // the next planes load while this one is evaluated
     if (k < k1) {
       tile_ld(ua, k + 1, nu);
       tile_ld(va, k + 1, nv);
@@ -442,25 +587,65 @@ __global__ __launch_bounds__(256, 3) void k_predict_tile(Geo g, Vel V, const dou
       ov = va[o];
       ow = wa[o];
     }
-    const double* U1 = Ush + ((k - 1) & 1) * PT_PL;
-    const double* V1 = Vsh + ((k - 1) & 1) * PT_PL;
-    const double* W0 = Wsh + (k & 1) * PT_PL;
-    auto ld = [&](int c, int di, int dj, int dk) -> double {
-      if (di == 0 && dj == 0) {
-        if (c == 0) return uz[dk + 2];
-        if (c == 1) return vz[dk + 2];
-        if (c == 2) return wz[dk + 2];
-        return pz[dk + 1];
+    plus_sides<0>(g, ld, Mu, Tu);
+    plus_sides<1>(g, ld, Mv, Tv);
+    plus_sides<2>(g, ld, Mw, Tw);
+    if (k == k0) {
+      // first plane of the column: the - sides along z evaluated on the plane below
+      auto ldz = [&](int c, int di, int dj, int dk) -> double { return ld(c, di, dj, dk - 1); };
+      plus_side<0, 2>(g, ldz, mzM[0], mzT[0]);
+      plus_side<1, 2>(g, ldz, mzM[1], mzT[1]);
+      plus_side<2, 2>(g, ldz, mzM[2], mzT[2]);
+    }
+    if (ty < PT_Y - 1) {
+      double* py = PY + (ty + 1) * 6 * PT_X + tx;
+      py[0 * PT_X] = Mu[1];
+      py[1 * PT_X] = Tu[1];
+      py[2 * PT_X] = Mv[1];
+      py[3 * PT_X] = Tv[1];
+      py[4 * PT_X] = Mw[1];
+      py[5 * PT_X] = Tw[1];
+    }
+    // halo sides, component ty on wave ty: along y on row j0 - 1 (all lanes),
+    // along x on column i0 - 1, row j0 + tx (lanes 0..3)
+    {
+      const int yb = PT_W + tx + 2;
+      const int xb = (tx + 2) * PT_W + 1;
+      auto ldy = [&](int c, int di, int dj, int dk) -> double { return lds(c, yb + dj * PT_W + di, dk); };
+      auto ldx = [&](int c, int di, int dj, int dk) -> double { return lds(c, xb + dj * PT_W + di, dk); };
+      double hm, ht;
+      if (ty == 0) {
+        plus_side<0, 1>(g, ldy, hm, ht);
+        PY[0 * PT_X + tx] = hm;
+        PY[1 * PT_X + tx] = ht;
+        if (tx < PT_Y) {
+          plus_side<0, 0>(g, ldx, hm, ht);
+          PX[tx * 6 + 0] = hm;
+          PX[tx * 6 + 1] = ht;
+        }
+      } else if (ty == 1) {
+        plus_side<1, 1>(g, ldy, hm, ht);
+        PY[2 * PT_X + tx] = hm;
+        PY[3 * PT_X + tx] = ht;
+        if (tx < PT_Y) {
+          plus_side<1, 0>(g, ldx, hm, ht);
+          PX[tx * 6 + 2] = hm;
+          PX[tx * 6 + 3] = ht;
+        }
+      } else if (ty == 2) {
+        plus_side<2, 1>(g, ldy, hm, ht);
+        PY[4 * PT_X + tx] = hm;
+        PY[5 * PT_X + tx] = ht;
+        if (tx < PT_Y) {
+          plus_side<2, 0>(g, ldx, hm, ht);
+          PX[tx * 6 + 4] = hm;
+          PX[tx * 6 + 5] = ht;
+        }
       }
-      const int o = own + dj * PT_W + di;
-      if (dk == 0) return c == 0 ? U0[o] : (c == 1 ? V0[o] : (c == 2 ? W0[o] : Psh[o]));
-      if (dk == 1) return W1[o];      // w one plane up
-      return c == 0 ? U1[o] : V1[o];  // u, v one plane down
-    };
-    const size_t id = kpl(k) + own_off;
-    us0[id] = predict_face_gen<0>(g, ld, dt);
-    us1[id] = predict_face_gen<1>(g, ld, dt);
-    us2[id] = predict_face_gen<2>(g, ld, dt);
+    }
+    // pressure one cell down in x and y, read before the next step's stores
+    pm0 = Psh[own - 1];
+    pm1 = Psh[own - PT_W];
   }
 }
 
