@@ -91,6 +91,7 @@ struct smac_ctx {
   double* fstage = nullptr;  // 3*1024 lane sums of the two-stage reductions
   std::vector<Level> lev;
   double* cinv = nullptr;
+  double* cinv_ws = nullptr;  // the factor of the coarsest operator (n x n)
   int cinv_n = 0;
   int mg_L = 0;
   int mg_prec = -1;
@@ -491,6 +492,8 @@ static void mg_free(smac_ctx* c) {
   c->mg_prec = -1;
   dfree(c->cinv);
   c->cinv = nullptr;
+  dfree(c->cinv_ws);
+  c->cinv_ws = nullptr;
   c->cinv_n = 0;
   c->mg_L = 0;
 }
@@ -899,93 +902,6 @@ static int mg_check_levels(const smac_ctx* c, int L) {
   return 1;
 }
 
-// level operator of a uniform-coefficient level (same rule as coef_uniform)
-// Level coefficients on the host (uniform, or from host copies hb[3] of the
-// level's face coefficients): the rule of level_coef.
-static void host_coef(const Lv& L, const std::vector<double>* hb, int i, int j, int k, double a6[6],
-                      double* diag) {
-  int q[3] = {i, j, k};
-  for (int d = 0; d < 3; d++) {
-    bool wp = !L.per[d] && (q[d] + 1 == 1 || q[d] + 1 == L.n[d] + 1);
-    bool wm = !L.per[d] && (q[d] == 1 || q[d] == L.n[d] + 1);
-    double bp, bm;
-    if (hb) {
-      int qp[3] = {i, j, k};
-      qp[d] = L.per[d] ? wrapi(q[d] + 1, L.n[d]) : q[d] + 1;
-      bp = hb[d][((size_t)qp[2] * L.py + qp[1]) * L.px + qp[0]];
-      bm = hb[d][((size_t)k * L.py + j) * L.px + i];
-    } else {
-      bp = wp ? 0.0 : L.beta0;
-      bm = wm ? 0.0 : L.beta0;
-    }
-    a6[2 * d] = bp * L.inv_d2[d];
-    a6[2 * d + 1] = bm * L.inv_d2[d];
-  }
-  *diag = a6[0] + a6[1] + a6[2] + a6[3] + a6[4] + a6[5];
-  for (int d = 0; d < 3; d++) {
-    bool wp = !L.per[d] && (q[d] + 1 == 1 || q[d] + 1 == L.n[d] + 1);
-    bool wm = !L.per[d] && (q[d] == 1 || q[d] == L.n[d] + 1);
-    if (wp) a6[2 * d] = 0.;
-    if (wm) a6[2 * d + 1] = 0.;
-  }
-}
-
-// Dense (pseudo-)inverse of the coarsest operator: Cholesky of A + c*J (c =
-// A00/n when A is singular), column-by-column inverse, minus J/(c n^2).
-static void dense_pinv(const Lv& L, const std::vector<double>* hb, double* out, int singular) {
-  int n = L.n[0] * L.n[1] * L.n[2];
-  std::vector<double> A((size_t)n * n, 0.0);
-  for (int k = 1; k <= L.n[2]; k++)
-    for (int j = 1; j <= L.n[1]; j++)
-      for (int i = 1; i <= L.n[0]; i++) {
-        int row = ((k - 1) * L.n[1] + (j - 1)) * L.n[0] + (i - 1);
-        double a6[6], diag;
-        host_coef(L, hb, i, j, k, a6, &diag);
-        A[(size_t)row * n + row] += diag;
-        int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
-        for (int m = 0; m < 6; m++) {
-          if (a6[m] == 0.) continue;
-          int q[3] = {nb[m][0], nb[m][1], nb[m][2]};
-          for (int d = 0; d < 3; d++)
-            if (L.per[d]) q[d] = wrapi(q[d], L.n[d]);
-          int col = ((q[2] - 1) * L.n[1] + (q[1] - 1)) * L.n[0] + (q[0] - 1);
-          A[(size_t)row * n + col] -= a6[m];
-        }
-      }
-  double cshift = singular ? A[0] / (double)n : 0.0;
-  if (singular)
-    for (size_t t = 0; t < (size_t)n * n; t++) A[t] += cshift;
-  for (int j = 0; j < n; j++) {
-    double s = A[(size_t)j * n + j];
-    for (int k = 0; k < j; k++) s -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-    double djj = sqrt(s);
-    A[(size_t)j * n + j] = djj;
-    for (int i = j + 1; i < n; i++) {
-      double t = A[(size_t)i * n + j];
-      for (int k = 0; k < j; k++) t -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-      A[(size_t)i * n + j] = t / djj;
-    }
-  }
-  std::vector<double> y((size_t)n);
-  for (int col = 0; col < n; col++) {
-    for (int i = 0; i < n; i++) {
-      double t = (i == col) ? 1.0 : 0.0;
-      for (int k = 0; k < i; k++) t -= A[(size_t)i * n + k] * y[k];
-      y[i] = t / A[(size_t)i * n + i];
-    }
-    for (int i = n - 1; i >= 0; i--) {
-      double t = y[i];
-      for (int k = i + 1; k < n; k++) t -= A[(size_t)k * n + i] * y[k];
-      y[i] = t / A[(size_t)i * n + i];
-    }
-    for (int i = 0; i < n; i++) out[(size_t)i * n + col] = y[i];
-  }
-  if (singular) {
-    double corr = 1.0 / (cshift * (double)n * (double)n);
-    for (size_t t = 0; t < (size_t)n * n; t++) out[t] -= corr;
-  }
-}
-
 static size_t prec_bytes(int prec) { return prec == SMAC_PREC_FP64 ? 8 : (prec == SMAC_PREC_MIXED_FP32 ? 4 : 2); }
 static size_t prec_cbytes(int prec) { return prec == SMAC_PREC_FP64 ? 8 : 4; }
 
@@ -1005,29 +921,21 @@ static int mg_coeffs(smac_ctx* c) {
       }
   HIPOK(hipGetLastError());
   const Lv& C = c->lev[L - 1].lv;
-  std::vector<double> hb[3];
-  if (var) {
-    const size_t t = (size_t)C.px * C.py * (size_t)(C.n[2] + 3);
-    for (int d = 0; d < 3; d++) {
-      hb[d].resize(t);
-      HIPOK(hipMemcpyAsync(hb[d].data(), C.b[d], t * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    }
-    HIPOK(hipStreamSynchronize(c->st));
-  }
-  int n = C.n[0] * C.n[1] * C.n[2];
-  std::vector<double> h((size_t)n * n);
-  // an outlet makes the operator regular: a plain inverse, else the
-  // pseudo-inverse of the singular pure-Neumann operator
-  dense_pinv(C, var ? hb : nullptr, h.data(), c->has_outlet ? 0 : 1);
+  const int n = C.n[0] * C.n[1] * C.n[2];
   if (!c->cinv) {
     int rc = dalloc(&c->cinv, (size_t)n * n);
     if (rc) return rc;
   }
-  // row-major: the wave of row r reads the row's entries at consecutive addresses
-  // (on the context's stream, waited for: h is freed on return and the coarse
-  // solve kernels run on that stream)
-  HIPOK(hipMemcpyAsync(c->cinv, h.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice, c->st));
-  HIPOK(hipStreamSynchronize(c->st));
+  if (!c->cinv_ws) {
+    int rc = dalloc(&c->cinv_ws, (size_t)n * n);
+    if (rc) return rc;
+  }
+  // an outlet makes the operator regular: a plain inverse, else the
+  // pseudo-inverse of the singular pure-Neumann operator; on the device, in
+  // stream order before the coarse solves that read it (row-major: the wave of
+  // row r reads the row's entries at consecutive addresses)
+  launch_dense_pinv(C, c->has_outlet ? 0 : 1, c->cinv_ws, c->cinv, c->st);
+  HIPOK(hipGetLastError());
   c->cinv_n = n;
   c->mg_ver = c->props_ver;
   return SMAC_OK;

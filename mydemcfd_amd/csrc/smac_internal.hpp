@@ -270,9 +270,11 @@ template <typename T>
 void launch_mg_coarse(const Lv& C, const double* ainv, const T* f, T* e, hipStream_t st);
 template <typename T>
 void launch_fill(const Lv& L, T* x, hipStream_t st);
-// coarse face coefficients of axis d on level C from level F: mean of the four
-// fine faces covering each coarse face, ((b00 + b10) + (b01 + b11)) / 4
+// coarse face coefficients of axis d on level C from level F (k_mg_coarsen_beta)
 void launch_mg_coarsen_beta(const Lv& F, const Lv& C, int d, double* cb, hipStream_t st);
+// dense (pseudo-)inverse of the coarsest level's operator into out (n x n,
+// n = cells of C; work: n x n scratch), on the device
+void launch_dense_pinv(const Lv& C, int singular, double* work, double* out, hipStream_t st);
 
 // the reference's multigrid (smac_refgmg.hip); fb == nullptr: coefficients
 // of level 0 from the grid

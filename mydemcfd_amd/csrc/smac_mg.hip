@@ -1190,6 +1190,219 @@ void launch_mg_coarsen_beta(const Lv& F, const Lv& C, int d, double* cb, hipStre
   hipLaunchKernelGGL(k_mg_coarsen_beta, g, dim3(64, 4), 0, st, F, C, d, cb);
 }
 
+// Dense (pseudo-)inverse of the coarsest operator in one workgroup: the
+// operator from the level's coefficients (the uniform rule, or the face
+// coefficient arrays), shifted by c J (c = A00 / n) when singular, its
+// Cholesky factor column by column, the inverse column by column by forward
+// and back substitution, minus J / (c n^2). Every sum runs in the order of
+// the sequential algorithm (dot products in increasing k), with one thread per
+// row (factor) or per column (substitution), so the result is that of the
+// plain host loops bit for bit. A and out are n x n, row-major.
+__device__ void coarsest_coef(const Lv& L, int i, int j, int k, double a6[6], double* diag) {
+  const int q[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const bool wp = !L.per[d] && (q[d] + 1 == 1 || q[d] + 1 == L.n[d] + 1);
+    const bool wm = !L.per[d] && (q[d] == 1 || q[d] == L.n[d] + 1);
+    double bp, bm;
+    if (L.b[0]) {
+      int qp[3] = {i, j, k};
+      qp[d] = L.per[d] ? wrapi(q[d] + 1, L.n[d]) : q[d] + 1;
+      bp = L.b[d][((size_t)qp[2] * L.py + qp[1]) * L.px + qp[0]];
+      bm = L.b[d][((size_t)k * L.py + j) * L.px + i];
+    } else {
+      bp = wp ? 0.0 : L.beta0;
+      bm = wm ? 0.0 : L.beta0;
+    }
+    a6[2 * d] = bp * L.inv_d2[d];
+    a6[2 * d + 1] = bm * L.inv_d2[d];
+  }
+  *diag = a6[0] + a6[1] + a6[2] + a6[3] + a6[4] + a6[5];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const bool wp = !L.per[d] && (q[d] + 1 == 1 || q[d] + 1 == L.n[d] + 1);
+    const bool wm = !L.per[d] && (q[d] == 1 || q[d] == L.n[d] + 1);
+    if (wp) a6[2 * d] = 0.;
+    if (wm) a6[2 * d + 1] = 0.;
+  }
+}
+
+constexpr int PINV_LN = 128, PINV_LB = 64;
+__device__ __forceinline__ int tri(int i, int k) { return i * (i + 1) / 2 + k; }
+// acc - a(k0) b(k0) - a(k0+1) b(k0+1) - ... - a(k1-1) b(k1-1), subtracted in
+// that order; the operands are read 16 at a time ahead of the subtractions
+template <class FA, class FB>
+__device__ __forceinline__ double sub_dot(double acc, int k0, int k1, const FA& a, const FB& b) {
+  int k = k0;
+  for (; k + 16 <= k1; k += 16) {
+    double x[16], y[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      x[u] = a(k + u);
+      y[u] = b(k + u);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc -= x[u] * y[u];
+  }
+  for (; k < k1; k++) acc -= a(k) * b(k);
+  return acc;
+}
+constexpr int PINV_T = 1024;
+__global__ __launch_bounds__(PINV_T) void k_dense_pinv(Lv L, int singular, double* __restrict__ A,
+                                                    double* __restrict__ out) {
+  const int n = L.n[0] * L.n[1] * L.n[2];
+  const int t = (int)threadIdx.x;
+  const size_t nn = (size_t)n * n;
+  __shared__ double djj_s, cshift_s;
+  for (size_t e = t; e < nn; e += PINV_T) A[e] = 0.0;
+  __syncthreads();
+  for (int row = t; row < n; row += PINV_T) {
+    const int i = row % L.n[0] + 1, j = (row / L.n[0]) % L.n[1] + 1, k = row / (L.n[0] * L.n[1]) + 1;
+    double a6[6], diag;
+    coarsest_coef(L, i, j, k, a6, &diag);
+    A[(size_t)row * n + row] += diag;
+    const int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+    for (int m = 0; m < 6; m++) {
+      if (a6[m] == 0.) continue;
+      int q[3] = {nb[m][0], nb[m][1], nb[m][2]};
+      for (int d = 0; d < 3; d++)
+        if (L.per[d]) q[d] = wrapi(q[d], L.n[d]);
+      const int col = ((q[2] - 1) * L.n[1] + (q[1] - 1)) * L.n[0] + (q[0] - 1);
+      A[(size_t)row * n + col] -= a6[m];
+    }
+  }
+  __syncthreads();
+  if (t == 0) cshift_s = singular ? A[0] / (double)n : 0.0;
+  __syncthreads();
+  const double cshift = cshift_s;
+  if (singular) {
+    for (size_t e = t; e < nn; e += PINV_T) A[e] += cshift;
+    __syncthreads();
+  }
+  for (int jc = 0; jc < n; jc++) {
+    if (t == 0) {
+      const double* aj = A + (size_t)jc * n;
+      auto lj = [&](int k) { return aj[k]; };
+      const double s = sub_dot(aj[jc], 0, jc, lj, lj);
+      const double d = sqrt(s);
+      A[(size_t)jc * n + jc] = d;
+      djj_s = d;
+    }
+    __syncthreads();
+    const double djj = djj_s;
+    const double* aj = A + (size_t)jc * n;
+    for (int i = jc + 1 + t; i < n; i += PINV_T) {
+      double* ai = A + (size_t)i * n;
+      const double tt = sub_dot(ai[jc], 0, jc, [&](int k) { return ai[k]; }, [&](int k) { return aj[k]; });
+      ai[jc] = tt / djj;
+    }
+    __syncthreads();
+  }
+  for (int col = t; col < n; col += PINV_T) {
+    // y = the column of the inverse, kept in out's column col
+    auto yk = [&](int k) { return out[(size_t)k * n + col]; };
+    for (int i = 0; i < n; i++) {
+      const double* ai = A + (size_t)i * n;
+      const double tt = sub_dot((i == col) ? 1.0 : 0.0, 0, i, [&](int k) { return ai[k]; }, yk);
+      out[(size_t)i * n + col] = tt / ai[i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      const double tt = sub_dot(out[(size_t)i * n + col], i + 1, n, [&](int k) { return A[(size_t)k * n + i]; }, yk);
+      out[(size_t)i * n + col] = tt / A[(size_t)i * n + i];
+    }
+  }
+  if (singular) {
+    __syncthreads();
+    const double corr = 1.0 / (cshift * (double)n * (double)n);
+    for (size_t e = t; e < nn; e += PINV_T) out[e] -= corr;
+  }
+}
+// The same for n <= 128 (a coarsest level of up to 128 cells, rebuilt with
+// every change of the coefficients) with the operations on LDS: the packed
+// lower triangle (all the factorization reads; the assembled operator is
+// symmetric entry for entry) and the solution columns 64 at a time (k-major,
+// so the column threads read consecutive words). Same arithmetic, same order.
+__global__ __launch_bounds__(PINV_T) void k_dense_pinv_lds(Lv L, int singular, double* __restrict__ out) {
+  __shared__ double Ls[PINV_LN * (PINV_LN + 1) / 2];
+  __shared__ double Ys[PINV_LN * PINV_LB];
+  __shared__ double djj_s, cshift_s;
+  const int n = L.n[0] * L.n[1] * L.n[2];
+  const int t = (int)threadIdx.x;
+  const int ntri = n * (n + 1) / 2;
+  for (int e = t; e < ntri; e += PINV_T) Ls[e] = 0.0;
+  __syncthreads();
+  for (int row = t; row < n; row += PINV_T) {
+    const int i = row % L.n[0] + 1, j = (row / L.n[0]) % L.n[1] + 1, k = row / (L.n[0] * L.n[1]) + 1;
+    double a6[6], diag;
+    coarsest_coef(L, i, j, k, a6, &diag);
+    Ls[tri(row, row)] += diag;
+    const int nb[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+    for (int m = 0; m < 6; m++) {
+      if (a6[m] == 0.) continue;
+      int q[3] = {nb[m][0], nb[m][1], nb[m][2]};
+      for (int d = 0; d < 3; d++)
+        if (L.per[d]) q[d] = wrapi(q[d], L.n[d]);
+      const int col = ((q[2] - 1) * L.n[1] + (q[1] - 1)) * L.n[0] + (q[0] - 1);
+      if (col <= row) Ls[tri(row, col)] -= a6[m];
+    }
+  }
+  __syncthreads();
+  if (t == 0) cshift_s = singular ? Ls[0] / (double)n : 0.0;
+  __syncthreads();
+  const double cshift = cshift_s;
+  if (singular) {
+    for (int e = t; e < ntri; e += PINV_T) Ls[e] += cshift;
+    __syncthreads();
+  }
+  for (int jc = 0; jc < n; jc++) {
+    if (t == 0) {
+      const int rj = tri(jc, 0);
+      auto lj = [&](int k) { return Ls[rj + k]; };
+      const double s = sub_dot(Ls[tri(jc, jc)], 0, jc, lj, lj);
+      const double d = sqrt(s);
+      Ls[tri(jc, jc)] = d;
+      djj_s = d;
+    }
+    __syncthreads();
+    const double djj = djj_s;
+    const int rj = tri(jc, 0);
+    for (int i = jc + 1 + t; i < n; i += PINV_T) {
+      const int ri = tri(i, 0);
+      const double tt = sub_dot(Ls[ri + jc], 0, jc, [&](int k) { return Ls[ri + k]; }, [&](int k) { return Ls[rj + k]; });
+      Ls[ri + jc] = tt / djj;
+    }
+    __syncthreads();
+  }
+  for (int c0 = 0; c0 < n; c0 += PINV_LB) {
+    const int cl = t, col = c0 + t;
+    if (t < PINV_LB && col < n) {
+      auto yk = [&](int k) { return Ys[k * PINV_LB + cl]; };
+      for (int i = 0; i < n; i++) {
+        const int ri = tri(i, 0);
+        const double tt = sub_dot((i == col) ? 1.0 : 0.0, 0, i, [&](int k) { return Ls[ri + k]; }, yk);
+        Ys[i * PINV_LB + cl] = tt / Ls[ri + i];
+      }
+      for (int i = n - 1; i >= 0; i--) {
+        const double tt = sub_dot(Ys[i * PINV_LB + cl], i + 1, n, [&](int k) { return Ls[tri(k, i)]; }, yk);
+        Ys[i * PINV_LB + cl] = tt / Ls[tri(i, i)];
+      }
+      for (int i = 0; i < n; i++) out[(size_t)i * n + col] = Ys[i * PINV_LB + cl];
+    }
+  }
+  if (singular) {
+    __syncthreads();
+    const double corr = 1.0 / (cshift * (double)n * (double)n);
+    for (size_t e = t; e < (size_t)n * n; e += PINV_T) out[e] -= corr;
+  }
+}
+void launch_dense_pinv(const Lv& C, int singular, double* work, double* out, hipStream_t st) {
+  const int n = C.n[0] * C.n[1] * C.n[2];
+  if (n <= PINV_LN)
+    hipLaunchKernelGGL(k_dense_pinv_lds, dim3(1), dim3(PINV_T), 0, st, C, singular, out);
+  else
+    hipLaunchKernelGGL(k_dense_pinv, dim3(1), dim3(PINV_T), 0, st, C, singular, work, out);
+}
+
 template <typename T>
 void launch_fill(const Lv& L, T* x, hipStream_t st) {
   hipLaunchKernelGGL(k_fill<T>, lrows_grid(L), ROWB, 0, st, L, x);
