@@ -93,6 +93,12 @@ struct smac_ctx {
   double* cinv = nullptr;
   double* cinv_ws = nullptr;  // the factor of the coarsest operator (n x n)
   int cinv_n = 0;
+  // the coarsest inverse is rebuilt on a second stream (default priority),
+  // next to the predictor; the solve waits for ev_side_out before its first
+  // V-cycle while pinv_pending
+  hipStream_t st_side = nullptr;
+  hipEvent_t ev_side_in = nullptr, ev_side_out = nullptr;
+  bool pinv_pending = false;
   int mg_L = 0;
   int mg_prec = -1;
   std::vector<RgLevel> rg;
@@ -450,6 +456,9 @@ extern "C" int smac_create(const smac_grid_desc* gd, const smac_fluid_desc* fl, 
   if (hipGetDevice(&c->dev) != hipSuccess) rc = SMAC_ERR_HIP;
   if (rc == SMAC_OK) rc = scratch_resolve(c->dev);
   if (rc == SMAC_OK && hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) rc = SMAC_ERR_HIP;
+  if (rc == SMAC_OK && hipStreamCreateWithFlags(&c->st_side, hipStreamNonBlocking) != hipSuccess) rc = SMAC_ERR_HIP;
+  if (rc == SMAC_OK && hipEventCreateWithFlags(&c->ev_side_in, hipEventDisableTiming) != hipSuccess) rc = SMAC_ERR_HIP;
+  if (rc == SMAC_OK && hipEventCreateWithFlags(&c->ev_side_out, hipEventDisableTiming) != hipSuccess) rc = SMAC_ERR_HIP;
   double** arrs[] = {&c->u[0], &c->u[1], &c->u[2], &c->us[0], &c->us[1], &c->us[2], &c->p, &c->dp, &c->rhs,
                      &c->r, &c->z, &c->dv, &c->Ad, &c->res0};
   for (auto a : arrs)
@@ -490,6 +499,9 @@ static void mg_free(smac_ctx* c) {
   }
   c->lev.clear();
   c->mg_prec = -1;
+  // a rebuild of the coarsest inverse may still run on the side stream
+  if (c->st_side) (void)hipStreamSynchronize(c->st_side);
+  c->pinv_pending = false;
   dfree(c->cinv);
   c->cinv = nullptr;
   dfree(c->cinv_ws);
@@ -534,6 +546,9 @@ extern "C" void smac_destroy(smac_ctx* c) {
   if (c->Sh) (void)hipHostFree(c->Sh);
   if (c->ev_ok)
     for (int e = 0; e < 8; e++) (void)hipEventDestroy(c->ev[e]);
+  if (c->ev_side_in) (void)hipEventDestroy(c->ev_side_in);
+  if (c->ev_side_out) (void)hipEventDestroy(c->ev_side_out);
+  if (c->st_side) (void)hipStreamDestroy(c->st_side);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -931,11 +946,17 @@ static int mg_coeffs(smac_ctx* c) {
     if (rc) return rc;
   }
   // an outlet makes the operator regular: a plain inverse, else the
-  // pseudo-inverse of the singular pure-Neumann operator; on the device, in
-  // stream order before the coarse solves that read it (row-major: the wave of
-  // row r reads the row's entries at consecutive addresses)
-  launch_dense_pinv(C, c->has_outlet ? 0 : 1, c->cinv_ws, c->cinv, c->st);
+  // pseudo-inverse of the singular pure-Neumann operator; on the device
+  // (row-major: the wave of row r reads the row's entries at consecutive
+  // addresses), on the side stream after everything enqueued so far on the
+  // context stream (the coarse coefficients above, the last solve's reads of
+  // the old inverse); mg_wait_pinv orders the next solve after it
+  HIPOK(hipEventRecord(c->ev_side_in, c->st));
+  HIPOK(hipStreamWaitEvent(c->st_side, c->ev_side_in, 0));
+  launch_dense_pinv(C, c->has_outlet ? 0 : 1, c->cinv_ws, c->cinv, c->st_side);
   HIPOK(hipGetLastError());
+  HIPOK(hipEventRecord(c->ev_side_out, c->st_side));
+  c->pinv_pending = true;
   c->cinv_n = n;
   c->mg_ver = c->props_ver;
   return SMAC_OK;
@@ -1545,6 +1566,10 @@ static int pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, 
     if (!mg_check_levels(c, L)) return SMAC_ERR_INVALID_GRID;
     int rc = mg_setup(c, L, o.precision);
     if (rc) return rc;
+    if (c->pinv_pending) {  // the coarsest inverse of the current coefficients
+      HIPOK(hipStreamWaitEvent(c->st, c->ev_side_out, 0));
+      c->pinv_pending = false;
+    }
   } else if (o.precond == SMAC_PRECOND_REF_GMG) {
     L = rg_setup(c, o.mg_levels > 0 ? o.mg_levels : 4);
     if (L < 0) return L;
@@ -1580,6 +1605,16 @@ extern "C" int smac_step(smac_ctx* c, double dt, const smac_solver_opts* o, smac
   if (!c->props_ready) rc = smac_update_properties(c, 1);
   else if (var_props(c) && (c->fl.rho0 != c->fl.rho1 || c->fl.mu0 != c->fl.mu1)) rc = smac_update_properties(c, 0);
   if (rc) return rc;
+  // the multigrid coefficients of the new properties before the predictor,
+  // so that the coarsest inverse builds on the side stream meanwhile
+  if (o && o->precond == SMAC_PRECOND_MG && o->precision >= SMAC_PREC_FP64 && o->precision <= SMAC_PREC_MIXED_BF16 &&
+      pois_var(c) && c->n[0] <= 4000) {
+    const int L = o->mg_levels > 0 ? o->mg_levels : mg_auto_levels(c);
+    if (mg_check_levels(c, L)) {
+      rc = mg_setup(c, L, o->precision);
+      if (rc) return rc;
+    }
+  }
   rc = smac_predictor(c, dt);
   if (rc) return rc;
   double t_pred = prof_end(c, 0);
