@@ -39,7 +39,11 @@ __device__ __forceinline__ CoefC<C> level_coef(const Lv& L, int i, int j, int k)
       int qp[3] = {i, j, k};
       qp[d] = L.per[d] ? wrapi(q[d] + 1, L.n[d]) : q[d] + 1;
       const size_t op = lrow(L, qp[1], qp[2]) + qp[0], om = lrow(L, j, k) + i;
-      if (sizeof(C) == 4 && L.bs[d]) {  // the mixed V-cycles: FP32 copies
+      // the mixed V-cycles (compute type float) read the FP32 copies, which
+      // the multigrid set-up allocates on every level and axis of a
+      // variable-coefficient hierarchy in those forms; chosen at compile time,
+      // so the six loads of a cell have no branch between them
+      if constexpr (sizeof(C) == 4) {
         c.a[2 * d] = (double)L.bs[d][op] * L.inv_d2[d];
         c.a[2 * d + 1] = (double)L.bs[d][om] * L.inv_d2[d];
       } else {
@@ -271,6 +275,19 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
     return;
   }
   C ov[NM][CPL], yp[NM][CPL], ym[NM][CPL], zp[NM][CPL], zm[NM][CPL];
+  // variable coefficients of the mixed forms: read with the values, ahead of
+  // the barrier (FP64 keeps them after it: 14 more registers per cell)
+  constexpr bool HC = VB && sizeof(C) == 4;
+  CoefC<C> cv[HC ? NM : 1][HC ? CPL : 1];
+  if constexpr (HC) {
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+      for (int t = 0; t < CPL; t++) {
+        const int h = 1 + CPL * lane + 64 * CPL * m + t;
+        cv[m][t] = coef(par ? 2 * h - 1 : 2 * h);
+      }
+  }
   if constexpr (LDSY) {
     const int w = (int)(threadIdx.x >> 6);
     C hv[NM][CPL];
@@ -341,7 +358,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
         xm = t > 0 ? ov[m][t > 0 ? t - 1 : 0] : nb;
         if (!perx && i == 1) xm = (C)0;
       }
-      CoefC<C> c = coef(i);
+      const CoefC<C> c = HC ? cv[HC ? m : 0][HC ? t : 0] : coef(i);
       C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] + c.a[5] * zm[m][t];
       ev[t] = (fv[m][t] + S) / c.diag;
     }
@@ -416,6 +433,15 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, con
   Vec<T, CPL>::ld(e + zpr + off + p, zp);
   Vec<T, CPL>::ld(e + zmr + off + p, zm);
   if (lo || hi) Vec<T, CPL>::ld(e + (lo ? ymr : ypr) + off + p, hv);
+  // variable coefficients: read with the values, ahead of the barrier
+  CoefC<C> cv[VB ? CPL : 1];
+  if constexpr (VB) {
+#pragma unroll
+    for (int t = 0; t < CPL; t++) {
+      const int h = p + t;
+      cv[t] = coef(par ? 2 * h - 1 : 2 * h);
+    }
+  }
 #pragma unroll
   for (int t = 0; t < CPL; t++) {
     const int o = CPL * l + t;
@@ -451,7 +477,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, con
       xm = t > 0 ? ov[t > 0 ? t - 1 : 0] : nb;
       if (!perx && i == 1) xm = (C)0;
     }
-    CoefC<C> c = coef(i);
+    const CoefC<C> c = VB ? cv[VB ? t : 0] : coef(i);
     C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[t] + c.a[3] * ym[t] + c.a[4] * zp[t] + c.a[5] * zm[t];
     ev[t] = sm_div(fv[t] + S, c.diag);
   }
