@@ -594,6 +594,19 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
         Vec<T, CPL>::ld(e + zmr + off + hb, zm[m]);
         if (w == 0 || w == 3) Vec<T, CPL>::ld(e + hr + off + hb, hv[m]);
       }
+      // variable coefficients of the mixed forms: read with the values, ahead
+      // of the barrier
+      constexpr bool HC = VB && sizeof(C) == 4;
+      CoefC<C> cv[HC ? NM : 1][HC ? CPL : 1];
+      if constexpr (HC) {
+#pragma unroll
+        for (int m = 0; m < NM; m++)
+#pragma unroll
+          for (int t = 0; t < CPL; t++) {
+            const int h = 1 + CPL * lane + 64 * CPL * m + t;
+            cv[m][t] = level_coef<C, VB>(L, par ? 2 * h - 1 : 2 * h, j, k);
+          }
+      }
 #pragma unroll
       for (int m = 0; m < NM; m++)
 #pragma unroll
@@ -641,8 +654,8 @@ __global__ __launch_bounds__(256) void k_mg_resid_rx(Lv F, Lv Cl, const T* __res
             xm = t > 0 ? ov[m][t > 0 ? t - 1 : 0] : nb;
             if (!perx && i == 1) xm = (C)0;
           }
-          const CoefC<C> c =
-              (VB || (!perx && (i == 1 || i == nx))) ? level_coef<C, VB>(L, i, j, k) : crow;
+          const CoefC<C> c = HC ? cv[HC ? m : 0][HC ? t : 0]
+                                : ((VB || (!perx && (i == 1 || i == nx))) ? level_coef<C, VB>(L, i, j, k) : crow);
           C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] +
                 c.a[5] * zm[m][t];
           buf[i] = ef[m][t] - (c.diag * ec[m][t] - S);
