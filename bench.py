@@ -36,6 +36,9 @@ width of the V-cycle):
          mg_rbgs_l0 1.5*W_s (a half sweep: other colour, own rhs, own
          colour written), mg_rbgs_l0_zs 1.0*W_s (the first red half sweep of
          a V-cycle starts from zero: rhs read, red cells written),
+         mg_rbgs_dot 1.5*W_s + 8 (+8 flexible) (the last red half sweep of a
+         V-cycle with dot2's products: r and Ad read, z not read again; the
+         FP32 form),
          mg_restrict_l0 2.125*W_s
          (residual + restriction), mg_prolong_l0 1.125*W_s (red cells),
          mg_fill_l0 W_s, predict 56, rhs 32, correct 80 (u* and dp read, u
@@ -69,6 +72,7 @@ KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (
     "dp_flush": lambda W, flex: 24.0,
     "mg_rbgs_l0": lambda W, flex: 1.5 * W,
     "mg_rbgs_l0_zs": lambda W, flex: 1.0 * W,
+    "mg_rbgs_dot": lambda W, flex: 1.5 * W + 8.0 + (8.0 if flex else 0.0),
     "mg_restrict_l0": lambda W, flex: 2.125 * W,
     "mg_prolong_l0": lambda W, flex: 1.125 * W,
     "mg_fill_l0": lambda W, flex: 1.0 * W,

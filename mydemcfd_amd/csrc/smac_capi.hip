@@ -13,6 +13,7 @@
 
 #include <map>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <mutex>
 #include <unordered_map>
@@ -78,6 +79,13 @@ struct smac_ctx {
   // dp is zero (as the correction leaves it, and at creation): the next solve
   // need not clear it
   bool dp_zero = true;
+  // the PCG's dot2 of the next V-cycle's z rides on its last fine half sweep
+  // (FP32 form): the operands while dot_req, dot_done once it ran
+  bool dot_req = false, dot_done = false;
+  const double* dot_r = nullptr;
+  const double* dot_Ad = nullptr;
+  int dot_flex = 0;
+  double *dot_p0 = nullptr, *dot_p1 = nullptr;
   // the residual update has done level 0's zero-start half sweep: the next
   // V-cycle starts with the black half sweep
   bool zs_done = false;
@@ -1117,13 +1125,21 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   KT(c, fine ? "mg_prolong_l0" : "mg_coarse_levels",
      launch_mg_prolong<T>(C.lv, V.lv, (const T*)C.e, e, nu2 > 0 ? 0 : -1, c->st));
   if (fine) {
+    bool dot = false;
+    if constexpr (std::is_same<T, float>::value) dot = c->dot_req && nu2 > 0 && mg_rbgs_dot_applies(V.lv, e, f);
     if (nu2 > 0)
-      KT_N(c, "mg_rbgs_l0", 2 * nu2, {
+      KT_N(c, "mg_rbgs_l0", dot ? 2 * nu2 - 1 : 2 * nu2, {
         for (int s = 0; s < nu2; s++) {
           launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st);
-          launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st);
+          if (!dot || s + 1 < nu2) launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st);
         }
       });
+    if constexpr (std::is_same<T, float>::value)
+      if (dot) {
+        KT(c, "mg_rbgs_dot",
+           launch_mg_rbgs_dot(V.lv, e, f, c->dot_r, c->dot_Ad, c->dot_flex, c->dot_p0, c->dot_p1, c->st));
+        c->dot_done = true;
+      }
   } else {
     for (int s = 0; s < nu2; s++) {
       KT(c, "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
@@ -1435,8 +1451,16 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
       launch_finalize(fa, c->S, s);
       KT(c, "add_scalar_norms", launch_add_scalar_norms(g, c->r, -1. * size_inv, c->S, S_SUMR, P[0], P[1], s));
     }
+    c->dot_req = !stat;
+    c->dot_done = false;
+    c->dot_r = c->r;
+    c->dot_Ad = c->Ad;
+    c->dot_flex = o.flexible;
+    c->dot_p0 = P[2];
+    c->dot_p1 = P[3];
     precond();
-    if (!stat) KT(c, "dot2", launch_dot2<T>(g, c->r, z, c->Ad, o.flexible, P[2], P[3], s));
+    c->dot_req = false;
+    if (!stat && !c->dot_done) KT(c, "dot2", launch_dot2<T>(g, c->r, z, c->Ad, o.flexible, P[2], P[3], s));
     bool check = (iter % o.check_every) == 0;
     {
       if (!stat) {

@@ -366,6 +366,131 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
   }
 }
 
+// The last fine-level half sweep of a V-cycle (colour 0) with the PCG's two
+// dot products of the new z in the same pass: k_mg_rbgs_vec<float, VPL,
+// false, 4, true> followed by k_dot2_ch's row partials. After this sweep the
+// row's z is final in both halves, the colour-0 half computed here and the
+// other half the values the sweep loaded as x neighbours; the row's z goes
+// to LDS, and each lane then reads r, Ad and z at the split positions
+// 1 + 2l + 128m (+1) of k_dot2_ch and forms r z and z Ad, summed in that
+// kernel's order (m, then the pair; wave tree over the lanes), so the
+// partials are bit-identical and z is not read again from memory.
+template <int VPL>
+__global__ __launch_bounds__(256) void k_mg_rbgs_dot(Lv L, float* __restrict__ e, const float* __restrict__ f,
+                                                     const double* __restrict__ r, const double* __restrict__ Ad,
+                                                     int flexible, double* __restrict__ part0,
+                                                     double* __restrict__ part1) {
+  using C = float;
+  constexpr int CPL = 4;
+  constexpr int NM = VPL / CPL;  // smoother chunks per lane
+  constexpr int NH = 64 * VPL;   // cells per half row
+  constexpr int NX = 2 * NH;
+  constexpr int DM = NX / 128;  // dot chunks (k_dot2_ch: 128 positions each)
+  __shared__ C ysm[6 * NH];
+  __shared__ float zs[4][NX];
+  LROW_SETUP
+  const int w = (int)(threadIdx.x >> 6);
+  const int nx = L.n[0], nE = nx >> 1;
+  const int par = (j + k) & 1;  // parity of the active i (colour 0)
+  const int off = par ? nE : 0;
+  const int oo = par ? 0 : nE;
+  const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
+  const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+  const bool perx = L.per[0] != 0;
+  const CoefC<C> crow = level_coef<C, false>(L, 2, j, k);
+  auto coef = [&](int i) -> CoefC<C> {
+    if (!perx && (i == 1 || i == nx)) return level_coef<C, false>(L, i, j, k);
+    return crow;
+  };
+  // the dot operands at the lane's dot positions, loaded first
+  double vr[DM][2], vA[DM][2];
+#pragma unroll
+  for (int m = 0; m < DM; m++) {
+    const size_t o = rb + 1 + 2 * lane + 128 * m;
+    Vec<double, 2>::ld(r + o, vr[m]);
+    if (flexible) Vec<double, 2>::ld(Ad + o, vA[m]);
+  }
+  C fv[NM][CPL], ov[NM][CPL], yp[NM][CPL], ym[NM][CPL], zp[NM][CPL], zm[NM][CPL], hv[NM][CPL];
+  const size_t hr = w == 0 ? ymr : ypr;  // wave 0: row j0-1, wave 3: row j0+4
+#pragma unroll
+  for (int m = 0; m < NM; m++) {
+    const int p = 1 + CPL * lane + 64 * CPL * m;
+    Vec<float, CPL>::ld(f + rb + off + p, fv[m]);
+    Vec<float, CPL>::ld(e + rb + oo + p, ov[m]);
+    Vec<float, CPL>::ld(e + zpr + off + p, zp[m]);
+    Vec<float, CPL>::ld(e + zmr + off + p, zm[m]);
+    if (w == 0 || w == 3) Vec<float, CPL>::ld(e + hr + off + p, hv[m]);
+  }
+#pragma unroll
+  for (int m = 0; m < NM; m++)
+#pragma unroll
+    for (int t = 0; t < CPL; t++) {
+      const int o = CPL * lane + 64 * CPL * m + t;
+      ysm[(w + 1) * NH + o] = ov[m][t];
+      if (w == 0) ysm[o] = hv[m][t];
+      if (w == 3) ysm[5 * NH + o] = hv[m][t];
+      zs[w][oo + o] = ov[m][t];  // the other half's z (split position oo + 1 + o, stored at oo + o)
+    }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NM; m++)
+#pragma unroll
+    for (int t = 0; t < CPL; t++) {
+      const int o = CPL * lane + 64 * CPL * m + t;
+      ym[m][t] = ysm[w * NH + o];
+      yp[m][t] = ysm[(w + 2) * NH + o];
+    }
+  const int up = (lane + 1) & 63, dn = (lane + 63) & 63;
+#pragma unroll
+  for (int m = 0; m < NM; m++) {
+    C nb;
+    if (!par) {
+      const C src = lane == 0 ? ov[(m + 1) % NM][0] : ov[m][0];
+      nb = __shfl(src, up);
+    } else {
+      const C src = lane == 63 ? ov[(m + NM - 1) % NM][CPL - 1] : ov[m][CPL - 1];
+      nb = __shfl(src, dn);
+    }
+    C ev[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; t++) {
+      const int h = 1 + CPL * lane + 64 * CPL * m + t;
+      const int i = par ? 2 * h - 1 : 2 * h;
+      C xp, xm;
+      if (!par) {
+        xm = ov[m][t];
+        xp = t + 1 < CPL ? ov[m][t + 1 < CPL ? t + 1 : 0] : nb;
+        if (!perx && i == nx) xp = (C)0;
+      } else {
+        xp = ov[m][t];
+        xm = t > 0 ? ov[m][t > 0 ? t - 1 : 0] : nb;
+        if (!perx && i == 1) xm = (C)0;
+      }
+      const CoefC<C> c = coef(i);
+      C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[m][t] + c.a[3] * ym[m][t] + c.a[4] * zp[m][t] + c.a[5] * zm[m][t];
+      ev[t] = (fv[m][t] + S) / c.diag;
+      zs[w][off + h - 1] = ev[t];
+    }
+    Vec<float, CPL>::st(e + rb + off + 1 + CPL * lane + 64 * CPL * m, ev);
+  }
+  __syncthreads();
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+  for (int m = 0; m < DM; m++)
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const double zv = (double)zs[w][2 * lane + 128 * m + t];
+      a0 += vr[m][t] * zv;
+      if (flexible) a1 += zv * vA[m][t];
+    }
+  a0 = wave_tree_sum(a0);
+  if (flexible) a1 = wave_tree_sum(a1);
+  if (lane == 0) {
+    part0[row] = a0;
+    if (flexible) part1[row] = a1;
+  }
+}
+
 // a / b in the smoother: IEEE division in FP64 (the oracle's arithmetic),
 // reciprocal times a in FP32 (within 1.5 ulp; the mixed forms store the
 // result in FP32 or BF16 and their V-cycle is a preconditioner only)
@@ -1100,6 +1225,21 @@ void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipS
   }
 #undef SMAC_RBGS_V
 #undef SMAC_RBGS
+}
+
+bool mg_rbgs_dot_applies(const Lv& L, const float* e, const float* f) {
+  const int half = L.n[0] - (L.n[0] >> 1);
+  const int vpl = row_vpl(half);
+  return !L.b[0] && (vpl == 4 || vpl == 8) && L.n[0] == 128 * vpl && L.n[1] % 4 == 0 && rows_aligned(L, e) &&
+         rows_aligned(L, f);
+}
+void launch_mg_rbgs_dot(const Lv& L, float* e, const float* f, const double* r, const double* Ad, int flexible,
+                        double* p0, double* p1, hipStream_t st) {
+  const int vpl = row_vpl(L.n[0] - (L.n[0] >> 1));
+  if (vpl == 4)
+    hipLaunchKernelGGL(k_mg_rbgs_dot<4>, lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
+  else
+    hipLaunchKernelGGL(k_mg_rbgs_dot<8>, lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
 }
 
 template <typename T>
