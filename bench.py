@@ -38,7 +38,7 @@ width of the V-cycle):
          a V-cycle starts from zero: rhs read, red cells written),
          mg_rbgs_dot 1.5*W_s + 8 (+8 flexible) (the last red half sweep of a
          V-cycle with dot2's products: r and Ad read, z not read again; the
-         FP32 form),
+         FP32 and FP64 forms),
          mg_restrict_l0 2.125*W_s
          (residual + restriction), mg_prolong_l0 1.125*W_s (red cells),
          mg_fill_l0 W_s, predict 56, rhs 32, correct 80 (u* and dp read, u

@@ -258,10 +258,12 @@ void launch_vof_substep(const Geo& g, const double h[3], const Vel& V, double** 
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st);
 // the last colour-0 half sweep of a V-cycle with the PCG's dot2 row partials
-// (FP32 storage, uniform coefficients, the vector form with LDS y rows)
-bool mg_rbgs_dot_applies(const Lv& L, const float* e, const float* f);
-void launch_mg_rbgs_dot(const Lv& L, float* e, const float* f, const double* r, const double* Ad, int flexible,
-                        double* p0, double* p1, hipStream_t st);
+// (FP32 or FP64 storage, uniform coefficients, the vector form with LDS y rows)
+template <typename T>
+bool mg_rbgs_dot_applies(const Lv& L, const T* e, const T* f);
+template <typename T>
+void launch_mg_rbgs_dot(const Lv& L, T* e, const T* f, const double* r, const double* Ad, int flexible, double* p0,
+                        double* p1, hipStream_t st);
 // dst = (float) src over n elements (the FP32 copies of the face coefficients)
 void launch_d2f(const double* src, float* dst, size_t n, hipStream_t st);
 // red != 0: the residual is 0 on black cells (the pre-smoothing ended with a

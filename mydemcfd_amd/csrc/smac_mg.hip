@@ -367,27 +367,28 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_vec(Lv L, T* __restrict__ e, co
 }
 
 // The last fine-level half sweep of a V-cycle (colour 0) with the PCG's two
-// dot products of the new z in the same pass: k_mg_rbgs_vec<float, VPL,
-// false, 4, true> followed by k_dot2_ch's row partials. After this sweep the
+// dot products of the new z in the same pass: k_mg_rbgs_vec<T, VPL, false,
+// 16 / sizeof(T), true> (T = float or double) followed by k_dot2_ch's row
+// partials. After this sweep the
 // row's z is final in both halves, the colour-0 half computed here and the
 // other half the values the sweep loaded as x neighbours; the row's z goes
 // to LDS, and each lane then reads r, Ad and z at the split positions
 // 1 + 2l + 128m (+1) of k_dot2_ch and forms r z and z Ad, summed in that
 // kernel's order (m, then the pair; wave tree over the lanes), so the
 // partials are bit-identical and z is not read again from memory.
-template <int VPL>
-__global__ __launch_bounds__(256) void k_mg_rbgs_dot(Lv L, float* __restrict__ e, const float* __restrict__ f,
+template <typename T, int VPL>
+__global__ __launch_bounds__(256) void k_mg_rbgs_dot(Lv L, T* __restrict__ e, const T* __restrict__ f,
                                                      const double* __restrict__ r, const double* __restrict__ Ad,
                                                      int flexible, double* __restrict__ part0,
                                                      double* __restrict__ part1) {
-  using C = float;
-  constexpr int CPL = 4;
+  using C = T;
+  constexpr int CPL = (int)(16 / sizeof(T));
   constexpr int NM = VPL / CPL;  // smoother chunks per lane
   constexpr int NH = 64 * VPL;   // cells per half row
   constexpr int NX = 2 * NH;
   constexpr int DM = NX / 128;  // dot chunks (k_dot2_ch: 128 positions each)
   __shared__ C ysm[6 * NH];
-  __shared__ float zs[4][NX];
+  __shared__ C zs[4][NX];
   LROW_SETUP
   const int w = (int)(threadIdx.x >> 6);
   const int nx = L.n[0], nE = nx >> 1;
@@ -415,11 +416,11 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_dot(Lv L, float* __restrict__ e
 #pragma unroll
   for (int m = 0; m < NM; m++) {
     const int p = 1 + CPL * lane + 64 * CPL * m;
-    Vec<float, CPL>::ld(f + rb + off + p, fv[m]);
-    Vec<float, CPL>::ld(e + rb + oo + p, ov[m]);
-    Vec<float, CPL>::ld(e + zpr + off + p, zp[m]);
-    Vec<float, CPL>::ld(e + zmr + off + p, zm[m]);
-    if (w == 0 || w == 3) Vec<float, CPL>::ld(e + hr + off + p, hv[m]);
+    Vec<T, CPL>::ld(f + rb + off + p, fv[m]);
+    Vec<T, CPL>::ld(e + rb + oo + p, ov[m]);
+    Vec<T, CPL>::ld(e + zpr + off + p, zp[m]);
+    Vec<T, CPL>::ld(e + zmr + off + p, zm[m]);
+    if (w == 0 || w == 3) Vec<T, CPL>::ld(e + hr + off + p, hv[m]);
   }
 #pragma unroll
   for (int m = 0; m < NM; m++)
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_dot(Lv L, float* __restrict__ e
       ev[t] = (fv[m][t] + S) / c.diag;
       zs[w][off + h - 1] = ev[t];
     }
-    Vec<float, CPL>::st(e + rb + off + 1 + CPL * lane + 64 * CPL * m, ev);
+    Vec<T, CPL>::st(e + rb + off + 1 + CPL * lane + 64 * CPL * m, ev);
   }
   __syncthreads();
   double a0 = 0.0, a1 = 0.0;
@@ -1227,20 +1228,28 @@ void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipS
 #undef SMAC_RBGS
 }
 
-bool mg_rbgs_dot_applies(const Lv& L, const float* e, const float* f) {
+template <typename T>
+bool mg_rbgs_dot_applies(const Lv& L, const T* e, const T* f) {
   const int half = L.n[0] - (L.n[0] >> 1);
   const int vpl = row_vpl(half);
   return !L.b[0] && (vpl == 4 || vpl == 8) && L.n[0] == 128 * vpl && L.n[1] % 4 == 0 && rows_aligned(L, e) &&
          rows_aligned(L, f);
 }
-void launch_mg_rbgs_dot(const Lv& L, float* e, const float* f, const double* r, const double* Ad, int flexible,
-                        double* p0, double* p1, hipStream_t st) {
+template <typename T>
+void launch_mg_rbgs_dot(const Lv& L, T* e, const T* f, const double* r, const double* Ad, int flexible, double* p0,
+                        double* p1, hipStream_t st) {
   const int vpl = row_vpl(L.n[0] - (L.n[0] >> 1));
   if (vpl == 4)
-    hipLaunchKernelGGL(k_mg_rbgs_dot<4>, lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
+    hipLaunchKernelGGL((k_mg_rbgs_dot<T, 4>), lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
   else
-    hipLaunchKernelGGL(k_mg_rbgs_dot<8>, lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
+    hipLaunchKernelGGL((k_mg_rbgs_dot<T, 8>), lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
 }
+template bool mg_rbgs_dot_applies<float>(const Lv&, const float*, const float*);
+template bool mg_rbgs_dot_applies<double>(const Lv&, const double*, const double*);
+template void launch_mg_rbgs_dot<float>(const Lv&, float*, const float*, const double*, const double*, int, double*,
+                                        double*, hipStream_t);
+template void launch_mg_rbgs_dot<double>(const Lv&, double*, const double*, const double*, const double*, int,
+                                         double*, double*, hipStream_t);
 
 template <typename T>
 void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc, int red,

@@ -107,6 +107,27 @@ def test_periodic_tg_mg(hip, oracle_mod):
     _compare(g, o, (n, n, n))
 
 
+@pytest.mark.parametrize("flexible", [0, 1])
+def test_wide_rows_mg_fp64(hip, oracle_mod, flexible):
+    """Rows of 512 cells: the V-cycle's last fine half sweep then also forms
+    the PCG's r.z (and z.Ad) row partials (the fused form of the 512- and
+    1024-cell rows); FP64 steps exact against the oracle, iteration counts
+    equal."""
+    from mydemcfd_amd import cases
+
+    n = (512, 8, 8)
+    print(f"case: periodic {n} mg fp64 flexible={flexible} x3", flush=True)
+    desc = cases.periodic_desc(n)
+    g, o = _pair(hip, oracle_mod, desc)
+    rng = np.random.default_rng(11)
+    _upload_both(g, o, {f: rng.standard_normal(g.shape(f)) for f in ("u", "v", "w")})
+    for s in range(3):
+        sg = g.step(1e-3, precond="mg", tol=1e-10, max_iter=300, flexible=flexible)
+        so = o.step(1e-3, precond="mg", tol=1e-10, max_iter=300, flexible=flexible)
+        assert sg.iters == so.iters and sg.converged, (s, sg.as_dict(), so.as_dict())
+    _compare(g, o, n, tol=0.0)
+
+
 def test_cfl(hip, oracle_mod):
     from mydemcfd_amd import cases
 
