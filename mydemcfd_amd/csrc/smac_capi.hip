@@ -80,7 +80,8 @@ struct smac_ctx {
   // need not clear it
   bool dp_zero = true;
   // the PCG's dot2 of the next V-cycle's z rides on its last fine half sweep
-  // (FP32 and FP64 forms): the operands while dot_req, dot_done once it ran
+  // (uniform coefficients, the vector and two-rows-per-wave smoother forms):
+  // the operands while dot_req, dot_done once it ran
   bool dot_req = false, dot_done = false;
   const double* dot_r = nullptr;
   const double* dot_Ad = nullptr;
@@ -1126,8 +1127,8 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
      launch_mg_prolong<T>(C.lv, V.lv, (const T*)C.e, e, nu2 > 0 ? 0 : -1, c->st));
   if (fine) {
     bool dot = false;
-    constexpr bool dotT = std::is_same<T, float>::value || std::is_same<T, double>::value;
-    if constexpr (dotT) dot = c->dot_req && nu2 > 0 && mg_rbgs_dot_applies<T>(V.lv, e, f);
+    constexpr bool dotT = true;
+    dot = c->dot_req && nu2 > 0 && mg_rbgs_dot_applies<T>(V.lv, e, f);
     if (nu2 > 0)
       KT_N(c, "mg_rbgs_l0", dot ? 2 * nu2 - 1 : 2 * nu2, {
         for (int s = 0; s < nu2; s++) {

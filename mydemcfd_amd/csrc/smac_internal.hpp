@@ -258,7 +258,8 @@ void launch_vof_substep(const Geo& g, const double h[3], const Vel& V, double** 
 template <typename T>
 void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipStream_t st);
 // the last colour-0 half sweep of a V-cycle with the PCG's dot2 row partials
-// (FP32 or FP64 storage, uniform coefficients, the vector form with LDS y rows)
+// (uniform coefficients; the vector form with LDS y rows for FP32 and FP64,
+// the two-rows-per-wave form for every storage type)
 template <typename T>
 bool mg_rbgs_dot_applies(const Lv& L, const T* e, const T* f);
 template <typename T>

@@ -610,6 +610,128 @@ __global__ __launch_bounds__(256) void k_mg_rbgs_v2(Lv L, T* __restrict__ e, con
   Vec<T, CPL>::st(e + rb + off + p, ev);
 }
 
+// k_mg_rbgs_v2<T, false, CPL, 2> as the V-cycle's last fine half sweep
+// (colour 0) with dot2's row partials, as k_mg_rbgs_dot does for the vector
+// form: the eight rows' z (the stored values) go to LDS, then each wave forms
+// the partials of its two rows with all 64 lanes at k_dot2_ch's positions.
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_mg_rbgs_v2_dot(Lv L, T* __restrict__ e, const T* __restrict__ f,
+                                                        const double* __restrict__ r, const double* __restrict__ Ad,
+                                                        int flexible, double* __restrict__ part0,
+                                                        double* __restrict__ part1) {
+  using C = typename St<T>::C;
+  constexpr int RPW = 2;
+  constexpr int LPR = 64 / RPW;  // lanes per row
+  constexpr int RB = 4 * RPW;    // rows per block
+  constexpr int NH = LPR * CPL;  // cells per half row
+  constexpr int NX = 2 * NH;
+  constexpr int DM = NX / 128;   // dot chunks
+  __shared__ C ysm[(RB + 2) * NH];
+  __shared__ C zs[RB][NX];
+  const int lane = threadIdx.x & 63;
+  const int w = (int)(threadIdx.x >> 6);
+  const int sub = lane / LPR, l = lane % LPR;
+  const int r8 = RPW * w + sub;
+  const int vb = xcd_block((int)blockIdx.x, (int)gridDim.x) * RB;
+  int j, k;
+  band_jk(vb + r8, L.n[1], L.n[2], j, k);
+  const size_t rb = lrow(L, j, k);
+  const int nx = L.n[0], nE = nx >> 1;
+  const int par = (j + k) & 1;
+  const int off = par ? nE : 0;
+  const int oo = par ? 0 : nE;
+  const size_t ypr = lrow(L, LW(L, 1, j + 1), k), ymr = lrow(L, LW(L, 1, j - 1), k);
+  const size_t zpr = lrow(L, j, LW(L, 2, k + 1)), zmr = lrow(L, j, LW(L, 2, k - 1));
+  const int p = 1 + CPL * l;
+  const bool perx = L.per[0] != 0;
+  const CoefC<C> crow = level_coef<C, false>(L, 2, j, k);
+  auto coef = [&](int i) -> CoefC<C> {
+    if (!perx && (i == 1 || i == nx)) return level_coef<C, false>(L, i, j, k);
+    return crow;
+  };
+  C fv[CPL], ov[CPL], yp[CPL], ym[CPL], zp[CPL], zm[CPL], hv[CPL];
+  const bool lo = r8 == 0, hi = r8 == RB - 1;
+  Vec<T, CPL>::ld(f + rb + off + p, fv);
+  Vec<T, CPL>::ld(e + rb + oo + p, ov);
+  Vec<T, CPL>::ld(e + zpr + off + p, zp);
+  Vec<T, CPL>::ld(e + zmr + off + p, zm);
+  if (lo || hi) Vec<T, CPL>::ld(e + (lo ? ymr : ypr) + off + p, hv);
+#pragma unroll
+  for (int t = 0; t < CPL; t++) {
+    const int o = CPL * l + t;
+    ysm[(r8 + 1) * NH + o] = ov[t];
+    if (lo) ysm[o] = hv[t];
+    if (hi) ysm[(RB + 1) * NH + o] = hv[t];
+    zs[r8][oo + o] = ov[t];  // the other half's z (split position oo + 1 + o)
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < CPL; t++) {
+    const int o = CPL * l + t;
+    ym[t] = ysm[r8 * NH + o];
+    yp[t] = ysm[(r8 + 2) * NH + o];
+  }
+  C nb;
+  if (!par) nb = __shfl(ov[0], sub * LPR + (l + 1) % LPR);
+  else nb = __shfl(ov[CPL - 1], sub * LPR + (l + LPR - 1) % LPR);
+  C ev[CPL];
+#pragma unroll
+  for (int t = 0; t < CPL; t++) {
+    const int h = p + t;
+    const int i = par ? 2 * h - 1 : 2 * h;
+    C xp, xm;
+    if (!par) {
+      xm = ov[t];
+      xp = t + 1 < CPL ? ov[t + 1 < CPL ? t + 1 : 0] : nb;
+      if (!perx && i == nx) xp = (C)0;
+    } else {
+      xp = ov[t];
+      xm = t > 0 ? ov[t > 0 ? t - 1 : 0] : nb;
+      if (!perx && i == 1) xm = (C)0;
+    }
+    const CoefC<C> c = coef(i);
+    C S = c.a[0] * xp + c.a[1] * xm + c.a[2] * yp[t] + c.a[3] * ym[t] + c.a[4] * zp[t] + c.a[5] * zm[t];
+    ev[t] = sm_div(fv[t] + S, c.diag);
+    zs[r8][off + CPL * l + t] = (C)stored_value<T>((double)ev[t]);
+  }
+  Vec<T, CPL>::st(e + rb + off + p, ev);
+  // the dot operands of the wave's two rows (issued before the barrier)
+  double vr[RPW][DM][2], vA[RPW][DM][2];
+  int drow[RPW];
+#pragma unroll
+  for (int q = 0; q < RPW; q++) {
+    int jq, kq;
+    band_jk(vb + RPW * w + q, L.n[1], L.n[2], jq, kq);
+    drow[q] = (kq - 1) * L.n[1] + (jq - 1);
+    const size_t rq = lrow(L, jq, kq);
+#pragma unroll
+    for (int m = 0; m < DM; m++) {
+      const size_t o = rq + 1 + 2 * lane + 128 * m;
+      Vec<double, 2>::ld(r + o, vr[q][m]);
+      if (flexible) Vec<double, 2>::ld(Ad + o, vA[q][m]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < RPW; q++) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < DM; m++)
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const double zv = (double)zs[RPW * w + q][2 * lane + 128 * m + t];
+        a0 += vr[q][m][t] * zv;
+        if (flexible) a1 += zv * vA[q][m][t];
+      }
+    a0 = wave_tree_sum(a0);
+    if (flexible) a1 = wave_tree_sum(a1);
+    if (lane == 0) {
+      part0[drow[q]] = a0;
+      if (flexible) part1[drow[q]] = a1;
+    }
+  }
+}
+
 // 1-D restriction stencil of coarse index I (transpose of cell-centred linear
 // prolongation; at a wall the mirrored weight folds onto the boundary cell)
 __device__ __forceinline__ int rweights(int per, int nc, int I, int ix[4], double w[4]) {
@@ -1228,28 +1350,49 @@ void launch_mg_rbgs(const Lv& L, T* e, const T* f, int col, int zero_start, hipS
 #undef SMAC_RBGS
 }
 
+// which fused form takes the level's colour-0 half sweep (the form
+// launch_mg_rbgs would pick, uniform coefficients): 2 = two rows per wave,
+// 1 = the vector form (FP32, FP64), 0 = none
+template <typename T>
+static int rbgs_dot_form(const Lv& L, const T* e, const T* f) {
+  constexpr int C16 = (int)(16 / sizeof(T));
+  if (L.b[0] || !rows_aligned(L, e) || !rows_aligned(L, f)) return 0;
+  if (L.n[0] == 64 * C16 && L.n[1] % 8 == 0) return 2;
+  if (L.n[0] == 32 * C16 && L.n[1] % 16 == 0) return 0;
+  const int vpl = row_vpl(L.n[0] - (L.n[0] >> 1));
+  if (sizeof(T) >= 4 && (vpl == 4 || vpl == 8) && L.n[0] == 128 * vpl && L.n[1] % 4 == 0) return 1;
+  return 0;
+}
 template <typename T>
 bool mg_rbgs_dot_applies(const Lv& L, const T* e, const T* f) {
-  const int half = L.n[0] - (L.n[0] >> 1);
-  const int vpl = row_vpl(half);
-  return !L.b[0] && (vpl == 4 || vpl == 8) && L.n[0] == 128 * vpl && L.n[1] % 4 == 0 && rows_aligned(L, e) &&
-         rows_aligned(L, f);
+  return rbgs_dot_form(L, e, f) != 0;
 }
 template <typename T>
 void launch_mg_rbgs_dot(const Lv& L, T* e, const T* f, const double* r, const double* Ad, int flexible, double* p0,
                         double* p1, hipStream_t st) {
-  const int vpl = row_vpl(L.n[0] - (L.n[0] >> 1));
-  if (vpl == 4)
-    hipLaunchKernelGGL((k_mg_rbgs_dot<T, 4>), lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
-  else
-    hipLaunchKernelGGL((k_mg_rbgs_dot<T, 8>), lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
+  constexpr int C16 = (int)(16 / sizeof(T));
+  if (rbgs_dot_form(L, e, f) == 2) {
+    hipLaunchKernelGGL((k_mg_rbgs_v2_dot<T, C16>), dim3((unsigned)(L.n[1] * L.n[2] / 8)), ROWB, 0, st, L, e, f, r,
+                       Ad, flexible, p0, p1);
+    return;
+  }
+  if constexpr (sizeof(T) >= 4) {
+    const int vpl = row_vpl(L.n[0] - (L.n[0] >> 1));
+    if (vpl == 4)
+      hipLaunchKernelGGL((k_mg_rbgs_dot<T, 4>), lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
+    else
+      hipLaunchKernelGGL((k_mg_rbgs_dot<T, 8>), lrows_grid(L), ROWB, 0, st, L, e, f, r, Ad, flexible, p0, p1);
+  }
 }
 template bool mg_rbgs_dot_applies<float>(const Lv&, const float*, const float*);
 template bool mg_rbgs_dot_applies<double>(const Lv&, const double*, const double*);
+template bool mg_rbgs_dot_applies<bf16s>(const Lv&, const bf16s*, const bf16s*);
 template void launch_mg_rbgs_dot<float>(const Lv&, float*, const float*, const double*, const double*, int, double*,
                                         double*, hipStream_t);
 template void launch_mg_rbgs_dot<double>(const Lv&, double*, const double*, const double*, const double*, int,
                                          double*, double*, hipStream_t);
+template void launch_mg_rbgs_dot<bf16s>(const Lv&, bf16s*, const bf16s*, const double*, const double*, int, double*,
+                                        double*, hipStream_t);
 
 template <typename T>
 void launch_mg_restrict(const Lv& F, const Lv& C, const T* e, const T* f, typename St<T>::C* tb, T* fc, int red,

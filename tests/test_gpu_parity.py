@@ -107,15 +107,15 @@ def test_periodic_tg_mg(hip, oracle_mod):
     _compare(g, o, (n, n, n))
 
 
+@pytest.mark.parametrize("n", [(512, 8, 8), (128, 8, 8)])
 @pytest.mark.parametrize("flexible", [0, 1])
-def test_wide_rows_mg_fp64(hip, oracle_mod, flexible):
-    """Rows of 512 cells: the V-cycle's last fine half sweep then also forms
-    the PCG's r.z (and z.Ad) row partials (the fused form of the 512- and
-    1024-cell rows); FP64 steps exact against the oracle, iteration counts
-    equal."""
+def test_wide_rows_mg_fp64(hip, oracle_mod, flexible, n):
+    """Rows of 512 cells (the vector smoother) and of 128 cells with ny % 8 == 0
+    (two rows per wave): the V-cycle's last fine half sweep then also forms the
+    PCG's r.z (and z.Ad) row partials; FP64 steps exact against the oracle,
+    iteration counts equal."""
     from mydemcfd_amd import cases
 
-    n = (512, 8, 8)
     print(f"case: periodic {n} mg fp64 flexible={flexible} x3", flush=True)
     desc = cases.periodic_desc(n)
     g, o = _pair(hip, oracle_mod, desc)
