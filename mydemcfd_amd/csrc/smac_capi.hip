@@ -959,7 +959,8 @@ static int mg_coeffs(smac_ctx* c) {
   // (row-major: the wave of row r reads the row's entries at consecutive
   // addresses), on the side stream after everything enqueued so far on the
   // context stream (the coarse coefficients above, the last solve's reads of
-  // the old inverse); mg_wait_pinv orders the next solve after it
+  // the old inverse); the pressure solve waits for ev_side_out before its
+  // first V-cycle
   HIPOK(hipEventRecord(c->ev_side_in, c->st));
   HIPOK(hipStreamWaitEvent(c->st_side, c->ev_side_in, 0));
   launch_dense_pinv(C, c->has_outlet ? 0 : 1, c->cinv_ws, c->cinv, c->st_side);
@@ -1126,9 +1127,8 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
   KT(c, fine ? "mg_prolong_l0" : "mg_coarse_levels",
      launch_mg_prolong<T>(C.lv, V.lv, (const T*)C.e, e, nu2 > 0 ? 0 : -1, c->st));
   if (fine) {
-    bool dot = false;
-    constexpr bool dotT = true;
-    dot = c->dot_req && nu2 > 0 && mg_rbgs_dot_applies<T>(V.lv, e, f);
+    // the last half sweep also forms the PCG dots of the new z when asked
+    const bool dot = c->dot_req && nu2 > 0 && mg_rbgs_dot_applies<T>(V.lv, e, f);
     if (nu2 > 0)
       KT_N(c, "mg_rbgs_l0", dot ? 2 * nu2 - 1 : 2 * nu2, {
         for (int s = 0; s < nu2; s++) {
@@ -1136,12 +1136,11 @@ static void vcycle(smac_ctx* c, int l, int nu1, int nu2) {
           if (!dot || s + 1 < nu2) launch_mg_rbgs<T>(V.lv, e, f, 0, 0, c->st);
         }
       });
-    if constexpr (dotT)
-      if (dot) {
-        KT(c, "mg_rbgs_dot",
-           launch_mg_rbgs_dot<T>(V.lv, e, f, c->dot_r, c->dot_Ad, c->dot_flex, c->dot_p0, c->dot_p1, c->st));
-        c->dot_done = true;
-      }
+    if (dot) {
+      KT(c, "mg_rbgs_dot",
+         launch_mg_rbgs_dot<T>(V.lv, e, f, c->dot_r, c->dot_Ad, c->dot_flex, c->dot_p0, c->dot_p1, c->st));
+      c->dot_done = true;
+    }
   } else {
     for (int s = 0; s < nu2; s++) {
       KT(c, "mg_coarse_levels", launch_mg_rbgs<T>(V.lv, e, f, 1, 0, c->st));
