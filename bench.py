@@ -31,7 +31,8 @@ width of the V-cycle):
          V-cycle's zero-start red half sweep, e = f/diag), dot2 8 + W_z (+8 flexible), xpay 32 + W_z (z, d, dp: the
          dp += alpha d of the previous update rides on the direction update),
          dir_spmv 40 + W_z (xpay fused into the next SpMV: z, d_old, dp read,
-         dp, d, Ad written), dp_flush 24 (that dp update after the last iteration; a step
+         dp, d, Ad written), first_spmv 24 + W_z (the first SpMV of a
+         solve with d = z and r.d: z, r read, d, Ad written), dp_flush 24 (that dp update after the last iteration; a step
          leaves it to the correction, which then also reads d: +8 there),
          mg_rbgs_l0 1.5*W_s (a half sweep: other colour, own rhs, own
          colour written), mg_rbgs_l0_zs 1.0*W_s (the first red half sweep of
@@ -69,6 +70,7 @@ KERNEL_BYTES = {  # bytes per cell of the level, per launch; W = storage width (
     "dot2": lambda W, flex: 8.0 + W + (8.0 if flex else 0.0),
     "xpay": lambda W, flex: 32.0 + W,
     "dir_spmv": lambda W, flex: 40.0 + W,
+    "first_spmv": lambda W, flex: 24.0 + W,
     "dp_flush": lambda W, flex: 24.0,
     "mg_rbgs_l0": lambda W, flex: 1.5 * W,
     "mg_rbgs_l0_zs": lambda W, flex: 1.0 * W,

@@ -1385,8 +1385,14 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
     fin_add(fa, P[0], 0, S_SUMZ);
     launch_finalize(fa, c->S, s);
   }
-  // z -= mean(z) (pure Neumann) in the same pass as d = z and r.z
-  launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s, -1. * size_inv, c->S, pure ? S_SUMZ : -1);
+  // z -= mean(z) (pure Neumann) in the same pass as d = z and r.z; rows of the
+  // pair form also take the first Ad = A d there (its d.Ad partials in P[4])
+  const bool first_fused = first_spmv_applies<T>(g, z, c->r, c->dv, c->Ad);
+  if (first_fused)
+    KT(c, "first_spmv", launch_first_spmv<T>(g, bfp, z, c->r, c->dv, c->Ad, P[4], P[0], -1. * size_inv, c->S,
+                                             pure ? S_SUMZ : -1, s));
+  else
+    launch_copy_dot<T>(g, z, c->dv, c->r, P[0], s, -1. * size_inv, c->S, pure ? S_SUMZ : -1);
   {
     FinalizeArgs fa = fin(c);
     fin_add(fa, P[0], 0, S_RZ);
@@ -1422,7 +1428,7 @@ static int solve_t(smac_ctx* c, double dt, const smac_solver_opts& o, smac_solve
                           ? mg_zs_division<T>(c->lev[0].lv, (const T*)c->lev[0].e, (const T*)c->lev[0].f)
                           : 0;
   c->zs_done = false;
-  bool ad_ready = false;
+  bool ad_ready = first_fused;
   for (iter = 0; iter < o.max_iter && !conv && !nan; ++iter) {
     if (!ad_ready) KT(c, "spmv_dot", launch_spmv_dot(g, bfp, c->dv, c->Ad, P[0], s));
     double* pap = ad_ready ? P[4] : P[0];

@@ -171,7 +171,7 @@ void launch_sum(const Geo& g, const Z* x, double* p0, hipStream_t st);
 // bf == nullptr: one-phase operator (beta0); else face coefficients bf->b
 void launch_jacobi(const Geo& g, const BetaF* bf, const double* r, double* z, hipStream_t st);
 template <typename Z>
-// d = z, part0 <- sum r*z (natural order); slot >= 0: z + a S[slot] first, as stored in Z
+// d = z, part0 <- sum r*z (pair order); slot >= 0: z + a S[slot] first, as stored in Z
 void launch_copy_dot(const Geo& g, const Z* z, double* d, const double* r, double* p0, hipStream_t st,
                      double a = 0.0, const double* S = nullptr, int slot = -1);
 void launch_spmv_dot(const Geo& g, const BetaF* bf, const double* d, double* Ad, double* p0, hipStream_t st);
@@ -184,6 +184,16 @@ bool dir_spmv_applies(const Geo& g, const Z* z, const double* dold, const double
 template <typename Z>
 void launch_dir_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* dold, double* dnew, double* dp,
                      double* Ad, double* p0, const double* S, int slot_alpha, int slot_beta, hipStream_t st);
+// k_copy_dot and the SpMV after it in one pass, the first SpMV of a solve:
+// d = z (slot >= 0: z + a S[slot] rounded to the storage of z), Ad = A d,
+// p_dad <- the rows' d.Ad and p_rz <- their r.d, both in pair order; the same
+// pair form as launch_dir_spmv (every array's rows aligned, z's included):
+// first_spmv_applies
+template <typename Z>
+bool first_spmv_applies(const Geo& g, const Z* z, const double* r, const double* d, const double* Ad);
+template <typename Z>
+void launch_first_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* r, double* d, double* Ad,
+                       double* p_dad, double* p_rz, double a, const double* S, int slot, hipStream_t st);
 // r -= alpha Ad with |r|^2, max|r| and f0 = (F) r; e0 != nullptr and zs != 0:
 // also the zero-start red half sweep of level 0 (uniform coefficients;
 // zs = 1: IEEE division, 2: the FP32 reciprocal). Returns whether it did that

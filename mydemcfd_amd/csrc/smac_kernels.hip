@@ -841,23 +841,30 @@ __global__ __launch_bounds__(256) void k_jacobi(Geo g, BetaF bf, const double* _
   }
 }
 
-// d = z; part0 <- sum r*z. slot >= 0: z + a S[slot] first (k_add_scalar's
-// arithmetic) rounded to the storage of z, the value a separate pass would
-// store and read back (z itself is not written: the next V-cycle overwrites it
-// before anything reads it)
+// d = z; part0 <- sum r*z in pair order (lane l: split positions 1+2l+128m,
+// then +1; the oracle's RED_DOT_S2, the order of the iteration's r.z).
+// slot >= 0: z + a S[slot] first (k_add_scalar's arithmetic) rounded to the
+// storage of z, the value a separate pass would store and read back (z itself
+// is not written: the next V-cycle overwrites it before anything reads it)
 template <typename Z>
 __global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z, double* __restrict__ d,
                                                   const double* __restrict__ r, double* __restrict__ part0,
                                                   double a, const double* __restrict__ S, int slot) {
   ROW_SETUP
+  const int nx = g.n[0];
   double acc = 0.0;
   const double s = slot >= 0 ? S[slot] : 0.0;
-  for (int i = 1 + lane; i <= g.n[0]; i += 64) {
-    const size_t o = rb + spos(i, g.n[0]);
-    double zz = (double)St<Z>::ld(z, o);
-    if (slot >= 0) zz = stored_value<Z>((double)(typename St<Z>::C)(zz + a * s));
-    d[o] = zz;
-    acc += r[o] * zz;
+  for (int q0 = 1 + 2 * lane; q0 <= nx; q0 += 128) {
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int q = q0 + t;
+      if (q > nx) continue;
+      const size_t o = rb + q;
+      double zz = (double)St<Z>::ld(z, o);
+      if (slot >= 0) zz = stored_value<Z>((double)(typename St<Z>::C)(zz + a * s));
+      d[o] = zz;
+      acc += r[o] * zz;
+    }
   }
   acc = wave_tree_sum(acc);
   if (lane == 0) part0[row] = acc;
@@ -869,6 +876,10 @@ __global__ __launch_bounds__(256) void k_copy_dot(Geo g, const Z* __restrict__ z
 // block may still read d_old), and dp += alpha d_old (the deferred half of
 // k_update_p_r) rides along; the lane's pairs of both half rows are four
 // consecutive natural cells, two 16-byte dp pairs.
+// FIRST (the first SpMV of a solve, in place of k_copy_dot and the SpMV after
+// it): d = z, shifted by a S[slot_beta] and rounded as k_copy_dot does when
+// slot_beta >= 0; no d_old, no dp; the row's r.d in k_copy_dot's order goes to
+// prz.
 template <typename Z>
 struct DirArgs {
   const Z* z;
@@ -876,6 +887,9 @@ struct DirArgs {
   double* dp;
   const double* S;
   int slot_alpha, slot_beta;
+  const double* r = nullptr;
+  double* prz = nullptr;
+  double a = 0.0;
 };
 
 // Ad = A d, part0 <- sum d*Ad (k_make_pAp, G_PCGSolver.cu:87-124), the dot in
@@ -888,11 +902,12 @@ struct DirArgs {
 // j0-1 and j0+4, through LDS for the y neighbours. NM = 0: any nx, loads per
 // neighbour, same order. DIR (NM > 0 only): d is written, not read; its values
 // come from da as described above.
-template <bool VAR, int NM, bool LDSY, typename Z = double, bool DIR = false>
+template <bool VAR, int NM, bool LDSY, typename Z = double, bool DIR = false, bool FIRST = false>
 __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, double* __restrict__ d,
                                                   double* __restrict__ Ad, double* __restrict__ part0,
                                                   DirArgs<Z> da) {
   static_assert(!DIR || NM > 0, "the fused direction update needs the pair form");
+  static_assert(!FIRST || DIR, "the first direction is formed as in DIR");
   constexpr int NMC = NM > 0 ? NM : 1;
   __shared__ double ysm[LDSY ? 6 * 128 * NMC : 1];
   ROW_SETUP
@@ -921,11 +936,26 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, double* __res
     constexpr int NH = NM / 2;  // pair chunks per half row
     constexpr int NX = 128 * NM;
     double xc[NM][2], yp[NM][2], ym[NM][2], zp[NM][2], zm[NM][2];
-    const double beta = DIR ? da.S[da.slot_beta] : 0.0;
-    // a pair of d at split position a: loaded, or (DIR) z + beta d_old
+    using ZC = typename St<Z>::C;
+    const double beta = (DIR && !FIRST) ? da.S[da.slot_beta] : 0.0;
+    const bool shift = FIRST && da.slot_beta >= 0;
+    const double zsum = shift ? da.S[da.slot_beta] : 0.0;
+    // (FIRST) the direction value of z, as k_copy_dot forms it
+    auto first_d = [&](ZC zc) -> double {
+      double v = (double)zc;
+      if (shift) v = stored_value<Z>((double)(ZC)(v + da.a * zsum));
+      return v;
+    };
+    // a pair of d at split position a: loaded, or (DIR) z + beta d_old, or
+    // (FIRST) the value of z
     auto ld2 = [&](size_t a, double* o2) {
-      if constexpr (DIR) {
-        typename St<Z>::C zz[2];
+      if constexpr (FIRST) {
+        ZC zz[2];
+        Vec<Z, 2>::ld(da.z + a, zz);
+        o2[0] = first_d(zz[0]);
+        o2[1] = first_d(zz[1]);
+      } else if constexpr (DIR) {
+        ZC zz[2];
         double dd[2];
         Vec<Z, 2>::ld(da.z + a, zz);
         Vec<double, 2>::ld(da.dold + a, dd);
@@ -935,7 +965,22 @@ __global__ __launch_bounds__(256) void k_spmv_dot(Geo g, BetaF bf, double* __res
         Vec<double, 2>::ld(d + a, o2);
       }
     };
-    if constexpr (DIR) {
+    if constexpr (FIRST) {
+      // the row itself: d stored and r.d in pair order
+      double rz = 0.0;
+#pragma unroll
+      for (int m = 0; m < NM; m++) {
+        const size_t a = rb + 1 + 2 * lane + 128 * m;
+        double rr[2];
+        ld2(a, xc[m]);
+        Vec<double, 2>::ld(da.r + a, rr);
+        Vec<double, 2>::st(d + a, xc[m]);
+        rz += rr[0] * xc[m][0];
+        rz += rr[1] * xc[m][1];
+      }
+      rz = wave_tree_sum(rz);
+      if (lane == 0) da.prz[row] = rz;
+    } else if constexpr (DIR) {
       // the row itself: d = z + beta d_old stored, dp += alpha d_old on the
       // natural cells 4l+256m+1..+4 (odd cells from the second half row)
       const double alpha = da.S[da.slot_alpha];
@@ -1797,6 +1842,41 @@ void launch_dir_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* do
 #undef SMAC_DSP_V
 #undef SMAC_DSP
 }
+template <typename Z>
+bool first_spmv_applies(const Geo& g, const Z* z, const double* r, const double* d, const double* Ad) {
+  int nm = pair_nm(g, d);
+  if (nm && (pair_nm(g, r) != nm || pair_nm(g, Ad) != nm)) nm = 0;
+  if (nm && ((uintptr_t)(z + 1) % 128) != 0) nm = 0;
+  return nm == 2 || nm == 4;
+}
+template <typename Z>
+void launch_first_spmv(const Geo& g, const BetaF* bf, const Z* z, const double* r, double* d, double* Ad,
+                       double* p_dad, double* p_rz, double a, const double* S, int slot, hipStream_t st) {
+  const int nm = pair_nm(g, d);
+  const bool ldsy = g.n[1] % 4 == 0;
+  DirArgs<Z> da{z, nullptr, nullptr, S, -1, slot};
+  da.r = r;
+  da.prz = p_rz;
+  da.a = a;
+#define SMAC_FSP(V, N, L)                                                                                   \
+  hipLaunchKernelGGL((k_spmv_dot<V, N, L, Z, true, true>), rows_grid(g), ROWB, 0, st, g, bf ? *bf : BetaF{}, d, \
+                     Ad, p_dad, da)
+#define SMAC_FSP_V(V)                  \
+  if (nm == 2) {                       \
+    if (ldsy) SMAC_FSP(V, 2, true);    \
+    else SMAC_FSP(V, 2, false);        \
+  } else {                             \
+    if (ldsy) SMAC_FSP(V, 4, true);    \
+    else SMAC_FSP(V, 4, false);        \
+  }
+  if (bf) {
+    SMAC_FSP_V(true)
+  } else {
+    SMAC_FSP_V(false)
+  }
+#undef SMAC_FSP_V
+#undef SMAC_FSP
+}
 template <typename F>
 bool launch_update(const Geo& g, double* r, const double* Ad, const double* S, int slot_alpha, double* p0,
                    double* p1, F* f0, hipStream_t st, F* e0, int zs, const Lv* L0) {
@@ -1867,7 +1947,10 @@ void launch_xpay(const Geo& g, const Z* z, double* d, double* dp, const double* 
   template bool dir_spmv_applies<Z>(const Geo&, const Z*, const double*, const double*, const double*,     \
                                     const double*);                                                       \
   template void launch_dir_spmv<Z>(const Geo&, const BetaF*, const Z*, const double*, double*, double*, double*, \
-                                   double*, const double*, int, int, hipStream_t);
+                                   double*, const double*, int, int, hipStream_t);                         \
+  template bool first_spmv_applies<Z>(const Geo&, const Z*, const double*, const double*, const double*);  \
+  template void launch_first_spmv<Z>(const Geo&, const BetaF*, const Z*, const double*, double*, double*,   \
+                                     double*, double*, double, const double*, int, hipStream_t);
 SMAC_INST_Z(double)
 SMAC_INST_Z(float)
 SMAC_INST_Z(bf16s)

@@ -1533,7 +1533,8 @@ int smac_pressure_solve(smac_ctx* c, double dt, const smac_solver_opts* opts, sm
     precond(c, &o);
     if (pure) add_scalar_interior(c, c->z, -1. * size_inv, csum(c, RED_SUM, c->z, NULL));
     memcpy(c->dvec, c->z, c->tot * sizeof(double));
-    double rz = csum(c, RED_DOT, c->r, c->z);
+    /* pair order, the order of every later r.z */
+    double rz = csum(c, RED_DOT_S2, c->r, c->z);
 
     int iter = 0, conv = 0;
     double rr = bb, linf = 0.0;
